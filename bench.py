@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X projector path on one GPU.
+
+Workload (N=1): plain FP32 OS-SART on a 512x512x256 volume from 121 projections
+of 512x512 (-60..+60 deg, 1 deg step), 10 subsets (13 projections per update,
+TomoJ's update = 13), relaxation 1.0, seeded 40-ellipsoid phantom with analytic
+projections + 1% Gaussian noise generated on the device.  One step = one full
+OS-SART iteration (every projection forward-projected, differenced and
+backprojected once, 10 volume updates).  Inputs are resident in HBM before the
+timed region.  Extras on the same line: voxel-ray GUPS, the per-kernel HBM
+roofline from HIP events, the literal 512^3 SIRT iteration rate, the weighted
+backprojection pass, and TomoJ's CPU algorithm (the C restatement in oracle/)
+timed on the host cores for one full iteration of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SIRT iters/s + voxel-ray GUPS at 512³ (HBM %roofline); serve p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(w):
+    """Algorithmic HBM bytes per iteration of each kernel (FP32 storage):
+    forward projection streams the volume once per projection and writes proj+norm;
+    the residual reads exp/proj/norm and writes diff; the backprojection reads and
+    writes the volume once per subset and reads every diff image."""
+    nvox, npx, n, s = w.nvox, w.npx, w.n, w.subsets()
+    return {
+        "fp": n * nvox * 4 + n * npx * 8,
+        "res": n * npx * 16,
+        "bp": s * 2 * nvox * 4 + n * npx * 4,
+    }
+
+
+def run_gpu(w, steps, warmup, tj):
+    from tomoj_amd.workloads import eulers_for
+    E = eulers_for(w)
+    P = tj.VoxelProjector3DHIP((w.W, w.H, w.D))
+    ell = tj.phantom_ellipsoids(w.seed, w.ellipsoids, w.W, w.H, w.D)
+    P.synth_tilt_series(ell, E, w.noise_rel, w.noise_seed)
+    if warmup > 0:
+        P.run_ossart(warmup, w.relaxation, w.update, first_index=0)
+    P.reset_volume()
+    P.enable_timing(True)
+    t0 = time.perf_counter()
+    err = P.run_ossart(steps, w.relaxation, w.update, first_index=0)
+    t1 = time.perf_counter()
+    tim = P.timing()
+    return P, E, err, (t1 - t0), tim
+
+
+def cpu_baseline(w, P, E, oracle):
+    """TomoJ's CPU algorithm (C restatement, TomoJ's 2-row task decomposition with a
+    join per projection) for one full OS-SART iteration of the same workload."""
+    cores = max(1, min(16, os.cpu_count() or 1))
+    exps = np.stack([P.get_loaded_projection(k) for k in range(w.n)])
+    g = oracle.make_geom(w.W, w.H, w.D, nthreads=cores)
+    vol = np.zeros((w.D, w.H, w.W), np.float32)
+    t0 = time.perf_counter()
+    oracle.ossart(g, vol, exps, E, 1, w.relaxation, w.update, first_index=0)
+    dt = time.perf_counter() - t0
+    return {
+        "value": 1.0 / dt,
+        "unit": "OS-SART iterations/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"one full OS-SART iteration ({w.subsets()} subsets, {w.n} projections) of "
+                  f"{w.W}x{w.H}x{w.D}, C restatement of VoxelProjector3D/TomoReconstruction2, "
+                  f"{cores} OpenMP threads, {dt:.1f} s",
+        "gups": w.voxel_ray_updates_per_iteration() / dt / 1e9,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import tomoj_amd as tj
+    from tomoj_amd import workloads as WL
+
+    w = WL.OSSART_512
+    log(f"[bench] workload {w.name}: {w.W}x{w.H}x{w.D}, {w.n} projections, update {w.update}")
+    P, E, err, dt, tim = run_gpu(w, args.steps, args.warmup, tj)
+    ms = dt / args.steps * 1e3
+    iters = args.steps / dt
+    gups = w.voxel_ray_updates_per_iteration() * iters / 1e9
+    log(f"[bench] {args.steps} iterations in {dt:.3f} s: {iters:.2f} it/s, {gups:.1f} GUPS; "
+        f"fp {tim['fp_ms']:.1f} ms, res {tim['res_ms']:.1f} ms, bp {tim['bp_ms']:.1f} ms, "
+        f"other {tim['other_ms']:.1f} ms")
+
+    kb = kernel_bytes(w)
+    kernels = {}
+    for k in ("fp", "res", "bp"):
+        launches = tim[f"{k}_launches"]
+        ms_k = tim[f"{k}_ms"]
+        total_bytes = kb[k] * args.steps
+        kernels[k] = {
+            "launches": launches,
+            "avg_ms": ms_k / max(launches, 1),
+            "bytes_per_launch": total_bytes / max(launches, 1),
+            "achieved_GBs": total_bytes / (ms_k * 1e-3) / 1e9 if ms_k > 0 else None,
+        }
+    dom = max(("fp", "bp"), key=lambda k: tim[f"{k}_ms"])
+    achieved = kernels[dom]["achieved_GBs"]
+    roofline = {
+        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "kernel": {"fp": "k_fp_axis", "bp": "k_bp_axis"}[dom],
+        "bytes_per_launch": kernels[dom]["bytes_per_launch"],
+        "avg_launch_ms": kernels[dom]["avg_ms"],
+    }
+    alg = WL.algorithmic_bytes_per_iteration(w)
+    out = {
+        "metric": METRIC,
+        "value": iters,
+        "unit": "OS-SART iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 40-ellipsoid phantom, analytic FP64 projections + 1% noise, generated on device)",
+        "config": {"workload": w.name, "volume": [w.W, w.H, w.D], "projections": w.n,
+                   "detector": [w.W, w.H], "tilts_deg": [w.tilts[0], w.tilts[-1], 1.0],
+                   "algorithm": "OS-SART", "update": w.update, "subsets": w.subsets(),
+                   "relaxation": w.relaxation, "first_index": 0},
+        "gups": gups,
+        "roofline": roofline,
+        "iteration_roofline": {"model_bytes": alg, "achieved_GBs": alg * iters / 1e9,
+                               "frac": alg * iters / 1e9 / HBM_PEAK_GBS,
+                               "compulsory_bytes": WL.compulsory_bytes_per_iteration(w)},
+        "kernels": kernels,
+        "final_residual": float(err[-1, 0]),
+    }
+
+    if not args.no_extras:
+        # weighted backprojection pass (Radermacher weighting, diameter 0.5*W) on the same data
+        P.reset_volume()
+        P.setWeightingFilter(0.5 * w.W, w.tilts)
+        P.run_wbp()
+        t0 = time.perf_counter()
+        P.run_wbp()
+        out["wbp_radermacher_ms"] = (time.perf_counter() - t0) * 1e3
+        P.setRampFilter(hamming=True, cutoff=1.0)
+        t0 = time.perf_counter()
+        P.run_wbp()
+        out["wbp_ramp_hamming_ms"] = (time.perf_counter() - t0) * 1e3
+        log(f"[bench] WBP: radermacher {out['wbp_radermacher_ms']:.1f} ms, "
+            f"ramp+hamming {out['wbp_ramp_hamming_ms']:.1f} ms")
+        # literal 512^3 SIRT
+        c = WL.SIRT_512_CUBE
+        Q = tj.VoxelProjector3DHIP((c.W, c.H, c.D))
+        Q.synth_tilt_series(tj.phantom_ellipsoids(c.seed, c.ellipsoids, c.W, c.H, c.D),
+                            WL.eulers_for(c), c.noise_rel, c.noise_seed)
+        Q.run_ossart(1, 1.0, 0)
+        k = 3
+        t0 = time.perf_counter()
+        Q.run_ossart(k, 1.0, 0)
+        dtc = time.perf_counter() - t0
+        out["sirt_512cube"] = {"iters_per_s": k / dtc,
+                               "gups": c.voxel_ray_updates_per_iteration() * k / dtc / 1e9,
+                               "roofline_frac": WL.algorithmic_bytes_per_iteration(c) * k / dtc / 1e9 / HBM_PEAK_GBS}
+        log(f"[bench] SIRT 512^3: {k / dtc:.2f} it/s")
+        Q.close()
+
+    if not args.no_cpu_baseline:
+        import oracle
+        oracle.build()
+        log("[bench] timing TomoJ's CPU algorithm (one iteration) ...")
+        out["cpu_baseline"] = cpu_baseline(w, P, E, oracle)
+        log(f"[bench] CPU: {out['cpu_baseline']['value']:.4f} it/s on {out['cpu_baseline']['cores']} cores")
+    P.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,233 @@
+/*
+ * tomoj_hip.h -- C ABI of the MI355X projector library (libtomoj_hip.so).
+ *
+ * The library stands where TomoJ's OpenCL projector stands today: behind the
+ * Java `Projector` plugin interface
+ *   src/TomoJ_Base/src/main/java/fr/curie/tomoj/tomography/projectors/Projector.java:14-159
+ * and the GPU lifecycle that ReconstructionThreadManagerGPU drives on
+ *   src/TomoJ_GPU/src/main/java/fr/curie/gpu/tomoj/tomography/projectors/ProjectorGPU.java.
+ * Every entry point below names the reference method it replaces.
+ *
+ * Conventions
+ *   - Plain C types only; host arrays are owned by the caller and copied during
+ *     the call (never retained).  Device memory belongs to the tj_ctx.
+ *   - Host volume layout is TomoJ's ImageStack layout: slice z, row y, x fastest
+ *     (`float zyx[D][H][W]`).  Projections are `float [n][ph][pw]`.
+ *   - Euler matrices are row-major 3x3 doubles, as returned by
+ *     Alignment.getEulerMatrix (align/AffineAlignment.java:382-395).
+ *   - Every call returns 0 on success or a negative TJ_E* code; the message of the
+ *     last failure on the calling thread is returned by tj_last_error().
+ *   - A tj_ctx is driven by one host thread at a time (as one Java thread drives
+ *     one ProjectorGPU); tj_get_completion may be polled from another thread.
+ *   - All calls are synchronous: when they return, the device work is done
+ *     (TomoJ ends each step with clFinish).
+ */
+#ifndef TOMOJ_HIP_H
+#define TOMOJ_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TJ_OK 0
+#define TJ_EINVAL -1     /* bad argument (sizes, indices, null pointers)          */
+#define TJ_ENOMEM -2     /* device or host allocation failed / over HBM budget    */
+#define TJ_EDEVICE -3    /* HIP runtime error                                     */
+#define TJ_ESTATE -4     /* call not valid in the current state                  */
+#define TJ_EUNSUP -5     /* geometry or option not supported                     */
+#define TJ_ECANCELLED -6 /* tj_interrupt() was called during the loop             */
+
+/* Storage type of the resident volume and projections.  Arithmetic is FP32
+ * (positions FP64, residual norms FP64) for every storage type. */
+#define TJ_STORAGE_F32 0
+#define TJ_STORAGE_BF16 1
+#define TJ_STORAGE_F16 2
+
+typedef struct tj_ctx tj_ctx;
+
+/* Projector state TomoJ keeps in Projector / VoxelProjector3D fields. */
+typedef struct {
+    int vol_w, vol_h, vol_d;  /* TomoReconstruction2 width, height, #slices             */
+    int proj_w, proj_h;       /* TiltSeries width, height                                */
+    double proj_center[4];    /* Projector.projectionCenters: [0..1] used by the forward
+                                 projection (ts.getCenterX/Y), [2..3] by the
+                                 backprojection (ts.getProjectionCenterX/Y)
+                                 (Projector.java:52-58)                                  */
+    double vol_center[3];     /* Projector.volumeCenters (Projector.java:60-65)         */
+    double scale;             /* VoxelProjector3D.setScale (VoxelProjector3D.java:89-91) */
+    int positivity;           /* setPositivityConstraint (VoxelProjector3D.java:102-104) */
+    int long_object_comp;     /* setLongObjectCompensation (VoxelProjector3D.java:78-80) */
+    int storage;              /* TJ_STORAGE_*                                            */
+} tj_geometry;
+
+/* Fills g with TomoJ's defaults for a vol_w x vol_h x vol_d reconstruction of a
+ * proj_w x proj_h tilt series: centres (n-1)/2 (TiltSeries.java:161-170,
+ * TomoReconstruction2.java:63-71), scale 1, no positivity, no long-object
+ * compensation, FP32 storage. */
+void tj_geometry_default(tj_geometry *g, int vol_w, int vol_h, int vol_d, int proj_w, int proj_h);
+
+const char *tj_last_error(void);
+const char *tj_version(void);
+
+/* Creates a projector context on HIP device `device` and allocates the resident
+ * volume.  Replaces `new VoxelProjector3DGPU(...)` + ProjectorGPU.initForIterative
+ * buffer allocation (ProjectorGPU.java:195-254).  The volume starts at zero. */
+int tj_open(int device, const tj_geometry *g, tj_ctx **out);
+
+/* Frees all device memory.  Replaces ProjectorGPU.releaseCL_Memory (ProjectorGPU.java:376-403). */
+int tj_close(tj_ctx *ctx);
+
+/* Makes n projections (already supplied, i.e. TiltSeries.getPixels(i) for each i,
+ * TiltSeries.java:678-687) and their Euler matrices resident on the device.
+ * `stack` is n x proj_h x proj_w, `euler` n x 9.  TomoJ re-uploads each image on
+ * every addProjection (ProjectorGPU.java:411-438); here they are uploaded once. */
+int tj_load_projections(tj_ctx *ctx, const float *stack, int n, const double *euler);
+
+/* Replaces the resident Euler matrices only (alignment changed). */
+int tj_set_eulers(tj_ctx *ctx, const double *euler, int n);
+
+/* Copies volume rows [start_y,end_y) of the host volume `zyx` (full D x H x W
+ * array) to the device.  Replaces the slab upload in ProjectorGPU.initForIterative
+ * (ProjectorGPU.java:195-254) / copyInGPUBuffer (:317-330). */
+int tj_set_volume(tj_ctx *ctx, const float *zyx, int start_y, int end_y);
+
+/* Copies volume rows [start_y,end_y) back into the host volume `zyx` (full
+ * D x H x W array; other rows untouched).  Replaces ProjectorGPU.updateFromGPU
+ * (ProjectorGPU.java:297-315). */
+int tj_get_volume(tj_ctx *ctx, float *zyx, int start_y, int end_y);
+
+/* Sets every voxel to zero (TomoReconstruction2.resetPixels, :414-428). */
+int tj_reset_volume(tj_ctx *ctx);
+
+/* Projector.addProjection(int index) (Projector.java:130; VoxelProjector3D.java:111-118):
+ * appends resident projection `index` to the ordered list of added projections. */
+int tj_add_projection(tj_ctx *ctx, int index);
+
+/* Projector.clearAllProjections() (Projector.java:132; VoxelProjector3D.java:123-129). */
+int tj_clear_projections(tj_ctx *ctx);
+
+/* Number of projections currently added. */
+int tj_num_added(tj_ctx *ctx);
+
+/* Projector.project(startY,endY) (Projector.java:144; VoxelProjector3D.java:333-376):
+ * forward-projects the volume for every added projection; results are read with
+ * tj_get_projection. */
+int tj_project(tj_ctx *ctx, int start_y, int end_y);
+
+/* Projector.projectAndDifference(factor,startY,endY) (Projector.java:150;
+ * VoxelProjector3D.java:387-433): projects every added projection, forms
+ * diff = (float)(((exp - th) / norm) * factor) and writes one FP64 error
+ * (sum of diff^2, computed before the float cast) per added projection into
+ * `errors` (length tj_num_added). */
+int tj_project_and_difference(tj_ctx *ctx, double factor, int start_y, int end_y, double *errors);
+
+/* Projector.getProjection(int) (Projector.java:156; VoxelProjector3D.java:131-133):
+ * copies the projection (and, if `norm` is not NULL, the norm image) of added
+ * slot `slot` from the last tj_project / tj_project_and_difference.  Either
+ * output may be NULL. */
+int tj_get_projection(tj_ctx *ctx, int slot, float *proj, float *norm);
+
+/* Copies the difference image of added slot `slot` from the last
+ * tj_project_and_difference (VoxelProjector3D.getDifferenceImage, :139-141). */
+int tj_get_difference(tj_ctx *ctx, int slot, float *diff);
+
+/* Projector.backproject(startY,endY) (Projector.java:138; VoxelProjector3D.java:155-308):
+ * adds the difference images of every added projection (or, when no difference
+ * was computed since the last clear, the projections themselves, weighted if a
+ * weighting filter is set) into volume rows [start_y,end_y). */
+int tj_backproject(tj_ctx *ctx, int start_y, int end_y);
+
+/* Projector.setWeightingFilter(new FFTWeighting(ts, diameter))
+ * (Projector.java:48-50; FFTWeighting.java:21-65): Radermacher exact weighting
+ * for weighted backprojection.  tilt_deg / axis_deg: n angles (axis_deg may be
+ * NULL for a vertical axis).  diameter = weightingRadius * ts.getWidth()
+ * (ReconstructionParameters.java:320).  diameter <= 0 removes the filter. */
+int tj_set_weighting(tj_ctx *ctx, double diameter, const double *tilt_deg, const double *axis_deg, int n);
+
+/* Ramp x Hamming filter for weighted backprojection.  No counterpart in TomoJ
+ * (which has only the Radermacher weighting); provided for the WBP pass of the
+ * production workload.  Filters each detector row in Fourier space with
+ * |k|/(N/2) * (0.54 + 0.46 cos(pi k/(N/2)))-style weights (see the library
+ * source for the exact table).  cutoff in (0,1] scales the Hamming window. */
+int tj_set_ramp_filter(tj_ctx *ctx, int hamming, double cutoff);
+
+/* Copies the current weighting mask (proj_h x proj_w floats). */
+int tj_get_weighting_mask(tj_ctx *ctx, float *mask);
+
+/* Applies the current weighting filter to one image in place (host in/out);
+ * FFTWeighting.weighting(ImageProcessor) (FFTWeighting.java:87-112). */
+int tj_apply_weighting(tj_ctx *ctx, float *image);
+
+/* Projector.startOfIteration / endOfIteration (Projector.java:152-154). */
+int tj_start_iteration(tj_ctx *ctx);
+int tj_end_iteration(tj_ctx *ctx);
+
+/* Projector.getCompletion (Projector.java:122-124); may be called from any thread. */
+double tj_get_completion(tj_ctx *ctx);
+
+/* TomoReconstruction2.interrupt() (:137-140): the running tj_run_* loop stops at
+ * its next subset boundary and returns TJ_ECANCELLED.  Any thread. */
+int tj_interrupt(tj_ctx *ctx);
+
+/* TomoReconstruction2.OSSART(ts, projector, params, startY, endY)
+ * (TomoReconstruction2.java:548-733) run inside the library over the resident
+ * projections: `iterations` iterations of osartIteration with relaxation
+ * `lambda` and `update` projections per subset (0 = all: SIRT; 1 = ART).  The
+ * projection order of each iteration is orderShuffleART (:735-781) started at
+ * `first_index` instead of Math.random().  err (iterations x 5, may be NULL)
+ * receives err1 = [avg, min, image of min, max, image of max] per iteration. */
+int tj_run_ossart(tj_ctx *ctx, int iterations, double lambda, int update, int first_index,
+                  int start_y, int end_y, double *err);
+
+/* TomoReconstruction2.WBP(ts, projector, params, startY, endY)
+ * (TomoReconstruction2.java:499-538) over the resident projections: each one
+ * weighted with the current filter (if any) and backprojected into the
+ * volume (which is not cleared first). */
+int tj_run_wbp(tj_ctx *ctx, int start_y, int end_y);
+
+/* Residual norms of the last tj_project_and_difference in FP64: sum over added
+ * projections of sum diff^2 (what osartIteration accumulates in `som`). */
+int tj_last_error_sum(tj_ctx *ctx, double *som);
+
+/* ---------------------------------------------------------------------------
+ * Synthetic inputs (no TomoJ counterpart: TomoJ reads tilt series from disk).
+ * Seeded ellipsoid phantom with analytic FP64 line integrals in TomoJ's
+ * geometry plus Gaussian noise, generated on the device straight into the
+ * resident projection stack.
+ * ------------------------------------------------------------------------- */
+
+/* Ellipsoid parameters (k x 16 doubles: density, centre xyz, semi-axes xyz,
+ * rotation 3x3 row-major) from a SplitMix64 stream seeded with `seed`, for a
+ * W x H x D volume. Host-only; no context needed. */
+void tj_phantom_ellipsoids(uint64_t seed, int k, int vol_w, int vol_h, int vol_d, double *out);
+
+/* Generates n analytic projections of the ellipsoids `ell` (k x 16) for Euler
+ * matrices `euler` (n x 9), adds N(0, noise_rel * max) noise drawn from
+ * `noise_seed`, and makes them the resident projections (as tj_load_projections). */
+int tj_synth_projections(tj_ctx *ctx, const double *ell, int k, const double *euler, int n,
+                         double noise_rel, uint64_t noise_seed);
+
+/* Copies resident projection `index` to the host. */
+int tj_get_loaded_projection(tj_ctx *ctx, int index, float *out);
+
+/* Timing of the most recent tj_run_ossart / tj_run_wbp, from HIP events on the
+ * context's stream: total milliseconds, and the summed milliseconds and launch
+ * counts of the forward-projection kernel, the residual kernels (difference and
+ * error sums) and the backprojection kernel (with its row table). */
+typedef struct {
+    double total_ms;
+    double fp_ms, res_ms, bp_ms, other_ms;
+    int fp_launches, res_launches, bp_launches;
+} tj_timing;
+int tj_get_timing(tj_ctx *ctx, tj_timing *t);
+
+/* Turns per-kernel event timing on/off for tj_run_* (off by default). */
+int tj_enable_timing(tj_ctx *ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOMOJ_HIP_H */

@@ -1,0 +1,329 @@
+"""CPU oracle for the TomoJ projector path -- TEST INFRASTRUCTURE ONLY.
+
+Python bindings (ctypes + numpy) over ``oracle/libtomoj_oracle.so``, built from
+``oracle/tomoj_oracle.c`` by ``oracle/Makefile``.  Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg import this
+package, and only to check (or time beside) the HIP path; the product never
+routes through it.
+
+Each function names the TomoJ source it restates (paths relative to
+``src/TomoJ_Base/src/main/java/fr/curie/tomoj/``).  Volumes use TomoJ's host
+layout ``[z][y][x]`` (ImageStack: one slice per z, x fastest); projections are
+``[n][ph][pw]``; Euler matrices ``[n][3][3]`` row-major float64.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libtomoj_oracle.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, no FMA contraction)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE],
+                       check=True)
+    return _LIB_PATH
+
+
+class Geom(C.Structure):
+    """``or_geom``: the projector state TomoJ keeps in ``Projector`` /
+    ``VoxelProjector3D`` fields (Projector.java:15-21, VoxelProjector3D.java:33-47)."""
+    _fields_ = [("vw", C.c_int), ("vh", C.c_int), ("vd", C.c_int),
+                ("pw", C.c_int), ("ph", C.c_int),
+                ("pc", C.c_double * 4), ("vc", C.c_double * 3),
+                ("scale", C.c_double), ("positivity", C.c_int), ("loc", C.c_int),
+                ("nthreads", C.c_int)]
+
+
+def make_geom(vw, vh, vd, pw=None, ph=None, pc=None, vc=None, scale=1.0,
+              positivity=False, loc=False, nthreads=1) -> Geom:
+    """Geometry with TomoJ's default centres: projection centres ``(n-1)/2``
+    (TiltSeries.java:161-170) and volume centres ``(n-1)/2``
+    (TomoReconstruction2.java:63-71)."""
+    pw = vw if pw is None else pw
+    ph = vh if ph is None else ph
+    if pc is None:
+        pc = ((pw - 1.0) / 2, (ph - 1.0) / 2, (pw - 1) / 2.0, (ph - 1) / 2.0)
+    if vc is None:
+        vc = ((vw - 1) / 2.0, (vh - 1) / 2.0, (vd - 1) / 2.0)
+    g = Geom()
+    g.vw, g.vh, g.vd, g.pw, g.ph = vw, vh, vd, pw, ph
+    for k in range(4):
+        g.pc[k] = pc[k]
+    for k in range(3):
+        g.vc[k] = vc[k]
+    g.scale = scale
+    g.positivity = int(bool(positivity))
+    g.loc = int(bool(loc))
+    g.nthreads = int(nthreads)
+    return g
+
+
+def _L():
+    global _lib
+    if _lib is None:
+        build()
+        lib = C.CDLL(_LIB_PATH)
+        P = C.c_void_p
+        G = C.POINTER(Geom)
+        d, i, l, u64 = C.c_double, C.c_int, C.c_long, C.c_uint64
+        sigs = {
+            "or_euler_matrix": (None, [d, d, d, P]),
+            "or_tilt_euler": (None, [d, d, i, P]),
+            "or_ij_interpolated_value": (d, [P, i, i, d, d]),
+            "or_getpixel": (C.c_float, [P, i, i, i, d, d, i]),
+            "or_fp_factors": (None, [P, P]),
+            "or_project_one": (None, [G, P, P, P, P, i, i]),
+            "or_project_one_f64": (None, [G, P, P, P, P, i, i]),
+            "or_difference": (d, [P, P, P, P, l, d]),
+            "or_difference_f64": (d, [P, P, P, P, l, d]),
+            "or_backproject_one": (None, [G, P, P, P, i, i]),
+            "or_backproject_one_f64": (None, [G, P, P, P, i, i]),
+            "or_order_shuffle": (None, [P, i, i, P]),
+            "or_ossart": (i, [G, P, P, P, i, i, d, i, i, i, i, P]),
+            "or_ossart_f64": (i, [G, P, P, P, i, i, d, i, i, i, i, P]),
+            "or_weighting_mask": (None, [i, i, P, P, i, d, P]),
+            "or_fft2": (None, [P, P, i, i, i]),
+            "or_weighting": (None, [P, P, i, i]),
+            "or_weighting_f64": (None, [P, P, i, i]),
+            "or_wbp": (None, [G, P, P, P, i, P, i, i, i]),
+            "or_wbp_f64": (None, [G, P, P, P, i, P, i, i]),
+            "or_ellipsoids": (None, [u64, i, i, i, i, P]),
+            "or_phantom_project": (None, [P, i, P, i, i, i, d, d, P]),
+            "or_gaussian_noise": (None, [u64, l, P]),
+            "or_phantom_volume": (None, [P, i, i, i, i, d, d, d, P]),
+            "or_num_procs": (i, []),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"], "arrays passed to the oracle must be C-contiguous"
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# --------------------------------------------------------------------------
+# geometry
+# --------------------------------------------------------------------------
+
+def euler_matrix(phi, theta, psi) -> np.ndarray:
+    """MatrixUtils.eulerAngles2Matrix (Utility .../MatrixUtils.java:226-251), degrees."""
+    E = np.zeros(9)
+    _L().or_euler_matrix(phi, theta, psi, _p(E))
+    return E.reshape(3, 3)
+
+
+def tilt_eulers(tilts_deg, tilt_axis_deg=0.0, axis_vertical=True) -> np.ndarray:
+    """AffineAlignment.getEulerMatrix for each tilt (align/AffineAlignment.java:382-395)."""
+    out = np.zeros((len(tilts_deg), 9))
+    for k, t in enumerate(tilts_deg):
+        _L().or_tilt_euler(float(t), float(tilt_axis_deg), int(axis_vertical), _p(out[k]))
+    return out.reshape(-1, 3, 3)
+
+
+def fp_factors(E) -> np.ndarray:
+    """Per-projection constants of projectJosephPartial (VoxelProjector3D.java:682-694):
+    [icf, zcf, divy, yfx, zfx, divx, dirX, dirY, normIncrement]."""
+    E = _c(E, np.float64)
+    out = np.zeros(9)
+    _L().or_fp_factors(_p(E), _p(out))
+    return out
+
+
+def getpixel(vol, x, y, z) -> float:
+    """TomoReconstruction2.getPixel (TomoReconstruction2.java:154-190)."""
+    vol = _c(vol, np.float32)
+    D, H, W = vol.shape
+    return _L().or_getpixel(_p(vol), W, H, D, float(x), float(y), int(z))
+
+
+def ij_interpolated_value(img, x, y) -> float:
+    """ImageJ ImageProcessor.getInterpolatedValue restatement (third-party, ij 1.53a)."""
+    img = _c(img, np.float32)
+    H, W = img.shape
+    return _L().or_ij_interpolated_value(_p(img), W, H, float(x), float(y))
+
+
+# --------------------------------------------------------------------------
+# projector SPI pieces
+# --------------------------------------------------------------------------
+
+def project(g: Geom, vol, eulers, start_y=0, end_y=None, f64=False):
+    """VoxelProjector3D.project (:333-376): one projection + norm image per Euler matrix."""
+    end_y = g.vh if end_y is None else end_y
+    dt = np.float64 if f64 else np.float32
+    vol = _c(vol, dt)
+    eulers = _c(eulers, np.float64).reshape(-1, 9)
+    n = eulers.shape[0]
+    proj = np.zeros((n, g.ph, g.pw), dt)
+    norm = np.zeros((n, g.ph, g.pw), dt)
+    fn = _L().or_project_one_f64 if f64 else _L().or_project_one
+    for p in range(n):
+        fn(C.byref(g), _p(vol), _p(eulers[p]), _p(proj[p]), _p(norm[p]), start_y, end_y)
+    return proj, norm
+
+
+def difference(exp, th, norm, factor, f64=False):
+    """VoxelProjector3D.difference (:414-433) -> (diff, error)."""
+    dt = np.float64 if f64 else np.float32
+    exp, th, norm = _c(exp, dt), _c(th, dt), _c(norm, dt)
+    diff = np.zeros_like(exp)
+    fn = _L().or_difference_f64 if f64 else _L().or_difference
+    err = fn(_p(exp), _p(th), _p(norm), _p(diff), exp.size, float(factor))
+    return diff, err
+
+
+def project_and_difference(g: Geom, vol, exps, eulers, factor, start_y=0, end_y=None, f64=False):
+    """VoxelProjector3D.projectAndDifference (:387-401) -> (diffs, errors, projs, norms)."""
+    projs, norms = project(g, vol, eulers, start_y, end_y, f64)
+    dt = np.float64 if f64 else np.float32
+    exps = _c(exps, dt)
+    diffs = np.zeros_like(projs)
+    errors = np.zeros(len(projs))
+    for p in range(len(projs)):
+        diffs[p], errors[p] = difference(exps[p], projs[p], norms[p], factor, f64)
+    return diffs, errors, projs, norms
+
+
+def backproject(g: Geom, vol, images, eulers, start_y=0, end_y=None, f64=False):
+    """VoxelProjector3D.backproject (:155-208) without weighting: adds every image
+    into ``vol`` (in place, returned)."""
+    end_y = g.vh if end_y is None else end_y
+    dt = np.float64 if f64 else np.float32
+    assert vol.dtype == dt and vol.flags["C_CONTIGUOUS"]
+    images = _c(images, dt)
+    eulers = _c(eulers, np.float64).reshape(-1, 9)
+    fn = _L().or_backproject_one_f64 if f64 else _L().or_backproject_one
+    for p in range(images.shape[0]):
+        fn(C.byref(g), _p(vol), _p(images[p]), _p(eulers[p]), start_y, end_y)
+    return vol
+
+
+def order_shuffle(eulers, first_index=0) -> np.ndarray:
+    """TomoReconstruction2.orderShuffleART (:735-781) with a fixed first index."""
+    eulers = _c(eulers, np.float64).reshape(-1, 9)
+    order = np.zeros(eulers.shape[0], np.int32)
+    _L().or_order_shuffle(_p(eulers), eulers.shape[0], int(first_index), _p(order))
+    return order
+
+
+def ossart(g: Geom, vol, projs, eulers, iterations, lam, update, first_index=0,
+           start_y=0, end_y=None, f64=False):
+    """TomoReconstruction2.OSSART (:548-634).  Updates ``vol`` in place; returns the
+    per-iteration ``err1`` rows [avg, min, imin, max, imax] (iterations x 5)."""
+    end_y = g.vh if end_y is None else end_y
+    dt = np.float64 if f64 else np.float32
+    assert vol.dtype == dt and vol.flags["C_CONTIGUOUS"]
+    projs = _c(projs, dt)
+    eulers = _c(eulers, np.float64).reshape(-1, 9)
+    err = np.zeros((iterations, 5))
+    fn = _L().or_ossart_f64 if f64 else _L().or_ossart
+    rc = fn(C.byref(g), _p(vol), _p(projs), _p(eulers), eulers.shape[0], iterations,
+            float(lam), int(update), int(first_index), start_y, end_y, _p(err))
+    if rc != 0:
+        raise MemoryError("oracle OS-SART allocation failed")
+    return err
+
+
+def weighting_mask(sx, sy, tilts_deg, diameter, axis_deg=None) -> np.ndarray:
+    """FFTWeighting.createWeightingMask (filters/FFTWeighting.java:31-65)."""
+    tilts = _c(tilts_deg, np.float64)
+    axis = np.zeros_like(tilts) if axis_deg is None else _c(axis_deg, np.float64)
+    mask = np.zeros((sy, sx), np.float32)
+    _L().or_weighting_mask(sx, sy, _p(tilts), _p(axis), len(tilts), float(diameter), _p(mask))
+    return mask
+
+
+def fft2(re, im, sign=-1):
+    """2D complex DFT (double), in place on copies; returns (re, im)."""
+    re = _c(re, np.float64).copy()
+    im = _c(im, np.float64).copy()
+    sy, sx = re.shape
+    _L().or_fft2(_p(re), _p(im), sx, sy, int(sign))
+    return re, im
+
+
+def weighting(img, mask, f64=False):
+    """FFTWeighting.weighting (:87-112): real(ifft2(mask * fft2(img)))."""
+    dt = np.float64 if f64 else np.float32
+    img = _c(img, dt).copy()
+    mask = _c(mask, dt)
+    sy, sx = img.shape
+    (_L().or_weighting_f64 if f64 else _L().or_weighting)(_p(img), _p(mask), sx, sy)
+    return img
+
+
+def wbp(g: Geom, vol, projs, eulers, mask=None, update=0, start_y=0, end_y=None, f64=False):
+    """TomoReconstruction2.WBP (:499-538); ``vol`` updated in place."""
+    end_y = g.vh if end_y is None else end_y
+    dt = np.float64 if f64 else np.float32
+    assert vol.dtype == dt and vol.flags["C_CONTIGUOUS"]
+    projs = _c(projs, dt)
+    eulers = _c(eulers, np.float64).reshape(-1, 9)
+    n = eulers.shape[0]
+    if f64:
+        m = None if mask is None else _c(mask, np.float64)
+        _L().or_wbp_f64(C.byref(g), _p(vol), _p(projs), _p(eulers), n,
+                        None if m is None else _p(m), start_y, end_y)
+    else:
+        m = None if mask is None else _c(mask, np.float32)
+        _L().or_wbp(C.byref(g), _p(vol), _p(projs), _p(eulers), n,
+                    None if m is None else _p(m), int(update), start_y, end_y)
+    return vol
+
+
+# --------------------------------------------------------------------------
+# synthetic data
+# --------------------------------------------------------------------------
+
+def ellipsoids(seed, k, W, H, D) -> np.ndarray:
+    """Seeded ellipsoid parameters, k x 16 (density, centre[3], semi-axes[3], R[9])."""
+    out = np.zeros((k, 16))
+    _L().or_ellipsoids(C.c_uint64(seed), k, W, H, D, _p(out))
+    return out
+
+
+def phantom_project(ell, eulers, pw, ph, pc0=None, pc1=None) -> np.ndarray:
+    """Analytic FP64 line integrals of the ellipsoid phantom (n x ph x pw)."""
+    ell = _c(ell, np.float64)
+    eulers = _c(eulers, np.float64).reshape(-1, 9)
+    pc0 = (pw - 1.0) / 2 if pc0 is None else pc0
+    pc1 = (ph - 1.0) / 2 if pc1 is None else pc1
+    out = np.zeros((eulers.shape[0], ph, pw))
+    _L().or_phantom_project(_p(ell), ell.shape[0], _p(eulers), eulers.shape[0], pw, ph,
+                            float(pc0), float(pc1), _p(out))
+    return out
+
+
+def gaussian_noise(seed, count) -> np.ndarray:
+    out = np.zeros(count)
+    _L().or_gaussian_noise(C.c_uint64(seed), int(count), _p(out))
+    return out
+
+
+def phantom_volume(ell, W, H, D) -> np.ndarray:
+    ell = _c(ell, np.float64)
+    vol = np.zeros((D, H, W), np.float32)
+    _L().or_phantom_volume(_p(ell), ell.shape[0], W, H, D, (W - 1) / 2.0, (H - 1) / 2.0,
+                           (D - 1) / 2.0, _p(vol))
+    return vol
+
+
+def num_procs() -> int:
+    return _L().or_num_procs()

@@ -1,0 +1,326 @@
+"""Parity of the HIP path with the oracle on the same inputs (GPU tests).
+
+The FP32 forward projection, residual and backprojection follow TomoJ's
+arithmetic operation by operation, so their results must be bit-identical to
+the oracle's FP32 restatement (np.array_equal, NaN == NaN).  FP64 error sums
+are reduced in a different order than Java's sequential loop: they must agree
+to 1e-12 relative.  The weighted backprojection uses an FP32 FFT where the
+oracle uses FP64, so it is held to relative L2 <= 1e-5 (the parity bound for
+floating-point results), and each result's distance to the FP64 oracle is
+reported.
+"""
+import threading
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def same(a, b):
+    return np.array_equal(np.asarray(a), np.asarray(b), equal_nan=True)
+
+
+def make(tj, vol_shape, proj_shape=None, **kw):
+    return tj.VoxelProjector3DHIP(vol_shape, proj_shape, **kw)
+
+
+def ogeom(oracle, P, scale=1.0, positivity=False, loc=False):
+    g = P.geometry
+    return oracle.make_geom(P.W, P.H, P.D, P.pw, P.ph, pc=list(g.proj_center), vc=list(g.vol_center),
+                            scale=scale, positivity=positivity, loc=loc, nthreads=8)
+
+
+CASES = {
+    # name: (vol WHD, proj WH, tilts, kwargs, euler kwargs)
+    "axis_default": ((37, 9, 13), None, np.arange(-60, 61, 15.0), {}, {}),
+    "axis_even_odd": ((40, 12, 21), None, [-59.5, -33.0, -1.0, 0.0, 7.25, 44.0, 60.0], {}, {}),
+    "scale2": ((20, 10, 8), (40, 20), np.arange(-45, 46, 15.0), {"scale": 2.0}, {}),
+    "proj_taller": ((24, 10, 9), (24, 16), np.arange(-30, 31, 30.0), {}, {}),
+    "shifted_centres": ((26, 11, 10), None, np.arange(-50, 51, 25.0),
+                        {"vol_center": (12.2, 5.3, 4.5), "proj_center": (12.5, 5.0, 12.75, 5.4)}, {}),
+    "tilt_axis_10deg": ((22, 18, 10), None, np.arange(-40, 41, 20.0), {}, {"tilt_axis_deg": 10.0,
+                                                                           "axis_vertical": False}),
+}
+
+
+def case_data(oracle, name, seed=0):
+    vshape, pshape, tilts, kw, ekw = CASES[name]
+    W, H, D = vshape
+    rng = np.random.default_rng(seed)
+    vol = rng.random((D, H, W), dtype=np.float32)
+    E = oracle.tilt_eulers(tilts, **ekw)
+    pw, ph = (W, H) if pshape is None else pshape
+    exps = (rng.random((len(tilts), ph, pw), dtype=np.float32) * D).astype(np.float32)
+    return vshape, pshape, E, kw, vol, exps
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_forward_projection_bit_exact(tj, oracle, name):
+    vshape, pshape, E, kw, vol, exps = case_data(oracle, name)
+    with make(tj, vshape, pshape, **kw) as P:
+        P.load_tilt_series(exps, E)
+        P.set_volume(vol)
+        for k in range(len(E)):
+            P.addProjection(k)
+        P.project()
+        g = ogeom(oracle, P, scale=kw.get("scale", 1.0))
+        ref_p, ref_n = oracle.project(g, vol, E)
+        for k in range(len(E)):
+            assert same(P.getProjection(k), ref_p[k]), (name, k)
+            assert same(P.getNormImage(k), ref_n[k]), (name, k)
+
+
+@pytest.mark.parametrize("name", ["axis_default", "scale2", "tilt_axis_10deg", "shifted_centres"])
+def test_project_and_difference(tj, oracle, name):
+    vshape, pshape, E, kw, vol, exps = case_data(oracle, name, seed=1)
+    exps[0, 1, :3] = np.nan
+    factor = float(np.float32(1.0 / len(E)))
+    with make(tj, vshape, pshape, **kw) as P:
+        P.load_tilt_series(exps, E)
+        P.set_volume(vol)
+        order = list(range(len(E)))[::-1]
+        for k in order:
+            P.addProjection(k)
+        errs = P.projectAndDifference(factor)
+        g = ogeom(oracle, P, scale=kw.get("scale", 1.0))
+        diffs, ref_errs, _, _ = oracle.project_and_difference(g, vol, exps[order], E[order], factor)
+        for s in range(len(order)):
+            assert same(P.getDifferenceImage(s), diffs[s]), (name, s)
+        assert np.allclose(errs, ref_errs, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("positivity", [False, True])
+def test_backprojection_bit_exact(tj, oracle, name, positivity):
+    vshape, pshape, E, kw, vol, exps = case_data(oracle, name, seed=2)
+    exps = exps - np.float32(0.5 * vshape[2])        # signed images
+    with make(tj, vshape, pshape, positivity=positivity, **kw) as P:
+        P.load_tilt_series(exps, E)
+        P.set_volume(vol)
+        for k in range(len(E)):
+            P.addProjection(k)
+        P.backproject()
+        got = P.get_volume()
+        g = ogeom(oracle, P, scale=kw.get("scale", 1.0), positivity=positivity)
+        ref = vol.copy()
+        oracle.backproject(g, ref, exps, E)
+        assert same(got, ref), (name, int((got != ref).sum()))
+
+
+def test_backprojection_nan_inf_rows(tj, oracle):
+    """NaN pixels (and the zero-weight next-row NaN rule) and Inf sums follow the oracle."""
+    W, H, D = 18, 7, 6
+    rng = np.random.default_rng(5)
+    E = oracle.tilt_eulers([0.0, 20.0, -35.0])
+    exps = rng.standard_normal((3, H, W)).astype(np.float32)
+    exps[0, 2, 5] = np.nan
+    exps[1, 4, :] = np.nan
+    exps[2, 0, 7] = np.float32(3e38)
+    vol = rng.standard_normal((D, H, W)).astype(np.float32)
+    vol[:, 3, :] = np.float32(3e38)
+    with make(tj, (W, H, D)) as P:
+        P.load_tilt_series(exps, E)
+        P.set_volume(vol)
+        for k in range(3):
+            P.addProjection(k)
+        P.backproject()
+        ref = vol.copy()
+        oracle.backproject(ogeom(oracle, P), ref, exps, E)
+        assert same(P.get_volume(), ref)
+
+
+def test_forward_projection_nonfinite_volume(tj, oracle):
+    """Planes holding NaN/Inf make the forward projection take the full 4-tap path."""
+    W, H, D = 20, 8, 9
+    rng = np.random.default_rng(6)
+    vol = rng.random((D, H, W), dtype=np.float32)
+    vol[3, 4, 7] = np.nan
+    vol[5, 6, 2] = np.inf
+    E = oracle.tilt_eulers([-40.0, 0.0, 15.0])
+    with make(tj, (W, H, D)) as P:
+        P.load_tilt_series(np.zeros((3, H, W), np.float32), E)
+        P.set_volume(vol)
+        for k in range(3):
+            P.addProjection(k)
+        P.project()
+        ref_p, ref_n = oracle.project(ogeom(oracle, P), vol, E)
+        for k in range(3):
+            assert same(P.getProjection(k), ref_p[k])
+            assert same(P.getNormImage(k), ref_n[k])
+
+
+def test_slab_ranges(tj, oracle):
+    """startY/endY restrict the backprojection to a slab; the forward projection
+    starts at round(startY*scale) as VoxelProjector3D.project does."""
+    vshape, pshape, E, kw, vol, exps = case_data(oracle, "axis_default", seed=3)
+    with make(tj, vshape) as P:
+        P.load_tilt_series(exps, E)
+        P.set_volume(vol)
+        for k in range(len(E)):
+            P.addProjection(k)
+        errs = P.projectAndDifference(0.25, 3, 7)
+        P.backproject(3, 7)
+        got = P.get_volume()
+        g = ogeom(oracle, P)
+        ref = vol.copy()
+        diffs, ref_errs, _, _ = oracle.project_and_difference(g, vol, exps, E, 0.25, 3, 7)
+        oracle.backproject(g, ref, diffs, E, 3, 7)
+        assert same(got, ref)
+        assert np.allclose(errs, ref_errs, rtol=1e-12, equal_nan=True)
+
+
+@pytest.mark.parametrize("update", [0, 1, 4])
+def test_ossart_loop_bit_exact(tj, oracle, update):
+    """tj_run_ossart (the library's OS-SART/SIRT/ART loop) reproduces the oracle's
+    volume bit for bit; err1 rows agree to FP64 summation order."""
+    W, H, D = 40, 10, 20
+    tilts = np.arange(-60, 61, 6.0)
+    E = oracle.tilt_eulers(tilts)
+    ell = oracle.ellipsoids(11, 5, W, H, D)
+    exps = (oracle.phantom_project(ell, E, W, H) + 0.05 * oracle.gaussian_noise(3, len(tilts) * W * H)
+            .reshape(len(tilts), H, W)).astype(np.float32)
+    with make(tj, (W, H, D)) as P:
+        P.load_tilt_series(exps, E)
+        err = P.run_ossart(3, 0.7, update, first_index=5)
+        got = P.get_volume()
+    ref = np.zeros((D, H, W), np.float32)
+    ref_err = oracle.ossart(ogeom(oracle, P), ref, exps, E, 3, 0.7, update, first_index=5)
+    assert same(got, ref)
+    assert np.allclose(err[:, [0, 1, 3]], ref_err[:, [0, 1, 3]], rtol=1e-12)
+    assert np.array_equal(err[:, [2, 4]], ref_err[:, [2, 4]])
+    ref64 = np.zeros((D, H, W), np.float64)
+    oracle.ossart(ogeom(oracle, P), ref64, exps.astype(np.float64), E, 3, 0.7, update, first_index=5, f64=True)
+    print(f"update={update}: rel L2 vs FP64 oracle = {rel_l2(got, ref64):.3e}")
+    assert rel_l2(got, ref64) < 1e-5
+
+
+def test_spi_loop_equals_fused_loop(tj, oracle):
+    """The OS-SART loop driven through the projector interface (as TomoJ's Java
+    loop calls it) and the library's fused loop give the same volume and errors."""
+    W, H, D = 32, 8, 16
+    tilts = np.arange(-60, 61, 10.0)
+    E = oracle.tilt_eulers(tilts)
+    ell = oracle.ellipsoids(12, 4, W, H, D)
+    exps = oracle.phantom_project(ell, E, W, H).astype(np.float32)
+    with make(tj, (W, H, D)) as P, make(tj, (W, H, D)) as Q:
+        P.load_tilt_series(exps, E)
+        Q.load_tilt_series(exps, E)
+        e1 = tj.ossart(P, E, 2, 1.0, 3, first_index=2)
+        e2 = Q.run_ossart(2, 1.0, 3, first_index=2)
+        assert same(P.get_volume(), Q.get_volume())
+        assert np.allclose(e1, e2, rtol=1e-13)
+
+
+def test_wbp_radermacher(tj, oracle):
+    """Weighted backprojection with TomoJ's exact weighting, vs the oracle (FP64 FFT)."""
+    W, H, D = 64, 16, 32
+    tilts = np.arange(-60, 61, 5.0)
+    E = oracle.tilt_eulers(tilts)
+    ell = oracle.ellipsoids(21, 5, W, H, D)
+    exps = oracle.phantom_project(ell, E, W, H).astype(np.float32)
+    dia = 0.5 * W
+    with make(tj, (W, H, D)) as P:
+        P.load_tilt_series(exps, E)
+        P.setWeightingFilter(dia, tilts)
+        mask = P.getWeightingMask()
+        ref_mask = oracle.weighting_mask(W, H, tilts, dia)
+        assert rel_l2(mask, ref_mask) < 1e-7
+        w0 = P.weighting(exps[3])
+        assert rel_l2(w0, oracle.weighting(exps[3], ref_mask)) < 1e-5
+        P.run_wbp()
+        got = P.get_volume()
+        P.reset_volume()
+        tj.wbp(P, len(tilts))            # through the projector interface, one image per call
+        spi = P.get_volume()
+    ref = np.zeros((D, H, W), np.float32)
+    oracle.wbp(ogeom(oracle, P), ref, exps, E, ref_mask)
+    ref64 = np.zeros((D, H, W), np.float64)
+    oracle.wbp(ogeom(oracle, P), ref64, exps.astype(np.float64), E, ref_mask.astype(np.float64), f64=True)
+    print(f"WBP rel L2 vs FP32 oracle {rel_l2(got, ref):.3e}, vs FP64 oracle {rel_l2(got, ref64):.3e}")
+    assert rel_l2(got, ref) < 1e-5
+    assert same(got, spi)
+
+
+def test_ramp_filter_closed_form(tj, oracle):
+    """Ramp x Hamming: a constant image filters to 0; a cosine of frequency k along x
+    is scaled by H(k) = (pi/N) (k/W) (0.54 + 0.46 cos(2 pi k / W))."""
+    W, H, D, N = 64, 8, 8, 9
+    E = oracle.tilt_eulers(np.linspace(-60, 60, N))
+    with make(tj, (W, H, D)) as P:
+        P.load_tilt_series(np.zeros((N, H, W), np.float32), E)
+        P.setRampFilter(hamming=True, cutoff=1.0)
+        assert np.max(np.abs(P.weighting(np.full((H, W), 3.0, np.float32)))) < 1e-5
+        k = 5
+        x = np.arange(W)
+        img = np.broadcast_to(np.cos(2 * np.pi * k * x / W), (H, W)).astype(np.float32)
+        f = k / W
+        Hk = (np.pi / N) * f * (0.54 + 0.46 * np.cos(np.pi * f / 0.5))
+        assert np.allclose(P.weighting(img), Hk * img, atol=1e-6)
+
+
+def test_synthetic_projections_match_oracle(tj, oracle):
+    """Device-generated analytic projections + noise equal the oracle's FP64 values
+    rounded to FP32 (within one FP32 ulp where libm and the device differ)."""
+    W, H, D = 48, 12, 24
+    tilts = np.arange(-60, 61, 20.0)
+    E = oracle.tilt_eulers(tilts)
+    ell = oracle.ellipsoids(99, 6, W, H, D)
+    with make(tj, (W, H, D)) as P:
+        P.synth_tilt_series(ell, E, 0.01, 1234)
+        got = np.stack([P.get_loaded_projection(k) for k in range(len(tilts))])
+    clean = oracle.phantom_project(ell, E, W, H)
+    noisy = clean + 0.01 * clean.max() * oracle.gaussian_noise(1234, clean.size).reshape(clean.shape)
+    assert np.allclose(got, noisy.astype(np.float32), rtol=2e-7, atol=1e-6 * clean.max())
+
+
+def test_deterministic_reruns(tj, oracle):
+    W, H, D = 48, 16, 24
+    tilts = np.arange(-60, 61, 4.0)
+    E = oracle.tilt_eulers(tilts)
+    ell = oracle.ellipsoids(4, 6, W, H, D)
+    outs = []
+    for _ in range(2):
+        with make(tj, (W, H, D)) as P:
+            P.synth_tilt_series(ell, E, 0.01, 77)
+            e = P.run_ossart(2, 1.0, 5)
+            outs.append((P.get_volume(), e))
+    assert same(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_errors_are_reported(tj):
+    from tomoj_amd import TjError
+    with make(tj, (16, 8, 8)) as P:
+        with pytest.raises(TjError):
+            P.addProjection(0)                  # nothing loaded
+        P.load_tilt_series(np.zeros((2, 8, 16), np.float32), np.stack([np.eye(3)] * 2))
+        with pytest.raises(TjError):
+            P.addProjection(2)
+        with pytest.raises(TjError):
+            P.backproject(0, 9)
+        with pytest.raises(TjError):
+            P.run_ossart(1, 1.0, 0, first_index=5)
+
+
+def test_interrupt_stops_the_loop(tj, oracle):
+    """TomoReconstruction2.interrupt(): the loop stops at a subset boundary."""
+    W, H, D = 128, 64, 64
+    tilts = np.arange(-60, 61, 1.0)
+    E = oracle.tilt_eulers(tilts)
+    ell = oracle.ellipsoids(5, 8, W, H, D)
+    from tomoj_amd import TjCancelled
+    with make(tj, (W, H, D)) as P:
+        P.synth_tilt_series(ell, E, 0.01, 5)
+        timer = threading.Timer(0.2, P.interrupt)
+        timer.start()
+        with pytest.raises(TjCancelled):
+            P.run_ossart(2000, 1.0, 1)
+        timer.join()
+        # the context stays usable
+        P.run_ossart(1, 1.0, 0)

@@ -1,0 +1,374 @@
+"""Pins the CPU oracle (oracle/tomoj_oracle.c) before it is trusted as the checker.
+
+TomoJ ships no tests, fixtures or golden vectors for this path, and its Java
+code cannot run here (no JDK, dependencies not vendored).  Each oracle function
+is therefore checked against what the mathematics fixes: closed forms,
+invariants, special cases that reduce to an independent library (numpy's FFT),
+and analytic line integrals.  Each check is chosen so that a plausible slip in
+the restatement (a swapped index, a wrong sign of the tilt, a dropped edge
+rule, a missing exit step) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+
+def geom(O, W, H, D, **kw):
+    return O.make_geom(W, H, D, nthreads=4, **kw)
+
+
+# ---------------------------------------------------------------------------
+# geometry
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("theta", [-60.0, -17.5, 0.0, 33.0, 60.0])
+def test_euler_single_axis_closed_form(oracle, theta):
+    """E(0,theta,0) = [[c,0,-s],[0,1,0],[s,0,c]] (MatrixUtils.java:238-246)."""
+    E = oracle.tilt_eulers([theta])[0]
+    t = theta / 180.0 * math.pi
+    ref = np.array([[math.cos(t), 0, -math.sin(t)], [0, 1, 0], [math.sin(t), 0, math.cos(t)]])
+    assert np.allclose(E, ref, atol=1e-15)
+
+
+def test_euler_general_is_rotation(oracle):
+    E = oracle.euler_matrix(23.0, 41.0, -77.0)
+    assert np.allclose(E @ E.T, np.eye(3), atol=1e-14)
+    assert abs(np.linalg.det(E) - 1) < 1e-14
+
+
+def test_python_euler_matches_oracle(oracle):
+    """The package's Euler matrices (used to drive the GPU) equal the oracle's bit for bit."""
+    from tomoj_amd.reconstruction import tilt_eulers
+    tilts = np.arange(-60, 61, 1.0)
+    assert np.array_equal(tilt_eulers(tilts), oracle.tilt_eulers(tilts))
+    assert np.array_equal(tilt_eulers(tilts, 12.5, False), oracle.tilt_eulers(tilts, 12.5, False))
+
+
+def test_fp_factors_single_axis(oracle):
+    """dirX = tan(theta), dirY = 0, normIncrement = 1/|cos(theta)| (VoxelProjector3D.java:692-694)."""
+    for th in (-60.0, -5.0, 25.0, 60.0):
+        f = oracle.fp_factors(oracle.tilt_eulers([th])[0])
+        t = th / 180 * math.pi
+        assert f[6] == pytest.approx(math.tan(t), rel=1e-15)
+        assert f[7] == 0.0
+        assert f[8] == pytest.approx(1 / abs(math.cos(t)), rel=1e-15)
+
+
+# ---------------------------------------------------------------------------
+# samplers
+# ---------------------------------------------------------------------------
+
+def test_getpixel_linear_exact_and_edges(oracle):
+    """Bilinear of a linear function is exact inside; the last column, last row and
+    last slice return the nearest sample (TomoReconstruction2.java:163-164)."""
+    W, H, D = 8, 6, 4
+    z, y, x = np.meshgrid(np.arange(D), np.arange(H), np.arange(W), indexing="ij")
+    vol = (0.5 * x + 0.25 * y + 2.0 * z).astype(np.float32)
+    assert oracle.getpixel(vol, 2.5, 3.25, 1) == pytest.approx(0.5 * 2.5 + 0.25 * 3.25 + 2, abs=1e-6)
+    # x in [W-1, W): nearest = value at xbase
+    assert oracle.getpixel(vol, 7.6, 2.0, 1) == pytest.approx(0.5 * 7 + 0.25 * 2 + 2)
+    # last slice: no interpolation in x or y
+    assert oracle.getpixel(vol, 2.5, 3.25, D - 1) == pytest.approx(0.5 * 2 + 0.25 * 3 + 2 * (D - 1))
+    # slightly negative x: xbase 0, fraction 0
+    assert oracle.getpixel(vol, -1e-12, 1.0, 0) == pytest.approx(0.25)
+
+
+def test_ij_interpolation_closed_form(oracle):
+    """ImageJ getInterpolatedValue: bilinear inside, clamped in the last-pixel band."""
+    img = np.arange(20, dtype=np.float32).reshape(4, 5)  # value = 5*y + x
+    assert oracle.ij_interpolated_value(img, 1.25, 2.5) == pytest.approx(5 * 2.5 + 1.25)
+    assert oracle.ij_interpolated_value(img, 4.5, 1.0) == pytest.approx(5 + 4)       # x clamped
+    assert oracle.ij_interpolated_value(img, 2.0, 3.75) == pytest.approx(15 + 2)     # y clamped
+    assert oracle.ij_interpolated_value(img, 5.0, 1.0) == 0.0                         # outside
+
+
+# ---------------------------------------------------------------------------
+# forward projection
+# ---------------------------------------------------------------------------
+
+def test_fp_constant_volume_zero_tilt(oracle):
+    """theta = 0, constant c: every ray stays inside, proj = c*D, norm = D, no exit step."""
+    W, H, D = 12, 5, 7
+    g = geom(oracle, W, H, D)
+    vol = np.full((D, H, W), 0.75, np.float32)
+    p, n = oracle.project(g, vol, oracle.tilt_eulers([0.0]))
+    assert np.all(p == np.float32(0.75 * D)) and np.all(n == D)
+
+
+@pytest.mark.parametrize("theta", [-50.0, 20.0, 37.0])
+def test_fp_constant_volume_tilted_counts(oracle, theta):
+    """Constant c = 0.5 (sums exact in float): proj = c * normInc * n_in and
+    norm = normInc * (n_in + exited), n_in and 'exited' counted from the closed-form
+    ray x(z) = x0 + (z - vcz) tan(theta) (pixels whose samples come within 1e-6
+    of a border are skipped)."""
+    W, H, D = 40, 3, 24
+    g = geom(oracle, W, H, D)
+    vol = np.full((D, H, W), 0.5, np.float32)
+    E = oracle.tilt_eulers([theta])
+    p, n = oracle.project(g, vol, E)
+    t = theta / 180 * math.pi
+    ni = 1 / abs(math.cos(t))
+    pc0, vcx, vcz = (W - 1) / 2, (W - 1) / 2, (D - 1) / 2
+    zs = np.arange(D)
+    checked = 0
+    for i in range(W):
+        x = (i - pc0) / math.cos(t) + (zs - vcz) * math.tan(t) + vcx
+        if np.any(np.abs(x) < 1e-6) or np.any(np.abs(x - W) < 1e-6):
+            continue
+        inside = (x >= 0) & (x < W)
+        n_in = int(inside.sum())
+        if n_in == 0:
+            assert math.isnan(p[0, 1, i]) and n[0, 1, i] == 0
+            continue
+        last = np.nonzero(inside)[0].max()
+        exited = 1 if last < D - 1 else 0
+        assert p[0, 1, i] == pytest.approx(0.5 * ni * n_in, rel=1e-6)
+        assert n[0, 1, i] == pytest.approx(ni * (n_in + exited), rel=1e-6)
+        checked += 1
+    assert checked > W // 2
+
+
+def test_fp_linear_ramp(oracle):
+    """f = a*x + b is interpolated exactly, except the nearest rule on the last slice
+    and at x >= W-1: the projection equals normInc * sum_z f(x_z) with those rules."""
+    W, H, D, th = 30, 4, 10, 25.0
+    g = geom(oracle, W, H, D)
+    x = np.arange(W, dtype=np.float64)
+    vol = np.broadcast_to((0.125 * x + 1.0).astype(np.float32), (D, H, W)).copy()
+    p, _ = oracle.project(g, vol, oracle.tilt_eulers([th]))
+    t = th / 180 * math.pi
+    ni = 1 / math.cos(t)
+    for i in range(W):
+        xs = (i - (W - 1) / 2) / math.cos(t) + (np.arange(D) - (D - 1) / 2) * math.tan(t) + (W - 1) / 2
+        inside = (xs >= 0) & (xs < W)
+        if not inside.any():
+            continue
+        vals = []
+        for z, xv in zip(range(D), xs):
+            if not inside[z]:
+                continue
+            xb = int(xv)
+            if xb == W - 1 or z == D - 1:
+                vals.append(0.125 * xb + 1.0)
+            else:
+                vals.append(0.125 * xv + 1.0)
+        assert p[0, 2, i] == pytest.approx(ni * sum(vals), rel=2e-5), i
+
+
+def test_fp64_matches_analytic_line_integrals(oracle):
+    """Projection of a voxelised phantom matches the analytic line integrals in the
+    same geometry (orientation, tilt sign and centres); a flipped tilt sign is far off."""
+    W, H, D = 64, 8, 48
+    ell = np.zeros((1, 16))
+    ell[0, 0] = 1.0
+    ell[0, 1:4] = [9.0, 0.0, -6.0]       # off-centre in x and z: sensitive to the tilt sign
+    ell[0, 4:7] = [10.0, 30.0, 7.0]
+    ell[0, 7:16] = np.eye(3).ravel()
+    vol = oracle.phantom_volume(ell, W, H, D).astype(np.float64)
+    g = geom(oracle, W, H, D)
+    E = oracle.tilt_eulers([35.0])
+    p, _ = oracle.project(g, vol, E, f64=True)
+    ref = oracle.phantom_project(ell, E, W, H)
+    wrong = oracle.phantom_project(ell, oracle.tilt_eulers([-35.0]), W, H)
+    row = slice(2, 6)
+    good = np.linalg.norm(p[0, row] - ref[0, row]) / np.linalg.norm(ref[0, row])
+    bad = np.linalg.norm(p[0, row] - wrong[0, row]) / np.linalg.norm(ref[0, row])
+    assert good < 0.08 and bad > 0.5
+
+
+def test_analytic_sphere_closed_form(oracle):
+    """Chord of a sphere of radius r at distance d from its centre: 2*sqrt(r^2-d^2)."""
+    ell = np.zeros((1, 16))
+    ell[0, 0] = 0.5
+    ell[0, 4:7] = 5.0
+    ell[0, 7:16] = np.eye(3).ravel()
+    W = H = 15
+    p = oracle.phantom_project(ell, oracle.tilt_eulers([0.0, 40.0]), W, H)
+    for k in range(2):
+        for (i, j) in [(7, 7), (9, 7), (7, 3), (10, 9), (12, 10)]:
+            d2 = (i - 7) ** 2 + (j - 7) ** 2
+            ref = 0.5 * 2 * math.sqrt(25 - d2) if d2 < 25 else 0.0
+            assert p[k, j, i] == pytest.approx(ref, abs=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# residual
+# ---------------------------------------------------------------------------
+
+def test_difference_known_answers(oracle):
+    """diff = ((exp - th) / norm) * factor; NaN in exp or th leaves 0 and no error term."""
+    exp = np.array([3.0, 1.0, np.nan, 2.0], np.float32)
+    th = np.array([1.0, 1.5, 1.0, np.nan], np.float32)
+    nm = np.array([2.0, 0.5, 1.0, 1.0], np.float32)
+    d, e = oracle.difference(exp, th, nm, 0.5)
+    assert list(d) == [0.5, -0.5, 0.0, 0.0]
+    assert e == pytest.approx(0.5)
+
+
+# ---------------------------------------------------------------------------
+# backprojection
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("theta", [0.0, 30.0, -45.0])
+def test_bp_constant_image(oracle, theta):
+    """A constant image v adds exactly v to every voxel whose projection lands in
+    [0,W) x [0,H) and nothing elsewhere."""
+    W, H, D = 20, 6, 9
+    g = geom(oracle, W, H, D)
+    vol = np.zeros((D, H, W), np.float32)
+    oracle.backproject(g, vol, np.full((1, H, W), 0.25, np.float32), oracle.tilt_eulers([theta]))
+    t = theta / 180 * math.pi
+    z, y, x = np.meshgrid(np.arange(D) - (D - 1) / 2, np.arange(H), np.arange(W) - (W - 1) / 2, indexing="ij")
+    xx = x * math.cos(t) - z * math.sin(t) + (W - 1) / 2
+    inside = (xx >= 0) & (xx < W)
+    far = (np.abs(xx) > 1e-9) & (np.abs(xx - W) > 1e-9)
+    assert np.all(vol[inside & far] == np.float32(0.25))
+    assert np.all(vol[~inside & far] == 0)
+
+
+def test_bp_delta_image_tent(oracle):
+    """A single-pixel image spreads as the bilinear tent max(0, 1 - |xx - i0|) on row j0."""
+    W, H, D, th = 24, 5, 11, 28.0
+    g = geom(oracle, W, H, D)
+    img = np.zeros((1, H, W), np.float32)
+    img[0, 2, 9] = 1.0
+    vol = np.zeros((D, H, W), np.float32)
+    oracle.backproject(g, vol, img, oracle.tilt_eulers([th]))
+    t = th / 180 * math.pi
+    z, x = np.meshgrid(np.arange(D) - (D - 1) / 2, np.arange(W) - (W - 1) / 2, indexing="ij")
+    xx = x * math.cos(t) - z * math.sin(t) + (W - 1) / 2
+    tent = np.maximum(0, 1 - np.abs(xx - 9)).astype(np.float32)
+    assert np.allclose(vol[:, 2, :], tent, atol=1e-6)
+    assert np.all(vol[:, [0, 1, 3, 4], :] == 0)
+
+
+def test_bp_scrubs_nan_and_inf(oracle):
+    """NaN samples are skipped; a sum that becomes Inf is reset to 0 (VoxelProjector3D.java:291-300).
+    ImageJ's bilinear formula multiplies the next row by its (zero) weight, so a NaN
+    in the row below also makes the sample NaN: row 0 skips image 0 too."""
+    W, H, D = 6, 3, 2
+    g = geom(oracle, W, H, D)
+    vol = np.zeros((D, H, W), np.float32)
+    img = np.full((2, H, W), 1.0, np.float32)
+    img[0, 1, :] = np.nan
+    img[1, 2, :] = np.float32(3e38)
+    vol[:, 2, :] = np.float32(3e38)
+    oracle.backproject(g, vol, img, oracle.tilt_eulers([0.0, 0.0]))
+    assert np.all(vol[:, 0, :] == 1.0)
+    assert np.all(vol[:, 1, :] == 1.0)
+    assert np.all(vol[:, 2, :] == 0.0)
+
+
+# ---------------------------------------------------------------------------
+# loops
+# ---------------------------------------------------------------------------
+
+def test_order_shuffle_perpendicular_chain(oracle):
+    """Each next projection is the remaining one closest to perpendicular to the
+    previous: from -60 the chain is -60, 30, -58, 32, ... (orderShuffleART)."""
+    tilts = np.arange(-60, 61, 2.0)
+    order = oracle.order_shuffle(oracle.tilt_eulers(tilts), 0)
+    assert sorted(order) == list(range(61))
+    assert list(order[:6]) == [0, 45, 1, 46, 2, 47]
+    rem = set(range(61))
+    prev = order[0]
+    rem.discard(prev)
+    for o in order[1:]:
+        best = min(abs(math.cos(math.radians(tilts[prev] - tilts[r]))) for r in rem)
+        assert abs(math.cos(math.radians(tilts[prev] - tilts[o]))) <= best + 1e-12
+        rem.discard(o)
+        prev = o
+
+
+def test_python_order_shuffle_matches_oracle(oracle):
+    from tomoj_amd.reconstruction import order_shuffle
+    E = oracle.tilt_eulers(np.arange(-60, 61, 1.0))
+    for first in (0, 17, 60, 120):
+        assert list(order_shuffle(E, first)) == list(oracle.order_shuffle(E, first))
+
+
+def test_sirt_fixed_point(oracle):
+    """If the measured projections are the projections of x*, SIRT leaves x* unchanged
+    (every difference is exactly 0)."""
+    W, H, D = 16, 4, 8
+    g = geom(oracle, W, H, D)
+    rng = np.random.default_rng(1)
+    x = rng.random((D, H, W), dtype=np.float32)
+    E = oracle.tilt_eulers(np.arange(-40, 41, 20.0))
+    exps, _ = oracle.project(g, x, E)
+    vol = x.copy()
+    err = oracle.ossart(g, vol, exps, E, 1, 1.0, 0)
+    assert np.array_equal(vol, x)
+    assert err[0, 0] == 0.0
+
+
+def test_sirt_residual_decreases(oracle):
+    W, H, D = 32, 4, 16
+    g = geom(oracle, W, H, D)
+    tilts = np.arange(-60, 61, 5.0)
+    E = oracle.tilt_eulers(tilts)
+    ell = oracle.ellipsoids(7, 4, W, H, D)
+    exps = oracle.phantom_project(ell, E, W, H).astype(np.float32)
+    vol = np.zeros((D, H, W), np.float32)
+    err = oracle.ossart(g, vol, exps, E, 6, 1.0, 0)
+    assert np.all(np.diff(err[:, 0]) < 0)
+
+
+def test_ossart_f32_close_to_f64(oracle):
+    """The FP32-faithful loop and the FP64 twin agree to FP32 rounding level."""
+    W, H, D = 24, 4, 12
+    g = geom(oracle, W, H, D)
+    E = oracle.tilt_eulers(np.arange(-60, 61, 10.0))
+    ell = oracle.ellipsoids(3, 3, W, H, D)
+    exps = oracle.phantom_project(ell, E, W, H)
+    v32 = np.zeros((D, H, W), np.float32)
+    v64 = np.zeros((D, H, W), np.float64)
+    oracle.ossart(g, v32, exps.astype(np.float32), E, 3, 1.0, 4)
+    oracle.ossart(g, v64, exps, E, 3, 1.0, 4, f64=True)
+    assert np.linalg.norm(v32 - v64) / np.linalg.norm(v64) < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# weighting
+# ---------------------------------------------------------------------------
+
+def test_fft2_matches_numpy(oracle):
+    rng = np.random.default_rng(3)
+    for (sy, sx) in [(8, 16), (6, 10), (32, 32)]:
+        re = rng.standard_normal((sy, sx))
+        im = rng.standard_normal((sy, sx))
+        r, i = oracle.fft2(re, im, -1)
+        ref = np.fft.fft2(re + 1j * im)
+        assert np.allclose(r + 1j * i, ref, atol=1e-10)
+
+
+def test_weighting_matches_numpy(oracle):
+    rng = np.random.default_rng(4)
+    img = rng.standard_normal((16, 32)).astype(np.float32)
+    mask = rng.random((16, 32)).astype(np.float32)
+    mask = 0.5 * (mask + np.roll(mask[::-1, ::-1], (1, 1), (0, 1)))  # Hermitian-symmetric
+    out = oracle.weighting(img, mask)
+    ref = np.real(np.fft.ifft2(np.fft.fft2(img.astype(np.float64)) * mask))
+    assert np.allclose(out, ref, atol=1e-5)
+
+
+def test_weighting_mask_special_cases(oracle):
+    """Column ci = 0 sees sinc(0) = 1 for every tilt: weight = N * diameter.  For one
+    tilt theta: mask = 1/(diameter * max(0.005, sinc(pi*diameter*ci*sin(theta)/sx)))."""
+    sx, sy, dia = 32, 8, 16.0
+    tilts = np.arange(-60, 61, 20.0)
+    m = oracle.weighting_mask(sx, sy, tilts, dia)
+    assert np.allclose(m[:, 0], np.float32(1 / (len(tilts) * dia)))
+    th = 30.0
+    m1 = oracle.weighting_mask(sx, sy, [th], dia)
+    for i in [1, 3, 7, 16, 20]:
+        ci = i if i <= sx / 2 else i - sx
+        v = dia * math.pi * ci * math.sin(math.radians(th)) / sx
+        w = max(math.sin(v) / v, 0.005)
+        assert m1[2, i] == pytest.approx(1 / (w * dia), rel=1e-5)
+
+
+def test_noise_statistics(oracle):
+    z = oracle.gaussian_noise(12345, 200000)
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01

@@ -1,0 +1,15 @@
+"""tomoj_amd -- MI355X-native parallel-beam projector path for TomoJ.
+
+Hand-written HIP kernels for gfx950 behind the C ABI in ``include/tomoj_hip.h``
+(``libtomoj_hip.so``): Joseph forward projection with fused residual,
+voxel-driven backprojection, OS-SART/SIRT/ART and weighted-backprojection
+loops, mirroring TomoJ's ``Projector`` plugin interface.
+"""
+from . import _lib
+from ._lib import TjError, TjCancelled, load as load_library
+from .projector import VoxelProjector3DHIP, phantom_ellipsoids
+from .reconstruction import tilt_euler, tilt_eulers, order_shuffle, ossart, wbp
+from . import workloads
+
+__all__ = ["VoxelProjector3DHIP", "TjError", "TjCancelled", "load_library", "phantom_ellipsoids",
+           "tilt_euler", "tilt_eulers", "order_shuffle", "ossart", "wbp", "workloads"]

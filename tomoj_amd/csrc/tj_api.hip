@@ -1,0 +1,1018 @@
+// tj_api.hip -- host side of libtomoj_hip.so: the C ABI declared in
+// include/tomoj_hip.h, the projector context, and the C++ restatement of
+// TomoJ's update loops (TomoReconstruction2.OSSART / osartIteration /
+// orderShuffleART / WBP) that drive the kernels in tj_kernels.hip.
+//
+// Reference paths are relative to src/TomoJ_Base/src/main/java/fr/curie/tomoj/.
+#include "tj_device.h"
+#include "../../include/tomoj_hip.h"
+
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+
+namespace tjk {
+hipError_t fp(hipStream_t, const TjGeomDev&, const float*, const TjProj*, const int*, int, int, bool,
+              const uint8_t*, float*, float*, int);
+hipError_t norm_max_fill(hipStream_t, int, int, float*);
+hipError_t residual(hipStream_t, int, int, int, const float*, const int*, const float*, const float*, double,
+                    float*, double*, uint8_t*, double*);
+hipError_t row_flags(hipStream_t, int, int, int, const float*, const int*, uint8_t*);
+hipError_t bp(hipStream_t, const TjGeomDev&, float*, const TjProj*, const int*, int, const float*, const int*,
+              const uint8_t*, TjBpRow*, int, int, bool, int, int);
+hipError_t zyx_to_yzx(hipStream_t, const float*, float*, int, int, int, int);
+hipError_t yzx_to_zyx(hipStream_t, const float*, float*, int, int, int, int);
+hipError_t plane_flags(hipStream_t, const float*, size_t, int, int, uint8_t*);
+hipError_t phantom_project(hipStream_t, const double*, int, const TjProj*, int, int, int, double, double, double*);
+hipError_t add_noise(hipStream_t, const double*, size_t, uint64_t, double, double*, int, float*);
+hipError_t weighting_init(hipStream_t, int, int, double, const double*, const double*, int, float*);
+hipError_t ramp_init(hipStream_t, int, int, int, double, double, float*);
+hipError_t weighting_apply(hipStream_t, int, int, int, const float*, const int*, const float*, float*, void*, size_t);
+size_t weighting_workspace(int, int, int);
+}  // namespace tjk
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+static int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define TJ_HIP(call)                                                                               \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) return fail(TJ_EDEVICE, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                          __FILE__, __LINE__);                                     \
+    } while (0)
+
+template <typename T>
+static hipError_t dalloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+template <typename T>
+static void dfree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct tj_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    tj_geometry geo{};
+    TjGeomDev g{};
+    size_t npx = 0, plane = 0, nvox = 0;
+
+    float* vol = nullptr;       // [vh][vd][vw]
+    uint8_t* pflag = nullptr;   // [vh+1] non-finite flag per volume plane
+
+    int nproj = 0;              // resident projections
+    float* exps = nullptr;      // [nproj][ph][pw]
+    TjProj* pp = nullptr;       // [nproj]
+    std::vector<TjProj> hpp;
+    int* order = nullptr;       // [nproj] device projection order for tj_run_ossart
+
+    int cap = 0;                // slot capacity
+    float *sproj = nullptr, *snorm = nullptr, *sdiff = nullptr, *swgt = nullptr;
+    int swgt_cap = 0;
+    double* row_err = nullptr;  // [cap][ph]
+    uint8_t* rowflag = nullptr; // [cap][ph]
+    TjBpRow* bprows = nullptr;  // [cap][vh]
+    double* err = nullptr;      // [max(cap,nproj)]
+    int* slot_proj = nullptr;   // [cap]
+    int* iota = nullptr;        // [max(cap,nproj)] 0,1,2,...
+    int iota_n = 0;
+    double* h_err = nullptr;    // pinned [max(cap,nproj)]
+    int h_err_n = 0;
+
+    std::vector<int> added;
+    bool have_diff = false;
+    bool have_proj = false;
+
+    // weighting filter
+    int weight_kind = 0;        // 0 none, 1 Radermacher, 2 ramp
+    float* mask = nullptr;      // [ph][pw]
+    void* wspace = nullptr;
+    size_t wspace_size = 0;
+
+    // tuning
+    int fp_rows = 4, bp_rows = 4, bp_kchunk = 0;
+
+    // progress / cancellation
+    std::atomic<double> completion{0.0};
+    std::atomic<int> interrupted{0};
+
+    // timing
+    int timing_on = 0;
+    tj_timing last_timing{};
+    std::vector<hipEvent_t> events;
+};
+
+static void ctx_free(tj_ctx* c)
+{
+    dfree(c->vol);
+    dfree(c->pflag);
+    dfree(c->exps);
+    dfree(c->pp);
+    dfree(c->order);
+    dfree(c->sproj);
+    dfree(c->snorm);
+    dfree(c->sdiff);
+    dfree(c->swgt);
+    dfree(c->row_err);
+    dfree(c->rowflag);
+    dfree(c->bprows);
+    dfree(c->err);
+    dfree(c->slot_proj);
+    dfree(c->iota);
+    dfree(c->mask);
+    if (c->wspace) (void)hipFree(c->wspace);
+    c->wspace = nullptr;
+    if (c->h_err) (void)hipHostFree(c->h_err);
+    c->h_err = nullptr;
+    for (auto e : c->events) (void)hipEventDestroy(e);
+    c->events.clear();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->stream = nullptr;
+}
+
+// Per-projection constants of projectJosephPartial (VoxelProjector3D.java:682-694),
+// same operation order as the Java source (host code is built without FMA
+// contraction).
+static TjProj make_proj(const double* r)
+{
+    TjProj p{};
+    for (int k = 0; k < 9; ++k) p.e[k] = r[k];
+    p.icf = r[3] / r[0];
+    p.zcf = (r[5] - (r[3] * r[2] / r[0]));
+    p.divy = r[4] - (r[3] * r[1] / r[0]);
+    p.yfx = r[1];
+    p.zfx = r[2];
+    p.divx = r[0];
+    p.diry = -(r[5] - r[3] * r[2] / r[0]) / (r[4] - r[3] * r[1] / r[0]);
+    p.dirx = (r[1] * (-p.diry) - r[2]) / r[0];
+    p.norminc = std::sqrt(p.dirx * p.dirx + p.diry * p.diry + 1);
+    p.axis = (r[1] == 0.0 && r[3] == 0.0 && r[5] == 0.0 && r[0] != 0.0 && r[4] != 0.0) ? 1 : 0;
+    return p;
+}
+
+static bool all_axis(const tj_ctx* c, const std::vector<int>& idx)
+{
+    for (int i : idx)
+        if (!c->hpp[i].axis) return false;
+    return true;
+}
+
+static int ensure_iota(tj_ctx* c, int n)
+{
+    if (n <= c->iota_n) return TJ_OK;
+    dfree(c->iota);
+    dfree(c->err);
+    if (c->h_err) (void)hipHostFree(c->h_err);
+    c->h_err = nullptr;
+    std::vector<int> h(n);
+    for (int k = 0; k < n; ++k) h[k] = k;
+    TJ_HIP(dalloc(&c->iota, n));
+    TJ_HIP(dalloc(&c->err, n));
+    TJ_HIP(hipHostMalloc((void**)&c->h_err, sizeof(double) * n, hipHostMallocDefault));
+    TJ_HIP(hipMemcpy(c->iota, h.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    c->iota_n = n;
+    c->h_err_n = n;
+    return TJ_OK;
+}
+
+static int ensure_slots(tj_ctx* c, int n)
+{
+    if (n <= c->cap) return ensure_iota(c, n);
+    dfree(c->sproj);
+    dfree(c->snorm);
+    dfree(c->sdiff);
+    dfree(c->swgt);
+    dfree(c->row_err);
+    dfree(c->rowflag);
+    dfree(c->bprows);
+    dfree(c->slot_proj);
+    c->cap = 0;
+    const size_t npx = c->npx;
+    if (dalloc(&c->sproj, npx * n) || dalloc(&c->snorm, npx * n) || dalloc(&c->sdiff, npx * n) ||
+        dalloc(&c->row_err, (size_t)c->g.ph * n) || dalloc(&c->rowflag, (size_t)c->g.ph * n) ||
+        dalloc(&c->bprows, (size_t)c->g.vh * n) || dalloc(&c->slot_proj, n)) {
+        (void)hipGetLastError();
+        return fail(TJ_ENOMEM, "cannot allocate %d projection slots of %dx%d", n, c->g.pw, c->g.ph);
+    }
+    c->cap = n;
+    return ensure_iota(c, n);
+}
+
+static int ensure_weight_buffers(tj_ctx* c, int n)
+{
+    size_t need = tjk::weighting_workspace(c->g.pw, c->g.ph, n);
+    if (need > c->wspace_size) {
+        if (c->wspace) (void)hipFree(c->wspace);
+        c->wspace = nullptr;
+        c->wspace_size = 0;
+        if (hipMalloc(&c->wspace, need) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(TJ_ENOMEM, "cannot allocate the weighting workspace (%zu bytes)", need);
+        }
+        c->wspace_size = need;
+    }
+    return TJ_OK;
+}
+
+static int check_range(const tj_ctx* c, int start_y, int end_y)
+{
+    if (start_y < 0 || end_y > c->g.vh || start_y > end_y)
+        return fail(TJ_EINVAL, "row range [%d,%d) outside the volume height %d", start_y, end_y, c->g.vh);
+    return TJ_OK;
+}
+
+// First detector row VoxelProjector3D.project computes (:338, :347).
+static int fp_start_row(const tj_ctx* c, int start_y)
+{
+    const tj_geometry& g = c->geo;
+    const int offset = (g.proj_h > g.vol_h && g.scale == 1) ? (g.proj_h - g.vol_h) / 2 : 0;
+    return (int)std::floor((start_y + offset) * g.scale + 0.5);
+}
+
+static hipEvent_t ev(tj_ctx* c, size_t k)
+{
+    while (c->events.size() <= k) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->events.push_back(e);
+    }
+    return c->events[k];
+}
+
+// Forward projection of `nslot` slots whose projection indices are in device
+// array `slots` (host copy `hslots`), into sproj/snorm.
+static int do_project(tj_ctx* c, const int* slots, const std::vector<int>& hslots, int start_y)
+{
+    const int nslot = (int)hslots.size();
+    const int start = fp_start_row(c, start_y);
+    const bool axis = all_axis(c, hslots);
+    TJ_HIP(tjk::fp(c->stream, c->g, c->vol, c->pp, slots, nslot, start, axis, c->pflag, c->sproj, c->snorm,
+                   c->fp_rows));
+    if (c->geo.long_object_comp) TJ_HIP(tjk::norm_max_fill(c->stream, (int)c->npx, nslot, c->snorm));
+    return TJ_OK;
+}
+
+static int do_residual(tj_ctx* c, const int* slots, int nslot, double factor, double* err_out)
+{
+    TJ_HIP(tjk::residual(c->stream, c->g.pw, c->g.ph, nslot, c->exps, slots, c->sproj, c->snorm, factor, c->sdiff,
+                         c->row_err, c->rowflag, err_out));
+    return TJ_OK;
+}
+
+static int do_backproject(tj_ctx* c, const int* slots, const std::vector<int>& hslots, const float* images,
+                          const int* image_index, int start_y, int end_y)
+{
+    const bool axis = all_axis(c, hslots);
+    TJ_HIP(tjk::bp(c->stream, c->g, c->vol, c->pp, slots, (int)hslots.size(), images, image_index, c->rowflag,
+                   c->bprows, start_y, end_y, axis, c->bp_rows, c->bp_kchunk));
+    return TJ_OK;
+}
+
+// TomoReconstruction2.orderShuffleART (:735-781) with the random first pick
+// replaced by `first`: each next image is the remaining one whose row 2 of the
+// Euler matrix has the smallest |dot product| with the previous one's.
+static void order_shuffle(const tj_ctx* c, int first, std::vector<int>& order)
+{
+    const int n = c->nproj;
+    order.assign(n, 0);
+    std::vector<char> todo(n, 1);
+    int cc = 0, im = 0;
+    while (cc < n) {
+        if (cc == 0) {
+            im = first;
+        } else {
+            double mini = DBL_MAX;
+            int omini = -1;
+            const double* a = c->hpp[im].e + 6;
+            for (int o = 0; o < n; ++o) {
+                if (!todo[o]) continue;
+                const double* b = c->hpp[o].e + 6;
+                double sum = 0;
+                sum += a[0] * b[0];
+                sum += a[1] * b[1];
+                sum += a[2] * b[2];
+                const double sp = std::fabs(sum);
+                if (sp < mini) {
+                    mini = sp;
+                    omini = o;
+                }
+            }
+            im = omini;
+        }
+        if (todo[im]) {
+            order[cc] = im;
+            todo[im] = 0;
+            ++cc;
+        }
+    }
+}
+
+extern "C" {
+
+const char* tj_last_error(void) { return g_last_error.c_str(); }
+const char* tj_version(void) { return "tomoj-mi355x 0.1 (gfx950)"; }
+
+void tj_geometry_default(tj_geometry* g, int vw, int vh, int vd, int pw, int ph)
+{
+    std::memset(g, 0, sizeof *g);
+    g->vol_w = vw;
+    g->vol_h = vh;
+    g->vol_d = vd;
+    g->proj_w = pw;
+    g->proj_h = ph;
+    g->proj_center[0] = (pw - 1.0) / 2;
+    g->proj_center[1] = (ph - 1.0) / 2;
+    g->proj_center[2] = (pw - 1) / 2.0;
+    g->proj_center[3] = (ph - 1) / 2.0;
+    g->vol_center[0] = (vw - 1) / 2.0;
+    g->vol_center[1] = (vh - 1) / 2.0;
+    g->vol_center[2] = (vd - 1) / 2.0;
+    g->scale = 1.0;
+    g->storage = TJ_STORAGE_F32;
+}
+
+int tj_open(int device, const tj_geometry* geo, tj_ctx** out)
+{
+    if (!geo || !out) return fail(TJ_EINVAL, "tj_open: null argument");
+    *out = nullptr;
+    if (geo->vol_w <= 0 || geo->vol_h <= 0 || geo->vol_d <= 0 || geo->proj_w <= 0 || geo->proj_h <= 0)
+        return fail(TJ_EINVAL, "tj_open: non-positive size");
+    if (!(geo->scale > 0)) return fail(TJ_EINVAL, "tj_open: scale must be > 0");
+    if (geo->storage != TJ_STORAGE_F32) return fail(TJ_EUNSUP, "tj_open: storage type %d not supported", geo->storage);
+    int ndev = 0;
+    TJ_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(TJ_EINVAL, "tj_open: device %d of %d", device, ndev);
+    TJ_HIP(hipSetDevice(device));
+    tj_ctx* c = new tj_ctx();
+    c->device = device;
+    c->geo = *geo;
+    TjGeomDev& g = c->g;
+    g.vw = geo->vol_w;
+    g.vh = geo->vol_h;
+    g.vd = geo->vol_d;
+    g.pw = geo->proj_w;
+    g.ph = geo->proj_h;
+    g.positivity = geo->positivity ? 1 : 0;
+    g.pc0 = geo->proj_center[0];
+    g.pc1 = geo->proj_center[1];
+    g.pc2 = geo->proj_center[2];
+    g.pc3 = geo->proj_center[3];
+    g.vcx = geo->vol_center[0];
+    g.vcy = geo->vol_center[1];
+    g.vcz = geo->vol_center[2];
+    g.scale = geo->scale;
+    g.bxmax = (double)((float)g.vw + 1e-10f);
+    g.bymax = (double)((float)g.vh + 1e-10f);
+    c->npx = (size_t)g.pw * g.ph;
+    c->plane = (size_t)g.vd * g.vw;
+    c->nvox = c->plane * g.vh;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        dalloc(&c->vol, c->nvox) != hipSuccess || dalloc(&c->pflag, (size_t)g.vh + 1) != hipSuccess) {
+        (void)hipGetLastError();
+        ctx_free(c);
+        delete c;
+        return fail(TJ_ENOMEM, "tj_open: cannot allocate a %dx%dx%d volume", g.vw, g.vh, g.vd);
+    }
+    if (hipMemsetAsync(c->vol, 0, c->nvox * sizeof(float), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->pflag, 0, g.vh + 1, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        ctx_free(c);
+        delete c;
+        return fail(TJ_EDEVICE, "tj_open: clearing the volume failed");
+    }
+    *out = c;
+    return TJ_OK;
+}
+
+int tj_close(tj_ctx* c)
+{
+    if (!c) return TJ_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    ctx_free(c);
+    delete c;
+    return TJ_OK;
+}
+
+int tj_set_eulers(tj_ctx* c, const double* euler, int n)
+{
+    if (!c || !euler) return fail(TJ_EINVAL, "tj_set_eulers: null argument");
+    if (n != c->nproj) return fail(TJ_EINVAL, "tj_set_eulers: %d matrices for %d projections", n, c->nproj);
+    TJ_HIP(hipSetDevice(c->device));
+    c->hpp.resize(n);
+    for (int k = 0; k < n; ++k) c->hpp[k] = make_proj(euler + 9 * k);
+    TJ_HIP(hipMemcpy(c->pp, c->hpp.data(), sizeof(TjProj) * n, hipMemcpyHostToDevice));
+    return TJ_OK;
+}
+
+static int alloc_projections(tj_ctx* c, int n, const double* euler)
+{
+    dfree(c->exps);
+    dfree(c->pp);
+    dfree(c->order);
+    c->nproj = 0;
+    c->added.clear();
+    c->have_diff = c->have_proj = false;
+    if (dalloc(&c->exps, c->npx * n) || dalloc(&c->pp, n) || dalloc(&c->order, n)) {
+        (void)hipGetLastError();
+        return fail(TJ_ENOMEM, "cannot allocate %d projections of %dx%d", n, c->g.pw, c->g.ph);
+    }
+    c->nproj = n;
+    int rc = tj_set_eulers(c, euler, n);
+    if (rc) return rc;
+    return ensure_iota(c, n);
+}
+
+int tj_load_projections(tj_ctx* c, const float* stack, int n, const double* euler)
+{
+    if (!c || !stack || !euler || n <= 0) return fail(TJ_EINVAL, "tj_load_projections: bad argument");
+    TJ_HIP(hipSetDevice(c->device));
+    int rc = alloc_projections(c, n, euler);
+    if (rc) return rc;
+    TJ_HIP(hipMemcpy(c->exps, stack, sizeof(float) * c->npx * n, hipMemcpyHostToDevice));
+    return TJ_OK;
+}
+
+int tj_get_loaded_projection(tj_ctx* c, int index, float* out)
+{
+    if (!c || !out) return fail(TJ_EINVAL, "tj_get_loaded_projection: null argument");
+    if (index < 0 || index >= c->nproj) return fail(TJ_EINVAL, "projection index %d of %d", index, c->nproj);
+    TJ_HIP(hipSetDevice(c->device));
+    TJ_HIP(hipMemcpy(out, c->exps + (size_t)index * c->npx, sizeof(float) * c->npx, hipMemcpyDeviceToHost));
+    return TJ_OK;
+}
+
+int tj_set_volume(tj_ctx* c, const float* zyx, int start_y, int end_y)
+{
+    if (!c || !zyx) return fail(TJ_EINVAL, "tj_set_volume: null argument");
+    int rc = check_range(c, start_y, end_y);
+    if (rc) return rc;
+    const int nr = end_y - start_y;
+    if (nr == 0) return TJ_OK;
+    TJ_HIP(hipSetDevice(c->device));
+    const TjGeomDev& g = c->g;
+    float* stage = nullptr;
+    TJ_HIP(dalloc(&stage, (size_t)g.vd * nr * g.vw));
+    hipError_t e = hipMemcpy2DAsync(stage, sizeof(float) * nr * g.vw, zyx + (size_t)start_y * g.vw,
+                                    sizeof(float) * g.vh * g.vw, sizeof(float) * nr * g.vw, g.vd,
+                                    hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = tjk::zyx_to_yzx(c->stream, stage, c->vol, g.vw, g.vd, nr, start_y);
+    if (e == hipSuccess) e = tjk::plane_flags(c->stream, c->vol, c->plane, start_y, nr, c->pflag);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(stage);
+    if (e != hipSuccess) return fail(TJ_EDEVICE, "tj_set_volume: %s", hipGetErrorString(e));
+    return TJ_OK;
+}
+
+int tj_get_volume(tj_ctx* c, float* zyx, int start_y, int end_y)
+{
+    if (!c || !zyx) return fail(TJ_EINVAL, "tj_get_volume: null argument");
+    int rc = check_range(c, start_y, end_y);
+    if (rc) return rc;
+    const int nr = end_y - start_y;
+    if (nr == 0) return TJ_OK;
+    TJ_HIP(hipSetDevice(c->device));
+    const TjGeomDev& g = c->g;
+    float* stage = nullptr;
+    TJ_HIP(dalloc(&stage, (size_t)g.vd * nr * g.vw));
+    hipError_t e = tjk::yzx_to_zyx(c->stream, c->vol, stage, g.vw, g.vd, nr, start_y);
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(zyx + (size_t)start_y * g.vw, sizeof(float) * g.vh * g.vw, stage,
+                             sizeof(float) * nr * g.vw, sizeof(float) * nr * g.vw, g.vd, hipMemcpyDeviceToHost,
+                             c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(stage);
+    if (e != hipSuccess) return fail(TJ_EDEVICE, "tj_get_volume: %s", hipGetErrorString(e));
+    return TJ_OK;
+}
+
+int tj_reset_volume(tj_ctx* c)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_reset_volume: null context");
+    TJ_HIP(hipSetDevice(c->device));
+    TJ_HIP(hipMemsetAsync(c->vol, 0, c->nvox * sizeof(float), c->stream));
+    TJ_HIP(hipMemsetAsync(c->pflag, 0, c->g.vh + 1, c->stream));
+    TJ_HIP(hipStreamSynchronize(c->stream));
+    return TJ_OK;
+}
+
+int tj_add_projection(tj_ctx* c, int index)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_add_projection: null context");
+    if (index < 0 || index >= c->nproj)
+        return fail(TJ_EINVAL, "tj_add_projection: index %d outside the %d resident projections", index, c->nproj);
+    c->added.push_back(index);
+    return TJ_OK;
+}
+
+int tj_clear_projections(tj_ctx* c)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_clear_projections: null context");
+    c->added.clear();
+    c->have_diff = c->have_proj = false;
+    return TJ_OK;
+}
+
+int tj_num_added(tj_ctx* c) { return c ? (int)c->added.size() : 0; }
+
+static int upload_added(tj_ctx* c)
+{
+    int rc = ensure_slots(c, (int)c->added.size());
+    if (rc) return rc;
+    TJ_HIP(hipMemcpyAsync(c->slot_proj, c->added.data(), sizeof(int) * c->added.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    return TJ_OK;
+}
+
+int tj_project(tj_ctx* c, int start_y, int end_y)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_project: null context");
+    int rc = check_range(c, start_y, end_y);
+    if (rc) return rc;
+    TJ_HIP(hipSetDevice(c->device));
+    c->completion.store(0.0);
+    if (c->added.empty()) {
+        c->have_proj = true;
+        return TJ_OK;
+    }
+    if ((rc = upload_added(c))) return rc;
+    if ((rc = do_project(c, c->slot_proj, c->added, start_y))) return rc;
+    TJ_HIP(hipStreamSynchronize(c->stream));
+    c->have_proj = true;
+    return TJ_OK;
+}
+
+int tj_project_and_difference(tj_ctx* c, double factor, int start_y, int end_y, double* errors)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_project_and_difference: null context");
+    int rc = check_range(c, start_y, end_y);
+    if (rc) return rc;
+    TJ_HIP(hipSetDevice(c->device));
+    c->completion.store(0.0);
+    const int n = (int)c->added.size();
+    if (n == 0) {
+        c->have_proj = c->have_diff = true;
+        return TJ_OK;
+    }
+    if ((rc = upload_added(c))) return rc;
+    if ((rc = do_project(c, c->slot_proj, c->added, start_y))) return rc;
+    if ((rc = do_residual(c, c->slot_proj, n, factor, c->err))) return rc;
+    TJ_HIP(hipMemcpyAsync(c->h_err, c->err, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    TJ_HIP(hipStreamSynchronize(c->stream));
+    if (errors) std::memcpy(errors, c->h_err, sizeof(double) * n);
+    c->have_proj = c->have_diff = true;
+    return TJ_OK;
+}
+
+int tj_last_error_sum(tj_ctx* c, double* som)
+{
+    if (!c || !som) return fail(TJ_EINVAL, "tj_last_error_sum: null argument");
+    if (!c->have_diff) return fail(TJ_ESTATE, "no difference computed since the last clear");
+    double s = 0;
+    for (size_t k = 0; k < c->added.size(); ++k) s += c->h_err[k];
+    *som = s;
+    return TJ_OK;
+}
+
+static int get_slot_image(tj_ctx* c, int slot, const float* base, float* out, const char* what)
+{
+    if (slot < 0 || slot >= (int)c->added.size())
+        return fail(TJ_EINVAL, "%s: slot %d of %zu added projections", what, slot, c->added.size());
+    TJ_HIP(hipSetDevice(c->device));
+    TJ_HIP(hipMemcpy(out, base + (size_t)slot * c->npx, sizeof(float) * c->npx, hipMemcpyDeviceToHost));
+    return TJ_OK;
+}
+
+int tj_get_projection(tj_ctx* c, int slot, float* proj, float* norm)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_get_projection: null context");
+    if (!c->have_proj) return fail(TJ_ESTATE, "tj_get_projection: no projection computed since the last clear");
+    int rc;
+    if (proj && (rc = get_slot_image(c, slot, c->sproj, proj, "tj_get_projection"))) return rc;
+    if (norm && (rc = get_slot_image(c, slot, c->snorm, norm, "tj_get_projection"))) return rc;
+    return TJ_OK;
+}
+
+int tj_get_difference(tj_ctx* c, int slot, float* diff)
+{
+    if (!c || !diff) return fail(TJ_EINVAL, "tj_get_difference: null argument");
+    if (!c->have_diff) return fail(TJ_ESTATE, "tj_get_difference: no difference computed since the last clear");
+    return get_slot_image(c, slot, c->sdiff, diff, "tj_get_difference");
+}
+
+// Weighted (or plain) images of the added projections into swgt and their row
+// flags; returns the image array the backprojection reads.
+static int prepare_originals(tj_ctx* c, const int* slots, const std::vector<int>& hslots, const float** images,
+                             const int** image_index)
+{
+    const int n = (int)hslots.size();
+    if (c->weight_kind == 0) {
+        *images = c->exps;
+        *image_index = slots;
+    } else {
+        if (c->swgt_cap < n) {
+            dfree(c->swgt);
+            c->swgt_cap = 0;
+            TJ_HIP(dalloc(&c->swgt, c->npx * n));
+            c->swgt_cap = n;
+        }
+        int rc = ensure_weight_buffers(c, n);
+        if (rc) return rc;
+        TJ_HIP(tjk::weighting_apply(c->stream, c->g.pw, c->g.ph, n, c->exps, slots, c->mask, c->swgt, c->wspace,
+                                    c->wspace_size));
+        *images = c->swgt;
+        *image_index = c->iota;
+    }
+    TJ_HIP(tjk::row_flags(c->stream, c->g.pw, c->g.ph, n, *images, *image_index, c->rowflag));
+    return TJ_OK;
+}
+
+int tj_backproject(tj_ctx* c, int start_y, int end_y)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_backproject: null context");
+    int rc = check_range(c, start_y, end_y);
+    if (rc) return rc;
+    TJ_HIP(hipSetDevice(c->device));
+    c->completion.store(0.0);
+    if (c->added.empty()) return TJ_OK;
+    if ((rc = upload_added(c))) return rc;
+    const float* images;
+    const int* index;
+    if (c->have_diff) {
+        images = c->sdiff;
+        index = c->iota;
+    } else {
+        if ((rc = prepare_originals(c, c->slot_proj, c->added, &images, &index))) return rc;
+    }
+    if ((rc = do_backproject(c, c->slot_proj, c->added, images, index, start_y, end_y))) return rc;
+    TJ_HIP(hipStreamSynchronize(c->stream));
+    return TJ_OK;
+}
+
+int tj_set_weighting(tj_ctx* c, double diameter, const double* tilt_deg, const double* axis_deg, int n)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_set_weighting: null context");
+    TJ_HIP(hipSetDevice(c->device));
+    if (!(diameter > 0)) {
+        c->weight_kind = 0;
+        return TJ_OK;
+    }
+    if (!tilt_deg || n <= 0) return fail(TJ_EINVAL, "tj_set_weighting: no tilt angles");
+    if (!c->mask) TJ_HIP(dalloc(&c->mask, c->npx));
+    TJ_HIP(tjk::weighting_init(c->stream, c->g.pw, c->g.ph, diameter, tilt_deg, axis_deg, n, c->mask));
+    c->weight_kind = 1;
+    return TJ_OK;
+}
+
+int tj_set_ramp_filter(tj_ctx* c, int hamming, double cutoff)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_set_ramp_filter: null context");
+    if (!(cutoff > 0 && cutoff <= 1)) return fail(TJ_EINVAL, "tj_set_ramp_filter: cutoff %g not in (0,1]", cutoff);
+    TJ_HIP(hipSetDevice(c->device));
+    if (!c->mask) TJ_HIP(dalloc(&c->mask, c->npx));
+    if (c->nproj <= 0) return fail(TJ_ESTATE, "tj_set_ramp_filter: load the projections first");
+    TJ_HIP(tjk::ramp_init(c->stream, c->g.pw, c->g.ph, hamming, cutoff, M_PI / c->nproj, c->mask));
+    TJ_HIP(hipStreamSynchronize(c->stream));
+    c->weight_kind = 2;
+    return TJ_OK;
+}
+
+int tj_get_weighting_mask(tj_ctx* c, float* mask)
+{
+    if (!c || !mask) return fail(TJ_EINVAL, "tj_get_weighting_mask: null argument");
+    if (c->weight_kind == 0) return fail(TJ_ESTATE, "tj_get_weighting_mask: no weighting filter set");
+    TJ_HIP(hipSetDevice(c->device));
+    TJ_HIP(hipMemcpy(mask, c->mask, sizeof(float) * c->npx, hipMemcpyDeviceToHost));
+    return TJ_OK;
+}
+
+int tj_apply_weighting(tj_ctx* c, float* image)
+{
+    if (!c || !image) return fail(TJ_EINVAL, "tj_apply_weighting: null argument");
+    if (c->weight_kind == 0) return fail(TJ_ESTATE, "tj_apply_weighting: no weighting filter set");
+    TJ_HIP(hipSetDevice(c->device));
+    int rc = ensure_slots(c, 1);
+    if (rc) return rc;
+    if ((rc = ensure_weight_buffers(c, 1))) return rc;
+    float* tmp = nullptr;
+    TJ_HIP(dalloc(&tmp, c->npx * 2));
+    hipError_t e = hipMemcpy(tmp, image, sizeof(float) * c->npx, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = tjk::weighting_apply(c->stream, c->g.pw, c->g.ph, 1, tmp, c->iota, c->mask, tmp + c->npx, c->wspace,
+                                 c->wspace_size);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(image, tmp + c->npx, sizeof(float) * c->npx, hipMemcpyDeviceToHost);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(TJ_EDEVICE, "tj_apply_weighting: %s", hipGetErrorString(e));
+    return TJ_OK;
+}
+
+int tj_start_iteration(tj_ctx* c) { return c ? TJ_OK : fail(TJ_EINVAL, "null context"); }
+int tj_end_iteration(tj_ctx* c) { return c ? TJ_OK : fail(TJ_EINVAL, "null context"); }
+
+double tj_get_completion(tj_ctx* c) { return c ? c->completion.load() : 0.0; }
+
+int tj_interrupt(tj_ctx* c)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_interrupt: null context");
+    c->interrupted.store(1);
+    return TJ_OK;
+}
+
+int tj_enable_timing(tj_ctx* c, int on)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_enable_timing: null context");
+    c->timing_on = on ? 1 : 0;
+    return TJ_OK;
+}
+
+int tj_get_timing(tj_ctx* c, tj_timing* t)
+{
+    if (!c || !t) return fail(TJ_EINVAL, "tj_get_timing: null argument");
+    *t = c->last_timing;
+    return TJ_OK;
+}
+
+// TomoReconstruction2.OSSART + osartIteration (:548-733) over the resident
+// projections; all device work is queued on the context stream and the host
+// synchronises once per iteration to read the errors back.
+int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int first_index, int start_y, int end_y,
+                  double* err)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_run_ossart: null context");
+    int rc = check_range(c, start_y, end_y);
+    if (rc) return rc;
+    const int n = c->nproj;
+    if (n <= 0) return fail(TJ_ESTATE, "tj_run_ossart: no resident projections");
+    if (iterations < 0 || update_nb < 0) return fail(TJ_EINVAL, "tj_run_ossart: negative count");
+    if (first_index < 0 || first_index >= n) return fail(TJ_EINVAL, "tj_run_ossart: first index %d of %d", first_index, n);
+    TJ_HIP(hipSetDevice(c->device));
+    const int update = (update_nb == 0) ? n : update_nb;
+    if ((rc = ensure_slots(c, std::min(update, n)))) return rc;
+    if ((rc = ensure_iota(c, n))) return rc;
+    c->interrupted.store(0);
+    c->added.clear();
+    c->have_diff = c->have_proj = false;
+
+    std::vector<int> order;
+    order_shuffle(c, first_index, order);
+    TJ_HIP(hipMemcpy(c->order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    const bool axis = all_axis(c, order);
+    const float factorimg = (float)(lambda / update);
+    const double completion_inc = (end_y - start_y) / (double)c->g.vh;
+
+    tj_timing tm{};
+    size_t nev = 0;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (c->timing_on) {
+        t0 = ev(c, nev++);
+        t1 = ev(c, nev++);
+        TJ_HIP(hipEventRecord(t0, c->stream));
+    }
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> fpev, resev, bpev;
+    for (int it = 0; it < iterations; ++it) {
+        if (c->interrupted.load()) return fail(TJ_ECANCELLED, "tj_run_ossart: interrupted");
+        int diffIndex = 0;
+        for (int t = 0; t < n; ++t) {
+            ++diffIndex;
+            if (diffIndex == update || t == n - 1) {
+                const int off = t - diffIndex + 1;
+                const int* slots = c->order + off;
+                std::vector<int> hs(order.begin() + off, order.begin() + off + diffIndex);
+                hipEvent_t a = nullptr, b = nullptr, r = nullptr, d = nullptr;
+                if (c->timing_on) {
+                    a = ev(c, nev++);
+                    b = ev(c, nev++);
+                    r = ev(c, nev++);
+                    d = ev(c, nev++);
+                    TJ_HIP(hipEventRecord(a, c->stream));
+                }
+                const int start = fp_start_row(c, start_y);
+                TJ_HIP(tjk::fp(c->stream, c->g, c->vol, c->pp, slots, diffIndex, start, axis, c->pflag, c->sproj,
+                               c->snorm, c->fp_rows));
+                if (c->geo.long_object_comp)
+                    TJ_HIP(tjk::norm_max_fill(c->stream, (int)c->npx, diffIndex, c->snorm));
+                if (c->timing_on) TJ_HIP(hipEventRecord(b, c->stream));
+                if ((rc = do_residual(c, slots, diffIndex, (double)factorimg, c->err + off))) return rc;
+                if (c->timing_on) TJ_HIP(hipEventRecord(r, c->stream));
+                TJ_HIP(tjk::bp(c->stream, c->g, c->vol, c->pp, slots, diffIndex, c->sdiff, c->iota, c->rowflag,
+                               c->bprows, start_y, end_y, axis, c->bp_rows, c->bp_kchunk));
+                if (c->timing_on) {
+                    TJ_HIP(hipEventRecord(d, c->stream));
+                    fpev.emplace_back(a, b);
+                    resev.emplace_back(b, r);
+                    bpev.emplace_back(r, d);
+                }
+                diffIndex = 0;
+                if (c->interrupted.load()) {
+                    (void)hipStreamSynchronize(c->stream);
+                    return fail(TJ_ECANCELLED, "tj_run_ossart: interrupted");
+                }
+            }
+        }
+        TJ_HIP(hipMemcpyAsync(c->h_err, c->err, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        TJ_HIP(hipStreamSynchronize(c->stream));
+        // error statistics, in the order osartIteration accumulates them (:692-701, :726-730)
+        double som = 0, errmin = 3.4028234663852886e38, errmax = 0;
+        int imin = 0, imax = 0;
+        for (int t = 0; t < n; ++t) {
+            const double e = c->h_err[t];
+            som += e;
+            if (e > errmax) {
+                errmax = e;
+                imax = order[t];
+            } else if (e < errmin) {
+                errmin = e;
+                imin = order[t];
+            }
+        }
+        if (err) {
+            const double wh = (double)c->g.vw * c->g.vh;
+            double* e5 = err + 5 * it;
+            e5[0] = std::sqrt(som / (double)((long)n * c->g.vw * c->g.vh));
+            e5[1] = std::sqrt(errmin / wh);
+            e5[2] = imin;
+            e5[3] = std::sqrt(errmax / wh);
+            e5[4] = imax;
+        }
+        c->completion.store(c->completion.load() + completion_inc);
+    }
+    if (c->timing_on) {
+        TJ_HIP(hipEventRecord(t1, c->stream));
+        TJ_HIP(hipEventSynchronize(t1));
+        float ms = 0;
+        TJ_HIP(hipEventElapsedTime(&ms, t0, t1));
+        tm.total_ms = ms;
+        for (auto& p : fpev) {
+            TJ_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+            tm.fp_ms += ms;
+            tm.fp_launches++;
+        }
+        for (auto& p : resev) {
+            TJ_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+            tm.res_ms += ms;
+            tm.res_launches++;
+        }
+        for (auto& p : bpev) {
+            TJ_HIP(hipEventElapsedTime(&ms, p.first, p.second));
+            tm.bp_ms += ms;
+            tm.bp_launches++;
+        }
+        tm.other_ms = tm.total_ms - tm.fp_ms - tm.res_ms - tm.bp_ms;
+        c->last_timing = tm;
+    }
+    c->completion.store(0.0);
+    return TJ_OK;
+}
+
+// TomoReconstruction2.WBP (:499-538) with update = 0: every resident projection
+// is weighted (when a filter is set) and backprojected in index order.  The
+// per-voxel accumulation order, with its scrub after each image, is the same as
+// backprojecting one image per call.
+int tj_run_wbp(tj_ctx* c, int start_y, int end_y)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_run_wbp: null context");
+    int rc = check_range(c, start_y, end_y);
+    if (rc) return rc;
+    const int n = c->nproj;
+    if (n <= 0) return fail(TJ_ESTATE, "tj_run_wbp: no resident projections");
+    TJ_HIP(hipSetDevice(c->device));
+    if ((rc = ensure_slots(c, n))) return rc;
+    if ((rc = ensure_iota(c, n))) return rc;
+    c->added.clear();
+    c->have_diff = c->have_proj = false;
+    std::vector<int> idx(n);
+    for (int k = 0; k < n; ++k) idx[k] = k;
+    hipEvent_t t0 = nullptr, t1 = nullptr, tb = nullptr;
+    if (c->timing_on) {
+        t0 = ev(c, 0);
+        tb = ev(c, 1);
+        t1 = ev(c, 2);
+        TJ_HIP(hipEventRecord(t0, c->stream));
+    }
+    const float* images;
+    const int* index;
+    if ((rc = prepare_originals(c, c->iota, idx, &images, &index))) return rc;
+    if (c->timing_on) TJ_HIP(hipEventRecord(tb, c->stream));
+    if ((rc = do_backproject(c, c->iota, idx, images, index, start_y, end_y))) return rc;
+    if (c->timing_on) {
+        TJ_HIP(hipEventRecord(t1, c->stream));
+        TJ_HIP(hipEventSynchronize(t1));
+        float a = 0, b = 0;
+        TJ_HIP(hipEventElapsedTime(&a, t0, t1));
+        TJ_HIP(hipEventElapsedTime(&b, tb, t1));
+        tj_timing tm{};
+        tm.total_ms = a;
+        tm.bp_ms = b;
+        tm.bp_launches = 1;
+        tm.other_ms = a - b;
+        c->last_timing = tm;
+    }
+    TJ_HIP(hipStreamSynchronize(c->stream));
+    return TJ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// synthetic inputs
+// ---------------------------------------------------------------------------
+static inline uint64_t splitmix64_next(uint64_t* s)
+{
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+static inline double u01(uint64_t v) { return (double)(v >> 11) * 0x1.0p-53; }
+
+void tj_phantom_ellipsoids(uint64_t seed, int k, int W, int H, int D, double* out)
+{
+    uint64_t s = seed;
+    double L = std::min(std::min(W, H), D);
+    for (int e = 0; e < k; ++e) {
+        double* o = out + 16 * e;
+        o[0] = 0.1 + 0.9 * u01(splitmix64_next(&s));
+        double px, py, pz;
+        do {
+            px = 2 * u01(splitmix64_next(&s)) - 1;
+            py = 2 * u01(splitmix64_next(&s)) - 1;
+            pz = 2 * u01(splitmix64_next(&s)) - 1;
+        } while (px * px + py * py + pz * pz > 1.0);
+        o[1] = px * 0.4 * W;
+        o[2] = py * 0.4 * H;
+        o[3] = pz * 0.4 * D;
+        for (int a = 0; a < 3; ++a) o[4 + a] = (0.03 + 0.22 * u01(splitmix64_next(&s))) * L;
+        const double u1 = u01(splitmix64_next(&s)), u2 = u01(splitmix64_next(&s)), u3 = u01(splitmix64_next(&s));
+        const double a = std::sqrt(1 - u1), b = std::sqrt(u1);
+        const double qw = a * std::sin(2 * M_PI * u2), qx = a * std::cos(2 * M_PI * u2);
+        const double qy = b * std::sin(2 * M_PI * u3), qz = b * std::cos(2 * M_PI * u3);
+        double* R = o + 7;
+        R[0] = 1 - 2 * (qy * qy + qz * qz);
+        R[1] = 2 * (qx * qy - qz * qw);
+        R[2] = 2 * (qx * qz + qy * qw);
+        R[3] = 2 * (qx * qy + qz * qw);
+        R[4] = 1 - 2 * (qx * qx + qz * qz);
+        R[5] = 2 * (qy * qz - qx * qw);
+        R[6] = 2 * (qx * qz - qy * qw);
+        R[7] = 2 * (qy * qz + qx * qw);
+        R[8] = 1 - 2 * (qx * qx + qy * qy);
+    }
+}
+
+int tj_synth_projections(tj_ctx* c, const double* ell, int k, const double* euler, int n, double noise_rel,
+                         uint64_t noise_seed)
+{
+    if (!c || !ell || !euler || n <= 0 || k < 0) return fail(TJ_EINVAL, "tj_synth_projections: bad argument");
+    TJ_HIP(hipSetDevice(c->device));
+    int rc = alloc_projections(c, n, euler);
+    if (rc) return rc;
+    double *dell = nullptr, *clean = nullptr, *part = nullptr;
+    const int npart = 256;
+    const size_t total = c->npx * n;
+    if (dalloc(&dell, (size_t)16 * std::max(k, 1)) || dalloc(&clean, total) || dalloc(&part, npart)) {
+        (void)hipGetLastError();
+        dfree(dell);
+        dfree(clean);
+        dfree(part);
+        return fail(TJ_ENOMEM, "tj_synth_projections: cannot allocate %zu doubles", total);
+    }
+    hipError_t e = hipMemcpy(dell, ell, sizeof(double) * 16 * k, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = tjk::phantom_project(c->stream, dell, k, c->pp, n, c->g.pw, c->g.ph, c->g.pc0, c->g.pc1, clean);
+    if (e == hipSuccess) e = tjk::add_noise(c->stream, clean, total, noise_seed, noise_rel, part, npart, c->exps);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(dell);
+    dfree(clean);
+    dfree(part);
+    if (e != hipSuccess) return fail(TJ_EDEVICE, "tj_synth_projections: %s", hipGetErrorString(e));
+    return TJ_OK;
+}
+
+}  // extern "C"

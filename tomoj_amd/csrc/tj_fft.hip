@@ -1,0 +1,288 @@
+// tj_fft.hip -- weighting filters for weighted backprojection.
+//
+// FFTWeighting.weighting (filters/FFTWeighting.java:87-112) filters each image
+// with real(ifft2(mask * fft2(img))), the inverse scaled by 1/(W*H).  Here the
+// 2D transform is done as row FFTs, a tiled transpose, row FFTs of the
+// transposed data (the column transforms, with the mask multiplied in), the
+// inverse row FFTs, a transpose back and the inverse row FFTs writing the real
+// part.  Row transforms of power-of-two length run as a radix-2 Stockham FFT in
+// LDS (one block per row); other lengths use a direct DFT with FP64 sums.
+// Twiddles are computed in FP64 and rounded once to FP32.
+#include "tj_device.h"
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace tjf {
+
+constexpr int kMaxBatch = 16;   // images transformed per pass (bounds the workspace)
+
+__device__ inline float2 cmul(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+
+// tw[m] = exp(-2 pi i m / n), m < n.
+__global__ void k_twiddles(int n, float2* __restrict__ tw)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    double s, c;
+    sincospi(-2.0 * (double)m / (double)n, &s, &c);
+    tw[m] = make_float2((float)c, (float)s);
+}
+
+// Input/output forms of a row transform.
+enum RowMode {
+    kRealIn = 0,      // in: float image rows (image index table), out: complex
+    kComplex = 1,     // complex -> complex
+    kComplexMask = 2, // complex -> complex, then multiply element (row r, col c) by mask[c][r]
+    kRealOut = 3      // complex -> real part * scale into float images
+};
+
+template <int MODE>
+__device__ inline float2 load_in(const void* in, const int* idx, int rows_per_img, int n, int row, int k, size_t img_px)
+{
+    if (MODE == kRealIn) {
+        const int img = row / rows_per_img, r = row % rows_per_img;
+        const float* p = (const float*)in + (size_t)idx[img] * img_px + (size_t)r * n;
+        return make_float2(p[k], 0.0f);
+    } else {
+        return ((const float2*)in)[(size_t)row * n + k];
+    }
+}
+
+template <int MODE>
+__device__ inline void store_out(void* out, const float* mask, int rows_per_img, int n, int row, int k, float2 v,
+                                 float scale)
+{
+    if (MODE == kRealOut) {
+        ((float*)out)[(size_t)row * n + k] = v.x * scale;
+    } else if (MODE == kComplexMask) {
+        const int r = row % rows_per_img;  // transposed image: row r = image column, k = image row
+        const float m = mask[(size_t)k * rows_per_img + r];
+        ((float2*)out)[(size_t)row * n + k] = make_float2(v.x * m, v.y * m);
+    } else {
+        ((float2*)out)[(size_t)row * n + k] = v;
+    }
+}
+
+// Radix-2 Stockham FFT of one row of length n = 2^logn per block.  inverse:
+// conjugate twiddles (no scaling).  Dynamic LDS: 2*n float2.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_fft_row(const void* __restrict__ in, const int* __restrict__ idx,
+                                                 void* __restrict__ out, const float* __restrict__ mask,
+                                                 const float2* __restrict__ tw, int n, int logn, int rows_per_img,
+                                                 size_t img_px, int inverse, float scale)
+{
+    extern __shared__ float2 lds[];
+    float2* a = lds;
+    float2* b = lds + n;
+    const int row = blockIdx.x;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) a[k] = load_in<MODE>(in, idx, rows_per_img, n, row, k, img_px);
+    __syncthreads();
+    const int half = n >> 1;
+    for (int s = 0; s < logn; ++s) {
+        const int ns = 1 << s;
+        for (int j = threadIdx.x; j < half; j += blockDim.x) {
+            const int kk = j & (ns - 1);
+            float2 w = tw[kk * (half >> s)];
+            if (inverse) w.y = -w.y;
+            const float2 v0 = a[j];
+            const float2 v1 = cmul(a[j + half], w);
+            const int d = ((j >> s) << (s + 1)) + kk;
+            b[d] = make_float2(v0.x + v1.x, v0.y + v1.y);
+            b[d + ns] = make_float2(v0.x - v1.x, v0.y - v1.y);
+        }
+        __syncthreads();
+        float2* t = a;
+        a = b;
+        b = t;
+    }
+    for (int k = threadIdx.x; k < n; k += blockDim.x) store_out<MODE>(out, mask, rows_per_img, n, row, k, a[k], scale);
+}
+
+// Direct DFT of one row (any n): out[k] = sum_t x[t] exp(-+2 pi i k t / n), FP64 sums.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_dft_row(const void* __restrict__ in, const int* __restrict__ idx,
+                                                 void* __restrict__ out, const float* __restrict__ mask,
+                                                 const float2* __restrict__ tw, int n, int rows_per_img,
+                                                 size_t img_px, int inverse, float scale)
+{
+    const int row = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double sr = 0.0, si = 0.0;
+    for (int t = 0; t < n; ++t) {
+        const float2 x = load_in<MODE>(in, idx, rows_per_img, n, row, t, img_px);
+        float2 w = tw[(int)(((long long)k * t) % n)];
+        if (inverse) w.y = -w.y;
+        sr += (double)x.x * w.x - (double)x.y * w.y;
+        si += (double)x.x * w.y + (double)x.y * w.x;
+    }
+    store_out<MODE>(out, mask, rows_per_img, n, row, k, make_float2((float)sr, (float)si), scale);
+}
+
+// Batched complex transpose: nb images of rows x cols -> cols x rows.
+__global__ void k_transpose(const float2* __restrict__ in, float2* __restrict__ out, int rows, int cols)
+{
+    __shared__ float2 tile[32][33];
+    const size_t img = (size_t)blockIdx.z * rows * cols;
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    for (int dy = threadIdx.y; dy < 32; dy += blockDim.y) {
+        const int r = r0 + dy, c = c0 + threadIdx.x;
+        if (r < rows && c < cols) tile[dy][threadIdx.x] = in[img + (size_t)r * cols + c];
+    }
+    __syncthreads();
+    for (int dy = threadIdx.y; dy < 32; dy += blockDim.y) {
+        const int c = c0 + dy, r = r0 + threadIdx.x;
+        if (r < rows && c < cols) out[img + (size_t)c * rows + r] = tile[threadIdx.x][dy];
+    }
+}
+
+// FFTWeighting.createWeightingMask (FFTWeighting.java:31-65).  sint/cosp/sinp are
+// sin(tilt), cos(axis), sin(axis) per image, computed on the host.
+__global__ __launch_bounds__(256) void k_radermacher(int sx, int sy, double diameter, const double* __restrict__ sint,
+                                                     const double* __restrict__ cosp, const double* __restrict__ sinp,
+                                                     int n, float* __restrict__ mask)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= sx) return;
+    const double cx = sx / 2.0, cy = sy / 2.0;
+    const double cj = (j > cy) ? j - sy : j;
+    const double ci = (i > cx) ? i - sx : i;
+    double weight = 0;
+    for (int w = 0; w < n; ++w) {
+        const double zj = (ci * sint[w] * cosp[w] + cj * sint[w] * sinp[w]) / sx;
+        double value = diameter * M_PI * zj;
+        if (value == 0) value = 1;
+        else value = sin(value) / value;
+        weight += value;
+    }
+    if (weight < 0.005) weight = (double)0.005f;
+    weight *= diameter;
+    mask[(size_t)j * sx + i] = (float)(1 / weight);
+}
+
+// Ramp x Hamming along x (the axis perpendicular to a vertical tilt axis):
+// H(k) = (pi / nproj) * |k| / sx * window(|k| / sx), window = 0.54 + 0.46 cos(pi f / fc)
+// for f = |k|/sx <= fc = cutoff/2, 0 above; hamming == 0 keeps the bare ramp up to fc.
+__global__ void k_ramp(int sx, int sy, int hamming, double cutoff, double ang_step, float* __restrict__ mask)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= sx) return;
+    const double ci = (i > sx / 2.0) ? i - sx : i;
+    const double f = fabs(ci) / sx;
+    const double fc = 0.5 * cutoff;
+    double h = 0.0;
+    if (f <= fc) {
+        const double win = hamming ? 0.54 + 0.46 * cos(M_PI * f / fc) : 1.0;
+        h = ang_step * f * win;
+    }
+    mask[(size_t)j * sx + i] = (float)h;
+}
+
+static bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+static int ilog2(int n)
+{
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}
+
+template <int MODE>
+static hipError_t row_transform(hipStream_t st, const void* in, const int* idx, void* out, const float* mask,
+                                const float2* tw, int n, int nrows, int rows_per_img, size_t img_px, int inverse,
+                                float scale)
+{
+    if (pow2(n) && n <= 4096) {
+        const size_t lds = sizeof(float2) * 2 * n;
+        k_fft_row<MODE><<<nrows, 256, lds, st>>>(in, idx, out, mask, tw, n, ilog2(n), rows_per_img, img_px, inverse,
+                                                 scale);
+    } else {
+        k_dft_row<MODE><<<dim3((n + 255) / 256, nrows), 256, 0, st>>>(in, idx, out, mask, tw, n, rows_per_img, img_px,
+                                                                     inverse, scale);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace tjf
+
+namespace tjk {
+using namespace tjf;
+
+// workspace: two complex buffers of kMaxBatch images + twiddle tables (pw + ph entries)
+size_t weighting_workspace(int pw, int ph, int n)
+{
+    const size_t nb = (size_t)std::min(n, kMaxBatch);
+    return sizeof(float2) * (2 * nb * (size_t)pw * ph + (size_t)pw + (size_t)ph) + 256;
+}
+
+hipError_t weighting_init(hipStream_t st, int sx, int sy, double diameter, const double* tilt_deg,
+                          const double* axis_deg, int n, float* mask)
+{
+    // FFTWeighting.initWeighting (:67-81): sin(tilt), cos(axis), sin(axis) per
+    // image; Java 8's Math.toRadians is angdeg / 180.0 * PI.
+    std::vector<double> trig(3 * (size_t)n);
+    for (int w = 0; w < n; ++w) {
+        const double ax = axis_deg ? axis_deg[w] : 0.0;
+        trig[w] = std::sin(tilt_deg[w] / 180.0 * M_PI);
+        trig[n + w] = std::cos(ax / 180.0 * M_PI);
+        trig[2 * n + w] = std::sin(ax / 180.0 * M_PI);
+    }
+    double* d = nullptr;
+    hipError_t e = hipMalloc((void**)&d, sizeof(double) * 3 * n);
+    if (e != hipSuccess) return e;
+    e = hipMemcpy(d, trig.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        k_radermacher<<<dim3((sx + 255) / 256, sy), 256, 0, st>>>(sx, sy, diameter, d, d + n, d + 2 * n, n, mask);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(d);
+    return e;
+}
+
+hipError_t ramp_init(hipStream_t st, int sx, int sy, int hamming, double cutoff, double ang_step, float* mask)
+{
+    k_ramp<<<dim3((sx + 255) / 256, sy), 256, 0, st>>>(sx, sy, hamming, cutoff, ang_step, mask);
+    return hipGetLastError();
+}
+
+// Weights n images (exps[idx[s]]) into out[s] (float, [n][ph][pw]).
+hipError_t weighting_apply(hipStream_t st, int pw, int ph, int n, const float* exps, const int* idx, const float* mask,
+                           float* out, void* ws, size_t ws_size)
+{
+    if (n <= 0) return hipSuccess;
+    if (ws_size < weighting_workspace(pw, ph, n)) return hipErrorInvalidValue;
+    const size_t npx = (size_t)pw * ph;
+    const int nb = std::min(n, kMaxBatch);
+    float2* A = (float2*)ws;
+    float2* B = A + nb * npx;
+    float2* tww = B + nb * npx;
+    float2* twh = tww + pw;
+    k_twiddles<<<(pw + 255) / 256, 256, 0, st>>>(pw, tww);
+    k_twiddles<<<(ph + 255) / 256, 256, 0, st>>>(ph, twh);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const float scale = (float)(1.0 / (double)npx);
+    for (int b0 = 0; b0 < n; b0 += nb) {
+        const int m = std::min(nb, n - b0);
+        // rows of the images: forward transform along x
+        e = row_transform<kRealIn>(st, exps, idx + b0, A, nullptr, tww, pw, m * ph, ph, npx, 0, 1.0f);
+        if (e != hipSuccess) return e;
+        k_transpose<<<dim3((pw + 31) / 32, (ph + 31) / 32, m), dim3(32, 8), 0, st>>>(A, B, ph, pw);
+        // columns: forward along y, mask, inverse along y
+        e = row_transform<kComplexMask>(st, B, nullptr, A, mask, twh, ph, m * pw, pw, npx, 0, 1.0f);
+        if (e != hipSuccess) return e;
+        e = row_transform<kComplex>(st, A, nullptr, B, nullptr, twh, ph, m * pw, pw, npx, 1, 1.0f);
+        if (e != hipSuccess) return e;
+        k_transpose<<<dim3((ph + 31) / 32, (pw + 31) / 32, m), dim3(32, 8), 0, st>>>(B, A, pw, ph);
+        // inverse along x, real part, 1/(W*H)
+        e = row_transform<kRealOut>(st, A, nullptr, out + (size_t)b0 * npx, nullptr, tww, pw, m * ph, ph, npx, 1,
+                                    scale);
+        if (e != hipSuccess) return e;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace tjk

@@ -1,0 +1,702 @@
+// tj_kernels.hip -- CDNA4 (gfx950) kernels of the TomoJ projector path.
+//
+// Arithmetic follows TomoJ's CPU projector operation by operation so that the
+// FP32 results are the ones the Java code produces: positions in FP64, samples
+// and accumulators in FP32, casts at the same points, and no fused multiply-add
+// (this file is built with -ffp-contract=off).  Reference methods are cited per
+// kernel (paths relative to src/TomoJ_Base/src/main/java/fr/curie/tomoj/).
+#include "tj_device.h"
+#include <math.h>
+
+namespace tj {
+
+// ---------------------------------------------------------------------------
+// Block reductions with a fixed combination order (deterministic results).
+// ---------------------------------------------------------------------------
+template <typename T, typename Op>
+__device__ inline T block_reduce(T v, T* sh, Op op)
+{
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if (t < off) sh[t] = op(sh[t], sh[t + off]);
+        __syncthreads();
+    }
+    T r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+struct AddD { __device__ double operator()(double a, double b) const { return a + b; } };
+struct MaxF { __device__ float operator()(float a, float b) const { return a > b ? a : b; } };
+struct MaxD { __device__ double operator()(double a, double b) const { return a > b ? a : b; } };
+struct OrU { __device__ unsigned operator()(unsigned a, unsigned b) const { return a | b; } };
+
+// jcentered of detector row j: VoxelProjector3D.project splits the rows from
+// `start` into tasks of two (:350-352) and projectJosephPartial sets
+// jcentered = (starty - 1.0 - pc1)/scale, then adds 1/scale per row (:711-724).
+__device__ inline double jcentered(const TjGeomDev& g, int start, int j)
+{
+    const int sy = start + ((j - start) & ~1);
+    const double jinc = 1.0 / g.scale;
+    double jc = (sy - 1.0 - g.pc1) / g.scale;
+    jc += jinc;
+    if (j != sy) jc += jinc;
+    return jc;
+}
+
+// TomoReconstruction2.getPixel(double x, double y, int z) (TomoReconstruction2.java:154-190)
+// on the [y][z][x] device layout.
+__device__ inline float getpixel(const TjGeomDev& g, const float* __restrict__ vol, double x,
+                                 double y, int z)
+{
+    const int xb = (x < 0) ? 0 : (int)x;
+    const int yb = (y < 0) ? 0 : (int)y;
+    const size_t plane = (size_t)g.vd * g.vw;
+    const float* p = vol + (size_t)yb * plane + (size_t)z * g.vw + xb;
+    if (xb == g.vw - 1 || yb == g.vh - 1 || z == g.vd - 1) return p[0];
+    const float xf = (x < 0) ? 0.0f : (float)x - (float)xb;
+    const float yf = (y < 0) ? 0.0f : (float)y - (float)yb;
+    const float v00 = p[0], v10 = p[1];
+    const float v01 = p[plane], v11 = p[plane + 1];
+    const float lower = v00 + xf * (v10 - v00);
+    return lower + yf * ((v01 + xf * (v11 - v01)) - lower);
+}
+
+// ---------------------------------------------------------------------------
+// Forward projection, any geometry: one thread per ray (slot, i, j).
+// VoxelProjector3D.projectJosephPartial (VoxelProjector3D.java:671-821).
+// Rows below `start_row` are left at 0 (TomoJ's fresh FloatProcessor) (:343-347).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TJ_BLOCK) void k_fp_generic(
+    TjGeomDev g, const float* __restrict__ vol, const TjProj* __restrict__ pp,
+    const int* __restrict__ slot_proj, int start_row, float* __restrict__ proj,
+    float* __restrict__ norm)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    const int s = blockIdx.z;
+    if (i >= g.pw) return;
+    const size_t o = ((size_t)s * g.ph + j) * g.pw + i;
+    if (j < start_row) {
+        proj[o] = 0.0f;
+        norm[o] = 0.0f;
+        return;
+    }
+    const TjProj& P = pp[slot_proj[s]];
+    const double jc = jcentered(g, start_row, j);
+    const double ic = (i - g.pc0) / g.scale;
+    float ray_sum = 0.0f, norm_sum = 0.0f;
+    const double ijpart = jc - P.icf * ic;
+    double oldy = (ijpart - P.zcf * (-1 - g.vcz)) / P.divy;
+    double oldx = ic - P.yfx * oldy - P.zfx * (-1 - g.vcz);
+    oldx /= P.divx;
+    bool entered = false;
+    for (int z = 0; z < g.vd; ++z) {
+        const double xc = oldx + P.dirx;
+        const double yc = oldy + P.diry;
+        const double x = xc + g.vcx;
+        const double y = yc + g.vcy;
+        if (x + TJ_ACCURACY >= 0 && x < g.bxmax && y + TJ_ACCURACY >= 0 && y < g.bymax) {
+            ray_sum += getpixel(g, vol, x, y, z);
+            norm_sum = (float)((double)norm_sum + P.norminc);
+            entered = true;
+        } else if (entered) {
+            norm_sum = (float)((double)norm_sum + P.norminc);
+            break;
+        }
+        oldx = xc;
+        oldy = yc;
+    }
+    if (norm_sum > 0) {
+        proj[o] = (float)((double)ray_sum * P.norminc);
+        norm[o] = norm_sum;
+    } else {
+        proj[o] = __builtin_nanf("");
+        norm[o] = 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Forward projection, single tilt axis (e01 == e10 == e12 == 0): every ray of a
+// detector column follows the same x(z) sequence and keeps a constant y, so one
+// thread marches one column (slot, i) and accumulates R detector rows with it.
+// Same arithmetic as k_fp_generic, sample for sample.  pflag[y] != 0 marks a
+// volume plane holding non-finite values: rows touching such a plane (or with a
+// fractional y) take the full 4-tap getPixel path.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(TJ_BLOCK) void k_fp_axis(
+    TjGeomDev g, const float* __restrict__ vol, const TjProj* __restrict__ pp,
+    const int* __restrict__ slot_proj, int start_row, const uint8_t* __restrict__ pflag,
+    float* __restrict__ proj, float* __restrict__ norm)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j0 = blockIdx.y * R;
+    const int s = blockIdx.z;
+    if (i >= g.pw) return;
+    const TjProj& P = pp[slot_proj[s]];
+    const double ic = (i - g.pc0) / g.scale;
+    const size_t plane = (size_t)g.vd * g.vw;
+
+    const float* rowp[R];
+    float yf[R];
+    bool ok[R], nearY[R], full[R];
+    double oldy0 = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        ok[r] = false;
+        nearY[r] = false;
+        full[r] = false;
+        yf[r] = 0.0f;
+        rowp[r] = vol;
+        if (j < g.ph && j >= start_row) {
+            const double jc = jcentered(g, start_row, j);
+            const double ijpart = jc - P.icf * ic;
+            const double oldy = (ijpart - P.zcf * (-1 - g.vcz)) / P.divy;
+            oldy0 = oldy;
+            const double y = (oldy + P.diry) + g.vcy;
+            if (y + TJ_ACCURACY >= 0 && y < g.bymax) {
+                ok[r] = true;
+                const int yb = (y < 0) ? 0 : (int)y;
+                const float yfr = (y < 0) ? 0.0f : (float)y - (float)yb;
+                rowp[r] = vol + (size_t)yb * plane;
+                nearY[r] = (yb == g.vh - 1);
+                full[r] = !nearY[r] && (yfr != 0.0f || pflag[yb] != 0 || pflag[yb + 1] != 0);
+                yf[r] = yfr;
+            }
+        }
+    }
+    // yfx == 0 here, so oldx does not depend on the row's oldy.
+    double xc = ic - P.yfx * oldy0 - P.zfx * (-1 - g.vcz);
+    xc /= P.divx;
+
+    float rs[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) rs[r] = 0.0f;
+    float ns = 0.0f;
+    bool entered = false;
+    for (int z = 0; z < g.vd; ++z) {
+        xc = xc + P.dirx;
+        const double x = xc + g.vcx;
+        if (x + TJ_ACCURACY >= 0 && x < g.bxmax) {
+            const int xb = (x < 0) ? 0 : (int)x;
+            const float xf = (x < 0) ? 0.0f : (float)x - (float)xb;
+            const bool nearXZ = (xb == g.vw - 1) || (z == g.vd - 1);
+            const size_t zo = (size_t)z * g.vw + xb;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (ok[r]) {
+                    const float* p = rowp[r] + zo;
+                    float v;
+                    if (nearXZ || nearY[r]) {
+                        v = p[0];
+                    } else {
+                        const float v00 = p[0], v10 = p[1];
+                        const float lower = v00 + xf * (v10 - v00);
+                        if (full[r]) {
+                            const float v01 = p[plane], v11 = p[plane + 1];
+                            v = lower + yf[r] * ((v01 + xf * (v11 - v01)) - lower);
+                        } else {
+                            v = lower;
+                        }
+                    }
+                    rs[r] += v;
+                }
+            }
+            ns = (float)((double)ns + P.norminc);
+            entered = true;
+        } else if (entered) {
+            ns = (float)((double)ns + P.norminc);
+            break;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int j = j0 + r;
+        if (j >= g.ph) break;
+        const size_t o = ((size_t)s * g.ph + j) * g.pw + i;
+        if (j < start_row) {
+            proj[o] = 0.0f;
+            norm[o] = 0.0f;
+        } else if (ok[r] && ns > 0) {
+            proj[o] = (float)((double)rs[r] * P.norminc);
+            norm[o] = ns;
+        } else {
+            proj[o] = __builtin_nanf("");
+            norm[o] = 0.0f;
+        }
+    }
+}
+
+// Long-object compensation: the norm image becomes its maximum
+// (VoxelProjector3D.java:799-801, :813-819).  One block per slot.
+__global__ __launch_bounds__(TJ_BLOCK) void k_norm_max_fill(int npx, float* __restrict__ norm)
+{
+    __shared__ float sh[TJ_BLOCK];
+    float* n = norm + (size_t)blockIdx.x * npx;
+    float m = 0.0f;
+    for (int k = threadIdx.x; k < npx; k += blockDim.x) m = fmaxf(m, n[k]);
+    m = block_reduce(m, sh, MaxF());
+    for (int k = threadIdx.x; k < npx; k += blockDim.x) n[k] = m;
+}
+
+// ---------------------------------------------------------------------------
+// Residual: VoxelProjector3D.difference (VoxelProjector3D.java:414-433).  One
+// block per (row j, slot s).  diff = (float)((double)((exp - th) / norm) * factor),
+// left 0 where exp or th is NaN; row_err[s][j] = sum of value^2 over the row in
+// FP64; rowflag[s][j] = 1 if the row of diff holds a non-finite value.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TJ_BLOCK) void k_residual(
+    int pw, int ph, const float* __restrict__ exps, const int* __restrict__ exp_index,
+    const float* __restrict__ proj, const float* __restrict__ norm, double factor,
+    float* __restrict__ diff, double* __restrict__ row_err, uint8_t* __restrict__ rowflag)
+{
+    __shared__ double shd[TJ_BLOCK];
+    __shared__ unsigned shu[TJ_BLOCK];
+    const int j = blockIdx.x, s = blockIdx.y;
+    const size_t npx = (size_t)pw * ph;
+    const float* e = exps + (size_t)exp_index[s] * npx + (size_t)j * pw;
+    const size_t o = (size_t)s * npx + (size_t)j * pw;
+    double acc = 0.0;
+    unsigned bad = 0;
+    for (int i = threadIdx.x; i < pw; i += blockDim.x) {
+        const float ex = e[i], th = proj[o + i], n = norm[o + i];
+        float d = 0.0f;
+        if (!isnan(ex) && !isnan(th)) {
+            const double value = (double)((ex - th) / n) * factor;
+            d = (float)value;
+            acc += value * value;
+        }
+        diff[o + i] = d;
+        bad |= isfinite(d) ? 0u : 1u;
+    }
+    acc = block_reduce(acc, shd, AddD());
+    bad = block_reduce(bad, shu, OrU());
+    if (threadIdx.x == 0) {
+        row_err[(size_t)s * ph + j] = acc;
+        rowflag[(size_t)s * ph + j] = (uint8_t)bad;
+    }
+}
+
+// Sum of the per-row errors of each slot in a fixed order; one block per slot.
+__global__ __launch_bounds__(TJ_BLOCK) void k_err_sum(int ph, const double* __restrict__ row_err,
+                                                      double* __restrict__ err)
+{
+    __shared__ double sh[TJ_BLOCK];
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < ph; j += blockDim.x) acc += row_err[(size_t)blockIdx.x * ph + j];
+    acc = block_reduce(acc, sh, AddD());
+    if (threadIdx.x == 0) err[blockIdx.x] = acc;
+}
+
+// Non-finite flag per image row for images that did not come from k_residual
+// (weighted or original projections fed to the backprojection).
+__global__ __launch_bounds__(TJ_BLOCK) void k_row_flags(int pw, int ph, const float* __restrict__ images,
+                                                        const int* __restrict__ image_index,
+                                                        uint8_t* __restrict__ rowflag)
+{
+    __shared__ unsigned sh[TJ_BLOCK];
+    const int j = blockIdx.x, s = blockIdx.y;
+    const float* p = images + (size_t)image_index[s] * pw * ph + (size_t)j * pw;
+    unsigned bad = 0;
+    for (int i = threadIdx.x; i < pw; i += blockDim.x) bad |= isfinite(p[i]) ? 0u : 1u;
+    bad = block_reduce(bad, sh, OrU());
+    if (threadIdx.x == 0) rowflag[(size_t)s * ph + j] = (uint8_t)bad;
+}
+
+// ---------------------------------------------------------------------------
+// Backprojection.  VoxelProjector3D.backProjectPartial (VoxelProjector3D.java:221-308)
+// with ImageJ's getInterpolatedValue (bilinear in double, neighbours clamped to
+// the last column/row).  Each voxel takes the added images in order, with the
+// NaN/Inf scrub and the positivity clamp after every image, exactly as the
+// Java loop does one projection at a time.  Each voxel is owned by one thread:
+// no atomics, results independent of scheduling.
+// ---------------------------------------------------------------------------
+
+// Row table for the single-axis kernel: yy = ((x*e10 + y*e11) + z*e12)*scale + ciy
+// reduces to (y*e11)*scale + ciy when e10 == e12 == 0.
+__global__ void k_bp_rows(TjGeomDev g, const TjProj* __restrict__ pp, const int* __restrict__ slot_proj,
+                          int nslot, const uint8_t* __restrict__ rowflag, TjBpRow* __restrict__ rows)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (j >= g.vh || s >= nslot) return;
+    const TjProj& Q = pp[slot_proj[s]];
+    const double y = j - g.vcy;
+    const double yy = (y * Q.e[4]) * g.scale + g.pc3;
+    TjBpRow r;
+    r.valid = (yy >= 0 && yy < g.ph) ? 1 : 0;
+    r.yb = 0;
+    r.y1 = 0;
+    r.yf = 0.0;
+    r.shortcut = 1;
+    if (r.valid) {
+        r.yb = (int)yy;
+        r.y1 = min(r.yb + 1, g.ph - 1);
+        r.yf = yy - (double)r.yb;
+        r.shortcut = (r.y1 == r.yb) || (r.yf == 0.0 && rowflag[(size_t)s * g.ph + r.y1] == 0);
+    }
+    rows[(size_t)s * g.vh + j] = r;
+}
+
+template <bool POS>
+__device__ inline void bp_accumulate(float& acc, float value)
+{
+    if (!isnan(value)) {
+        float a = acc + value;
+        if (isnan(a) || isinf(a)) a = 0.0f;
+        if (POS && a < 0) a = 0.0f;
+        acc = a;
+    }
+}
+
+// Single-axis backprojection: thread = voxel column x=i over R volume rows and
+// a chunk of z; xx depends on (slot, i, k) only and yy on (slot, j) only.
+template <int R, bool POS, bool SCALE1>
+__global__ __launch_bounds__(TJ_BLOCK) void k_bp_axis(
+    TjGeomDev g, float* __restrict__ vol, const TjProj* __restrict__ pp,
+    const int* __restrict__ slot_proj, int nslot, const float* __restrict__ images,
+    const int* __restrict__ image_index, const TjBpRow* __restrict__ rows, int start_y, int end_y,
+    int kchunk)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.vw) return;
+    const int j0 = start_y + blockIdx.y * R;
+    const int k0 = blockIdx.z * kchunk;
+    const int k1 = min(k0 + kchunk, g.vd);
+    const int nr = min(R, end_y - j0);
+    const double x = i - g.vcx;
+    const size_t plane = (size_t)g.vd * g.vw;
+    const size_t npx = (size_t)g.pw * g.ph;
+    for (int k = k0; k < k1; ++k) {
+        const double z = k - g.vcz;
+        float acc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            acc[r] = (r < nr) ? vol[(size_t)(j0 + r) * plane + (size_t)k * g.vw + i] : 0.0f;
+        for (int s = 0; s < nslot; ++s) {
+            const TjProj& Q = pp[slot_proj[s]];
+            // ((x*e00 + y*e01) + z*e02): y*e01 is a signed zero here, which changes
+            // no value the comparisons and interpolation below can see.
+            double xx = x * Q.e[0] + z * Q.e[2];
+            if (!SCALE1) xx = xx * g.scale;
+            xx = xx + g.pc2;
+            if (!(xx >= 0 && xx < g.pw)) continue;
+            const int xb = (int)xx;
+            const double xf = xx - (double)xb;
+            const int x1 = min(xb + 1, g.pw - 1);
+            const float* img = images + (size_t)image_index[s] * npx;
+            const TjBpRow* rw = rows + (size_t)s * g.vh + j0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (r >= nr) break;
+                const TjBpRow row = rw[r];
+                if (!row.valid) continue;
+                const float* lo = img + (size_t)row.yb * g.pw;
+                const double ll = lo[xb], lr = lo[x1];
+                const double lower = ll + xf * (lr - ll);
+                float value;
+                if (row.shortcut) {
+                    // lower + 0*(upper - lower): lower when finite, NaN otherwise
+                    value = isfinite(lower) ? (float)lower : __builtin_nanf("");
+                } else {
+                    const float* up = img + (size_t)row.y1 * g.pw;
+                    const double ul = up[xb], ur = up[x1];
+                    const double upper = ul + xf * (ur - ul);
+                    value = (float)(lower + row.yf * (upper - lower));
+                }
+                bp_accumulate<POS>(acc[r], value);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (r < nr) vol[(size_t)(j0 + r) * plane + (size_t)k * g.vw + i] = acc[r];
+    }
+}
+
+// Any-geometry backprojection: one thread per voxel (i, j, k).
+template <bool POS>
+__global__ __launch_bounds__(TJ_BLOCK) void k_bp_generic(
+    TjGeomDev g, float* __restrict__ vol, const TjProj* __restrict__ pp,
+    const int* __restrict__ slot_proj, int nslot, const float* __restrict__ images,
+    const int* __restrict__ image_index, int start_y)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = start_y + blockIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.vw) return;
+    const double x = i - g.vcx, y = j - g.vcy, z = k - g.vcz;
+    const size_t npx = (size_t)g.pw * g.ph;
+    float* v = vol + ((size_t)j * g.vd + k) * g.vw + i;
+    float acc = *v;
+    for (int s = 0; s < nslot; ++s) {
+        const TjProj& Q = pp[slot_proj[s]];
+        const double ztx = z * Q.e[2], zty = z * Q.e[5];
+        const double ytx = y * Q.e[1], yty = y * Q.e[4];
+        const double xx = (x * Q.e[0] + ytx + ztx) * g.scale + g.pc2;
+        const double yy = (x * Q.e[3] + yty + zty) * g.scale + g.pc3;
+        if (xx >= 0 && xx < g.pw && yy >= 0 && yy < g.ph) {
+            const float* img = images + (size_t)image_index[s] * npx;
+            const int xb = (int)xx, yb = (int)yy;
+            const int x1 = min(xb + 1, g.pw - 1), y1 = min(yb + 1, g.ph - 1);
+            const double xf = xx - (double)xb, yf = yy - (double)yb;
+            const double ll = img[(size_t)yb * g.pw + xb], lr = img[(size_t)yb * g.pw + x1];
+            const double ul = img[(size_t)y1 * g.pw + xb], ur = img[(size_t)y1 * g.pw + x1];
+            const double upper = ul + xf * (ur - ul);
+            const double lower = ll + xf * (lr - ll);
+            bp_accumulate<POS>(acc, (float)(lower + yf * (upper - lower)));
+        }
+    }
+    *v = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Volume layout conversion between the host ImageStack layout [z][y][x] and the
+// device layout [y][z][x], for rows [y0, y0+nrows).
+// ---------------------------------------------------------------------------
+__global__ void k_zyx_to_yzx(const float* __restrict__ src, float* __restrict__ vol, int W, int D,
+                             int nrows, int y0)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int yy = blockIdx.y, z = blockIdx.z;
+    if (x >= W) return;
+    vol[((size_t)(y0 + yy) * D + z) * W + x] = src[((size_t)z * nrows + yy) * W + x];
+}
+
+__global__ void k_yzx_to_zyx(const float* __restrict__ vol, float* __restrict__ dst, int W, int D,
+                             int nrows, int y0)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int yy = blockIdx.y, z = blockIdx.z;
+    if (x >= W) return;
+    dst[((size_t)z * nrows + yy) * W + x] = vol[((size_t)(y0 + yy) * D + z) * W + x];
+}
+
+// Non-finite flag per volume plane y.
+__global__ __launch_bounds__(TJ_BLOCK) void k_plane_flags(const float* __restrict__ vol, size_t plane,
+                                                          int y0, uint8_t* __restrict__ pflag)
+{
+    __shared__ unsigned sh[TJ_BLOCK];
+    const int y = y0 + blockIdx.x;
+    const float* p = vol + (size_t)y * plane;
+    unsigned bad = 0;
+    for (size_t k = threadIdx.x; k < plane; k += blockDim.x) bad |= isfinite(p[k]) ? 0u : 1u;
+    bad = block_reduce(bad, sh, OrU());
+    if (threadIdx.x == 0) pflag[y] = (uint8_t)bad;
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic tilt series: analytic FP64 line integrals of an ellipsoid phantom
+// in TomoJ's geometry, then Gaussian noise.
+// ---------------------------------------------------------------------------
+__device__ inline uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// pixel (i,j) of image p: line u*(i-pc0) + v*(j-pc1) + t*w, u,v,w = rows of E_p.
+__global__ __launch_bounds__(TJ_BLOCK) void k_phantom_project(
+    const double* __restrict__ ell, int k, const TjProj* __restrict__ pp, int pw, int ph, double pc0,
+    double pc1, double* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, p = blockIdx.z;
+    if (i >= pw) return;
+    const double* E = pp[p].e;
+    const double a = i - pc0, b = j - pc1;
+    const double pt0 = E[0] * a + E[3] * b, pt1 = E[1] * a + E[4] * b, pt2 = E[2] * a + E[5] * b;
+    const double d0 = E[6], d1 = E[7], d2 = E[8];
+    double sum = 0.0;
+    for (int e = 0; e < k; ++e) {
+        const double* o = ell + 16 * e;
+        const double* R = o + 7;
+        const double q0 = pt0 - o[1], q1 = pt1 - o[2], q2 = pt2 - o[3];
+        double A = 0, B = 0, C = 0;
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const double qa = R[3 * ax] * q0 + R[3 * ax + 1] * q1 + R[3 * ax + 2] * q2;
+            const double da = R[3 * ax] * d0 + R[3 * ax + 1] * d1 + R[3 * ax + 2] * d2;
+            const double inv = 1.0 / (o[4 + ax] * o[4 + ax]);
+            A += da * da * inv;
+            B += 2 * da * qa * inv;
+            C += qa * qa * inv;
+        }
+        C -= 1;
+        const double disc = B * B - 4 * A * C;
+        if (disc > 0) sum += o[0] * sqrt(disc) / A;
+    }
+    out[((size_t)p * ph + j) * pw + i] = sum;
+}
+
+__global__ __launch_bounds__(TJ_BLOCK) void k_max_partial(const double* __restrict__ a, size_t n,
+                                                          double* __restrict__ part)
+{
+    __shared__ double sh[TJ_BLOCK];
+    double m = -INFINITY;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        m = fmax(m, a[k]);
+    m = block_reduce(m, sh, MaxD());
+    if (threadIdx.x == 0) part[blockIdx.x] = m;
+}
+
+// value n: a[n] + sigma * N(0,1), the normal deviate from mix64(seed ^ 2n) and
+// mix64(seed ^ (2n+1)) through Box-Muller.
+__global__ __launch_bounds__(TJ_BLOCK) void k_add_noise(const double* __restrict__ a, size_t n,
+                                                        uint64_t seed, const double* __restrict__ sigma_src,
+                                                        int nsig, double rel, float* __restrict__ out)
+{
+    __shared__ double s_sigma;
+    if (threadIdx.x == 0) {
+        double m = -INFINITY;
+        for (int t = 0; t < nsig; ++t) m = fmax(m, sigma_src[t]);
+        s_sigma = rel * m;
+    }
+    __syncthreads();
+    const double sigma = s_sigma;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+        const double u1 = (double)((mix64(seed ^ (uint64_t)(2 * k)) >> 11) + 1) * 0x1.0p-53;
+        const double u2 = (double)(mix64(seed ^ (uint64_t)(2 * k + 1)) >> 11) * 0x1.0p-53;
+        const double gz = sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+        out[k] = (float)(a[k] + sigma * gz);
+    }
+}
+
+}  // namespace tj
+
+// ---------------------------------------------------------------------------
+// Launch wrappers (C++ linkage, used by tj_api.hip).
+// ---------------------------------------------------------------------------
+namespace tjk {
+using namespace tj;
+
+hipError_t fp(hipStream_t st, const TjGeomDev& g, const float* vol, const TjProj* pp,
+              const int* slot_proj, int nslot, int start_row, bool axis, const uint8_t* pflag,
+              float* proj, float* norm, int rows_per_thread)
+{
+    if (nslot <= 0) return hipSuccess;
+    const int bx = (g.pw + TJ_BLOCK - 1) / TJ_BLOCK;
+    if (!axis) {
+        k_fp_generic<<<dim3(bx, g.ph, nslot), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, start_row, proj, norm);
+    } else {
+        switch (rows_per_thread) {
+        case 1:
+            k_fp_axis<1><<<dim3(bx, g.ph, nslot), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, start_row, pflag, proj, norm);
+            break;
+        case 2:
+            k_fp_axis<2><<<dim3(bx, (g.ph + 1) / 2, nslot), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, start_row, pflag, proj, norm);
+            break;
+        case 8:
+            k_fp_axis<8><<<dim3(bx, (g.ph + 7) / 8, nslot), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, start_row, pflag, proj, norm);
+            break;
+        default:
+            k_fp_axis<4><<<dim3(bx, (g.ph + 3) / 4, nslot), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, start_row, pflag, proj, norm);
+            break;
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t norm_max_fill(hipStream_t st, int npx, int nslot, float* norm)
+{
+    if (nslot <= 0) return hipSuccess;
+    k_norm_max_fill<<<nslot, TJ_BLOCK, 0, st>>>(npx, norm);
+    return hipGetLastError();
+}
+
+hipError_t residual(hipStream_t st, int pw, int ph, int nslot, const float* exps, const int* exp_index,
+                    const float* proj, const float* norm, double factor, float* diff, double* row_err,
+                    uint8_t* rowflag, double* err)
+{
+    if (nslot <= 0) return hipSuccess;
+    k_residual<<<dim3(ph, nslot), TJ_BLOCK, 0, st>>>(pw, ph, exps, exp_index, proj, norm, factor, diff, row_err, rowflag);
+    k_err_sum<<<nslot, TJ_BLOCK, 0, st>>>(ph, row_err, err);
+    return hipGetLastError();
+}
+
+hipError_t row_flags(hipStream_t st, int pw, int ph, int nslot, const float* images, const int* image_index,
+                     uint8_t* rowflag)
+{
+    if (nslot <= 0) return hipSuccess;
+    k_row_flags<<<dim3(ph, nslot), TJ_BLOCK, 0, st>>>(pw, ph, images, image_index, rowflag);
+    return hipGetLastError();
+}
+
+hipError_t bp(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp, const int* slot_proj,
+              int nslot, const float* images, const int* image_index, const uint8_t* rowflag,
+              TjBpRow* rows, int start_y, int end_y, bool axis, int rows_per_thread, int kchunk)
+{
+    if (nslot <= 0 || end_y <= start_y) return hipSuccess;
+    const int bx = (g.vw + TJ_BLOCK - 1) / TJ_BLOCK;
+    const int ny = end_y - start_y;
+    if (!axis) {
+        if (g.positivity)
+            k_bp_generic<true><<<dim3(bx, ny, g.vd), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, start_y);
+        else
+            k_bp_generic<false><<<dim3(bx, ny, g.vd), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, start_y);
+        return hipGetLastError();
+    }
+    k_bp_rows<<<dim3((g.vh + 255) / 256, nslot), 256, 0, st>>>(g, pp, slot_proj, nslot, rowflag, rows);
+    if (kchunk <= 0) kchunk = g.vd;
+    const int bz = (g.vd + kchunk - 1) / kchunk;
+    const bool s1 = (g.scale == 1.0);
+    const bool pos = g.positivity != 0;
+#define TJ_BP_LAUNCH(RR)                                                                                     \
+    {                                                                                                        \
+        dim3 grid(bx, (ny + RR - 1) / RR, bz);                                                               \
+        if (pos && s1) k_bp_axis<RR, true, true><<<grid, TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, kchunk); \
+        else if (pos) k_bp_axis<RR, true, false><<<grid, TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, kchunk); \
+        else if (s1) k_bp_axis<RR, false, true><<<grid, TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, kchunk); \
+        else k_bp_axis<RR, false, false><<<grid, TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, kchunk); \
+    }
+    switch (rows_per_thread) {
+    case 1: TJ_BP_LAUNCH(1) break;
+    case 2: TJ_BP_LAUNCH(2) break;
+    case 8: TJ_BP_LAUNCH(8) break;
+    default: TJ_BP_LAUNCH(4) break;
+    }
+#undef TJ_BP_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t zyx_to_yzx(hipStream_t st, const float* src, float* vol, int W, int D, int nrows, int y0)
+{
+    if (nrows <= 0) return hipSuccess;
+    k_zyx_to_yzx<<<dim3((W + 255) / 256, nrows, D), 256, 0, st>>>(src, vol, W, D, nrows, y0);
+    return hipGetLastError();
+}
+
+hipError_t yzx_to_zyx(hipStream_t st, const float* vol, float* dst, int W, int D, int nrows, int y0)
+{
+    if (nrows <= 0) return hipSuccess;
+    k_yzx_to_zyx<<<dim3((W + 255) / 256, nrows, D), 256, 0, st>>>(vol, dst, W, D, nrows, y0);
+    return hipGetLastError();
+}
+
+hipError_t plane_flags(hipStream_t st, const float* vol, size_t plane, int y0, int nrows, uint8_t* pflag)
+{
+    if (nrows <= 0) return hipSuccess;
+    k_plane_flags<<<nrows, TJ_BLOCK, 0, st>>>(vol, plane, y0, pflag);
+    return hipGetLastError();
+}
+
+hipError_t phantom_project(hipStream_t st, const double* ell, int k, const TjProj* pp, int n, int pw, int ph,
+                           double pc0, double pc1, double* out)
+{
+    k_phantom_project<<<dim3((pw + TJ_BLOCK - 1) / TJ_BLOCK, ph, n), TJ_BLOCK, 0, st>>>(ell, k, pp, pw, ph, pc0, pc1, out);
+    return hipGetLastError();
+}
+
+hipError_t add_noise(hipStream_t st, const double* a, size_t n, uint64_t seed, double rel, double* part,
+                     int npart, float* out)
+{
+    k_max_partial<<<npart, TJ_BLOCK, 0, st>>>(a, n, part);
+    k_add_noise<<<1024, TJ_BLOCK, 0, st>>>(a, n, seed, part, npart, rel, out);
+    return hipGetLastError();
+}
+
+}  // namespace tjk

@@ -1,0 +1,235 @@
+"""Host-side mirror of TomoJ's projector plugin interface over libtomoj_hip.
+
+``VoxelProjector3DHIP`` exposes the method names and argument meaning of
+``Projector`` (src/TomoJ_Base/.../tomography/projectors/Projector.java:14-159)
+as implemented by ``VoxelProjector3D`` (VoxelProjector3D.java:32-885), plus the
+GPU lifecycle calls ``ProjectorGPU`` offers its slab driver (initForIterative /
+updateFromGPU / releaseCL_Memory).  Every call goes through the C ABI; errors
+from the library raise ``TjError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, tj_geometry, tj_timing
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class VoxelProjector3DHIP:
+    """Parallel-beam voxel projector on one MI355X.
+
+    vol_shape = (W, H, D) of the reconstruction, proj_shape = (W, H) of the tilt
+    series.  Centres default to TomoJ's (n-1)/2; ``proj_center`` takes the four
+    ``projectionCenters`` and ``vol_center`` the three ``volumeCenters``.
+    """
+
+    def __init__(self, vol_shape: Sequence[int], proj_shape: Optional[Sequence[int]] = None, *,
+                 device: int = 0, scale: float = 1.0, positivity: bool = False,
+                 long_object_compensation: bool = False, proj_center=None, vol_center=None,
+                 storage: int = _lib.TJ_STORAGE_F32):
+        W, H, D = (int(v) for v in vol_shape)
+        pw, ph = (W, H) if proj_shape is None else (int(proj_shape[0]), int(proj_shape[1]))
+        g = tj_geometry()
+        _lib.load().tj_geometry_default(C.byref(g), W, H, D, pw, ph)
+        if proj_center is not None:
+            for k in range(4):
+                g.proj_center[k] = float(proj_center[k])
+        if vol_center is not None:
+            for k in range(3):
+                g.vol_center[k] = float(vol_center[k])
+        g.scale = float(scale)
+        g.positivity = int(bool(positivity))
+        g.long_object_comp = int(bool(long_object_compensation))
+        g.storage = int(storage)
+        self.geometry = g
+        self.W, self.H, self.D, self.pw, self.ph = W, H, D, pw, ph
+        self._ctx = C.c_void_p()
+        call("tj_open", int(device), C.byref(g), C.byref(self._ctx))
+        self.n_loaded = 0
+
+    # ------------------------------------------------------------------ lifecycle
+    def close(self) -> None:
+        """ProjectorGPU.releaseCL_Memory (ProjectorGPU.java:376-403)."""
+        if self._ctx:
+            call("tj_close", self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ data
+    def load_tilt_series(self, stack: np.ndarray, eulers: np.ndarray) -> None:
+        """Makes the supplied projections (TiltSeries.getPixels(i) for each i) and
+        their Euler matrices resident on the device."""
+        stack = _f32(stack)
+        eulers = _f64(eulers).reshape(-1, 9)
+        n = eulers.shape[0]
+        if stack.shape != (n, self.ph, self.pw):
+            raise ValueError(f"stack shape {stack.shape} != {(n, self.ph, self.pw)}")
+        call("tj_load_projections", self._ctx, _ptr(stack), n, _ptr(eulers))
+        self.n_loaded = n
+
+    def synth_tilt_series(self, ellipsoids: np.ndarray, eulers: np.ndarray, noise_rel: float,
+                          noise_seed: int) -> None:
+        """Generates analytic projections of an ellipsoid phantom on the device."""
+        ell = _f64(ellipsoids).reshape(-1, 16)
+        eulers = _f64(eulers).reshape(-1, 9)
+        call("tj_synth_projections", self._ctx, _ptr(ell), ell.shape[0], _ptr(eulers),
+             eulers.shape[0], float(noise_rel), C.c_uint64(int(noise_seed)))
+        self.n_loaded = eulers.shape[0]
+
+    def get_loaded_projection(self, index: int) -> np.ndarray:
+        out = np.empty((self.ph, self.pw), np.float32)
+        call("tj_get_loaded_projection", self._ctx, int(index), _ptr(out))
+        return out
+
+    def set_eulers(self, eulers: np.ndarray) -> None:
+        eulers = _f64(eulers).reshape(-1, 9)
+        call("tj_set_eulers", self._ctx, _ptr(eulers), eulers.shape[0])
+
+    def set_volume(self, vol_zyx: np.ndarray, startY: int = 0, endY: Optional[int] = None) -> None:
+        """ProjectorGPU.initForIterative's slab upload (ProjectorGPU.java:195-254).
+        ``vol_zyx`` is the full (D, H, W) host volume; rows [startY, endY) are sent."""
+        vol = _f32(vol_zyx)
+        if vol.shape != (self.D, self.H, self.W):
+            raise ValueError(f"volume shape {vol.shape} != {(self.D, self.H, self.W)}")
+        call("tj_set_volume", self._ctx, _ptr(vol), int(startY), self.H if endY is None else int(endY))
+
+    def get_volume(self, startY: int = 0, endY: Optional[int] = None,
+                   out: Optional[np.ndarray] = None) -> np.ndarray:
+        """ProjectorGPU.updateFromGPU (ProjectorGPU.java:297-315)."""
+        if out is None:
+            out = np.zeros((self.D, self.H, self.W), np.float32)
+        assert out.dtype == np.float32 and out.flags["C_CONTIGUOUS"] and out.shape == (self.D, self.H, self.W)
+        call("tj_get_volume", self._ctx, _ptr(out), int(startY), self.H if endY is None else int(endY))
+        return out
+
+    def reset_volume(self) -> None:
+        call("tj_reset_volume", self._ctx)
+
+    # ------------------------------------------------------------------ Projector SPI
+    def addProjection(self, index: int) -> None:
+        call("tj_add_projection", self._ctx, int(index))
+
+    def clearAllProjections(self) -> None:
+        call("tj_clear_projections", self._ctx)
+
+    def num_added(self) -> int:
+        return _lib.load().tj_num_added(self._ctx)
+
+    def project(self, startY: int = 0, endY: Optional[int] = None) -> None:
+        call("tj_project", self._ctx, int(startY), self.H if endY is None else int(endY))
+
+    def projectAndDifference(self, factor: float, startY: int = 0, endY: Optional[int] = None) -> np.ndarray:
+        errors = np.zeros(max(self.num_added(), 1), np.float64)
+        call("tj_project_and_difference", self._ctx, float(factor), int(startY),
+             self.H if endY is None else int(endY), _ptr(errors))
+        return errors[: self.num_added()]
+
+    def backproject(self, startY: int = 0, endY: Optional[int] = None) -> None:
+        call("tj_backproject", self._ctx, int(startY), self.H if endY is None else int(endY))
+
+    def getProjection(self, index: int) -> np.ndarray:
+        out = np.empty((self.ph, self.pw), np.float32)
+        call("tj_get_projection", self._ctx, int(index), _ptr(out), None)
+        return out
+
+    def getNormImage(self, index: int) -> np.ndarray:
+        out = np.empty((self.ph, self.pw), np.float32)
+        call("tj_get_projection", self._ctx, int(index), None, _ptr(out))
+        return out
+
+    def getDifferenceImage(self, index: int) -> np.ndarray:
+        out = np.empty((self.ph, self.pw), np.float32)
+        call("tj_get_difference", self._ctx, int(index), _ptr(out))
+        return out
+
+    def startOfIteration(self) -> None:
+        call("tj_start_iteration", self._ctx)
+
+    def endOfIteration(self) -> None:
+        call("tj_end_iteration", self._ctx)
+
+    def getCompletion(self) -> float:
+        return _lib.load().tj_get_completion(self._ctx)
+
+    def isWeighting(self) -> bool:
+        return getattr(self, "_weighting", False)
+
+    def setWeightingFilter(self, diameter: float, tilts_deg, axis_deg=None) -> None:
+        """setWeightingFilter(new FFTWeighting(ts, diameter)) (FFTWeighting.java:21-65)."""
+        t = _f64(tilts_deg)
+        a = None if axis_deg is None else _f64(axis_deg)
+        call("tj_set_weighting", self._ctx, float(diameter), _ptr(t),
+             None if a is None else _ptr(a), len(t))
+        self._weighting = diameter > 0
+
+    def setRampFilter(self, hamming: bool = True, cutoff: float = 1.0) -> None:
+        call("tj_set_ramp_filter", self._ctx, int(bool(hamming)), float(cutoff))
+        self._weighting = True
+
+    def getWeightingMask(self) -> np.ndarray:
+        out = np.empty((self.ph, self.pw), np.float32)
+        call("tj_get_weighting_mask", self._ctx, _ptr(out))
+        return out
+
+    def weighting(self, image: np.ndarray) -> np.ndarray:
+        """FFTWeighting.weighting on one image (returns a new array)."""
+        img = _f32(image).copy()
+        call("tj_apply_weighting", self._ctx, _ptr(img))
+        return img
+
+    def interrupt(self) -> None:
+        call("tj_interrupt", self._ctx)
+
+    # ------------------------------------------------------------------ fused loops
+    def run_ossart(self, iterations: int, relaxation: float, update: int, first_index: int = 0,
+                   startY: int = 0, endY: Optional[int] = None) -> np.ndarray:
+        """TomoReconstruction2.OSSART over the resident projections (iterations x 5 err1 rows)."""
+        err = np.zeros((max(iterations, 1), 5), np.float64)
+        call("tj_run_ossart", self._ctx, int(iterations), float(relaxation), int(update), int(first_index),
+             int(startY), self.H if endY is None else int(endY), _ptr(err))
+        return err[:iterations]
+
+    def run_wbp(self, startY: int = 0, endY: Optional[int] = None) -> None:
+        """TomoReconstruction2.WBP over the resident projections (volume not cleared)."""
+        call("tj_run_wbp", self._ctx, int(startY), self.H if endY is None else int(endY))
+
+    def enable_timing(self, on: bool = True) -> None:
+        call("tj_enable_timing", self._ctx, int(bool(on)))
+
+    def timing(self) -> dict:
+        t = tj_timing()
+        call("tj_get_timing", self._ctx, C.byref(t))
+        return {k: getattr(t, k) for k, _ in tj_timing._fields_}
+
+
+def phantom_ellipsoids(seed: int, k: int, W: int, H: int, D: int) -> np.ndarray:
+    """Seeded ellipsoid parameters (k x 16), generated by the library's host code."""
+    out = np.zeros((k, 16), np.float64)
+    _lib.load().tj_phantom_ellipsoids(C.c_uint64(int(seed)), int(k), int(W), int(H), int(D), _ptr(out))
+    return out

@@ -1,0 +1,124 @@
+"""Update loops of TomoReconstruction2 driven through the projector interface.
+
+``ossart`` and ``wbp`` restate TomoReconstruction2.OSSART / osartIteration /
+orderShuffleART / WBP (src/TomoJ_Base/.../tomography/TomoReconstruction2.java
+:499-781) on top of any object with the Projector methods (addProjection,
+clearAllProjections, projectAndDifference, backproject, start/endOfIteration),
+exactly as the Java loop calls them.  ``VoxelProjector3DHIP.run_ossart`` runs
+the same loop inside the library without the per-call host round trips; both
+must give the same volume.
+"""
+from __future__ import annotations
+
+import math
+from typing import List, Sequence
+
+import numpy as np
+
+JAVA_FLOAT_MAX = 3.4028234663852886e38
+
+
+def tilt_euler(tilt_deg: float, tilt_axis_deg: float = 0.0, axis_vertical: bool = True) -> np.ndarray:
+    """AffineAlignment.getEulerMatrix (align/AffineAlignment.java:382-395) =
+    MatrixUtils.eulerAngles2Matrix(ta, tilt, -ta) (MatrixUtils.java:226-251), with
+    Java 8's Math.toRadians (angdeg / 180.0 * PI)."""
+    ta = 0.0 if axis_vertical else float(tilt_axis_deg)
+    phi, theta, psi = ta / 180.0 * math.pi, float(tilt_deg) / 180.0 * math.pi, (-ta) / 180.0 * math.pi
+    cphi, sphi = math.cos(phi), math.sin(phi)
+    ct, st = math.cos(theta), math.sin(theta)
+    cpsi, spsi = math.cos(psi), math.sin(psi)
+    return np.array([
+        [cpsi * ct * cphi - spsi * sphi, cpsi * ct * sphi + spsi * cphi, -cpsi * st],
+        [-spsi * ct * cphi - cpsi * sphi, -spsi * ct * sphi + cpsi * cphi, spsi * st],
+        [st * cphi, st * sphi, ct],
+    ])
+
+
+def tilt_eulers(tilts_deg: Sequence[float], tilt_axis_deg: float = 0.0, axis_vertical: bool = True) -> np.ndarray:
+    return np.stack([tilt_euler(t, tilt_axis_deg, axis_vertical) for t in tilts_deg])
+
+
+def order_shuffle(eulers: np.ndarray, first_index: int) -> List[int]:
+    """orderShuffleART (:735-781) with the first pick fixed: each next image is the
+    remaining one with the smallest |row2(E_prev) . row2(E_o)|."""
+    rows = np.asarray(eulers, np.float64).reshape(-1, 3, 3)[:, 2, :]
+    n = rows.shape[0]
+    todo = [True] * n
+    order: List[int] = []
+    im = first_index
+    while len(order) < n:
+        if order:
+            mini, omini = float("inf"), -1
+            a = rows[im]
+            for o in range(n):
+                if todo[o]:
+                    s = 0.0
+                    s += a[0] * rows[o][0]
+                    s += a[1] * rows[o][1]
+                    s += a[2] * rows[o][2]
+                    sp = abs(s)
+                    if sp < mini:
+                        mini, omini = sp, o
+            im = omini
+        if todo[im]:
+            order.append(im)
+            todo[im] = False
+    return order
+
+
+def osart_iteration(projector, eulers: np.ndarray, vol_w: int, vol_h: int, relaxation: float,
+                    update_nb: int, first_index: int, startY: int, endY: int) -> List[float]:
+    """TomoReconstruction2.osartIteration (:636-733); returns err1 (5 values)."""
+    projector.clearAllProjections()
+    n = len(eulers)
+    indexes = order_shuffle(eulers, first_index)
+    update = n if update_nb == 0 else update_nb
+    factorimg = float(np.float32(relaxation / update))
+    diff_index = 0
+    som, errmin, errmax = 0.0, JAVA_FLOAT_MAX, 0.0
+    imin = imax = 0
+    for t in range(n):
+        projector.addProjection(indexes[t])
+        diff_index += 1
+        if diff_index == update or t == n - 1:
+            error = projector.projectAndDifference(factorimg, startY, endY)
+            for e in range(len(error)):
+                som += error[e]
+                if error[e] > errmax:
+                    errmax = error[e]
+                    imax = indexes[t - len(error) + e + 1]
+                elif error[e] < errmin:
+                    errmin = error[e]
+                    imin = indexes[t - len(error) + e + 1]
+            projector.backproject(startY, endY)
+            projector.clearAllProjections()
+            diff_index = 0
+    wh = vol_w * vol_h
+    return [math.sqrt(som / float(n * vol_w * vol_h)), math.sqrt(errmin / wh), imin,
+            math.sqrt(errmax / wh), imax]
+
+
+def ossart(projector, eulers: np.ndarray, iterations: int, relaxation: float, update_nb: int,
+           first_index: int = 0, startY: int = 0, endY=None) -> np.ndarray:
+    """TomoReconstruction2.OSSART (:548-634) through the projector interface."""
+    endY = projector.H if endY is None else endY
+    out = []
+    for _ in range(iterations):
+        projector.startOfIteration()
+        out.append(osart_iteration(projector, eulers, projector.W, projector.H, relaxation, update_nb,
+                                   first_index, startY, endY))
+        projector.endOfIteration()
+    return np.array(out, np.float64).reshape(iterations, 5)
+
+
+def wbp(projector, n: int, update_nb: int = 0, startY: int = 0, endY=None) -> None:
+    """TomoReconstruction2.WBP(ts, projector, params, startY, endY) (:499-538)."""
+    endY = projector.H if endY is None else endY
+    index = 0
+    for i in range(n):
+        projector.addProjection(i)
+        index += 1
+        if index >= update_nb or i == n - 1:
+            projector.backproject(startY, endY)
+            projector.clearAllProjections()
+            index = 0
