@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="also run a plain device copy of BYTES (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -128,7 +130,7 @@ def main():
     roofline = {
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-        "kernel": {"fp": "k_fp_axis", "bp": "k_bp_axis"}[dom],
+        "kernel": {"fp": "k_fp_lds", "bp": "k_bp_lds"}[dom],
         "bytes_per_launch": kernels[dom]["bytes_per_launch"],
         "avg_launch_ms": kernels[dom]["avg_ms"],
     }
@@ -188,6 +190,13 @@ def main():
                                "roofline_frac": WL.algorithmic_bytes_per_iteration(c) * k / dtc / 1e9 / HBM_PEAK_GBS}
         log(f"[bench] SIRT 512^3: {k / dtc:.2f} it/s")
         Q.close()
+
+    if args.copy_probe > 0:
+        cms = tj._lib.copy_probe(args.copy_probe, repeats=3)
+        out["copy_probe"] = {"bytes": args.copy_probe, "ms": cms,
+                             "read_plus_write_GBs": 2 * args.copy_probe / (cms * 1e-3) / 1e9}
+        log(f"[bench] copy probe {args.copy_probe} B: {cms:.3f} ms "
+            f"({out['copy_probe']['read_plus_write_GBs']:.0f} GB/s read+write)")
 
     if not args.no_cpu_baseline:
         import oracle

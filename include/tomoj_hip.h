@@ -71,6 +71,17 @@ void tj_geometry_default(tj_geometry *g, int vol_w, int vol_h, int vol_d, int pr
 const char *tj_last_error(void);
 const char *tj_version(void);
 
+/* JSON description of HIP device `device` (name, arch, CUs, clock, LDS sizes,
+ * HBM and L2 bytes) into buf.  No TomoJ counterpart (JOCLDeviceInfo plays this
+ * role for OpenCL devices). */
+int tj_device_info(int device, char *buf, int len);
+
+/* Copies `bytes` between two device buffers `repeats` times with a plain
+ * streaming kernel and reports the average milliseconds per copy.  No TomoJ
+ * counterpart: it is the known-size reference used to calibrate HBM byte
+ * counters (FETCH_SIZE / WRITE_SIZE) and the achievable copy bandwidth. */
+int tj_copy_probe(int device, size_t bytes, int repeats, double *ms);
+
 /* Creates a projector context on HIP device `device` and allocates the resident
  * volume.  Replaces `new VoxelProjector3DGPU(...)` + ProjectorGPU.initForIterative
  * buffer allocation (ProjectorGPU.java:195-254).  The volume starts at zero. */
@@ -225,6 +236,24 @@ int tj_get_timing(tj_ctx *ctx, tj_timing *t);
 
 /* Turns per-kernel event timing on/off for tj_run_* (off by default). */
 int tj_enable_timing(tj_ctx *ctx, int on);
+
+/* Kernel selection and launch shapes (no TomoJ counterpart: TomoJ lets the
+ * OpenCL driver pick the work-group, GPUDevice.java:303-313).  Used by the
+ * autotuner; every choice gives bit-identical results.  kernel: 0 automatic,
+ * 1 any geometry, 2 single tilt axis, 3 single tilt axis with LDS staging
+ * (used only where its preconditions hold; otherwise 2). */
+typedef struct {
+    int fp_kernel;
+    int fp_rows;    /* detector rows per thread in kernel 2: 1, 2, 4, 8 (kernel 3: 4) */
+    int fp_rays;    /* projections per thread in kernel 3: 2, 4, 8                 */
+    int fp_zchunk;  /* slices per LDS chunk in kernel 3 (0: as many as fit)        */
+    int bp_kernel;
+    int bp_rows;    /* volume rows per thread: 2, 4 (1, 8 for kernel 2; 8 for 3)   */
+    int bp_zchunk;  /* slices per thread: 4, 8 (16 with 2 rows in kernel 3)        */
+    int bp_group;   /* images per LDS pass in kernel 3: 2, 3, 4                    */
+} tj_tuning;
+int tj_set_tuning(tj_ctx *ctx, const tj_tuning *t);
+int tj_get_tuning(tj_ctx *ctx, tj_tuning *t);
 
 #ifdef __cplusplus
 }

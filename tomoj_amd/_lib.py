@@ -50,12 +50,19 @@ class tj_timing(C.Structure):
                 ("res_launches", C.c_int), ("bp_launches", C.c_int)]
 
 
+class tj_tuning(C.Structure):
+    _fields_ = [("fp_kernel", C.c_int), ("fp_rows", C.c_int), ("fp_rays", C.c_int), ("fp_zchunk", C.c_int),
+                ("bp_kernel", C.c_int), ("bp_rows", C.c_int), ("bp_zchunk", C.c_int), ("bp_group", C.c_int)]
+
+
 # name -> (restype, argtypes); every function returning int is status-checked by call()
 _P = C.c_void_p
 _SIGS = {
     "tj_geometry_default": (None, [C.POINTER(tj_geometry), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "tj_last_error": (C.c_char_p, []),
     "tj_version": (C.c_char_p, []),
+    "tj_device_info": (C.c_int, [C.c_int, C.c_char_p, C.c_int]),
+    "tj_copy_probe": (C.c_int, [C.c_int, C.c_size_t, C.c_int, C.POINTER(C.c_double)]),
     "tj_open": (C.c_int, [C.c_int, C.POINTER(tj_geometry), C.POINTER(_P)]),
     "tj_close": (C.c_int, [_P]),
     "tj_load_projections": (C.c_int, [_P, _P, C.c_int, _P]),
@@ -87,6 +94,8 @@ _SIGS = {
     "tj_get_loaded_projection": (C.c_int, [_P, C.c_int, _P]),
     "tj_get_timing": (C.c_int, [_P, C.POINTER(tj_timing)]),
     "tj_enable_timing": (C.c_int, [_P, C.c_int]),
+    "tj_set_tuning": (C.c_int, [_P, C.POINTER(tj_tuning)]),
+    "tj_get_tuning": (C.c_int, [_P, C.POINTER(tj_tuning)]),
 }
 
 EXPORTED = tuple(_SIGS)
@@ -113,6 +122,20 @@ def load() -> C.CDLL:
 
 def last_error() -> str:
     return load().tj_last_error().decode(errors="replace")
+
+
+def device_info(device: int = 0) -> dict:
+    import json
+    buf = C.create_string_buffer(1024)
+    call("tj_device_info", int(device), buf, len(buf))
+    return json.loads(buf.value.decode())
+
+
+def copy_probe(bytes_: int, repeats: int = 5, device: int = 0) -> float:
+    """Milliseconds per device-to-device copy of `bytes_` with a plain kernel."""
+    ms = C.c_double()
+    call("tj_copy_probe", int(device), C.c_size_t(int(bytes_)), int(repeats), C.byref(ms))
+    return ms.value
 
 
 def call(name: str, *args) -> int:

@@ -18,19 +18,22 @@
 #include <algorithm>
 
 namespace tjk {
-hipError_t fp(hipStream_t, const TjGeomDev&, const float*, const TjProj*, const int*, int, int, bool,
-              const uint8_t*, float*, float*, int);
+hipError_t fp(hipStream_t, const TjGeomDev&, const float*, const TjProj*, const int*, int, int, int,
+              const uint8_t*, const float*, float*, float*, int, int, int);
+hipError_t norm_table(hipStream_t, const TjGeomDev&, const TjProj*, int, float*);
+void set_max_lds(int);
 hipError_t norm_max_fill(hipStream_t, int, int, float*);
 hipError_t residual(hipStream_t, int, int, int, const float*, const int*, const float*, const float*, double,
                     float*, double*, uint8_t*, double*);
 hipError_t row_flags(hipStream_t, int, int, int, const float*, const int*, uint8_t*);
 hipError_t bp(hipStream_t, const TjGeomDev&, float*, const TjProj*, const int*, int, const float*, const int*,
-              const uint8_t*, TjBpRow*, int, int, bool, int, int);
+              const uint8_t*, TjBpRow*, int, int, int, int, int, int, int);
 hipError_t zyx_to_yzx(hipStream_t, const float*, float*, int, int, int, int);
 hipError_t yzx_to_zyx(hipStream_t, const float*, float*, int, int, int, int);
 hipError_t plane_flags(hipStream_t, const float*, size_t, int, int, uint8_t*);
 hipError_t phantom_project(hipStream_t, const double*, int, const TjProj*, int, int, int, double, double, double*);
 hipError_t add_noise(hipStream_t, const double*, size_t, uint64_t, double, double*, int, float*);
+hipError_t copy(hipStream_t, const void*, void*, size_t);
 hipError_t weighting_init(hipStream_t, int, int, double, const double*, const double*, int, float*);
 hipError_t ramp_init(hipStream_t, int, int, int, double, double, float*);
 hipError_t weighting_apply(hipStream_t, int, int, int, const float*, const int*, const float*, float*, void*, size_t);
@@ -87,6 +90,8 @@ struct tj_ctx {
 
     float* vol = nullptr;       // [vh][vd][vw]
     uint8_t* pflag = nullptr;   // [vh+1] non-finite flag per volume plane
+    bool any_nonfinite = false; // host copy: some plane flag is set
+    float* norm_tab = nullptr;  // [nproj][pw] ray norms of single-axis projections
 
     int nproj = 0;              // resident projections
     float* exps = nullptr;      // [nproj][ph][pw]
@@ -117,8 +122,8 @@ struct tj_ctx {
     void* wspace = nullptr;
     size_t wspace_size = 0;
 
-    // tuning
-    int fp_rows = 4, bp_rows = 4, bp_kchunk = 0;
+    // kernel selection and shapes (tj_set_tuning)
+    tj_tuning tune{0, 2, 4, 4, 0, 4, 8, 3};
 
     // progress / cancellation
     std::atomic<double> completion{0.0};
@@ -134,6 +139,7 @@ static void ctx_free(tj_ctx* c)
 {
     dfree(c->vol);
     dfree(c->pflag);
+    dfree(c->norm_tab);
     dfree(c->exps);
     dfree(c->pp);
     dfree(c->order);
@@ -183,6 +189,71 @@ static bool all_axis(const tj_ctx* c, const std::vector<int>& idx)
     for (int i : idx)
         if (!c->hpp[i].axis) return false;
     return true;
+}
+
+// The LDS forward projection stages R consecutive volume rows for R consecutive
+// detector rows and shares the row geometry between all projections of a
+// launch.  Checked here with the kernel's own formulas: every projection must
+// give the same y for each detector row, and the inside rows must map to
+// consecutive volume rows.
+static bool fp_lds_ok(const tj_ctx* c, const std::vector<int>& idx, int start_row)
+{
+    const TjGeomDev& g = c->g;
+    if (g.scale != 1.0 || idx.empty() || c->any_nonfinite || !c->norm_tab) return false;
+    const TjProj& a = c->hpp[idx[0]];
+    for (int i : idx) {
+        const TjProj& b = c->hpp[i];
+        if (!b.axis || b.e[4] != 1.0 || b.icf != a.icf || b.zcf != a.zcf || b.divy != a.divy || b.diry != a.diry)
+            return false;
+    }
+    long first_yb = -1, first_j = -1;
+    for (int j = std::max(start_row, 0); j < g.ph; ++j) {
+        const int sy = start_row + ((j - start_row) & ~1);
+        const double jinc = 1.0 / g.scale;
+        double jc = (sy - 1.0 - g.pc1) / g.scale;
+        jc += jinc;
+        if (j != sy) jc += jinc;
+        const double oldy = ((jc - a.icf * 0.0) - a.zcf * (-1 - g.vcz)) / a.divy;
+        const double y = (oldy + a.diry) + g.vcy;
+        if (!(y + TJ_ACCURACY >= 0 && y < g.bymax)) continue;
+        const long yb = (y < 0) ? 0 : (long)y;
+        if ((y < 0 ? 0.0f : (float)y - (float)yb) != 0.0f) return false;  // 4-tap rows
+        if (first_j < 0) {
+            first_j = j;
+            first_yb = yb;
+        } else if (yb != first_yb + (j - first_j)) {
+            return false;
+        }
+    }
+    return true;
+}
+
+// The LDS backprojection bounds the detector columns a block reaches by
+// 256*|e00| + KC*|e02| and drops the multiplication by scale.
+static bool bp_lds_ok(const tj_ctx* c, const std::vector<int>& idx)
+{
+    if (c->g.scale != 1.0) return false;
+    for (int i : idx) {
+        const TjProj& b = c->hpp[i];
+        if (!b.axis || std::fabs(b.e[0]) > 1.0 || std::fabs(b.e[2]) > 1.0) return false;
+    }
+    return true;
+}
+
+static int pick_fp_kernel(const tj_ctx* c, const std::vector<int>& idx, int start_row)
+{
+    if (!all_axis(c, idx)) return 1;
+    const int want = c->tune.fp_kernel;
+    if (want == 1 || want == 2) return want;
+    return fp_lds_ok(c, idx, start_row) ? 3 : 2;
+}
+
+static int pick_bp_kernel(const tj_ctx* c, const std::vector<int>& idx)
+{
+    if (!all_axis(c, idx)) return 1;
+    const int want = c->tune.bp_kernel;
+    if (want == 1 || want == 2) return want;
+    return bp_lds_ok(c, idx) ? 3 : 2;
 }
 
 static int ensure_iota(tj_ctx* c, int n)
@@ -273,9 +344,9 @@ static int do_project(tj_ctx* c, const int* slots, const std::vector<int>& hslot
 {
     const int nslot = (int)hslots.size();
     const int start = fp_start_row(c, start_y);
-    const bool axis = all_axis(c, hslots);
-    TJ_HIP(tjk::fp(c->stream, c->g, c->vol, c->pp, slots, nslot, start, axis, c->pflag, c->sproj, c->snorm,
-                   c->fp_rows));
+    const int kernel = pick_fp_kernel(c, hslots, start);
+    TJ_HIP(tjk::fp(c->stream, c->g, c->vol, c->pp, slots, nslot, start, kernel, c->pflag, c->norm_tab, c->sproj,
+                   c->snorm, c->tune.fp_rows, c->tune.fp_rays, c->tune.fp_zchunk));
     if (c->geo.long_object_comp) TJ_HIP(tjk::norm_max_fill(c->stream, (int)c->npx, nslot, c->snorm));
     return TJ_OK;
 }
@@ -290,9 +361,10 @@ static int do_residual(tj_ctx* c, const int* slots, int nslot, double factor, do
 static int do_backproject(tj_ctx* c, const int* slots, const std::vector<int>& hslots, const float* images,
                           const int* image_index, int start_y, int end_y)
 {
-    const bool axis = all_axis(c, hslots);
+    const int kernel = pick_bp_kernel(c, hslots);
     TJ_HIP(tjk::bp(c->stream, c->g, c->vol, c->pp, slots, (int)hslots.size(), images, image_index, c->rowflag,
-                   c->bprows, start_y, end_y, axis, c->bp_rows, c->bp_kchunk));
+                   c->bprows, start_y, end_y, kernel, c->tune.bp_rows, c->tune.bp_zchunk, c->tune.bp_group,
+                   c->any_nonfinite ? 0 : 1));
     return TJ_OK;
 }
 
@@ -340,6 +412,52 @@ extern "C" {
 const char* tj_last_error(void) { return g_last_error.c_str(); }
 const char* tj_version(void) { return "tomoj-mi355x 0.1 (gfx950)"; }
 
+int tj_copy_probe(int device, size_t bytes, int repeats, double* ms)
+{
+    if (bytes < 16 || repeats < 1) return fail(TJ_EINVAL, "tj_copy_probe: bad size");
+    TJ_HIP(hipSetDevice(device));
+    bytes &= ~(size_t)15;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&a, bytes);
+    if (e == hipSuccess) e = hipMalloc(&b, bytes);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 0, bytes, st);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int r = 0; r < repeats && e == hipSuccess; ++r) e = tjk::copy(st, a, b, bytes);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (ms) *ms = t / repeats;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e != hipSuccess) return fail(TJ_EDEVICE, "tj_copy_probe: %s", hipGetErrorString(e));
+    return TJ_OK;
+}
+
+int tj_device_info(int device, char* buf, int len)
+{
+    if (!buf || len <= 0) return fail(TJ_EINVAL, "tj_device_info: no buffer");
+    hipDeviceProp_t prop;
+    TJ_HIP(hipGetDeviceProperties(&prop, device));
+    int lds_optin = 0;
+    (void)hipDeviceGetAttribute(&lds_optin, hipDeviceAttributeSharedMemPerBlockOptin, device);
+    snprintf(buf, len,
+             "{\"name\": \"%s\", \"arch\": \"%s\", \"cus\": %d, \"clock_mhz\": %d, "
+             "\"lds_per_block\": %zu, \"lds_per_block_optin\": %d, \"lds_per_cu\": %zu, "
+             "\"hbm_bytes\": %zu, \"l2_bytes\": %d}",
+             prop.name, prop.gcnArchName, prop.multiProcessorCount, prop.clockRate / 1000, prop.sharedMemPerBlock,
+             lds_optin, prop.maxSharedMemoryPerMultiProcessor, prop.totalGlobalMem, prop.l2CacheSize);
+    return TJ_OK;
+}
+
 void tj_geometry_default(tj_geometry* g, int vw, int vh, int vd, int pw, int ph)
 {
     std::memset(g, 0, sizeof *g);
@@ -371,6 +489,9 @@ int tj_open(int device, const tj_geometry* geo, tj_ctx** out)
     TJ_HIP(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return fail(TJ_EINVAL, "tj_open: device %d of %d", device, ndev);
     TJ_HIP(hipSetDevice(device));
+    int max_lds = 65536;
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess)
+        tjk::set_max_lds(max_lds);
     tj_ctx* c = new tj_ctx();
     c->device = device;
     c->geo = *geo;
@@ -430,6 +551,8 @@ int tj_set_eulers(tj_ctx* c, const double* euler, int n)
     c->hpp.resize(n);
     for (int k = 0; k < n; ++k) c->hpp[k] = make_proj(euler + 9 * k);
     TJ_HIP(hipMemcpy(c->pp, c->hpp.data(), sizeof(TjProj) * n, hipMemcpyHostToDevice));
+    TJ_HIP(tjk::norm_table(c->stream, c->g, c->pp, n, c->norm_tab));
+    TJ_HIP(hipStreamSynchronize(c->stream));
     return TJ_OK;
 }
 
@@ -438,10 +561,12 @@ static int alloc_projections(tj_ctx* c, int n, const double* euler)
     dfree(c->exps);
     dfree(c->pp);
     dfree(c->order);
+    dfree(c->norm_tab);
     c->nproj = 0;
     c->added.clear();
     c->have_diff = c->have_proj = false;
-    if (dalloc(&c->exps, c->npx * n) || dalloc(&c->pp, n) || dalloc(&c->order, n)) {
+    if (dalloc(&c->exps, c->npx * n) || dalloc(&c->pp, n) || dalloc(&c->order, n) ||
+        dalloc(&c->norm_tab, (size_t)c->g.pw * n)) {
         (void)hipGetLastError();
         return fail(TJ_ENOMEM, "cannot allocate %d projections of %dx%d", n, c->g.pw, c->g.ph);
     }
@@ -489,6 +614,10 @@ int tj_set_volume(tj_ctx* c, const float* zyx, int start_y, int end_y)
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(stage);
     if (e != hipSuccess) return fail(TJ_EDEVICE, "tj_set_volume: %s", hipGetErrorString(e));
+    std::vector<uint8_t> hf(g.vh + 1);
+    TJ_HIP(hipMemcpy(hf.data(), c->pflag, g.vh + 1, hipMemcpyDeviceToHost));
+    c->any_nonfinite = false;
+    for (uint8_t f : hf) c->any_nonfinite |= (f != 0);
     return TJ_OK;
 }
 
@@ -521,6 +650,7 @@ int tj_reset_volume(tj_ctx* c)
     TJ_HIP(hipMemsetAsync(c->vol, 0, c->nvox * sizeof(float), c->stream));
     TJ_HIP(hipMemsetAsync(c->pflag, 0, c->g.vh + 1, c->stream));
     TJ_HIP(hipStreamSynchronize(c->stream));
+    c->any_nonfinite = false;
     return TJ_OK;
 }
 
@@ -735,6 +865,25 @@ int tj_apply_weighting(tj_ctx* c, float* image)
     return TJ_OK;
 }
 
+int tj_set_tuning(tj_ctx* c, const tj_tuning* t)
+{
+    if (!c || !t) return fail(TJ_EINVAL, "tj_set_tuning: null argument");
+    if (t->fp_kernel < 0 || t->fp_kernel > 3 || t->bp_kernel < 0 || t->bp_kernel > 3)
+        return fail(TJ_EINVAL, "tj_set_tuning: kernel index out of range");
+    if (t->fp_rows < 1 || t->bp_rows < 1 || t->fp_rays < 1 || t->bp_zchunk < 0 || t->fp_zchunk < 0 ||
+        t->bp_group < 1)
+        return fail(TJ_EINVAL, "tj_set_tuning: non-positive shape");
+    c->tune = *t;
+    return TJ_OK;
+}
+
+int tj_get_tuning(tj_ctx* c, tj_tuning* t)
+{
+    if (!c || !t) return fail(TJ_EINVAL, "tj_get_tuning: null argument");
+    *t = c->tune;
+    return TJ_OK;
+}
+
 int tj_start_iteration(tj_ctx* c) { return c ? TJ_OK : fail(TJ_EINVAL, "null context"); }
 int tj_end_iteration(tj_ctx* c) { return c ? TJ_OK : fail(TJ_EINVAL, "null context"); }
 
@@ -785,7 +934,6 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
     std::vector<int> order;
     order_shuffle(c, first_index, order);
     TJ_HIP(hipMemcpy(c->order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-    const bool axis = all_axis(c, order);
     const float factorimg = (float)(lambda / update);
     const double completion_inc = (end_y - start_y) / (double)c->g.vh;
 
@@ -816,15 +964,18 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
                     TJ_HIP(hipEventRecord(a, c->stream));
                 }
                 const int start = fp_start_row(c, start_y);
-                TJ_HIP(tjk::fp(c->stream, c->g, c->vol, c->pp, slots, diffIndex, start, axis, c->pflag, c->sproj,
-                               c->snorm, c->fp_rows));
+                const int fk = pick_fp_kernel(c, hs, start);
+                const int bk = pick_bp_kernel(c, hs);
+                TJ_HIP(tjk::fp(c->stream, c->g, c->vol, c->pp, slots, diffIndex, start, fk, c->pflag, c->norm_tab,
+                               c->sproj, c->snorm, c->tune.fp_rows, c->tune.fp_rays, c->tune.fp_zchunk));
                 if (c->geo.long_object_comp)
                     TJ_HIP(tjk::norm_max_fill(c->stream, (int)c->npx, diffIndex, c->snorm));
                 if (c->timing_on) TJ_HIP(hipEventRecord(b, c->stream));
                 if ((rc = do_residual(c, slots, diffIndex, (double)factorimg, c->err + off))) return rc;
                 if (c->timing_on) TJ_HIP(hipEventRecord(r, c->stream));
                 TJ_HIP(tjk::bp(c->stream, c->g, c->vol, c->pp, slots, diffIndex, c->sdiff, c->iota, c->rowflag,
-                               c->bprows, start_y, end_y, axis, c->bp_rows, c->bp_kchunk));
+                               c->bprows, start_y, end_y, bk, c->tune.bp_rows, c->tune.bp_zchunk, c->tune.bp_group,
+                               c->any_nonfinite ? 0 : 1));
                 if (c->timing_on) {
                     TJ_HIP(hipEventRecord(d, c->stream));
                     fpev.emplace_back(a, b);

@@ -41,6 +41,8 @@ struct TjBpRow {
     int yb, y1;    // row and clamped next row of the image
     int valid;     // 0 <= yy < ph
     int shortcut;  // the y-interpolation reduces to the lower row exactly
+    int finite;    // the lower image row holds only finite values
+    int pad;
 };
 
 // ACCURACY (VoxelProjector3D.java:33) is a float constant widened to double.

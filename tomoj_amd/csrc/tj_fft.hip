@@ -220,6 +220,7 @@ size_t weighting_workspace(int pw, int ph, int n)
 hipError_t weighting_init(hipStream_t st, int sx, int sy, double diameter, const double* tilt_deg,
                           const double* axis_deg, int n, float* mask)
 {
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     // FFTWeighting.initWeighting (:67-81): sin(tilt), cos(axis), sin(axis) per
     // image; Java 8's Math.toRadians is angdeg / 180.0 * PI.
     std::vector<double> trig(3 * (size_t)n);
@@ -244,6 +245,7 @@ hipError_t weighting_init(hipStream_t st, int sx, int sy, double diameter, const
 
 hipError_t ramp_init(hipStream_t st, int sx, int sy, int hamming, double cutoff, double ang_step, float* mask)
 {
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     k_ramp<<<dim3((sx + 255) / 256, sy), 256, 0, st>>>(sx, sy, hamming, cutoff, ang_step, mask);
     return hipGetLastError();
 }
@@ -252,6 +254,7 @@ hipError_t ramp_init(hipStream_t st, int sx, int sy, int hamming, double cutoff,
 hipError_t weighting_apply(hipStream_t st, int pw, int ph, int n, const float* exps, const int* idx, const float* mask,
                            float* out, void* ws, size_t ws_size)
 {
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (n <= 0) return hipSuccess;
     if (ws_size < weighting_workspace(pw, ph, n)) return hipErrorInvalidValue;
     const size_t npx = (size_t)pw * ph;

@@ -333,11 +333,14 @@ __global__ void k_bp_rows(TjGeomDev g, const TjProj* __restrict__ pp, const int*
     r.y1 = 0;
     r.yf = 0.0;
     r.shortcut = 1;
+    r.finite = 1;
+    r.pad = 0;
     if (r.valid) {
         r.yb = (int)yy;
         r.y1 = min(r.yb + 1, g.ph - 1);
         r.yf = yy - (double)r.yb;
         r.shortcut = (r.y1 == r.yb) || (r.yf == 0.0 && rowflag[(size_t)s * g.ph + r.y1] == 0);
+        r.finite = rowflag[(size_t)s * g.ph + r.yb] == 0;
     }
     rows[(size_t)s * g.vh + j] = r;
 }
@@ -454,6 +457,421 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_bp_generic(
 }
 
 // ---------------------------------------------------------------------------
+// LDS-staged single-axis kernels (the fast path for tilt series around a
+// vertical axis with scale 1).  Same arithmetic as the kernels above.
+// ---------------------------------------------------------------------------
+
+// Backprojection: a 256-thread block owns 256 voxel columns x, R volume rows and
+// KC slices, and keeps their KC*R sums in registers.  For each group of G added
+// images it stages, per (image, row), the detector columns its voxels can reach
+// as FP64 pairs (ll, lr - ll): exactly the two operands of ImageJ's
+// interpolation along x, so a voxel update is one LDS read, one multiply and one
+// add before the float cast.  Images are taken in order for every voxel.  The
+// update is branch-free: lanes whose projection falls outside the image read a
+// fixed LDS cell and keep their sum.  Rows whose y-interpolation does not reduce
+// to the lower row (fractional yy, or a non-finite value in the next image row)
+// take the exact 4-tap path with the next row read from global memory.
+template <int R, int KC, int G, bool POS>
+__global__ __launch_bounds__(TJ_BLOCK) void k_bp_lds(
+    TjGeomDev g, float* __restrict__ vol, const TjProj* __restrict__ pp,
+    const int* __restrict__ slot_proj, int nslot, const float* __restrict__ images,
+    const int* __restrict__ image_index, const TjBpRow* __restrict__ rows, int start_y, int end_y,
+    int finite_volume)
+{
+    constexpr int XW = TJ_BLOCK + KC + 8;  // reachable columns + a zero cell at XW-1
+    __shared__ double2 lds[G][R][XW];
+    // per-slot constants of the current group, staged once per pass
+    __shared__ double s_e0[G], s_e2[G];
+    __shared__ int s_x0[G], s_mode[G], s_img[G];
+    __shared__ double s_ryf[G][R];
+    __shared__ int s_rflag[G][R];  // bit0 valid, bit1 shortcut
+    __shared__ int s_ryb[G][R], s_ry1[G][R];
+    const int tid = threadIdx.x;
+    const int i0 = blockIdx.x * TJ_BLOCK;
+    const int i = i0 + tid;
+    const int j0 = start_y + blockIdx.y * R;
+    const int nr = min(R, end_y - j0);
+    const int k0 = blockIdx.z * KC;
+    const int nk = min(KC, g.vd - k0);
+    const int ilast = min(i0 + TJ_BLOCK - 1, g.vw - 1);
+    const bool active = i < g.vw;
+    const double x = (active ? i : ilast) - g.vcx;
+    const size_t plane = (size_t)g.vd * g.vw;
+    const size_t npx = (size_t)g.pw * g.ph;
+    const double pwd = g.pw;
+
+    float acc[KC][R];
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            acc[kk][r] = (active && kk < nk && r < nr)
+                             ? vol[(size_t)(j0 + r) * plane + (size_t)(k0 + kk) * g.vw + i]
+                             : 0.0f;
+
+    for (int s0 = 0; s0 < nslot; s0 += G) {
+        const int gn = min(G, nslot - s0);
+        __syncthreads();
+        if (tid < gn) {
+            // smallest xx over the block: xx is monotone in i and in k, so it is
+            // at a corner; the same expression as the per-voxel xx below.
+            const TjProj& Q = pp[slot_proj[s0 + tid]];
+            const double xa = (i0 - g.vcx) * Q.e[0], xb = (ilast - g.vcx) * Q.e[0];
+            const double za = (k0 - g.vcz) * Q.e[2], zb = (k0 + nk - 1 - g.vcz) * Q.e[2];
+            const double m = fmin(fmin((xa + za) + g.pc2, (xa + zb) + g.pc2),
+                                  fmin((xb + za) + g.pc2, (xb + zb) + g.pc2));
+            s_x0[tid] = (m > 0.0) ? (int)m : 0;
+            s_e0[tid] = Q.e[0];
+            s_e2[tid] = Q.e[2];
+            s_img[tid] = image_index[s0 + tid];
+            // mode 2: every row reduces to its lower row and holds finite values
+            //         (with a finite volume and no positivity: the fast update)
+            // mode 1: every row reduces to its lower row
+            // mode 0: some row needs the 4-tap interpolation
+            int sh = 1, fin = 1;
+            for (int r = 0; r < nr; ++r) {
+                const TjBpRow row = rows[(size_t)(s0 + tid) * g.vh + j0 + r];
+                if (row.valid && !row.shortcut) sh = 0;
+                if (row.valid && !row.finite) fin = 0;
+            }
+            s_mode[tid] = sh ? ((fin && finite_volume && !POS) ? 2 : 1) : 0;
+        }
+        if (tid < G * R) {
+            const int gg = tid / R, r = tid % R;
+            int fl = 0, yb = 0, y1 = 0;
+            double yf = 0.0;
+            if (gg < gn && r < nr) {
+                const TjBpRow row = rows[(size_t)(s0 + gg) * g.vh + j0 + r];
+                fl = (row.valid ? 1 : 0) | (row.shortcut ? 2 : 0);
+                yb = row.yb;
+                y1 = row.y1;
+                yf = row.yf;
+            }
+            s_rflag[gg][r] = fl;
+            s_ryb[gg][r] = yb;
+            s_ry1[gg][r] = y1;
+            s_ryf[gg][r] = yf;
+        }
+        __syncthreads();
+        // wave w stages image rows w, w+4, ... of the group; lanes take
+        // consecutive columns, so a row's constants are uniform in the wave
+        {
+            const int lane = tid & 63;
+            for (int gr = tid >> 6; gr < gn * R; gr += TJ_BLOCK / 64) {
+                const int gg = gr / R, r = gr - (gr / R) * R;
+                const int x0s = s_x0[gg];
+                const bool rok = (s_rflag[gg][r] & 1) != 0;
+                const float* img = images + (size_t)s_img[gg] * npx + (size_t)s_ryb[gg][r] * g.pw;
+                double2* dst = &lds[gg][r][0];
+                for (int xx = lane; xx < XW; xx += 64) {
+                    const int xs = x0s + xx;
+                    double2 v = make_double2(0.0, 0.0);
+                    if (rok && xs < g.pw && xx < XW - 1) {
+                        const double ll = img[xs];
+                        const double lr = img[min(xs + 1, g.pw - 1)];
+                        v = make_double2(ll, lr - ll);
+                    }
+                    dst[xx] = v;
+                }
+            }
+        }
+        __syncthreads();
+        for (int gg = 0; gg < gn; ++gg) {
+            const double e2 = s_e2[gg];
+            const double xe = x * s_e0[gg];
+            const int x0s = s_x0[gg];
+            const int mode = s_mode[gg];
+            if (mode == 2) {
+                // finite images, finite volume, no positivity: lanes off the image
+                // (and invalid rows) read zero pairs, and acc + 0 == acc
+#pragma unroll
+                for (int kk = 0; kk < KC; ++kk) {
+                    const double z = (k0 + kk) - g.vcz;
+                    const double xx = (xe + z * e2) + g.pc2;
+                    const bool vx = (xx >= 0) && (xx < pwd);
+                    const double xs = vx ? xx : 0.0;
+                    const int xb = (int)xs;
+                    const double xf = xs - (double)xb;
+                    const int li = vx ? xb - x0s : XW - 1;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const double2 q = lds[gg][r][li];
+                        float a = acc[kk][r] + (float)(q.x + xf * q.y);
+                        acc[kk][r] = (isnan(a) || isinf(a)) ? 0.0f : a;
+                    }
+                }
+            } else if (mode == 1) {
+                bool rv[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) rv[r] = (s_rflag[gg][r] & 1) != 0;
+#pragma unroll
+                for (int kk = 0; kk < KC; ++kk) {
+                    const double z = (k0 + kk) - g.vcz;
+                    const double xx = (xe + z * e2) + g.pc2;
+                    const bool vx = active && (kk < nk) && (xx >= 0) && (xx < pwd);
+                    const double xs = vx ? xx : 0.0;
+                    const int xb = (int)xs;
+                    const double xf = xs - (double)xb;
+                    const int li = vx ? xb - x0s : XW - 1;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const double2 q = lds[gg][r][li];
+                        const double lower = q.x + xf * q.y;
+                        // lower + 0*(upper - lower): no update unless lower is finite
+                        float a = acc[kk][r] + (float)lower;
+                        a = (isnan(a) || isinf(a)) ? 0.0f : a;
+                        if (POS) a = (a < 0) ? 0.0f : a;
+                        acc[kk][r] = (vx && rv[r] && isfinite(lower)) ? a : acc[kk][r];
+                    }
+                }
+            } else {
+                const float* img = images + (size_t)s_img[gg] * npx;
+#pragma unroll
+                for (int kk = 0; kk < KC; ++kk) {
+                    const double z = (k0 + kk) - g.vcz;
+                    const double xx = (xe + z * e2) + g.pc2;
+                    const bool vx = active && (kk < nk) && (xx >= 0) && (xx < pwd);
+                    if (!vx) continue;
+                    const int xb = (int)xx;
+                    const double xf = xx - (double)xb;
+                    const int li = xb - x0s;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        if (!(s_rflag[gg][r] & 1)) continue;
+                        const double2 q = lds[gg][r][li];
+                        const double lower = q.x + xf * q.y;
+                        float value;
+                        if (s_rflag[gg][r] & 2) {
+                            value = isfinite(lower) ? (float)lower : __builtin_nanf("");
+                        } else {
+                            const float* up = img + (size_t)s_ry1[gg][r] * g.pw;
+                            const double ul = up[xb], ur = up[min(xb + 1, g.pw - 1)];
+                            const double upper = ul + xf * (ur - ul);
+                            value = (float)(lower + s_ryf[gg][r] * (upper - lower));
+                        }
+                        bp_accumulate<POS>(acc[kk][r], value);
+                    }
+                }
+            }
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (kk < nk && r < nr) vol[(size_t)(j0 + r) * plane + (size_t)(k0 + kk) * g.vw + i] = acc[kk][r];
+}
+
+// Norm of every ray of a single-axis projection: with e01 = 0 the x(z) sequence
+// of a ray does not depend on its detector row, so norm_sum
+// (VoxelProjector3D.java:774-784) is a function of (projection, column) only.
+// One thread per (projection, column), same loop as projectJosephPartial.
+__global__ __launch_bounds__(TJ_BLOCK) void k_norm_table(TjGeomDev g, const TjProj* __restrict__ pp,
+                                                         float* __restrict__ tab)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = blockIdx.y;
+    if (i >= g.pw) return;
+    const TjProj& P = pp[p];
+    float ns = 0.0f;
+    if (P.axis) {
+        const double ic = (i - g.pc0) / g.scale;
+        double xc = ic - P.yfx * 0.0 - P.zfx * (-1 - g.vcz);
+        xc /= P.divx;
+        bool entered = false;
+        for (int z = 0; z < g.vd; ++z) {
+            xc = xc + P.dirx;
+            const double x = xc + g.vcx;
+            if (x + TJ_ACCURACY >= 0 && x < g.bxmax) {
+                ns = (float)((double)ns + P.norminc);
+                entered = true;
+            } else if (entered) {
+                ns = (float)((double)ns + P.norminc);
+                break;
+            }
+        }
+    }
+    tab[(size_t)p * g.pw + i] = ns;
+}
+
+// Forward projection for a single vertical tilt axis with integer volume rows:
+// a 256-thread block owns 4 consecutive detector rows (4 consecutive volume
+// rows), 256 detector columns and G added projections (thread = column, one ray
+// per projection).  The 4 volume rows are staged in LDS interleaved, one float4
+// per (z, x), with column W a copy of column W-1, so the two taps of all four
+// rows come from two 16-byte reads and the nearest-column rule of getPixel
+// needs no branch (lerp with equal taps).  Lanes outside the volume read a
+// zero cell.  Requires a finite volume and rows whose y is an integer (the
+// host checks both; otherwise k_fp_axis runs).  Blocks are numbered so the
+// blocks sharing one row group run on one XCD back to back and share its L2.
+template <int G, bool NEARY>
+__device__ __forceinline__ void fp_lds_chunks(const TjGeomDev& g, const float* __restrict__ vol, float4* lds4,
+                                              int zc, int rowbase, int ns, const float (&ym)[4], double (&xc)[G],
+                                              const double (&dirx)[G], float4 (&rs)[G])
+{
+    const int tid = threadIdx.x;
+    const size_t plane = (size_t)g.vd * g.vw;
+    const int W1 = g.vw + 1;
+    const int zero_idx = zc * W1;
+    const double nacc = -TJ_ACCURACY;  // x + ACCURACY >= 0  <=>  x >= -ACCURACY
+    const double bx = g.bxmax;
+    const double wm1 = g.vw - 1;
+    const bool rin0 = rowbase + 0 >= 0 && rowbase + 0 < g.vh, rin1 = rowbase + 1 >= 0 && rowbase + 1 < g.vh;
+    const bool rin2 = rowbase + 2 >= 0 && rowbase + 2 < g.vh, rin3 = rowbase + 3 >= 0 && rowbase + 3 < g.vh;
+    const float* r0 = vol + (size_t)(rin0 ? rowbase + 0 : 0) * plane;
+    const float* r1 = vol + (size_t)(rin1 ? rowbase + 1 : 0) * plane;
+    const float* r2 = vol + (size_t)(rin2 ? rowbase + 2 : 0) * plane;
+    const float* r3 = vol + (size_t)(rin3 ? rowbase + 3 : 0) * plane;
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int z0 = 0; z0 < g.vd; z0 += zc) {
+        const int zn = min(zc, g.vd - z0);
+        __syncthreads();
+        // wave w stages slices w, w+4, ...; lanes take consecutive columns
+        // (32-bit offsets within a plane: the plane D*W is far below 2^31)
+        for (int zz = wave; zz < zn; zz += TJ_BLOCK / 64) {
+            const int zo = (z0 + zz) * g.vw;
+            float4* dst = lds4 + zz * W1;
+            for (int xx = lane; xx < W1; xx += 64) {
+                const int o = zo + min(xx, g.vw - 1);
+                dst[xx] = make_float4(rin0 ? r0[o] : 0.0f, rin1 ? r1[o] : 0.0f, rin2 ? r2[o] : 0.0f,
+                                      rin3 ? r3[o] : 0.0f);
+            }
+        }
+        __syncthreads();
+        // the last slice takes the nearest value (getPixel's z == D-1 rule): its
+        // fraction is multiplied by 0
+        const int zlast = (z0 + zn == g.vd) ? zn - 1 : zn;
+        for (int zz = 0; zz < zn; ++zz) {
+            const float zl = (zz == zlast) ? 0.0f : 1.0f;
+            const int zo = zz * W1;
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                xc[k] = xc[k] + dirx[k];
+                const double xv = xc[k] + g.vcx;
+                const bool in = (xv >= nacc) && (xv < bx);
+                // inside, xv is in [-ACCURACY, W): clamping to [0, W-1] keeps xb and,
+                // for xb < W-1, the fraction; at xb = W-1 both taps are equal (the
+                // replicated column) so the fraction does not matter
+                const double xs = fmin(fmax(xv, 0.0), wm1);
+                const int xb = (int)xs;
+                const float xf = ((float)xs - (float)xb) * zl;
+                const int a = in ? zo + xb : zero_idx;
+                const float4 q0 = lds4[a];
+                const float4 q1 = lds4[a + 1];
+                if (NEARY) {
+                    rs[k].x += q0.x + (xf * ym[0]) * (q1.x - q0.x);
+                    rs[k].y += q0.y + (xf * ym[1]) * (q1.y - q0.y);
+                    rs[k].z += q0.z + (xf * ym[2]) * (q1.z - q0.z);
+                    rs[k].w += q0.w + (xf * ym[3]) * (q1.w - q0.w);
+                } else {
+                    rs[k].x += q0.x + xf * (q1.x - q0.x);
+                    rs[k].y += q0.y + xf * (q1.y - q0.y);
+                    rs[k].z += q0.z + xf * (q1.z - q0.z);
+                    rs[k].w += q0.w + xf * (q1.w - q0.w);
+                }
+            }
+        }
+    }
+}
+
+template <int G>
+__global__ __launch_bounds__(TJ_BLOCK) void k_fp_lds(
+    TjGeomDev g, const float* __restrict__ vol, const TjProj* __restrict__ pp,
+    const int* __restrict__ slot_proj, int nslot, int start_row, const float* __restrict__ norm_tab, int zc,
+    int nrb, int nsub, int ntile, float* __restrict__ proj, float* __restrict__ norm)
+{
+    extern __shared__ float4 lds4[];
+    const int tid = threadIdx.x;
+    const int b = blockIdx.x;
+    const int xcd = b & 7, local = b >> 3;
+    const int sub = local % nsub;
+    const int rb = (local / nsub) * 8 + xcd;
+    if (rb >= nrb) return;  // whole block
+    const int tile = sub % ntile, grp = sub / ntile;
+    const int i = tile * TJ_BLOCK + tid;
+    const int s0 = grp * G;
+    const int ns = min(G, nslot - s0);
+    const int j0 = rb * 4;
+    const bool col = i < g.pw;
+    const double ic = ((col ? i : g.pw - 1) - g.pc0) / g.scale;
+
+    const TjProj& P0 = pp[slot_proj[s0]];
+    bool ok[4];
+    float ym[4];  // 0 for the last volume row (nearest value), 1 otherwise
+    int rowbase = 0;
+    bool have_row = false;
+    double oldy_any = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        ok[r] = false;
+        ym[r] = 1.0f;
+        const int j = j0 + r;
+        if (j < g.ph && j >= start_row) {
+            const double jc = jcentered(g, start_row, j);
+            const double oldy = ((jc - P0.icf * ic) - P0.zcf * (-1 - g.vcz)) / P0.divy;
+            oldy_any = oldy;
+            const double y = (oldy + P0.diry) + g.vcy;
+            if (y + TJ_ACCURACY >= 0 && y < g.bymax) {
+                const int yb = (y < 0) ? 0 : (int)y;
+                ok[r] = true;
+                ym[r] = (yb == g.vh - 1) ? 0.0f : 1.0f;
+                if (!have_row) {
+                    rowbase = yb - r;
+                    have_row = true;
+                }
+            }
+        }
+    }
+    const bool neary = (ym[0] * ym[1] * ym[2] * ym[3]) == 0.0f;
+
+    double xc[G], dirx[G];
+    float4 rs[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        rs[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        const TjProj& Q = pp[slot_proj[s0 + min(k, ns - 1)]];
+        double ox = ic - Q.yfx * oldy_any - Q.zfx * (-1 - g.vcz);
+        ox /= Q.divx;
+        // rays beyond the launch's projections stay far outside the volume
+        xc[k] = (k < ns) ? ox : -1.0e300;
+        dirx[k] = (k < ns) ? Q.dirx : 0.0;
+    }
+    if (have_row) {
+        if (tid < 2) lds4[zc * (g.vw + 1) + tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (neary)
+            fp_lds_chunks<G, true>(g, vol, lds4, zc, rowbase, ns, ym, xc, dirx, rs);
+        else
+            fp_lds_chunks<G, false>(g, vol, lds4, zc, rowbase, ns, ym, xc, dirx, rs);
+    }
+    if (!col) return;
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        if (k >= ns) break;
+        const int p = slot_proj[s0 + k];
+        const double ninc = pp[p].norminc;
+        const float nt = norm_tab[(size_t)p * g.pw + i];
+        const float rv[4] = {rs[k].x, rs[k].y, rs[k].z, rs[k].w};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int j = j0 + r;
+            if (j >= g.ph) break;
+            const size_t o = ((size_t)(s0 + k) * g.ph + j) * g.pw + i;
+            if (j < start_row) {
+                proj[o] = 0.0f;
+                norm[o] = 0.0f;
+            } else if (ok[r] && nt > 0) {
+                proj[o] = (float)((double)rv[r] * ninc);
+                norm[o] = nt;
+            } else {
+                proj[o] = __builtin_nanf("");
+                norm[o] = 0.0f;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Volume layout conversion between the host ImageStack layout [z][y][x] and the
 // device layout [y][z][x], for rows [y0, y0+nrows).
 // ---------------------------------------------------------------------------
@@ -566,6 +984,14 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_add_noise(const double* __restrict
     }
 }
 
+// Plain streaming copy (16-byte vectors), used to calibrate HBM byte counters.
+__global__ __launch_bounds__(TJ_BLOCK) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                   size_t n4)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n4; k += (size_t)gridDim.x * blockDim.x)
+        dst[k] = src[k];
+}
+
 }  // namespace tj
 
 // ---------------------------------------------------------------------------
@@ -574,35 +1000,82 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_add_noise(const double* __restrict
 namespace tjk {
 using namespace tj;
 
-hipError_t fp(hipStream_t st, const TjGeomDev& g, const float* vol, const TjProj* pp,
-              const int* slot_proj, int nslot, int start_row, bool axis, const uint8_t* pflag,
-              float* proj, float* norm, int rows_per_thread)
+static int g_max_lds = 65536;
+
+void set_max_lds(int bytes) { g_max_lds = bytes; }
+
+template <int G>
+static hipError_t launch_fp_lds(hipStream_t st, const TjGeomDev& g, const float* vol, const TjProj* pp,
+                                const int* slot_proj, int nslot, int start_row, const float* norm_tab, int zc,
+                                float* proj, float* norm)
 {
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    const int per = (g.vw + 1) * (int)sizeof(float4);
+    int zmax = (g_max_lds - 2 * (int)sizeof(float4)) / per;
+    if (zmax < 1) return hipErrorInvalidValue;
+    if (zc <= 0 || zc > zmax) zc = zmax;
+    if (zc > g.vd) zc = g.vd;
+    const size_t lds = (size_t)(zc * (g.vw + 1) + 2) * sizeof(float4);
+    const int nrb = (g.ph + 3) / 4;
+    const int ntile = (g.pw + TJ_BLOCK - 1) / TJ_BLOCK;
+    const int ngrp = (nslot + G - 1) / G;
+    const int nsub = ntile * ngrp;
+    const int nblk = ((nrb + 7) / 8) * 8 * nsub;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_fp_lds<G>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    k_fp_lds<G><<<nblk, TJ_BLOCK, lds, st>>>(g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nrb, nsub,
+                                              ntile, proj, norm);
+    return hipGetLastError();
+}
+
+hipError_t norm_table(hipStream_t st, const TjGeomDev& g, const TjProj* pp, int nproj, float* tab)
+{
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    if (nproj <= 0) return hipSuccess;
+    k_norm_table<<<dim3((g.pw + TJ_BLOCK - 1) / TJ_BLOCK, nproj), TJ_BLOCK, 0, st>>>(g, pp, tab);
+    return hipGetLastError();
+}
+
+// kernel: 1 generic, 2 axis (global loads), 3 axis with LDS staging
+hipError_t fp(hipStream_t st, const TjGeomDev& g, const float* vol, const TjProj* pp, const int* slot_proj,
+              int nslot, int start_row, int kernel, const uint8_t* pflag, const float* norm_tab, float* proj,
+              float* norm, int rows, int rpt, int zc)
+{
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nslot <= 0) return hipSuccess;
     const int bx = (g.pw + TJ_BLOCK - 1) / TJ_BLOCK;
-    if (!axis) {
+    if (kernel == 1) {
         k_fp_generic<<<dim3(bx, g.ph, nslot), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, start_row, proj, norm);
-    } else {
-        switch (rows_per_thread) {
-        case 1:
-            k_fp_axis<1><<<dim3(bx, g.ph, nslot), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, start_row, pflag, proj, norm);
-            break;
-        case 2:
-            k_fp_axis<2><<<dim3(bx, (g.ph + 1) / 2, nslot), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, start_row, pflag, proj, norm);
-            break;
-        case 8:
-            k_fp_axis<8><<<dim3(bx, (g.ph + 7) / 8, nslot), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, start_row, pflag, proj, norm);
-            break;
-        default:
-            k_fp_axis<4><<<dim3(bx, (g.ph + 3) / 4, nslot), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, start_row, pflag, proj, norm);
-            break;
-        }
+        return hipGetLastError();
+    }
+    if (kernel == 3) {
+        if (rpt <= 2) return launch_fp_lds<2>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm);
+        if (rpt <= 4) return launch_fp_lds<4>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm);
+        return launch_fp_lds<8>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm);
+    }
+    switch (rows) {
+    case 1:
+        k_fp_axis<1><<<dim3(bx, g.ph, nslot), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, start_row, pflag, proj, norm);
+        break;
+    case 2:
+        k_fp_axis<2><<<dim3(bx, (g.ph + 1) / 2, nslot), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, start_row, pflag, proj, norm);
+        break;
+    case 8:
+        k_fp_axis<8><<<dim3(bx, (g.ph + 7) / 8, nslot), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, start_row, pflag, proj, norm);
+        break;
+    default:
+        k_fp_axis<4><<<dim3(bx, (g.ph + 3) / 4, nslot), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, start_row, pflag, proj, norm);
+        break;
     }
     return hipGetLastError();
 }
 
 hipError_t norm_max_fill(hipStream_t st, int npx, int nslot, float* norm)
 {
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nslot <= 0) return hipSuccess;
     k_norm_max_fill<<<nslot, TJ_BLOCK, 0, st>>>(npx, norm);
     return hipGetLastError();
@@ -612,6 +1085,7 @@ hipError_t residual(hipStream_t st, int pw, int ph, int nslot, const float* exps
                     const float* proj, const float* norm, double factor, float* diff, double* row_err,
                     uint8_t* rowflag, double* err)
 {
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nslot <= 0) return hipSuccess;
     k_residual<<<dim3(ph, nslot), TJ_BLOCK, 0, st>>>(pw, ph, exps, exp_index, proj, norm, factor, diff, row_err, rowflag);
     k_err_sum<<<nslot, TJ_BLOCK, 0, st>>>(ph, row_err, err);
@@ -621,19 +1095,37 @@ hipError_t residual(hipStream_t st, int pw, int ph, int nslot, const float* exps
 hipError_t row_flags(hipStream_t st, int pw, int ph, int nslot, const float* images, const int* image_index,
                      uint8_t* rowflag)
 {
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nslot <= 0) return hipSuccess;
     k_row_flags<<<dim3(ph, nslot), TJ_BLOCK, 0, st>>>(pw, ph, images, image_index, rowflag);
     return hipGetLastError();
 }
 
-hipError_t bp(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp, const int* slot_proj,
-              int nslot, const float* images, const int* image_index, const uint8_t* rowflag,
-              TjBpRow* rows, int start_y, int end_y, bool axis, int rows_per_thread, int kchunk)
+template <int R, int KC, int G>
+static hipError_t launch_bp_lds(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp,
+                                const int* slot_proj, int nslot, const float* images, const int* image_index,
+                                const TjBpRow* rows, int start_y, int end_y, int finite_volume)
 {
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    const int ny = end_y - start_y;
+    dim3 grid((g.vw + TJ_BLOCK - 1) / TJ_BLOCK, (ny + R - 1) / R, (g.vd + KC - 1) / KC);
+    if (g.positivity)
+        k_bp_lds<R, KC, G, true><<<grid, TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume);
+    else
+        k_bp_lds<R, KC, G, false><<<grid, TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume);
+    return hipGetLastError();
+}
+
+// kernel: 1 generic, 2 axis (global loads), 3 axis with LDS staging (scale 1)
+hipError_t bp(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp, const int* slot_proj, int nslot,
+              const float* images, const int* image_index, const uint8_t* rowflag, TjBpRow* rows, int start_y,
+              int end_y, int kernel, int rows_per_thread, int kchunk, int group, int finite_volume)
+{
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nslot <= 0 || end_y <= start_y) return hipSuccess;
     const int bx = (g.vw + TJ_BLOCK - 1) / TJ_BLOCK;
     const int ny = end_y - start_y;
-    if (!axis) {
+    if (kernel == 1) {
         if (g.positivity)
             k_bp_generic<true><<<dim3(bx, ny, g.vd), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, start_y);
         else
@@ -641,7 +1133,22 @@ hipError_t bp(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp, 
         return hipGetLastError();
     }
     k_bp_rows<<<dim3((g.vh + 255) / 256, nslot), 256, 0, st>>>(g, pp, slot_proj, nslot, rowflag, rows);
-    if (kchunk <= 0) kchunk = g.vd;
+    if (kernel == 3) {
+#define TJ_BPL(RR, KK, GG) return launch_bp_lds<RR, KK, GG>(st, g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume)
+        if (rows_per_thread == 2) {
+            if (kchunk >= 16) { if (group >= 4) TJ_BPL(2, 16, 4); TJ_BPL(2, 16, 2); }
+            if (group >= 8) TJ_BPL(2, 8, 8);
+            if (group >= 4) TJ_BPL(2, 8, 4);
+            TJ_BPL(2, 8, 2);
+        }
+        if (rows_per_thread == 8) TJ_BPL(8, 4, 1);
+        if (kchunk <= 4) TJ_BPL(4, 4, 2);
+        if (group >= 4) TJ_BPL(4, 8, 4);
+        if (group >= 3) TJ_BPL(4, 8, 3);
+        TJ_BPL(4, 8, 2);
+#undef TJ_BPL
+    }
+    if (kchunk <= 0) kchunk = 8;
     const int bz = (g.vd + kchunk - 1) / kchunk;
     const bool s1 = (g.scale == 1.0);
     const bool pos = g.positivity != 0;
@@ -663,8 +1170,16 @@ hipError_t bp(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp, 
     return hipGetLastError();
 }
 
+hipError_t copy(hipStream_t st, const void* src, void* dst, size_t bytes)
+{
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    k_copy<<<2048, TJ_BLOCK, 0, st>>>((const float4*)src, (float4*)dst, bytes / 16);
+    return hipGetLastError();
+}
+
 hipError_t zyx_to_yzx(hipStream_t st, const float* src, float* vol, int W, int D, int nrows, int y0)
 {
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nrows <= 0) return hipSuccess;
     k_zyx_to_yzx<<<dim3((W + 255) / 256, nrows, D), 256, 0, st>>>(src, vol, W, D, nrows, y0);
     return hipGetLastError();
@@ -672,6 +1187,7 @@ hipError_t zyx_to_yzx(hipStream_t st, const float* src, float* vol, int W, int D
 
 hipError_t yzx_to_zyx(hipStream_t st, const float* vol, float* dst, int W, int D, int nrows, int y0)
 {
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nrows <= 0) return hipSuccess;
     k_yzx_to_zyx<<<dim3((W + 255) / 256, nrows, D), 256, 0, st>>>(vol, dst, W, D, nrows, y0);
     return hipGetLastError();
@@ -679,6 +1195,7 @@ hipError_t yzx_to_zyx(hipStream_t st, const float* vol, float* dst, int W, int D
 
 hipError_t plane_flags(hipStream_t st, const float* vol, size_t plane, int y0, int nrows, uint8_t* pflag)
 {
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nrows <= 0) return hipSuccess;
     k_plane_flags<<<nrows, TJ_BLOCK, 0, st>>>(vol, plane, y0, pflag);
     return hipGetLastError();
@@ -687,6 +1204,7 @@ hipError_t plane_flags(hipStream_t st, const float* vol, size_t plane, int y0, i
 hipError_t phantom_project(hipStream_t st, const double* ell, int k, const TjProj* pp, int n, int pw, int ph,
                            double pc0, double pc1, double* out)
 {
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     k_phantom_project<<<dim3((pw + TJ_BLOCK - 1) / TJ_BLOCK, ph, n), TJ_BLOCK, 0, st>>>(ell, k, pp, pw, ph, pc0, pc1, out);
     return hipGetLastError();
 }
@@ -694,6 +1212,7 @@ hipError_t phantom_project(hipStream_t st, const double* ell, int k, const TjPro
 hipError_t add_noise(hipStream_t st, const double* a, size_t n, uint64_t seed, double rel, double* part,
                      int npart, float* out)
 {
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     k_max_partial<<<npart, TJ_BLOCK, 0, st>>>(a, n, part);
     k_add_noise<<<1024, TJ_BLOCK, 0, st>>>(a, n, seed, part, npart, rel, out);
     return hipGetLastError();

@@ -15,7 +15,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import _lib
-from ._lib import call, tj_geometry, tj_timing
+from ._lib import call, tj_geometry, tj_timing, tj_tuning
 
 
 def _ptr(a: np.ndarray):
@@ -221,6 +221,17 @@ class VoxelProjector3DHIP:
 
     def enable_timing(self, on: bool = True) -> None:
         call("tj_enable_timing", self._ctx, int(bool(on)))
+
+    def get_tuning(self) -> dict:
+        t = tj_tuning()
+        call("tj_get_tuning", self._ctx, C.byref(t))
+        return {k: getattr(t, k) for k, _ in tj_tuning._fields_}
+
+    def set_tuning(self, **kw) -> None:
+        cur = self.get_tuning()
+        cur.update(kw)
+        t = tj_tuning(**cur)
+        call("tj_set_tuning", self._ctx, C.byref(t))
 
     def timing(self) -> dict:
         t = tj_timing()
