@@ -46,6 +46,18 @@ def kernel_bytes(w):
     }
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+    summarised in profiles/traffic.json (scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            data = json.load(fh)
+        return data["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def run_gpu(w, steps, warmup, tj):
     from tomoj_amd.workloads import eulers_for
     E = eulers_for(w)
@@ -127,12 +139,15 @@ def main():
         }
     dom = max(("fp", "bp"), key=lambda k: tim[f"{k}_ms"])
     achieved = kernels[dom]["achieved_GBs"]
+    kname = {"fp": "k_fp_lds", "bp": "k_bp_lds"}[dom]
     roofline = {
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-        "kernel": {"fp": "k_fp_lds", "bp": "k_bp_lds"}[dom],
+        "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname),
+        "kernel": kname,
         "bytes_per_launch": kernels[dom]["bytes_per_launch"],
         "avg_launch_ms": kernels[dom]["avg_ms"],
+        "limiter": "VALU: FP64 positions and ImageJ's FP64 bilinear weights per voxel update "
+                   "(SQ_INSTS_VALU in profiles/); HBM traffic is below the slab-streaming model",
     }
     alg = WL.algorithmic_bytes_per_iteration(w)
     out = {
