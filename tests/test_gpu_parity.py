@@ -136,6 +136,37 @@ def test_backprojection_nan_inf_rows(tj, oracle):
         assert same(P.get_volume(), ref)
 
 
+@pytest.mark.parametrize("tuning", [{}, dict(bp_kernel=3, bp_rows=2, bp_zchunk=16, bp_group=2)])
+def test_backprojection_large_values(tj, oracle, tuning):
+    """Huge finite values follow the oracle, including sums that overflow to
+    infinity and are reset to 0 by the scrub after each addition."""
+    W, H, D = 300, 9, 40
+    rng = np.random.default_rng(8)
+    E = oracle.tilt_eulers([-30.0, 0.0, 10.0, 45.0])
+    exps = rng.standard_normal((4, H, W)).astype(np.float32)
+    exps[0, 2, :] = np.float32(2.0 ** 100)      # largest tame magnitude
+    exps[1, 5, 10:40] = np.float32(2.0 ** 101)  # beyond it
+    exps[2, 6, :] = np.float32(3.0e38)          # overflows the sums it reaches
+    exps[3, 6, :] = np.float32(3.0e38)
+    vol = rng.standard_normal((D, H, W)).astype(np.float32)
+    vol[:, 1, :] = np.float32(2.0 ** 125)       # largest small-block magnitude
+    vol[7, 3, 200] = np.float32(2.0 ** 126)     # beyond it
+    vol[:, 6, 0:150] = np.float32(3.0e38)
+    with make(tj, (W, H, D)) as P:
+        if tuning:
+            P.set_tuning(**tuning)
+        P.load_tilt_series(exps, E)
+        P.set_volume(vol)
+        for k in range(4):
+            P.addProjection(k)
+        P.backproject()
+        got = P.get_volume()
+        ref = vol.copy()
+        oracle.backproject(ogeom(oracle, P), ref, exps, E)
+        assert same(got, ref), int((got != ref).sum())
+        assert (ref[:, 6, :] == 0).any()  # the overflow case was reached
+
+
 def test_forward_projection_nonfinite_volume(tj, oracle):
     """Planes holding NaN/Inf make the forward projection take the full 4-tap path."""
     W, H, D = 20, 8, 9
@@ -333,6 +364,8 @@ TUNINGS = [
     dict(fp_kernel=3, fp_rows=2, fp_rays=16, fp_zchunk=0, bp_kernel=3, bp_rows=2, bp_zchunk=8, bp_group=4),
     dict(fp_kernel=3, fp_rows=4, fp_rays=8, fp_zchunk=5, bp_kernel=3, bp_rows=4, bp_zchunk=8, bp_group=3),
     dict(fp_kernel=3, fp_rows=2, fp_rays=8, fp_zchunk=7, bp_kernel=3, bp_rows=8, bp_zchunk=4, bp_group=1),
+    dict(fp_kernel=3, fp_rows=1, fp_rays=2, fp_zchunk=4, bp_kernel=3, bp_rows=2, bp_zchunk=16, bp_group=4),
+    dict(fp_kernel=3, fp_rows=1, fp_rays=4, fp_zchunk=0, bp_kernel=3, bp_rows=4, bp_zchunk=4, bp_group=2),
     dict(fp_kernel=1, bp_kernel=1),
 ]
 
