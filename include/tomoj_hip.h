@@ -39,11 +39,15 @@ extern "C" {
 #define TJ_EUNSUP -5     /* geometry or option not supported                     */
 #define TJ_ECANCELLED -6 /* tj_interrupt() was called during the loop             */
 
-/* Storage type of the resident volume and projections.  Arithmetic is FP32
- * (positions FP64, residual norms FP64) for every storage type. */
+/* Storage type of the resident volume and projections.  The projector context
+ * (tj_open) stores FP32 only.  The job engine (tj_batch_*) stores any of them:
+ * FP32, BF16 and FP16 storage compute in FP32, FP64 storage computes in FP64
+ * (the reference twin used to measure the others); residual norms are always
+ * reduced in FP64. */
 #define TJ_STORAGE_F32 0
 #define TJ_STORAGE_BF16 1
 #define TJ_STORAGE_F16 2
+#define TJ_STORAGE_F64 3
 
 typedef struct tj_ctx tj_ctx;
 
@@ -254,6 +258,94 @@ typedef struct {
 } tj_tuning;
 int tj_set_tuning(tj_ctx *ctx, const tj_tuning *t);
 int tj_get_tuning(tj_ctx *ctx, tj_tuning *t);
+
+/* ---------------------------------------------------------------------------
+ * Job engine: precision, batched and serving forms.  No TomoJ counterpart
+ * (TomoJ reconstructs one tilt series per ReconstructionThreadManager).  A
+ * tj_batch holds any number of independent SIRT jobs resident on one device
+ * within a fixed HBM budget; tj_batch_run advances a set of them together:
+ * per iteration one persistent forward-projection launch per storage type
+ * (with the residual fused: TomoJ's projectAndDifference,
+ * VoxelProjector3D.java:387-433), one FP64 error-reduction launch and one
+ * persistent backprojection launch per storage type and positivity setting
+ * (VoxelProjector3D.java:155-308), each walking a table of tiles over all the
+ * jobs.  Jobs are tilt series about the vertical axis with TomoJ's default
+ * centres (TiltSeries.java:161-170); SIRT is osartIteration with update = N
+ * (TomoReconstruction2.java:636-733, factor (float)(lambda/N)).  Positions are
+ * closed-form in the arithmetic type instead of Java's incremental doubles, so
+ * results match the FP64 oracle to a stated tolerance rather than bit for bit.
+ * A job's result does not depend on which other jobs share its launches.
+ * Failures are reported like every other call (status code, tj_last_error).
+ * ------------------------------------------------------------------------- */
+typedef struct tj_batch tj_batch;
+
+typedef struct {
+    int vol_w, vol_h, vol_d;   /* reconstruction size (width <= 4096)              */
+    int proj_w, proj_h;        /* detector size; proj_h <= vol_h, difference even   */
+    int n_proj;                /* projections                                       */
+    int storage;               /* TJ_STORAGE_*                                      */
+    int positivity;            /* clamp at 0 after each image (VoxelProjector3D:299) */
+    double relaxation;         /* lambda                                            */
+    int max_iterations;        /* length of the error history kept on the device    */
+    int fp32_master;           /* BF16/FP16 storage: keep the SIRT iterate in an FP32
+                                  master volume; the 16-bit copy feeds the forward
+                                  projection (half its traffic), the backprojection
+                                  updates both.  Ignored for FP32/FP64 storage.      */
+    int fp32_projections;      /* BF16/FP16 storage: keep the measured projections in
+                                  FP32 (they are read once per iteration).           */
+    int reserved;
+} tj_job_desc;
+
+/* Tile and slab shapes of the job kernels (picked by tj_batch_autotune). */
+typedef struct {
+    int fp_rays;        /* projections per forward tile: 2 or 4                     */
+    int fp_lds_kb;      /* LDS per forward block in KiB (16..128): slices per slab   */
+    int bp_rows;        /* volume rows per back tile: 2 or 4                         */
+    int bp_zchunk;      /* slices per back tile: 4, 8 or 16 (16 with 2 rows)         */
+    int bp_group;       /* images per LDS pass (derived from the shape)              */
+    int blocks_per_cu;  /* persistent blocks per compute unit: 1..16                 */
+} tj_batch_tuning;
+
+#define TJ_RUN_GRAPH 1  /* tj_batch_run: capture the iterations in one HIP graph */
+
+/* Device bytes a job of this description takes (0 if the description is invalid). */
+size_t tj_job_bytes(const tj_job_desc *d);
+int tj_batch_open(int device, size_t hbm_budget_bytes, tj_batch **out);
+int tj_batch_close(tj_batch *b);
+size_t tj_batch_used(const tj_batch *b);
+size_t tj_batch_budget(const tj_batch *b);
+/* Admits a job (TJ_ENOMEM when it does not fit the budget); euler is n_proj x 9
+ * (AffineAlignment.getEulerMatrix per tilt).  Volume and projections start at 0. */
+int tj_batch_add_job(tj_batch *b, const tj_job_desc *d, const double *euler, int *job);
+int tj_batch_remove_job(tj_batch *b, int job);
+/* Measured projections float[n][ph][pw], converted to the job's storage. */
+int tj_batch_set_projections(tj_batch *b, int job, const float *stack);
+/* Analytic projections of an ellipsoid phantom plus noise (as tj_synth_projections). */
+int tj_batch_synth_job(tj_batch *b, int job, const double *ell, int k, double noise_rel, uint64_t seed);
+int tj_batch_get_projection(tj_batch *b, int job, int index, float *out);
+/* Difference image `index` of the job's last forward pass, widened to double. */
+int tj_batch_get_difference(tj_batch *b, int job, int index, double *out);
+/* Volume in TomoJ's ImageStack layout float[D][H][W]. */
+int tj_batch_set_volume(tj_batch *b, int job, const float *zyx);
+int tj_batch_get_volume(tj_batch *b, int job, float *zyx);
+int tj_batch_get_volume_f64(tj_batch *b, int job, double *zyx);  /* FP64 jobs only */
+int tj_batch_reset_volume(tj_batch *b, int job);
+/* Changes a job's storage type in place (e.g. BF16 -> FP32 before refinement);
+ * the volume is taken from the FP32 master when there is one.  Projections
+ * are converted from their stored values: supply the FP32 originals again
+ * (tj_batch_set_projections / tj_batch_synth_job) to refine against them. */
+int tj_batch_convert(tj_batch *b, int job, int storage);
+/* `iterations` SIRT iterations of every listed job; synchronous. */
+int tj_batch_run(tj_batch *b, const int *jobs, int njobs, int iterations, int flags);
+double tj_batch_last_run_ms(const tj_batch *b);
+/* Returns the number of iterations the job has done and copies the sums of
+ * squared differences of the last min(that, cap, max_iterations) of them. */
+int tj_batch_errors(tj_batch *b, int job, double *err, int cap);
+int tj_batch_set_tuning(tj_batch *b, const tj_batch_tuning *t);
+int tj_batch_get_tuning(const tj_batch *b, tj_batch_tuning *t);
+/* Times candidate tile/slab shapes on the listed jobs (volumes are restored)
+ * and keeps the fastest; returns it and its forward + back milliseconds. */
+int tj_batch_autotune(tj_batch *b, const int *jobs, int njobs, tj_batch_tuning *best, double *best_ms);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,10 @@ TJ_ECANCELLED = -6
 TJ_STORAGE_F32 = 0
 TJ_STORAGE_BF16 = 1
 TJ_STORAGE_F16 = 2
+TJ_STORAGE_F64 = 3
+STORAGE_NAMES = {"f32": TJ_STORAGE_F32, "bf16": TJ_STORAGE_BF16, "f16": TJ_STORAGE_F16, "f64": TJ_STORAGE_F64}
+
+TJ_RUN_GRAPH = 1
 
 
 class TjError(RuntimeError):
@@ -53,6 +57,19 @@ class tj_timing(C.Structure):
 class tj_tuning(C.Structure):
     _fields_ = [("fp_kernel", C.c_int), ("fp_rows", C.c_int), ("fp_rays", C.c_int), ("fp_zchunk", C.c_int),
                 ("bp_kernel", C.c_int), ("bp_rows", C.c_int), ("bp_zchunk", C.c_int), ("bp_group", C.c_int)]
+
+
+class tj_job_desc(C.Structure):
+    _fields_ = [("vol_w", C.c_int), ("vol_h", C.c_int), ("vol_d", C.c_int),
+                ("proj_w", C.c_int), ("proj_h", C.c_int), ("n_proj", C.c_int),
+                ("storage", C.c_int), ("positivity", C.c_int), ("relaxation", C.c_double),
+                ("max_iterations", C.c_int), ("fp32_master", C.c_int),
+                ("fp32_projections", C.c_int), ("reserved", C.c_int)]
+
+
+class tj_batch_tuning(C.Structure):
+    _fields_ = [("fp_rays", C.c_int), ("fp_lds_kb", C.c_int), ("bp_rows", C.c_int),
+                ("bp_zchunk", C.c_int), ("bp_group", C.c_int), ("blocks_per_cu", C.c_int)]
 
 
 # name -> (restype, argtypes); every function returning int is status-checked by call()
@@ -96,6 +113,28 @@ _SIGS = {
     "tj_enable_timing": (C.c_int, [_P, C.c_int]),
     "tj_set_tuning": (C.c_int, [_P, C.POINTER(tj_tuning)]),
     "tj_get_tuning": (C.c_int, [_P, C.POINTER(tj_tuning)]),
+    "tj_job_bytes": (C.c_size_t, [C.POINTER(tj_job_desc)]),
+    "tj_batch_open": (C.c_int, [C.c_int, C.c_size_t, C.POINTER(_P)]),
+    "tj_batch_close": (C.c_int, [_P]),
+    "tj_batch_used": (C.c_size_t, [_P]),
+    "tj_batch_budget": (C.c_size_t, [_P]),
+    "tj_batch_add_job": (C.c_int, [_P, C.POINTER(tj_job_desc), _P, C.POINTER(C.c_int)]),
+    "tj_batch_remove_job": (C.c_int, [_P, C.c_int]),
+    "tj_batch_set_projections": (C.c_int, [_P, C.c_int, _P]),
+    "tj_batch_synth_job": (C.c_int, [_P, C.c_int, _P, C.c_int, C.c_double, C.c_uint64]),
+    "tj_batch_get_projection": (C.c_int, [_P, C.c_int, C.c_int, _P]),
+    "tj_batch_get_difference": (C.c_int, [_P, C.c_int, C.c_int, _P]),
+    "tj_batch_set_volume": (C.c_int, [_P, C.c_int, _P]),
+    "tj_batch_get_volume": (C.c_int, [_P, C.c_int, _P]),
+    "tj_batch_get_volume_f64": (C.c_int, [_P, C.c_int, _P]),
+    "tj_batch_reset_volume": (C.c_int, [_P, C.c_int]),
+    "tj_batch_convert": (C.c_int, [_P, C.c_int, C.c_int]),
+    "tj_batch_run": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int]),
+    "tj_batch_last_run_ms": (C.c_double, [_P]),
+    "tj_batch_errors": (C.c_int, [_P, C.c_int, _P, C.c_int]),
+    "tj_batch_set_tuning": (C.c_int, [_P, C.POINTER(tj_batch_tuning)]),
+    "tj_batch_get_tuning": (C.c_int, [_P, C.POINTER(tj_batch_tuning)]),
+    "tj_batch_autotune": (C.c_int, [_P, _P, C.c_int, C.POINTER(tj_batch_tuning), C.POINTER(C.c_double)]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -56,6 +56,9 @@ static int fail(int code, const char* fmt, ...)
     return code;
 }
 
+// The job engine (tj_batch.hip) reports its failures through tj_last_error too.
+void tj_record_error(const char* msg) { g_last_error = msg; }
+
 #define TJ_HIP(call)                                                                               \
     do {                                                                                           \
         hipError_t e_ = (call);                                                                    \
@@ -370,11 +373,10 @@ static int do_backproject(tj_ctx* c, const int* slots, const std::vector<int>& h
 
 // TomoReconstruction2.orderShuffleART (:735-781) with the random first pick
 // replaced by `first`: each next image is the remaining one whose row 2 of the
-// Euler matrix has the smallest |dot product| with the previous one's.
-static void order_shuffle(const tj_ctx* c, int first, std::vector<int>& order)
+// Euler matrix (euler, n x 9) has the smallest |dot product| with the previous
+// one's.
+void tj_order_shuffle(const double* euler, int n, int first, int* order)
 {
-    const int n = c->nproj;
-    order.assign(n, 0);
     std::vector<char> todo(n, 1);
     int cc = 0, im = 0;
     while (cc < n) {
@@ -383,10 +385,10 @@ static void order_shuffle(const tj_ctx* c, int first, std::vector<int>& order)
         } else {
             double mini = DBL_MAX;
             int omini = -1;
-            const double* a = c->hpp[im].e + 6;
+            const double* a = euler + 9 * (size_t)im + 6;
             for (int o = 0; o < n; ++o) {
                 if (!todo[o]) continue;
-                const double* b = c->hpp[o].e + 6;
+                const double* b = euler + 9 * (size_t)o + 6;
                 double sum = 0;
                 sum += a[0] * b[0];
                 sum += a[1] * b[1];
@@ -405,6 +407,16 @@ static void order_shuffle(const tj_ctx* c, int first, std::vector<int>& order)
             ++cc;
         }
     }
+}
+
+static void order_shuffle(const tj_ctx* c, int first, std::vector<int>& order)
+{
+    const int n = c->nproj;
+    std::vector<double> e((size_t)9 * n);
+    for (int p = 0; p < n; ++p)
+        for (int k = 0; k < 9; ++k) e[(size_t)9 * p + k] = c->hpp[p].e[k];
+    order.assign(n, 0);
+    if (n > 0) tj_order_shuffle(e.data(), n, first, order.data());
 }
 
 extern "C" {

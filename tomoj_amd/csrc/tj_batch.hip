@@ -1,0 +1,1630 @@
+// tj_batch.hip -- the job engine: many independent SIRT reconstructions of
+// tilt series around a vertical axis, resident on one GPU and advanced by
+// persistent kernels that walk one table of tiles over all the jobs of a run.
+//
+// The arithmetic is TomoJ's algorithm (VoxelProjector3D.projectJosephPartial,
+// difference and backProjectPartial; TomoReconstruction2.osartIteration with
+// update = N), evaluated with closed-form positions instead of Java's
+// incremental double sums:
+//   forward  x(i, z) = fma(z, tan, base_i)   base_i = i/cos + b   (x of slice z)
+//   back     xx(i, k) = fma(k, e02, xe_i)    xe_i = i*e00 + b2
+// in the storage's arithmetic type A (float for FP32/BF16/FP16 storage, double
+// for FP64 storage).  Sampling rules are TomoJ's: getPixel's nearest value on
+// the last column, the last row and the last slice (TomoReconstruction2.java:154-190), the
+// entry/exit norm of the ray (VoxelProjector3D.java:756-805), ImageJ's bilinear
+// value with the last column clamped in the backprojection, positivity after
+// each image.  Deviations, all of which vanish for finite data: the NaN/Inf
+// scrub of the backprojection runs once per voxel after all images instead of
+// after each image, and the ray norm is count*normIncrement rounded once.
+//
+// Layouts per job: volume T[y][z][x], measured projections T[n][ph][pw],
+// differences A[n][ph][pw], error partials double[n][ph][column tiles].
+#include "tj_device.h"
+#include "../../include/tomoj_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace tjk {
+hipError_t phantom_project(hipStream_t, const double*, int, const TjProj*, int, int, int, double, double, double*);
+hipError_t add_noise(hipStream_t, const double*, size_t, uint64_t, double, double*, int, float*);
+}  // namespace tjk
+
+namespace tjb {
+
+constexpr int NT = 256;  // threads per block: one column (forward) or voxel column (back) each
+constexpr int PRM = 10;  // doubles of per-projection constants
+
+struct Bf16 {
+    uint16_t u;
+};
+
+// ---------------------------------------------------------------------------
+// storage conversions (round to nearest even on store)
+// ---------------------------------------------------------------------------
+template <typename A>
+__device__ __forceinline__ A ld(const float* p, size_t i) { return (A)p[i]; }
+template <typename A>
+__device__ __forceinline__ A ld(const double* p, size_t i) { return (A)p[i]; }
+template <typename A>
+__device__ __forceinline__ A ld(const _Float16* p, size_t i) { return (A)(float)p[i]; }
+template <typename A>
+__device__ __forceinline__ A ld(const Bf16* p, size_t i) { return (A)__uint_as_float((uint32_t)p[i].u << 16); }
+
+__device__ __forceinline__ uint16_t f32_to_bf16(float f)
+{
+    uint32_t b = __float_as_uint(f);
+    if ((b & 0x7fffffffu) > 0x7f800000u) return (uint16_t)(((b >> 16) & 0x8000u) | 0x7fc0u);  // NaN
+    b += 0x7fffu + ((b >> 16) & 1u);
+    return (uint16_t)(b >> 16);
+}
+__device__ __forceinline__ void st(float* p, size_t i, float v) { p[i] = v; }
+__device__ __forceinline__ void st(double* p, size_t i, double v) { p[i] = v; }
+__device__ __forceinline__ void st(_Float16* p, size_t i, float v) { p[i] = (_Float16)v; }
+__device__ __forceinline__ void st(Bf16* p, size_t i, float v) { p[i].u = f32_to_bf16(v); }
+
+template <typename A> struct V4;
+template <> struct V4<float> { using t = float4; };
+template <> struct V4<double> { using t = double4; };
+template <typename A> struct V2;
+template <> struct V2<float> { using t = float2; };
+template <> struct V2<double> { using t = double2; };
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Device descriptor of one job (host fills it; see tj_batch_add_job).
+struct TjJob {
+    void* vol;
+    const void* exps;
+    void* diff;
+    double* err_part;
+    const double* prm;   // [n][PRM]: a, b, tan, normInc, e00, e02, b2, 1/tan, 1/e02
+    float* master;       // FP32 master volume [y][z][x] when the stored volume is 16-bit (or null)
+    const int* order;    // [n] order in which the backprojection adds the images
+    double* err_hist;    // [hist_cap] sum of diff^2 per iteration
+    int* iter;           // iterations done (device counter)
+    int W, H, D, pw, ph, n;
+    int dy;              // forward: volume row of detector row j is j + dy
+    int dyb;             // back: image row of volume row j is j + dyb
+    int nct;             // forward column tiles
+    int hist_cap;
+    double factor;       // (double)(float)(lambda / n)
+};
+
+// ---------------------------------------------------------------------------
+// Forward projection fused with the residual.  Tile (job, first projection,
+// block of 4 detector rows, 256-column tile): the 4 volume rows those detector
+// rows see are staged in LDS one slab of zc slices at a time, one A4 cell per
+// (z, x) plus a replicated last column and a zero cell; each thread follows
+// the rays of its column for G projections.
+// ---------------------------------------------------------------------------
+// Integers z in [0, n) where the line x(z) = fma(z, t, b), evaluated in double,
+// satisfies lo <= x(z) < hi: returned as [zlo, zhi] (empty when zhi < zlo).
+// x is monotone in z, so the set is an interval; its ends are estimated from
+// the crossings and then settled by evaluating the predicate itself, so every
+// arithmetic type decides inside/outside exactly as the FP64 evaluation does.
+__device__ inline void line_interval(double b, double t, double inv_t, double lo, double hi, int n, int& zlo,
+                                     int& zhi)
+{
+    auto inside = [&](int z) {
+        const double x = fma((double)z, t, b);
+        return x >= lo && x < hi;
+    };
+    int a, c;
+    if (t == 0.0) {
+        a = 0;
+        c = inside(0) ? n - 1 : -1;
+    } else {
+        const double za = (lo - b) * inv_t, zb = (hi - b) * inv_t;
+        const double zmin = fmin(za, zb), zmax = fmax(za, zb);
+        a = (int)fmin(fmax(ceil(zmin), 0.0), (double)(n - 1));
+        c = (int)fmin(fmax(floor(zmax), 0.0), (double)(n - 1));
+        for (int s = 0; s < 3 && a > 0 && inside(a - 1); ++s) --a;
+        for (int s = 0; s < 3 && a <= c && !inside(a); ++s) ++a;
+        for (int s = 0; s < 3 && c < n - 1 && inside(c + 1); ++s) ++c;
+        for (int s = 0; s < 3 && c >= a && !inside(c); ++s) --c;
+    }
+    zlo = a;
+    zhi = c;
+}
+
+// Adds the sample of slice z of ray k (LDS cell row zo) at position xs
+// (clamped to [0, W-1]) with x fraction xf to the 4 row sums.
+template <typename A, int G, bool NEARY>
+__device__ __forceinline__ void fp_add(const typename V4<A>::t* cell, int a, A xf, const A (&ym)[4],
+                                       A (&rs)[G][4], int k)
+{
+    using C4 = typename V4<A>::t;
+    const C4 q0 = cell[a];
+    const C4 q1 = cell[a + 1];
+    if (NEARY) {
+        rs[k][0] += fma(xf * ym[0], q1.x - q0.x, q0.x);
+        rs[k][1] += fma(xf * ym[1], q1.y - q0.y, q0.y);
+        rs[k][2] += fma(xf * ym[2], q1.z - q0.z, q0.z);
+        rs[k][3] += fma(xf * ym[3], q1.w - q0.w, q0.w);
+    } else {
+        rs[k][0] += fma(xf, q1.x - q0.x, q0.x);
+        rs[k][1] += fma(xf, q1.y - q0.y, q0.y);
+        rs[k][2] += fma(xf, q1.z - q0.z, q0.z);
+        rs[k][3] += fma(xf, q1.w - q0.w, q0.w);
+    }
+}
+
+// One LDS slab of the forward projection: zn slices for the G rays of the
+// thread.  Interpolated samples take their position in the arithmetic type A
+// (the bilinear value is continuous in x, so rounding moves it by a rounding
+// error).  getPixel's nearest-value samples -- the last slice (zlast) and, with
+// NEARY, the last volume row (ym = 0) -- are discontinuous in x, so there the
+// column comes from the FP64 position, as TomoJ computes it.
+template <typename A, int G, bool NEARY>
+__device__ __forceinline__ void fp_slab(const typename V4<A>::t* cell, int zn, int zlast, int z0, int W1, int zero,
+                                        A wm1, const A (&ym)[4], const A (&base)[G], const A (&tn)[G],
+                                        const double (&bd)[G], const double (&td)[G], const int (&zlo)[G],
+                                        const int (&zhi)[G], A (&rs)[G][4])
+{
+    const double wd = (double)wm1;
+    for (int zz = 0; zz < zn; ++zz) {
+        const int z = z0 + zz;
+        const int zo = zz * W1;
+        if (NEARY || zz == zlast) {
+            const A zl = (zz == zlast) ? (A)0 : (A)1;
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                const double xv = fma((double)z, td[k], bd[k]);
+                const bool in = (z >= zlo[k]) && (z <= zhi[k]);
+                const double xs = fmin(fmax(xv, 0.0), wd);
+                const int xb = (int)xs;
+                const A xf = (A)(xs - (double)xb) * zl;
+                fp_add<A, G, NEARY>(cell, in ? zo + xb : zero, xf, ym, rs, k);
+            }
+        } else {
+            const A zf = (A)z;
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                const A xv = fma(zf, tn[k], base[k]);
+                const bool in = (z >= zlo[k]) && (z <= zhi[k]);
+                const A xs = fmin(fmax(xv, (A)0), wm1);
+                const int xb = (int)xs;
+                const A xf = xs - (A)xb;
+                fp_add<A, G, false>(cell, in ? zo + xb : zero, xf, ym, rs, k);
+            }
+        }
+    }
+}
+
+template <typename T, typename TE, typename A, int G>
+__global__ __launch_bounds__(NT) void k_job_fp(const TjJob* __restrict__ jobs, const int4* __restrict__ tiles,
+                                               int ntiles, int lds_cells)
+{
+    using C4 = typename V4<A>::t;
+    extern __shared__ __align__(16) unsigned char smem[];
+    C4* cell = reinterpret_cast<C4*>(smem);
+    __shared__ double red[NT / 64][G * 4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int4 tl = tiles[t];
+        if (tl.x < 0) continue;  // padding entry (uniform in the block)
+        const TjJob& J = jobs[tl.x];
+        const T* vol = (const T*)J.vol;
+        const int W = J.W, D = J.D, pw = J.pw, ph = J.ph, n = J.n;
+        const int W1 = W + 1;
+        const int zc = min(D, (lds_cells - 2) / W1);
+        const int zero = zc * W1;
+        const int p0 = tl.y, j0 = tl.z * 4, ct = tl.w;
+        const int i = ct * NT + tid;
+        const bool col = i < pw;
+        const double ic = col ? i : pw - 1;
+        const size_t plane = (size_t)D * W;
+        bool rin[4];
+        const T* rp[4];
+        A ym[4];  // 0 on the last volume row: getPixel takes the nearest value there
+        bool neary = false;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int y = j0 + r + J.dy;
+            rin[r] = (j0 + r < ph) && y >= 0 && y < J.H;
+            rp[r] = vol + (size_t)(rin[r] ? y : 0) * plane;
+            ym[r] = (y == J.H - 1) ? (A)0 : (A)1;
+            neary = neary || (y == J.H - 1);
+        }
+        A base[G], tn[G], rs[G][4];
+        double bd[G], td[G];
+        int zlo[G], zhi[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const double* q = J.prm + PRM * min(p0 + k, n - 1);
+            bd[k] = ic * q[0] + q[1];
+            td[k] = q[2];
+            base[k] = (A)bd[k];
+            tn[k] = (A)td[k];
+            // inside test of VoxelProjector3D.java:761 (x + ACCURACY >= 0, x < W)
+            line_interval(bd[k], td[k], q[7], -TJ_ACCURACY, (double)W, D, zlo[k], zhi[k]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) rs[k][r] = (A)0;
+        }
+        const A wm1 = (A)(W - 1);
+        for (int z0 = 0; z0 < D; z0 += zc) {
+            const int zn = min(zc, D - z0);
+            __syncthreads();
+            if (tid < 2) {
+                C4 zc4;
+                zc4.x = zc4.y = zc4.z = zc4.w = (A)0;
+                cell[zero + tid] = zc4;
+            }
+            for (int zz = wave; zz < zn; zz += NT / 64) {
+                const size_t zo = (size_t)(z0 + zz) * W;
+                C4* dst = cell + zz * W1;
+                for (int xx = lane; xx < W1; xx += 64) {
+                    const size_t o = zo + min(xx, W - 1);
+                    C4 v;
+                    v.x = rin[0] ? ld<A>(rp[0], o) : (A)0;
+                    v.y = rin[1] ? ld<A>(rp[1], o) : (A)0;
+                    v.z = rin[2] ? ld<A>(rp[2], o) : (A)0;
+                    v.w = rin[3] ? ld<A>(rp[3], o) : (A)0;
+                    dst[xx] = v;
+                }
+            }
+            __syncthreads();
+            const int zlast = (z0 + zn == D) ? zn - 1 : zn;
+            if (neary)
+                fp_slab<A, G, true>(cell, zn, zlast, z0, W1, zero, wm1, ym, base, tn, bd, td, zlo, zhi, rs);
+            else
+                fp_slab<A, G, false>(cell, zn, zlast, z0, W1, zero, wm1, ym, base, tn, bd, td, zlo, zhi, rs);
+        }
+        // projection, norm and difference per (projection, row); error partials
+        double e2[G][4];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int p = p0 + k;
+            const double* q = J.prm + PRM * min(p, n - 1);
+            const double ninc = q[3];
+            // the in-volume slices of a ray are consecutive: once it has entered,
+            // it leaves (one more normIncrement) unless the last slice is inside
+            const int cnt = zhi[k] - zlo[k] + 1;
+            const double nrm = (cnt > 0) ? (double)(cnt + (zhi[k] == D - 1 ? 0 : 1)) * ninc : 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int j = j0 + r;
+                double v2 = 0.0;
+                if (p < n && col && j < ph) {
+                    const size_t o = ((size_t)p * ph + j) * pw + i;
+                    const A norm = rin[r] ? (A)nrm : (A)0;
+                    const A th = (norm > (A)0) ? (A)((double)rs[k][r] * ninc) : (A)NAN;
+                    const A ex = ld<A>((const TE*)J.exps, o);
+                    A d = (A)0;
+                    if (!isnan(ex) && !isnan(th)) {
+                        const double v = (double)((ex - th) / norm) * J.factor;
+                        d = (A)v;
+                        v2 = v * v;
+                    }
+                    ((A*)J.diff)[o] = d;
+                }
+                e2[k][r] = wave_sum(v2);
+            }
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < G; ++k)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) red[wave][k * 4 + r] = e2[k][r];
+        }
+        __syncthreads();
+        if (tid < G * 4) {
+            const int p = p0 + tid / 4, j = j0 + (tid & 3);
+            if (p < n && j < ph) {
+                double s = red[0][tid];
+#pragma unroll
+                for (int w = 1; w < NT / 64; ++w) s += red[w][tid];
+                J.err_part[((size_t)p * ph + j) * J.nct + ct] = s;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Backprojection of all N differences of a job (SIRT), added to each voxel in
+// the job's projection order (orderShuffleART), as osartIteration adds them.  Tile (job, 256-column
+// tile, R volume rows, KC slices): each thread keeps the KC*R sums of its voxel
+// column in registers; per group of G images the rows those voxels reach are
+// staged in LDS as pairs (ll, lr - ll) over the reachable detector columns.
+// ---------------------------------------------------------------------------
+template <typename T, typename A, int R, int KC, int G, int MODE>
+__global__ __launch_bounds__(NT) void k_job_bp(const TjJob* __restrict__ jobs, const int4* __restrict__ tiles,
+                                               int ntiles)
+{
+    using C2 = typename V2<A>::t;
+    constexpr bool POS = (MODE & 1) != 0;     // positivity after each image
+    constexpr bool MASTER = (MODE & 2) != 0;  // FP32 master volume beside the stored one
+    constexpr int XW = NT + KC + 8;  // reachable columns + a zero cell at XW-1
+    __shared__ C2 lds[G][R][XW];
+    __shared__ double s_c[G], s_ms[G], s_b2[G], s_ims[G];
+    __shared__ int s_x0[G], s_img[G];
+    __shared__ int s_row[G][R];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int4 tl = tiles[t];
+        if (tl.x < 0) continue;
+        const TjJob& J = jobs[tl.x];
+        T* vol = (T*)J.vol;
+        const A* diff = (const A*)J.diff;
+        const int W = J.W, D = J.D, pw = J.pw, ph = J.ph, n = J.n;
+        const int i0 = tl.y * NT, i = i0 + tid;
+        const bool active = i < W;
+        const int ilast = min(i0 + NT - 1, W - 1);
+        const double ie = active ? i : ilast;
+        const int j0 = tl.z * R, nr = min(R, J.H - j0);
+        const int k0 = tl.w * KC, nk = min(KC, D - k0);
+        const size_t plane = (size_t)D * W;
+        A acc[KC][R];
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                acc[kk][r] = (active && kk < nk && r < nr)
+                                 ? (MASTER ? (A)J.master[(size_t)(j0 + r) * plane + (size_t)(k0 + kk) * W + i]
+                                           : ld<A>(vol, (size_t)(j0 + r) * plane + (size_t)(k0 + kk) * W + i))
+                                 : (A)0;
+        const A pwm1 = (A)(pw - 1);
+        for (int p0 = 0; p0 < n; p0 += G) {
+            const int gn = min(G, n - p0);
+            __syncthreads();
+            if (tid < gn) {
+                const int pj = J.order[p0 + tid];
+                s_img[tid] = pj;
+                const double* q = J.prm + PRM * pj;
+                const double c = q[4], ms = q[5], b2 = q[6];
+                // smallest xx of the tile (xx is linear in i and k: a corner);
+                // one column of margin for the rounding of the arithmetic type
+                const double ka = k0, kb = k0 + KC - 1;
+                const double m = fmin(fmin((double)i0 * c + ka * ms, (double)i0 * c + kb * ms),
+                                      fmin((double)ilast * c + ka * ms, (double)ilast * c + kb * ms)) + b2;
+                s_x0[tid] = max(0, (int)floor(m) - 1);
+                s_c[tid] = c;
+                s_ms[tid] = ms;
+                s_b2[tid] = b2;
+                s_ims[tid] = q[8];
+            }
+            if (tid < G * R) {
+                const int g = tid / R, r = tid % R;
+                const int jj = j0 + r + J.dyb;
+                s_row[g][r] = (g < gn && r < nr && jj >= 0 && jj < ph) ? jj : -1;
+            }
+            __syncthreads();
+            for (int gr = wave; gr < gn * R; gr += NT / 64) {
+                const int g = gr / R, r = gr - (gr / R) * R;
+                const int jj = s_row[g][r], x0 = s_x0[g];
+                const A* row = diff + ((size_t)s_img[g] * ph + max(jj, 0)) * pw;
+                C2* dst = &lds[g][r][0];
+                for (int xx = lane; xx < XW; xx += 64) {
+                    const int xs = x0 + xx;
+                    C2 v;
+                    v.x = (A)0;
+                    v.y = (A)0;
+                    if (jj >= 0 && xs < pw && xx < XW - 1) {
+                        const A ll = row[xs];
+                        const A lr = row[min(xs + 1, pw - 1)];
+                        v.x = ll;
+                        v.y = lr - ll;
+                    }
+                    dst[xx] = v;
+                }
+            }
+            __syncthreads();
+            for (int g = 0; g < gn; ++g) {
+                const A ms = (A)s_ms[g];
+                const double xed = ie * s_c[g] + s_b2[g];
+                const A xe = (A)xed;
+                const int x0 = s_x0[g];
+                // slices whose xx lies in [0, pw) (VoxelProjector3D.java:285)
+                int klo, khi;
+                line_interval(xed, s_ms[g], s_ims[g], 0.0, (double)pw, D, klo, khi);
+#pragma unroll
+                for (int kk = 0; kk < KC; ++kk) {
+                    const A xx = fma((A)(k0 + kk), ms, xe);
+                    const bool vx = (k0 + kk >= klo) && (k0 + kk <= khi);
+                    const A xs = vx ? fmin(fmax(xx, (A)0), pwm1) : (A)0;
+                    const int xb = (int)xs;
+                    const A xf = xs - (A)xb;
+                    const int li = vx ? xb - x0 : XW - 1;
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const C2 q = lds[g][r][li];
+                        A a = acc[kk][r] + fma(xf, q.y, q.x);
+                        if (POS) a = (a < (A)0) ? (A)0 : a;
+                        acc[kk][r] = a;
+                    }
+                }
+            }
+        }
+        if (active) {
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (kk < nk && r < nr) {
+                        A a = acc[kk][r];
+                        if (isnan(a) || isinf(a)) a = (A)0;
+                        const size_t o = (size_t)(j0 + r) * plane + (size_t)(k0 + kk) * W + i;
+                        if (MASTER) J.master[o] = (float)a;
+                        st(vol, o, a);
+                    }
+        }
+    }
+}
+
+// Sum of squared differences of each job of a run, in a fixed order; appended
+// to the job's error history.  One block per job.
+__global__ __launch_bounds__(NT) void k_job_err(const TjJob* __restrict__ jobs, int njobs)
+{
+    __shared__ double sh[NT / 64];
+    const TjJob& J = jobs[blockIdx.x];
+    const size_t m = (size_t)J.n * J.ph * J.nct;
+    double s = 0.0;
+    for (size_t k = threadIdx.x; k < m; k += NT) s += J.err_part[k];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = sh[0];
+        for (int w = 1; w < NT / 64; ++w) tot += sh[w];
+        const int it = *J.iter;
+        J.err_hist[it % J.hist_cap] = tot;
+        *J.iter = it + 1;
+    }
+}
+
+// Storage conversions and layout changes.
+template <typename TI, typename TO>
+__global__ void k_convert(const TI* __restrict__ src, TO* __restrict__ dst, size_t n)
+{
+    using A = typename std::conditional<std::is_same<TO, double>::value, double, float>::type;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x)
+        st(dst, k, ld<A>(src, k));
+}
+
+// host [z][y][x] float -> device [y][z][x] T, and back
+template <typename T>
+__global__ void k_zyx_to_job(const float* __restrict__ src, T* __restrict__ vol, int W, int H, int D)
+{
+    using A = typename std::conditional<std::is_same<T, double>::value, double, float>::type;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+    if (x >= W) return;
+    st(vol, ((size_t)y * D + z) * W + x, (A)src[((size_t)z * H + y) * W + x]);
+}
+template <typename T>
+__global__ void k_job_to_zyx(const T* __restrict__ vol, float* __restrict__ dst, int W, int H, int D)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, z = blockIdx.z;
+    if (x >= W) return;
+    dst[((size_t)z * H + y) * W + x] = ld<float>(vol, ((size_t)y * D + z) * W + x);
+}
+
+}  // namespace tjb
+
+// ===========================================================================
+// host side
+// ===========================================================================
+using tjb::Bf16;
+using tjb::TjJob;
+
+static int bfail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    tj_record_error(buf);
+    return code;
+}
+
+#define TB_HIP(call)                                                                                \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) return bfail(TJ_EDEVICE, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                           __FILE__, __LINE__);                                     \
+    } while (0)
+
+static size_t storage_bytes(int s)
+{
+    switch (s) {
+    case TJ_STORAGE_F32: return 4;
+    case TJ_STORAGE_BF16: return 2;
+    case TJ_STORAGE_F16: return 2;
+    case TJ_STORAGE_F64: return 8;
+    default: return 0;
+    }
+}
+static size_t arith_bytes(int s) { return s == TJ_STORAGE_F64 ? 8 : 4; }
+static bool half_storage(int s) { return s == TJ_STORAGE_BF16 || s == TJ_STORAGE_F16; }
+// storage type of a job's measured projections
+static int proj_storage(const tj_job_desc& d)
+{
+    return (half_storage(d.storage) && d.fp32_projections) ? TJ_STORAGE_F32 : d.storage;
+}
+static int fp_col_tiles(int pw) { return (pw + tjb::NT - 1) / tjb::NT; }
+
+struct BJob {
+    int id = 0;
+    tj_job_desc d{};
+    void* vol = nullptr;
+    float* master = nullptr;  // FP32 master volume (16-bit storage with fp32_master)
+    void* exps = nullptr;
+    void* diff = nullptr;
+    double* err_part = nullptr;
+    double* prm = nullptr;
+    int* order = nullptr;
+    double* err_hist = nullptr;
+    int* iter = nullptr;
+    std::vector<double> euler;  // n x 9
+    std::vector<double> hprm;   // n x PRM
+    int dy = 0, dyb = 0;
+    size_t bytes = 0;
+};
+
+struct RunPlan {
+    std::vector<int> ids;
+    int iterations = 0;
+    tj_batch_tuning tune{};
+    long epoch = -1;
+    TjJob* djobs = nullptr;
+    std::vector<int4*> dtiles;  // owned tile tables
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    // launches of one iteration: kind 0 forward, 1 error, 2 back
+    struct L {
+        int kind, storage, pos, ntiles, grid;  // pos: back mode bits (back), widest volume (forward)
+        int4* tiles;
+        int job0, njobs;  // error launch: jobs [job0, job0+njobs) of djobs
+        int pstorage;     // forward launch: storage of the measured projections
+    };
+    std::vector<L> launches;
+};
+
+struct tj_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t budget = 0, used = 0;
+    int next_id = 1;
+    long epoch = 0;
+    std::map<int, BJob*> jobs;
+    tj_batch_tuning tune{4, 64, 4, 8, 4, 4};
+    RunPlan plan;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    int ncu = 256;
+};
+
+static void free_plan(RunPlan& p)
+{
+    if (p.exec) (void)hipGraphExecDestroy(p.exec);
+    if (p.graph) (void)hipGraphDestroy(p.graph);
+    p.exec = nullptr;
+    p.graph = nullptr;
+    if (p.djobs) (void)hipFree(p.djobs);
+    p.djobs = nullptr;
+    for (int4* t : p.dtiles) (void)hipFree(t);
+    p.dtiles.clear();
+    p.launches.clear();
+    p.ids.clear();
+    p.epoch = -1;
+}
+
+static void free_job(BJob* j)
+{
+    void* ptrs[] = {j->vol, j->master, j->exps, j->diff, j->err_part, j->prm, j->order, j->err_hist, j->iter};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    delete j;
+}
+
+static size_t job_bytes(const tj_job_desc* d)
+{
+    const size_t s = storage_bytes(d->storage), a = arith_bytes(d->storage), sp = storage_bytes(proj_storage(*d));
+    const size_t nvox = (size_t)d->vol_w * d->vol_h * d->vol_d;
+    const size_t npx = (size_t)d->proj_w * d->proj_h * d->n_proj;
+    const size_t parts = (size_t)d->n_proj * d->proj_h * fp_col_tiles(d->proj_w);
+    const bool master = d->fp32_master && (d->storage == TJ_STORAGE_BF16 || d->storage == TJ_STORAGE_F16);
+    return nvox * (s + (master ? 4 : 0)) + npx * (sp + a) + parts * 8 + (size_t)d->n_proj * (tjb::PRM * 8 + 4) +
+           (size_t)std::max(d->max_iterations, 1) * 8 + 256;
+}
+
+static int check_desc(const tj_job_desc* d)
+{
+    if (!d) return bfail(TJ_EINVAL, "null job description");
+    if (d->vol_w < 2 || d->vol_h < 1 || d->vol_d < 1 || d->proj_w < 2 || d->proj_h < 1 || d->n_proj < 1)
+        return bfail(TJ_EINVAL, "job sizes must be positive (volume width and detector width >= 2)");
+    if (d->vol_w > 4096 || d->proj_w > 4096) return bfail(TJ_EUNSUP, "widths above 4096 are not supported");
+    if (!storage_bytes(d->storage)) return bfail(TJ_EINVAL, "unknown storage type %d", d->storage);
+    if (d->proj_h > d->vol_h)
+        return bfail(TJ_EUNSUP, "detector taller than the volume is not supported by the job engine");
+    if ((d->vol_h - d->proj_h) % 2 != 0)
+        return bfail(TJ_EUNSUP, "volume and detector heights must differ by an even number (integer row shift)");
+    if (d->max_iterations < 1) return bfail(TJ_EINVAL, "max_iterations must be >= 1");
+    return TJ_OK;
+}
+
+// per-projection constants (see the forward/back formulas at the top of the file)
+static int job_params(BJob* j)
+{
+    const tj_job_desc& d = j->d;
+    const double pc0 = (d.proj_w - 1) / 2.0, pc1 = (d.proj_h - 1) / 2.0;
+    const double vcx = (d.vol_w - 1) / 2.0, vcy = (d.vol_h - 1) / 2.0, vcz = (d.vol_d - 1) / 2.0;
+    j->dy = (int)(vcy - pc1);
+    j->dyb = (int)(pc1 - vcy);
+    j->hprm.assign((size_t)d.n_proj * tjb::PRM, 0.0);
+    for (int p = 0; p < d.n_proj; ++p) {
+        const double* r = &j->euler[(size_t)9 * p];
+        if (r[1] != 0.0 || r[3] != 0.0 || r[5] != 0.0 || r[7] != 0.0 || r[4] != 1.0 || r[0] == 0.0)
+            return bfail(TJ_EUNSUP, "projection %d: the job engine needs a tilt about the vertical (y) axis", p);
+        const double divx = r[0], zfx = r[2];
+        const double dirx = (0.0 - r[2]) / r[0];
+        double* q = &j->hprm[(size_t)tjb::PRM * p];
+        q[0] = 1.0 / divx;
+        q[1] = (-pc0 - zfx * (-1.0 - vcz)) / divx + dirx + vcx;
+        q[2] = dirx;
+        q[3] = std::sqrt(dirx * dirx + 1.0);
+        q[4] = r[0];
+        q[5] = r[2];
+        q[6] = pc0 - vcx * r[0] - vcz * r[2];
+        q[7] = dirx != 0.0 ? 1.0 / dirx : 0.0;
+        q[8] = r[2] != 0.0 ? 1.0 / r[2] : 0.0;
+    }
+    return TJ_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers (template dispatch on storage and tuning)
+// ---------------------------------------------------------------------------
+// Dynamic LDS of the forward kernel: the tuned size, raised to what one slice of
+// the widest job needs (W + 1 cells, the replicated column, the zero cells).
+template <typename T, typename TE, typename A, int G>
+static hipError_t launch_fp_t(hipStream_t st, const TjJob* jobs, const int4* tiles, int ntiles, int grid,
+                              int lds_bytes, int max_w)
+{
+    using C4 = typename tjb::V4<A>::t;
+    lds_bytes = std::max(lds_bytes, (max_w + 3) * (int)sizeof(C4));
+    const int cells = lds_bytes / (int)sizeof(C4);
+    tjb::k_job_fp<T, TE, A, G><<<grid, tjb::NT, lds_bytes, st>>>(jobs, tiles, ntiles, cells);
+    return hipGetLastError();
+}
+
+template <typename T, typename TE, typename A, int G>
+static hipError_t allow_lds(int bytes)
+{
+    return hipFuncSetAttribute((const void*)tjb::k_job_fp<T, TE, A, G>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+// Largest dynamic LDS any forward launch may ask for (set once per device,
+// before any graph capture).
+static const int kMaxFpLds = 144 * 1024;
+
+template <typename T, typename TE>
+static hipError_t allow_lds2()
+{
+    hipError_t e = allow_lds<T, TE, float, 2>(kMaxFpLds);
+    if (e == hipSuccess) e = allow_lds<T, TE, float, 4>(kMaxFpLds);
+    return e;
+}
+
+static hipError_t allow_fp_lds()
+{
+    hipError_t e = allow_lds2<float, float>();
+    if (e == hipSuccess) e = allow_lds2<Bf16, Bf16>();
+    if (e == hipSuccess) e = allow_lds2<Bf16, float>();
+    if (e == hipSuccess) e = allow_lds2<_Float16, _Float16>();
+    if (e == hipSuccess) e = allow_lds2<_Float16, float>();
+    if (e == hipSuccess) e = allow_lds<double, double, double, 2>(kMaxFpLds);
+    return e;
+}
+
+template <typename T, typename TE>
+static hipError_t launch_fp_s(hipStream_t st, const tj_batch_tuning& tu, const TjJob* jobs, const int4* tiles,
+                              int ntiles, int grid, int max_w)
+{
+    const int lds = std::min(std::max(tu.fp_lds_kb, 16), 128) * 1024;
+    if (tu.fp_rays <= 2) return launch_fp_t<T, TE, float, 2>(st, jobs, tiles, ntiles, grid, lds, max_w);
+    return launch_fp_t<T, TE, float, 4>(st, jobs, tiles, ntiles, grid, lds, max_w);
+}
+
+static int fp_rays_of(const tj_batch_tuning& tu, int storage)
+{
+    if (storage == TJ_STORAGE_F64) return 2;
+    return tu.fp_rays <= 2 ? 2 : 4;
+}
+
+static hipError_t launch_fp(hipStream_t st, int storage, int pstorage, const tj_batch_tuning& tu,
+                            const TjJob* jobs, const int4* tiles, int ntiles, int grid, int max_w)
+{
+    (void)hipGetLastError();
+    const bool pf = pstorage == TJ_STORAGE_F32;
+    switch (storage) {
+    case TJ_STORAGE_F32: return launch_fp_s<float, float>(st, tu, jobs, tiles, ntiles, grid, max_w);
+    case TJ_STORAGE_BF16:
+        return pf ? launch_fp_s<Bf16, float>(st, tu, jobs, tiles, ntiles, grid, max_w)
+                  : launch_fp_s<Bf16, Bf16>(st, tu, jobs, tiles, ntiles, grid, max_w);
+    case TJ_STORAGE_F16:
+        return pf ? launch_fp_s<_Float16, float>(st, tu, jobs, tiles, ntiles, grid, max_w)
+                  : launch_fp_s<_Float16, _Float16>(st, tu, jobs, tiles, ntiles, grid, max_w);
+    default: return launch_fp_t<double, double, double, 2>(st, jobs, tiles, ntiles, grid, 128 * 1024, max_w);
+    }
+}
+
+// backprojection shapes: (rows, slices, images per LDS pass)
+struct BpShape {
+    int r, kc, g;
+};
+static BpShape bp_shape_of(const tj_batch_tuning& tu, int storage)
+{
+    if (storage == TJ_STORAGE_F64) return {2, 8, 2};
+    if (tu.bp_rows <= 2) return tu.bp_zchunk >= 16 ? BpShape{2, 16, 2} : BpShape{2, 8, 4};
+    return tu.bp_zchunk <= 4 ? BpShape{4, 4, 4} : BpShape{4, 8, 4};
+}
+
+template <typename T, int MODE>
+static hipError_t launch_bp_s(hipStream_t st, const BpShape& s, const TjJob* jobs, const int4* tiles, int ntiles,
+                              int grid)
+{
+    if (s.r == 2 && s.kc == 16) tjb::k_job_bp<T, float, 2, 16, 2, MODE><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
+    else if (s.r == 2) tjb::k_job_bp<T, float, 2, 8, 4, MODE><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
+    else if (s.kc == 4) tjb::k_job_bp<T, float, 4, 4, 4, MODE><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
+    else tjb::k_job_bp<T, float, 4, 8, 4, MODE><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_bp_m(hipStream_t st, int mode, const BpShape& s, const TjJob* jobs, const int4* tiles,
+                              int ntiles, int grid)
+{
+    switch (mode) {
+    case 0: return launch_bp_s<T, 0>(st, s, jobs, tiles, ntiles, grid);
+    case 1: return launch_bp_s<T, 1>(st, s, jobs, tiles, ntiles, grid);
+    case 2: return launch_bp_s<T, 2>(st, s, jobs, tiles, ntiles, grid);
+    default: return launch_bp_s<T, 3>(st, s, jobs, tiles, ntiles, grid);
+    }
+}
+
+// mode: bit 0 positivity, bit 1 FP32 master volume (16-bit storage only)
+static hipError_t launch_bp(hipStream_t st, int storage, int mode, const tj_batch_tuning& tu, const TjJob* jobs,
+                            const int4* tiles, int ntiles, int grid)
+{
+    (void)hipGetLastError();
+    const BpShape s = bp_shape_of(tu, storage);
+    switch (storage) {
+    case TJ_STORAGE_F32: return launch_bp_m<float>(st, mode & 1, s, jobs, tiles, ntiles, grid);
+    case TJ_STORAGE_BF16: return launch_bp_m<Bf16>(st, mode, s, jobs, tiles, ntiles, grid);
+    case TJ_STORAGE_F16: return launch_bp_m<_Float16>(st, mode, s, jobs, tiles, ntiles, grid);
+    default:
+        if (mode & 1) tjb::k_job_bp<double, double, 2, 8, 2, 1><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
+        else tjb::k_job_bp<double, double, 2, 8, 2, 0><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
+        return hipGetLastError();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// tile tables.  Tiles that read the same data (the same volume rows for the
+// forward projection, the same image rows for the backprojection) form a
+// group; groups are dealt, largest first, to the 8 XCD queues, and the queues
+// are interleaved so that tile t runs on XCD t % 8 (blocks are dealt to XCDs
+// round-robin and each block takes tiles t = b, b + grid, ... with grid a
+// multiple of 8).  Short queues are padded with empty tiles.
+// ---------------------------------------------------------------------------
+struct Group {
+    double cost;
+    std::vector<int4> tiles;
+};
+
+static std::vector<int4> interleave(std::vector<Group>& groups)
+{
+    std::stable_sort(groups.begin(), groups.end(), [](const Group& a, const Group& b) { return a.cost > b.cost; });
+    std::vector<int4> q[8];
+    double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (Group& g : groups) {
+        int best = 0;
+        for (int x = 1; x < 8; ++x)
+            if (load[x] < load[best]) best = x;
+        load[best] += g.cost;
+        q[best].insert(q[best].end(), g.tiles.begin(), g.tiles.end());
+    }
+    size_t len = 0;
+    for (auto& v : q) len = std::max(len, v.size());
+    std::vector<int4> out(len * 8, make_int4(-1, 0, 0, 0));
+    for (int x = 0; x < 8; ++x)
+        for (size_t m = 0; m < q[x].size(); ++m) out[m * 8 + x] = q[x][m];
+    return out;
+}
+
+static std::vector<int4> fp_tiles(const std::vector<BJob*>& js, const std::vector<int>& idx, int G)
+{
+    std::vector<Group> groups;
+    for (size_t m = 0; m < js.size(); ++m) {
+        const tj_job_desc& d = js[m]->d;
+        const int nct = fp_col_tiles(d.proj_w), nrb = (d.proj_h + 3) / 4;
+        for (int rb = 0; rb < nrb; ++rb) {
+            Group g;
+            g.cost = (double)d.vol_d * ((double)d.vol_w + (double)tjb::NT * G) * ((d.n_proj + G - 1) / G) * nct;
+            for (int p0 = 0; p0 < d.n_proj; p0 += G)
+                for (int ct = 0; ct < nct; ++ct) g.tiles.push_back(make_int4(idx[m], p0, rb, ct));
+            groups.push_back(std::move(g));
+        }
+    }
+    return interleave(groups);
+}
+
+static std::vector<int4> bp_tiles(const std::vector<BJob*>& js, const std::vector<int>& idx, const BpShape& s)
+{
+    std::vector<Group> groups;
+    for (size_t m = 0; m < js.size(); ++m) {
+        const tj_job_desc& d = js[m]->d;
+        const int nxt = (d.vol_w + tjb::NT - 1) / tjb::NT, nrb = (d.vol_h + s.r - 1) / s.r;
+        const int nkb = (d.vol_d + s.kc - 1) / s.kc;
+        for (int rb = 0; rb < nrb; ++rb) {
+            Group g;
+            g.cost = (double)d.n_proj * nxt * nkb * (s.kc * s.r * (double)tjb::NT + (tjb::NT + s.kc) * s.r);
+            for (int xt = 0; xt < nxt; ++xt)
+                for (int kb = 0; kb < nkb; ++kb) g.tiles.push_back(make_int4(idx[m], xt, rb, kb));
+            groups.push_back(std::move(g));
+        }
+    }
+    return interleave(groups);
+}
+
+static int grid_for(const tj_batch* b, int ntiles)
+{
+    const int want = std::max(1, std::min(b->tune.blocks_per_cu, 16)) * b->ncu;
+    int g = std::min(ntiles, want);
+    g = ((g + 7) / 8) * 8;
+    return std::max(g, 8);
+}
+
+static TjJob dev_desc(const BJob* j)
+{
+    TjJob t{};
+    t.vol = j->vol;
+    t.master = j->master;
+    t.exps = j->exps;
+    t.diff = j->diff;
+    t.err_part = j->err_part;
+    t.prm = j->prm;
+    t.order = j->order;
+    t.err_hist = j->err_hist;
+    t.iter = j->iter;
+    t.W = j->d.vol_w;
+    t.H = j->d.vol_h;
+    t.D = j->d.vol_d;
+    t.pw = j->d.proj_w;
+    t.ph = j->d.proj_h;
+    t.n = j->d.n_proj;
+    t.dy = j->dy;
+    t.dyb = j->dyb;
+    t.nct = fp_col_tiles(j->d.proj_w);
+    t.hist_cap = std::max(j->d.max_iterations, 1);
+    // TomoJ casts lambda/update to float (TomoReconstruction2.java:646); the
+    // FP64 twin keeps it in double
+    t.factor = j->d.storage == TJ_STORAGE_F64 ? j->d.relaxation / j->d.n_proj
+                                              : (double)(float)(j->d.relaxation / j->d.n_proj);
+    return t;
+}
+
+// Builds the launches of one iteration over `ids`: per storage type a forward
+// launch, one error launch over every job, per (storage, positivity) a back
+// launch.  Jobs are ordered by storage in the descriptor table.
+static int build_plan(tj_batch* b, const std::vector<int>& ids)
+{
+    RunPlan& p = b->plan;
+    free_plan(p);
+    std::vector<BJob*> js;
+    for (int id : ids) {
+        auto it = b->jobs.find(id);
+        if (it == b->jobs.end()) return bfail(TJ_EINVAL, "unknown job %d", id);
+        js.push_back(it->second);
+    }
+    std::stable_sort(js.begin(), js.end(), [](const BJob* a, const BJob* c) { return a->d.storage < c->d.storage; });
+    std::vector<TjJob> hj;
+    for (BJob* j : js) hj.push_back(dev_desc(j));
+    TB_HIP(hipMalloc((void**)&p.djobs, sizeof(TjJob) * hj.size()));
+    TB_HIP(hipMemcpy(p.djobs, hj.data(), sizeof(TjJob) * hj.size(), hipMemcpyHostToDevice));
+    auto upload = [&](const std::vector<int4>& v, int4** out) -> int {
+        *out = nullptr;
+        if (v.empty()) return TJ_OK;
+        TB_HIP(hipMalloc((void**)out, sizeof(int4) * v.size()));
+        p.dtiles.push_back(*out);
+        TB_HIP(hipMemcpy(*out, v.data(), sizeof(int4) * v.size(), hipMemcpyHostToDevice));
+        return TJ_OK;
+    };
+    std::vector<RunPlan::L> fwd, back;
+    for (int s = 0; s <= TJ_STORAGE_F64; ++s) {
+        std::vector<BJob*> sj;
+        std::vector<int> sidx;
+        for (size_t m = 0; m < js.size(); ++m)
+            if (js[m]->d.storage == s) {
+                sj.push_back(js[m]);
+                sidx.push_back((int)m);
+            }
+        if (sj.empty()) continue;
+        for (int ps = 0; ps <= TJ_STORAGE_F64; ++ps) {
+            std::vector<BJob*> fj;
+            std::vector<int> fidx;
+            for (size_t m = 0; m < sj.size(); ++m)
+                if (proj_storage(sj[m]->d) == ps) {
+                    fj.push_back(sj[m]);
+                    fidx.push_back(sidx[m]);
+                }
+            if (fj.empty()) continue;
+            std::vector<int4> ft = fp_tiles(fj, fidx, fp_rays_of(b->tune, s));
+            int4* dft = nullptr;
+            if (int rc = upload(ft, &dft)) return rc;
+            int max_w = 0;
+            for (BJob* j : fj) max_w = std::max(max_w, j->d.vol_w);
+            fwd.push_back({0, s, max_w, (int)ft.size(), grid_for(b, (int)ft.size()), dft, 0, 0, ps});
+        }
+        for (int mode = 0; mode <= 3; ++mode) {
+            std::vector<BJob*> pj;
+            std::vector<int> pidx;
+            for (size_t m = 0; m < sj.size(); ++m)
+                if (((sj[m]->d.positivity != 0 ? 1 : 0) | (sj[m]->master ? 2 : 0)) == mode) {
+                    pj.push_back(sj[m]);
+                    pidx.push_back(sidx[m]);
+                }
+            if (pj.empty()) continue;
+            std::vector<int4> bt = bp_tiles(pj, pidx, bp_shape_of(b->tune, s));
+            int4* dbt = nullptr;
+            if (int rc = upload(bt, &dbt)) return rc;
+            back.push_back({2, s, mode, (int)bt.size(), grid_for(b, (int)bt.size()), dbt, 0, 0, s});
+        }
+    }
+    p.launches = fwd;
+    p.launches.push_back({1, 0, 0, 0, (int)js.size(), nullptr, 0, (int)js.size(), 0});
+    p.launches.insert(p.launches.end(), back.begin(), back.end());
+    p.ids = ids;
+    p.tune = b->tune;
+    p.epoch = b->epoch;
+    return TJ_OK;
+}
+
+static hipError_t enqueue_launch(tj_batch* b, const RunPlan::L& l)
+{
+    const RunPlan& p = b->plan;
+    if (l.kind == 0)
+        return launch_fp(b->stream, l.storage, l.pstorage, p.tune, p.djobs, l.tiles, l.ntiles, l.grid, l.pos);
+    if (l.kind == 2) return launch_bp(b->stream, l.storage, l.pos, p.tune, p.djobs, l.tiles, l.ntiles, l.grid);
+    (void)hipGetLastError();
+    tjb::k_job_err<<<l.njobs, tjb::NT, 0, b->stream>>>(p.djobs + l.job0, l.njobs);
+    return hipGetLastError();
+}
+
+static hipError_t enqueue_iterations(tj_batch* b, int iterations)
+{
+    for (int it = 0; it < iterations; ++it)
+        for (const RunPlan::L& l : b->plan.launches) {
+            hipError_t e = enqueue_launch(b, l);
+            if (e != hipSuccess) return e;
+        }
+    return hipSuccess;
+}
+
+static bool plan_matches(const tj_batch* b, const std::vector<int>& ids)
+{
+    const RunPlan& p = b->plan;
+    return p.epoch == b->epoch && p.ids == ids && memcmp(&p.tune, &b->tune, sizeof(tj_batch_tuning)) == 0;
+}
+
+template <typename TO>
+static hipError_t convert_to(hipStream_t st, int from, const void* src, TO* dst, size_t n)
+{
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+    switch (from) {
+    case TJ_STORAGE_F32: tjb::k_convert<float, TO><<<blocks, 256, 0, st>>>((const float*)src, dst, n); break;
+    case TJ_STORAGE_BF16: tjb::k_convert<Bf16, TO><<<blocks, 256, 0, st>>>((const Bf16*)src, dst, n); break;
+    case TJ_STORAGE_F16: tjb::k_convert<_Float16, TO><<<blocks, 256, 0, st>>>((const _Float16*)src, dst, n); break;
+    default: tjb::k_convert<double, TO><<<blocks, 256, 0, st>>>((const double*)src, dst, n); break;
+    }
+    return hipGetLastError();
+}
+
+static hipError_t convert_any(hipStream_t st, int from, const void* src, int to, void* dst, size_t n)
+{
+    switch (to) {
+    case TJ_STORAGE_F32: return convert_to(st, from, src, (float*)dst, n);
+    case TJ_STORAGE_BF16: return convert_to(st, from, src, (Bf16*)dst, n);
+    case TJ_STORAGE_F16: return convert_to(st, from, src, (_Float16*)dst, n);
+    default: return convert_to(st, from, src, (double*)dst, n);
+    }
+}
+
+static BJob* find_job(tj_batch* b, int id)
+{
+    auto it = b->jobs.find(id);
+    return it == b->jobs.end() ? nullptr : it->second;
+}
+
+template <typename T>
+static hipError_t put_volume(hipStream_t st, const float* src, void* vol, int W, int H, int D)
+{
+    tjb::k_zyx_to_job<T><<<dim3((W + 255) / 256, H, D), 256, 0, st>>>(src, (T*)vol, W, H, D);
+    return hipGetLastError();
+}
+template <typename T>
+static hipError_t take_volume(hipStream_t st, const void* vol, float* dst, int W, int H, int D)
+{
+    tjb::k_job_to_zyx<T><<<dim3((W + 255) / 256, H, D), 256, 0, st>>>((const T*)vol, dst, W, H, D);
+    return hipGetLastError();
+}
+
+extern "C" {
+
+size_t tj_job_bytes(const tj_job_desc* d)
+{
+    if (check_desc(d)) return 0;
+    return job_bytes(d);
+}
+
+int tj_batch_open(int device, size_t hbm_budget, tj_batch** out)
+{
+    if (!out) return bfail(TJ_EINVAL, "tj_batch_open: null output");
+    *out = nullptr;
+    int ndev = 0;
+    TB_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return bfail(TJ_EINVAL, "tj_batch_open: no HIP device %d", device);
+    TB_HIP(hipSetDevice(device));
+    tj_batch* b = new tj_batch();
+    b->device = device;
+    b->budget = hbm_budget;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        b->ncu = prop.multiProcessorCount;
+    if (allow_fp_lds() != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+        delete b;
+        return bfail(TJ_EDEVICE, "tj_batch_open: cannot create stream/events");
+    }
+    *out = b;
+    return TJ_OK;
+}
+
+int tj_batch_close(tj_batch* b)
+{
+    if (!b) return TJ_OK;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    free_plan(b->plan);
+    for (auto& kv : b->jobs) free_job(kv.second);
+    (void)hipEventDestroy(b->ev0);
+    (void)hipEventDestroy(b->ev1);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return TJ_OK;
+}
+
+size_t tj_batch_used(const tj_batch* b) { return b ? b->used : 0; }
+size_t tj_batch_budget(const tj_batch* b) { return b ? b->budget : 0; }
+
+int tj_batch_add_job(tj_batch* b, const tj_job_desc* d, const double* euler, int* job)
+{
+    if (!b || !euler || !job) return bfail(TJ_EINVAL, "tj_batch_add_job: null argument");
+    if (int rc = check_desc(d)) return rc;
+    const size_t need = job_bytes(d);
+    if (b->used + need > b->budget)
+        return bfail(TJ_ENOMEM, "tj_batch_add_job: %zu bytes needed, %zu of %zu in use", need, b->used, b->budget);
+    TB_HIP(hipSetDevice(b->device));
+    BJob* j = new BJob();
+    j->d = *d;
+    j->euler.assign(euler, euler + (size_t)9 * d->n_proj);
+    if (int rc = job_params(j)) {
+        delete j;
+        return rc;
+    }
+    const size_t s = storage_bytes(d->storage), a = arith_bytes(d->storage), sp = storage_bytes(proj_storage(*d));
+    const size_t nvox = (size_t)d->vol_w * d->vol_h * d->vol_d;
+    const size_t npx = (size_t)d->proj_w * d->proj_h * d->n_proj;
+    const size_t parts = (size_t)d->n_proj * d->proj_h * fp_col_tiles(d->proj_w);
+    hipError_t e = hipMalloc(&j->vol, nvox * s);
+    if (e == hipSuccess && d->fp32_master && (d->storage == TJ_STORAGE_BF16 || d->storage == TJ_STORAGE_F16)) {
+        e = hipMalloc((void**)&j->master, nvox * 4);
+        if (e == hipSuccess) e = hipMemsetAsync(j->master, 0, nvox * 4, b->stream);
+    }
+    if (e == hipSuccess) e = hipMalloc((void**)&j->exps, npx * sp);
+    if (e == hipSuccess) e = hipMalloc(&j->diff, npx * a);
+    if (e == hipSuccess) e = hipMalloc((void**)&j->err_part, parts * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&j->prm, j->hprm.size() * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&j->err_hist, (size_t)d->max_iterations * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&j->iter, sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(j->prm, j->hprm.data(), j->hprm.size() * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        // SIRT adds the images in orderShuffleART order from image 0 (TomoJ draws
+        // the first image at random; the plain path takes first_index likewise)
+        std::vector<int> ord(d->n_proj);
+        tj_order_shuffle(j->euler.data(), d->n_proj, 0, ord.data());
+        e = hipMalloc((void**)&j->order, sizeof(int) * d->n_proj);
+        if (e == hipSuccess) e = hipMemcpy(j->order, ord.data(), sizeof(int) * d->n_proj, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMemsetAsync(j->vol, 0, nvox * s, b->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(j->exps, 0, npx * sp, b->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(j->iter, 0, sizeof(int), b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        free_job(j);
+        return bfail(TJ_ENOMEM, "tj_batch_add_job: device allocation failed: %s", hipGetErrorString(e));
+    }
+    j->id = b->next_id++;
+    j->bytes = need;
+    b->used += need;
+    b->jobs[j->id] = j;
+    b->epoch++;
+    *job = j->id;
+    return TJ_OK;
+}
+
+int tj_batch_remove_job(tj_batch* b, int id)
+{
+    if (!b) return bfail(TJ_EINVAL, "null batch");
+    BJob* j = find_job(b, id);
+    if (!j) return bfail(TJ_EINVAL, "tj_batch_remove_job: unknown job %d", id);
+    TB_HIP(hipSetDevice(b->device));
+    TB_HIP(hipStreamSynchronize(b->stream));
+    if (std::find(b->plan.ids.begin(), b->plan.ids.end(), id) != b->plan.ids.end()) free_plan(b->plan);
+    b->used -= j->bytes;
+    b->jobs.erase(id);
+    free_job(j);
+    b->epoch++;
+    return TJ_OK;
+}
+
+int tj_batch_set_projections(tj_batch* b, int id, const float* stack)
+{
+    if (!b || !stack) return bfail(TJ_EINVAL, "tj_batch_set_projections: null argument");
+    BJob* j = find_job(b, id);
+    if (!j) return bfail(TJ_EINVAL, "unknown job %d", id);
+    TB_HIP(hipSetDevice(b->device));
+    const size_t npx = (size_t)j->d.proj_w * j->d.proj_h * j->d.n_proj;
+    float* tmp = nullptr;
+    TB_HIP(hipMalloc((void**)&tmp, npx * 4));
+    hipError_t e = hipMemcpyAsync(tmp, stack, npx * 4, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = convert_any(b->stream, TJ_STORAGE_F32, tmp, proj_storage(j->d), j->exps, npx);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return bfail(TJ_EDEVICE, "tj_batch_set_projections: %s", hipGetErrorString(e));
+    return TJ_OK;
+}
+
+int tj_batch_synth_job(tj_batch* b, int id, const double* ell, int k, double noise_rel, uint64_t seed)
+{
+    if (!b || (!ell && k > 0) || k < 0) return bfail(TJ_EINVAL, "tj_batch_synth_job: bad argument");
+    BJob* j = find_job(b, id);
+    if (!j) return bfail(TJ_EINVAL, "unknown job %d", id);
+    TB_HIP(hipSetDevice(b->device));
+    const tj_job_desc& d = j->d;
+    const size_t npx = (size_t)d.proj_w * d.proj_h * d.n_proj;
+    std::vector<TjProj> hp(d.n_proj);
+    for (int p = 0; p < d.n_proj; ++p) {
+        memset(&hp[p], 0, sizeof(TjProj));
+        for (int q = 0; q < 9; ++q) hp[p].e[q] = j->euler[(size_t)9 * p + q];
+    }
+    TjProj* dp = nullptr;
+    double *dell = nullptr, *clean = nullptr, *part = nullptr;
+    float* tmp = nullptr;
+    const int npart = 256;
+    hipError_t e = hipMalloc((void**)&dp, sizeof(TjProj) * d.n_proj);
+    if (e == hipSuccess) e = hipMalloc((void**)&dell, sizeof(double) * 16 * std::max(k, 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&clean, sizeof(double) * npx);
+    if (e == hipSuccess) e = hipMalloc((void**)&part, sizeof(double) * npart);
+    if (e == hipSuccess) e = hipMalloc((void**)&tmp, sizeof(float) * npx);
+    if (e == hipSuccess) e = hipMemcpy(dp, hp.data(), sizeof(TjProj) * d.n_proj, hipMemcpyHostToDevice);
+    if (e == hipSuccess && k > 0) e = hipMemcpy(dell, ell, sizeof(double) * 16 * k, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = tjk::phantom_project(b->stream, dell, k, dp, d.n_proj, d.proj_w, d.proj_h, (d.proj_w - 1) / 2.0,
+                                 (d.proj_h - 1) / 2.0, clean);
+    if (e == hipSuccess) e = tjk::add_noise(b->stream, clean, npx, seed, noise_rel, part, npart, tmp);
+    if (e == hipSuccess) e = convert_any(b->stream, TJ_STORAGE_F32, tmp, proj_storage(d), j->exps, npx);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    void* ptrs[] = {dp, dell, clean, part, tmp};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    if (e != hipSuccess) return bfail(TJ_EDEVICE, "tj_batch_synth_job: %s", hipGetErrorString(e));
+    return TJ_OK;
+}
+
+int tj_batch_get_projection(tj_batch* b, int id, int index, float* out)
+{
+    if (!b || !out) return bfail(TJ_EINVAL, "null argument");
+    BJob* j = find_job(b, id);
+    if (!j || index < 0 || index >= j->d.n_proj) return bfail(TJ_EINVAL, "bad job or projection index");
+    TB_HIP(hipSetDevice(b->device));
+    const size_t npx = (size_t)j->d.proj_w * j->d.proj_h;
+    float* tmp = nullptr;
+    TB_HIP(hipMalloc((void**)&tmp, npx * 4));
+    const int ps = proj_storage(j->d);
+    const char* src = (const char*)j->exps + npx * index * storage_bytes(ps);
+    hipError_t e = convert_any(b->stream, ps, src, TJ_STORAGE_F32, tmp, npx);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, npx * 4, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return bfail(TJ_EDEVICE, "tj_batch_get_projection: %s", hipGetErrorString(e));
+    return TJ_OK;
+}
+
+int tj_batch_get_difference(tj_batch* b, int id, int index, double* out)
+{
+    if (!b || !out) return bfail(TJ_EINVAL, "null argument");
+    BJob* j = find_job(b, id);
+    if (!j || index < 0 || index >= j->d.n_proj) return bfail(TJ_EINVAL, "bad job or projection index");
+    TB_HIP(hipSetDevice(b->device));
+    const size_t npx = (size_t)j->d.proj_w * j->d.proj_h;
+    double* tmp = nullptr;
+    TB_HIP(hipMalloc((void**)&tmp, npx * 8));
+    const int as = j->d.storage == TJ_STORAGE_F64 ? TJ_STORAGE_F64 : TJ_STORAGE_F32;
+    const char* src = (const char*)j->diff + npx * index * arith_bytes(j->d.storage);
+    hipError_t e = convert_any(b->stream, as, src, TJ_STORAGE_F64, tmp, npx);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, npx * 8, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return bfail(TJ_EDEVICE, "tj_batch_get_difference: %s", hipGetErrorString(e));
+    return TJ_OK;
+}
+
+int tj_batch_set_volume(tj_batch* b, int id, const float* zyx)
+{
+    if (!b || !zyx) return bfail(TJ_EINVAL, "null argument");
+    BJob* j = find_job(b, id);
+    if (!j) return bfail(TJ_EINVAL, "unknown job %d", id);
+    TB_HIP(hipSetDevice(b->device));
+    const int W = j->d.vol_w, H = j->d.vol_h, D = j->d.vol_d;
+    const size_t nvox = (size_t)W * H * D;
+    float* tmp = nullptr;
+    TB_HIP(hipMalloc((void**)&tmp, nvox * 4));
+    hipError_t e = hipMemcpyAsync(tmp, zyx, nvox * 4, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) {
+        switch (j->d.storage) {
+        case TJ_STORAGE_F32: e = put_volume<float>(b->stream, tmp, j->vol, W, H, D); break;
+        case TJ_STORAGE_BF16: e = put_volume<Bf16>(b->stream, tmp, j->vol, W, H, D); break;
+        case TJ_STORAGE_F16: e = put_volume<_Float16>(b->stream, tmp, j->vol, W, H, D); break;
+        default: e = put_volume<double>(b->stream, tmp, j->vol, W, H, D); break;
+        }
+    }
+    if (e == hipSuccess && j->master) e = put_volume<float>(b->stream, tmp, j->master, W, H, D);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return bfail(TJ_EDEVICE, "tj_batch_set_volume: %s", hipGetErrorString(e));
+    return TJ_OK;
+}
+
+int tj_batch_get_volume(tj_batch* b, int id, float* zyx)
+{
+    if (!b || !zyx) return bfail(TJ_EINVAL, "null argument");
+    BJob* j = find_job(b, id);
+    if (!j) return bfail(TJ_EINVAL, "unknown job %d", id);
+    TB_HIP(hipSetDevice(b->device));
+    const int W = j->d.vol_w, H = j->d.vol_h, D = j->d.vol_d;
+    const size_t nvox = (size_t)W * H * D;
+    float* tmp = nullptr;
+    TB_HIP(hipMalloc((void**)&tmp, nvox * 4));
+    hipError_t e;
+    switch (j->master ? TJ_STORAGE_F32 : j->d.storage) {
+    case TJ_STORAGE_F32: e = take_volume<float>(b->stream, j->master ? (const void*)j->master : j->vol, tmp, W, H, D); break;
+    case TJ_STORAGE_BF16: e = take_volume<Bf16>(b->stream, j->vol, tmp, W, H, D); break;
+    case TJ_STORAGE_F16: e = take_volume<_Float16>(b->stream, j->vol, tmp, W, H, D); break;
+    default: e = take_volume<double>(b->stream, j->vol, tmp, W, H, D); break;
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(zyx, tmp, nvox * 4, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return bfail(TJ_EDEVICE, "tj_batch_get_volume: %s", hipGetErrorString(e));
+    return TJ_OK;
+}
+
+int tj_batch_get_volume_f64(tj_batch* b, int id, double* zyx)
+{
+    if (!b || !zyx) return bfail(TJ_EINVAL, "null argument");
+    BJob* j = find_job(b, id);
+    if (!j) return bfail(TJ_EINVAL, "unknown job %d", id);
+    if (j->d.storage != TJ_STORAGE_F64) return bfail(TJ_EINVAL, "job %d does not store FP64", id);
+    TB_HIP(hipSetDevice(b->device));
+    const int W = j->d.vol_w, H = j->d.vol_h, D = j->d.vol_d;
+    // [y][z][x] -> host, reordered on the host
+    std::vector<double> tmp((size_t)W * H * D);
+    TB_HIP(hipMemcpy(tmp.data(), j->vol, tmp.size() * 8, hipMemcpyDeviceToHost));
+    for (int y = 0; y < H; ++y)
+        for (int z = 0; z < D; ++z)
+            memcpy(zyx + ((size_t)z * H + y) * W, tmp.data() + ((size_t)y * D + z) * W, (size_t)W * 8);
+    return TJ_OK;
+}
+
+int tj_batch_convert(tj_batch* b, int id, int storage)
+{
+    if (!b) return bfail(TJ_EINVAL, "null batch");
+    BJob* j = find_job(b, id);
+    if (!j) return bfail(TJ_EINVAL, "unknown job %d", id);
+    if (!storage_bytes(storage)) return bfail(TJ_EINVAL, "unknown storage type %d", storage);
+    if (storage == j->d.storage) return TJ_OK;
+    tj_job_desc nd = j->d;
+    nd.storage = storage;
+    const size_t need = job_bytes(&nd);
+    if (b->used - j->bytes + need > b->budget)
+        return bfail(TJ_ENOMEM, "tj_batch_convert: %zu bytes needed over the budget", need);
+    TB_HIP(hipSetDevice(b->device));
+    TB_HIP(hipStreamSynchronize(b->stream));
+    const size_t nvox = (size_t)nd.vol_w * nd.vol_h * nd.vol_d;
+    const size_t npx = (size_t)nd.proj_w * nd.proj_h * nd.n_proj;
+    void *vol = nullptr, *exps = nullptr, *diff = nullptr;
+    hipError_t e = hipMalloc(&vol, nvox * storage_bytes(storage));
+    const int ps_old = proj_storage(j->d), ps_new = proj_storage(nd);
+    if (e == hipSuccess) e = hipMalloc(&exps, npx * storage_bytes(ps_new));
+    if (e == hipSuccess && arith_bytes(storage) != arith_bytes(j->d.storage))
+        e = hipMalloc(&diff, npx * arith_bytes(storage));
+    if (e == hipSuccess) {
+        if (j->master) e = convert_any(b->stream, TJ_STORAGE_F32, j->master, storage, vol, nvox);
+        else e = convert_any(b->stream, j->d.storage, j->vol, storage, vol, nvox);
+    }
+    if (e == hipSuccess) e = convert_any(b->stream, ps_old, j->exps, ps_new, exps, npx);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        if (vol) (void)hipFree(vol);
+        if (exps) (void)hipFree(exps);
+        if (diff) (void)hipFree(diff);
+        return bfail(TJ_EDEVICE, "tj_batch_convert: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(j->vol);
+    (void)hipFree((void*)j->exps);
+    j->vol = vol;
+    j->exps = exps;
+    if (diff) {
+        (void)hipFree(j->diff);
+        j->diff = diff;
+    }
+    // a 16-bit target keeps the master when the job asked for one
+    if (j->master && !(storage == TJ_STORAGE_BF16 || storage == TJ_STORAGE_F16)) {
+        (void)hipFree(j->master);
+        j->master = nullptr;
+    }
+    b->used = b->used - j->bytes + need;
+    j->bytes = need;
+    j->d.storage = storage;
+    free_plan(b->plan);
+    b->epoch++;
+    return TJ_OK;
+}
+
+int tj_batch_reset_volume(tj_batch* b, int id)
+{
+    if (!b) return bfail(TJ_EINVAL, "null batch");
+    BJob* j = find_job(b, id);
+    if (!j) return bfail(TJ_EINVAL, "unknown job %d", id);
+    TB_HIP(hipSetDevice(b->device));
+    const size_t nvox = (size_t)j->d.vol_w * j->d.vol_h * j->d.vol_d;
+    TB_HIP(hipMemsetAsync(j->vol, 0, nvox * storage_bytes(j->d.storage), b->stream));
+    if (j->master) TB_HIP(hipMemsetAsync(j->master, 0, nvox * 4, b->stream));
+    TB_HIP(hipMemsetAsync(j->iter, 0, sizeof(int), b->stream));
+    TB_HIP(hipStreamSynchronize(b->stream));
+    return TJ_OK;
+}
+
+int tj_batch_run(tj_batch* b, const int* ids, int nids, int iterations, int flags)
+{
+    if (!b || !ids || nids <= 0 || iterations < 0) return bfail(TJ_EINVAL, "tj_batch_run: bad argument");
+    TB_HIP(hipSetDevice(b->device));
+    std::vector<int> v(ids, ids + nids);
+    {
+        std::vector<int> s = v;
+        std::sort(s.begin(), s.end());
+        if (std::adjacent_find(s.begin(), s.end()) != s.end()) return bfail(TJ_EINVAL, "tj_batch_run: repeated job");
+    }
+    if (iterations == 0) return TJ_OK;
+    const bool graph = (flags & TJ_RUN_GRAPH) != 0;
+    const bool reuse = plan_matches(b, v);
+    if (!reuse) {
+        if (int rc = build_plan(b, v)) return rc;
+    }
+    RunPlan& p = b->plan;
+    TB_HIP(hipEventRecord(b->ev0, b->stream));
+    if (graph) {
+        if (!(reuse && p.exec && p.iterations == iterations)) {
+            if (p.exec) (void)hipGraphExecDestroy(p.exec);
+            if (p.graph) (void)hipGraphDestroy(p.graph);
+            p.exec = nullptr;
+            p.graph = nullptr;
+            TB_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = enqueue_iterations(b, iterations);
+            hipGraph_t g = nullptr;
+            hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return bfail(TJ_EDEVICE, "tj_batch_run: graph capture failed: %s",
+                             hipGetErrorString(e != hipSuccess ? e : e2));
+            }
+            p.graph = g;
+            TB_HIP(hipGraphInstantiate(&p.exec, g, nullptr, nullptr, 0));
+            p.iterations = iterations;
+        }
+        TB_HIP(hipGraphLaunch(p.exec, b->stream));
+    } else {
+        TB_HIP(enqueue_iterations(b, iterations));
+    }
+    TB_HIP(hipEventRecord(b->ev1, b->stream));
+    TB_HIP(hipStreamSynchronize(b->stream));
+    float ms = 0.0f;
+    TB_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    return TJ_OK;
+}
+
+double tj_batch_last_run_ms(const tj_batch* b) { return b ? b->last_ms : 0.0; }
+
+int tj_batch_errors(tj_batch* b, int id, double* err, int cap)
+{
+    if (!b) return bfail(TJ_EINVAL, "null batch");
+    BJob* j = find_job(b, id);
+    if (!j) return bfail(TJ_EINVAL, "unknown job %d", id);
+    TB_HIP(hipSetDevice(b->device));
+    int it = 0;
+    TB_HIP(hipMemcpy(&it, j->iter, sizeof(int), hipMemcpyDeviceToHost));
+    if (err && cap > 0) {
+        const int hc = std::max(j->d.max_iterations, 1);
+        std::vector<double> h(hc);
+        TB_HIP(hipMemcpy(h.data(), j->err_hist, (size_t)hc * 8, hipMemcpyDeviceToHost));
+        // the last min(it, hc, cap) iterations, oldest first
+        const int m = std::min(std::min(it, hc), cap);
+        for (int k = 0; k < m; ++k) err[k] = h[(size_t)(it - m + k) % hc];
+    }
+    return it;
+}
+
+int tj_batch_set_tuning(tj_batch* b, const tj_batch_tuning* t)
+{
+    if (!b || !t) return bfail(TJ_EINVAL, "null argument");
+    if (t->fp_rays < 1 || t->fp_lds_kb < 16 || t->fp_lds_kb > 128 || t->bp_rows < 1 || t->bp_zchunk < 1 ||
+        t->bp_group < 1 || t->blocks_per_cu < 1 || t->blocks_per_cu > 16)
+        return bfail(TJ_EINVAL, "tj_batch_set_tuning: value out of range");
+    b->tune = *t;
+    return TJ_OK;
+}
+
+int tj_batch_get_tuning(const tj_batch* b, tj_batch_tuning* t)
+{
+    if (!b || !t) return bfail(TJ_EINVAL, "null argument");
+    *t = b->tune;
+    return TJ_OK;
+}
+
+// Times one launch kind of the current plan (forward = 0, back = 2) over
+// `reps` repetitions; volumes are saved before and restored after when the
+// backprojection is timed.
+static int time_kind(tj_batch* b, int kind, int reps, double* ms_out)
+{
+    RunPlan& p = b->plan;
+    // (device buffer, saved copy, bytes) of every volume the backprojection writes
+    struct Saved {
+        void* buf;
+        void* copy;
+        size_t bytes;
+    };
+    std::vector<Saved> saved;
+    auto release = [&]() {
+        for (auto& q : saved) (void)hipFree(q.copy);
+    };
+    if (kind == 2) {
+        for (int id : p.ids) {
+            BJob* j = find_job(b, id);
+            const size_t nvox = (size_t)j->d.vol_w * j->d.vol_h * j->d.vol_d;
+            std::pair<void*, size_t> bufs[2] = {{j->vol, nvox * storage_bytes(j->d.storage)},
+                                                {j->master, j->master ? nvox * 4 : 0}};
+            for (auto& bf : bufs) {
+                if (!bf.first) continue;
+                void* c = nullptr;
+                if (hipMalloc(&c, bf.second) != hipSuccess) {
+                    (void)hipGetLastError();
+                    release();
+                    return bfail(TJ_ENOMEM, "autotune: no room to save the volumes");
+                }
+                saved.push_back({bf.first, c, bf.second});
+                if (hipMemcpyAsync(c, bf.first, bf.second, hipMemcpyDeviceToDevice, b->stream) != hipSuccess) {
+                    release();
+                    return bfail(TJ_EDEVICE, "autotune: cannot save a volume");
+                }
+            }
+        }
+    }
+    hipError_t e = hipSuccess;
+    for (int r = 0; r <= reps && e == hipSuccess; ++r) {
+        if (r == 1) e = hipEventRecord(b->ev0, b->stream);
+        for (const RunPlan::L& l : p.launches)
+            if (l.kind == kind && e == hipSuccess) e = enqueue_launch(b, l);
+    }
+    if (e == hipSuccess) e = hipEventRecord(b->ev1, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, b->ev0, b->ev1);
+    for (auto& q : saved) (void)hipMemcpyAsync(q.buf, q.copy, q.bytes, hipMemcpyDeviceToDevice, b->stream);
+    (void)hipStreamSynchronize(b->stream);
+    release();
+    if (e != hipSuccess) return bfail(TJ_EDEVICE, "autotune: %s", hipGetErrorString(e));
+    *ms_out = ms / std::max(reps, 1);
+    return TJ_OK;
+}
+
+int tj_batch_autotune(tj_batch* b, const int* ids, int nids, tj_batch_tuning* best, double* best_ms)
+{
+    if (!b || !ids || nids <= 0) return bfail(TJ_EINVAL, "tj_batch_autotune: bad argument");
+    TB_HIP(hipSetDevice(b->device));
+    std::vector<int> v(ids, ids + nids);
+    const tj_batch_tuning start = b->tune;
+    // forward: rays per tile x LDS per tile x persistent blocks per CU
+    const int fr[] = {2, 4}, fl[] = {32, 64}, bpc[] = {2, 4, 8};
+    tj_batch_tuning cur = start, bestf = start;
+    double tf = 1e300;
+    for (int r : fr)
+        for (int l : fl)
+            for (int c : bpc) {
+                cur.fp_rays = r;
+                cur.fp_lds_kb = l;
+                cur.blocks_per_cu = c;
+                b->tune = cur;
+                if (int rc = build_plan(b, v)) return rc;
+                double ms = 0;
+                if (int rc = time_kind(b, 0, 2, &ms)) return rc;
+                if (ms < tf) {
+                    tf = ms;
+                    bestf = cur;
+                }
+            }
+    // backprojection: (rows, slices) x persistent blocks per CU, forward fixed
+    const int shapes[][2] = {{4, 8}, {2, 8}, {4, 4}, {2, 16}};
+    cur = bestf;
+    tj_batch_tuning bestb = bestf;
+    double tb = 1e300;
+    for (auto& sh : shapes)
+        for (int c : bpc) {
+            cur.bp_rows = sh[0];
+            cur.bp_zchunk = sh[1];
+            cur.bp_group = (sh[1] >= 16) ? 2 : 4;
+            cur.blocks_per_cu = c;
+            b->tune = cur;
+            if (int rc = build_plan(b, v)) return rc;
+            double ms = 0;
+            if (int rc = time_kind(b, 2, 2, &ms)) return rc;
+            if (ms < tb) {
+                tb = ms;
+                bestb = cur;
+            }
+        }
+    // one persistent block count serves both kernels: keep the one that is
+    // better for the sum
+    tj_batch_tuning fin = bestb;
+    fin.fp_rays = bestf.fp_rays;
+    fin.fp_lds_kb = bestf.fp_lds_kb;
+    double tot_best = 1e300;
+    tj_batch_tuning pick = fin;
+    for (int c : {bestf.blocks_per_cu, bestb.blocks_per_cu}) {
+        fin.blocks_per_cu = c;
+        b->tune = fin;
+        if (int rc = build_plan(b, v)) return rc;
+        double a = 0, c2 = 0;
+        if (int rc = time_kind(b, 0, 2, &a)) return rc;
+        if (int rc = time_kind(b, 2, 2, &c2)) return rc;
+        if (a + c2 < tot_best) {
+            tot_best = a + c2;
+            pick = fin;
+        }
+    }
+    b->tune = pick;
+    free_plan(b->plan);
+    if (best) *best = pick;
+    if (best_ms) *best_ms = tot_best;
+    return TJ_OK;
+}
+
+}  // extern "C"

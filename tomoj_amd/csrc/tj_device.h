@@ -49,3 +49,9 @@ struct TjBpRow {
 #define TJ_ACCURACY ((double)1e-10f)
 
 #define TJ_BLOCK 256
+
+// Stores the message tj_last_error() returns on the calling thread.
+void tj_record_error(const char* msg);
+
+// TomoReconstruction2.orderShuffleART with the first image fixed (host code).
+void tj_order_shuffle(const double* euler, int n, int first, int* order);
