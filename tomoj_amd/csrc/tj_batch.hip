@@ -41,6 +41,7 @@ namespace tjb {
 
 constexpr int NT = 256;  // threads per block: one column (forward) or voxel column (back) each
 constexpr int PRM = 10;  // doubles of per-projection constants
+constexpr int SU = 4;    // columns per lane per staging batch of the forward projection
 
 struct Bf16 {
     uint16_t u;
@@ -49,14 +50,23 @@ struct Bf16 {
 // ---------------------------------------------------------------------------
 // storage conversions (round to nearest even on store)
 // ---------------------------------------------------------------------------
+// Every buffer of a job lives in device global memory: accesses go through
+// global (address space 1) pointers so the compiler emits global rather than
+// flat instructions (flat loads also count against the LDS wait counter).
+#define TJ_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const TJ_GLOBAL T* gp(const T* p) { return (const TJ_GLOBAL T*)p; }
+template <typename T>
+__device__ __forceinline__ TJ_GLOBAL T* gp(T* p) { return (TJ_GLOBAL T*)p; }
+
 template <typename A>
-__device__ __forceinline__ A ld(const float* p, size_t i) { return (A)p[i]; }
+__device__ __forceinline__ A ld(const float* p, size_t i) { return (A)gp(p)[i]; }
 template <typename A>
-__device__ __forceinline__ A ld(const double* p, size_t i) { return (A)p[i]; }
+__device__ __forceinline__ A ld(const double* p, size_t i) { return (A)gp(p)[i]; }
 template <typename A>
-__device__ __forceinline__ A ld(const _Float16* p, size_t i) { return (A)(float)p[i]; }
+__device__ __forceinline__ A ld(const _Float16* p, size_t i) { return (A)(float)gp(p)[i]; }
 template <typename A>
-__device__ __forceinline__ A ld(const Bf16* p, size_t i) { return (A)__uint_as_float((uint32_t)p[i].u << 16); }
+__device__ __forceinline__ A ld(const Bf16* p, size_t i) { return (A)__uint_as_float((uint32_t)gp(p)[i].u << 16); }
 
 __device__ __forceinline__ uint16_t f32_to_bf16(float f)
 {
@@ -65,10 +75,10 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f)
     b += 0x7fffu + ((b >> 16) & 1u);
     return (uint16_t)(b >> 16);
 }
-__device__ __forceinline__ void st(float* p, size_t i, float v) { p[i] = v; }
-__device__ __forceinline__ void st(double* p, size_t i, double v) { p[i] = v; }
-__device__ __forceinline__ void st(_Float16* p, size_t i, float v) { p[i] = (_Float16)v; }
-__device__ __forceinline__ void st(Bf16* p, size_t i, float v) { p[i].u = f32_to_bf16(v); }
+__device__ __forceinline__ void st(float* p, size_t i, float v) { gp(p)[i] = v; }
+__device__ __forceinline__ void st(double* p, size_t i, double v) { gp(p)[i] = v; }
+__device__ __forceinline__ void st(_Float16* p, size_t i, float v) { gp(p)[i] = (_Float16)v; }
+__device__ __forceinline__ void st(Bf16* p, size_t i, float v) { gp(p)[i].u = f32_to_bf16(v); }
 
 template <typename A> struct V4;
 template <> struct V4<float> { using t = float4; };
@@ -140,26 +150,60 @@ __device__ inline void line_interval(double b, double t, double inv_t, double lo
     zhi = c;
 }
 
-// Adds the sample of slice z of ray k (LDS cell row zo) at position xs
-// (clamped to [0, W-1]) with x fraction xf to the 4 row sums.
+// Fraction of a non-negative position xs with integer part xb.
+__device__ __forceinline__ float frac_of(float xs, int) { return __builtin_amdgcn_fractf(xs); }
+__device__ __forceinline__ double frac_of(double xs, int xb) { return xs - (double)xb; }
+
+// Adds the samples of one slice for the G rays of the thread: q0/q1 are the
+// cells at and right of each ray's column, xf its x fraction.
 template <typename A, int G, bool NEARY>
-__device__ __forceinline__ void fp_add(const typename V4<A>::t* cell, int a, A xf, const A (&ym)[4],
-                                       A (&rs)[G][4], int k)
+__device__ __forceinline__ void fp_accumulate(const typename V4<A>::t (&q0)[G], const typename V4<A>::t (&q1)[G],
+                                              const A (&xf)[G], const A (&ym)[4], A (&rs)[G][4])
+{
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        if (NEARY) {
+            rs[k][0] += fma(xf[k] * ym[0], q1[k].x - q0[k].x, q0[k].x);
+            rs[k][1] += fma(xf[k] * ym[1], q1[k].y - q0[k].y, q0[k].y);
+            rs[k][2] += fma(xf[k] * ym[2], q1[k].z - q0[k].z, q0[k].z);
+            rs[k][3] += fma(xf[k] * ym[3], q1[k].w - q0[k].w, q0[k].w);
+        } else {
+            rs[k][0] += fma(xf[k], q1[k].x - q0[k].x, q0[k].x);
+            rs[k][1] += fma(xf[k], q1[k].y - q0[k].y, q0[k].y);
+            rs[k][2] += fma(xf[k], q1[k].z - q0[k].z, q0[k].z);
+            rs[k][3] += fma(xf[k], q1[k].w - q0[k].w, q0[k].w);
+        }
+    }
+}
+
+// Slice z of the G rays with FP64 positions (nearest-value samples: the x
+// column of a discontinuous sample must be the one TomoJ's FP64 position
+// gives); zl = 0 on the last slice.
+template <typename A, int G, bool NEARY>
+__device__ __forceinline__ void fp_slice_f64(const typename V4<A>::t* cell, int z, int zo, int zero, A zl,
+                                             double wd, const A (&ym)[4], const double (&bd)[G],
+                                             const double (&td)[G], const int (&zlo)[G], const int (&zhi)[G],
+                                             A (&rs)[G][4])
 {
     using C4 = typename V4<A>::t;
-    const C4 q0 = cell[a];
-    const C4 q1 = cell[a + 1];
-    if (NEARY) {
-        rs[k][0] += fma(xf * ym[0], q1.x - q0.x, q0.x);
-        rs[k][1] += fma(xf * ym[1], q1.y - q0.y, q0.y);
-        rs[k][2] += fma(xf * ym[2], q1.z - q0.z, q0.z);
-        rs[k][3] += fma(xf * ym[3], q1.w - q0.w, q0.w);
-    } else {
-        rs[k][0] += fma(xf, q1.x - q0.x, q0.x);
-        rs[k][1] += fma(xf, q1.y - q0.y, q0.y);
-        rs[k][2] += fma(xf, q1.z - q0.z, q0.z);
-        rs[k][3] += fma(xf, q1.w - q0.w, q0.w);
+    int a[G];
+    A xf[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        const double xv = fma((double)z, td[k], bd[k]);
+        const bool in = (z >= zlo[k]) && (z <= zhi[k]);
+        const double xs = fmin(fmax(xv, 0.0), wd);
+        const int xb = (int)xs;
+        xf[k] = (A)(xs - (double)xb) * zl;
+        a[k] = in ? zo + xb : zero;
     }
+    C4 q0[G], q1[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        q0[k] = cell[a[k]];
+        q1[k] = cell[a[k] + 1];
+    }
+    fp_accumulate<A, G, NEARY>(q0, q1, xf, ym, rs);
 }
 
 // One LDS slab of the forward projection: zn slices for the G rays of the
@@ -167,41 +211,50 @@ __device__ __forceinline__ void fp_add(const typename V4<A>::t* cell, int a, A x
 // (the bilinear value is continuous in x, so rounding moves it by a rounding
 // error).  getPixel's nearest-value samples -- the last slice (zlast) and, with
 // NEARY, the last volume row (ym = 0) -- are discontinuous in x, so there the
-// column comes from the FP64 position, as TomoJ computes it.
+// column comes from the FP64 position, as TomoJ computes it.  The main loop
+// has no branch: addresses, then LDS reads, then arithmetic.
 template <typename A, int G, bool NEARY>
 __device__ __forceinline__ void fp_slab(const typename V4<A>::t* cell, int zn, int zlast, int z0, int W1, int zero,
                                         A wm1, const A (&ym)[4], const A (&base)[G], const A (&tn)[G],
                                         const double (&bd)[G], const double (&td)[G], const int (&zlo)[G],
                                         const int (&zhi)[G], A (&rs)[G][4])
 {
+    using C4 = typename V4<A>::t;
     const double wd = (double)wm1;
-    for (int zz = 0; zz < zn; ++zz) {
-        const int z = z0 + zz;
-        const int zo = zz * W1;
-        if (NEARY || zz == zlast) {
-            const A zl = (zz == zlast) ? (A)0 : (A)1;
-#pragma unroll
-            for (int k = 0; k < G; ++k) {
-                const double xv = fma((double)z, td[k], bd[k]);
-                const bool in = (z >= zlo[k]) && (z <= zhi[k]);
-                const double xs = fmin(fmax(xv, 0.0), wd);
-                const int xb = (int)xs;
-                const A xf = (A)(xs - (double)xb) * zl;
-                fp_add<A, G, NEARY>(cell, in ? zo + xb : zero, xf, ym, rs, k);
-            }
-        } else {
+    const int zend = min(zlast, zn);
+    if (NEARY) {
+        for (int zz = 0; zz < zend; ++zz)
+            fp_slice_f64<A, G, true>(cell, z0 + zz, zz * W1, zero, (A)1, wd, ym, bd, td, zlo, zhi, rs);
+    } else {
+        for (int zz = 0; zz < zend; ++zz) {
+            const int z = z0 + zz;
+            const int zo = zz * W1;
             const A zf = (A)z;
+            int a[G];
+            A xf[G];
 #pragma unroll
             for (int k = 0; k < G; ++k) {
-                const A xv = fma(zf, tn[k], base[k]);
-                const bool in = (z >= zlo[k]) && (z <= zhi[k]);
-                const A xs = fmin(fmax(xv, (A)0), wm1);
+                // inside the volume (decided in FP64 by zlo/zhi) xv lies in
+                // [-ACCURACY, W) up to rounding: x < 0 samples column 0 with
+                // fraction 0 (getPixel), x >= W-1 the last column with fraction
+                // 0 (its replicated neighbour makes the fraction irrelevant).
+                // Outside, the lane reads the zero cell.
+                const A xs = fmin(fmax(fma(zf, tn[k], base[k]), (A)0), wm1);
                 const int xb = (int)xs;
-                const A xf = xs - (A)xb;
-                fp_add<A, G, false>(cell, in ? zo + xb : zero, xf, ym, rs, k);
+                xf[k] = frac_of(xs, xb);
+                a[k] = ((z >= zlo[k]) && (z <= zhi[k])) ? zo + xb : zero;
             }
+            C4 q0[G], q1[G];
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                q0[k] = cell[a[k]];
+                q1[k] = cell[a[k] + 1];
+            }
+            fp_accumulate<A, G, false>(q0, q1, xf, ym, rs);
         }
     }
+    if (zlast < zn)
+        fp_slice_f64<A, G, NEARY>(cell, z0 + zlast, zlast * W1, zero, (A)0, wd, ym, bd, td, zlo, zhi, rs);
 }
 
 template <typename T, typename TE, typename A, int G>
@@ -263,17 +316,33 @@ __global__ __launch_bounds__(NT) void k_job_fp(const TjJob* __restrict__ jobs, c
                 zc4.x = zc4.y = zc4.z = zc4.w = (A)0;
                 cell[zero + tid] = zc4;
             }
+            // wave w stages slices w, w + NT/64, ...; lanes take consecutive
+            // columns, SU columns per lane per batch so that 4*SU loads are in
+            // flight before the first LDS write.  Rows outside the volume read
+            // row 0 (a valid address) and store 0; columns past W read W-1.
             for (int zz = wave; zz < zn; zz += NT / 64) {
                 const size_t zo = (size_t)(z0 + zz) * W;
                 C4* dst = cell + zz * W1;
-                for (int xx = lane; xx < W1; xx += 64) {
-                    const size_t o = zo + min(xx, W - 1);
-                    C4 v;
-                    v.x = rin[0] ? ld<A>(rp[0], o) : (A)0;
-                    v.y = rin[1] ? ld<A>(rp[1], o) : (A)0;
-                    v.z = rin[2] ? ld<A>(rp[2], o) : (A)0;
-                    v.w = rin[3] ? ld<A>(rp[3], o) : (A)0;
-                    dst[xx] = v;
+                for (int x0 = 0; x0 < W1; x0 += 64 * SU) {
+                    A v[SU][4];
+#pragma unroll
+                    for (int u = 0; u < SU; ++u) {
+                        const size_t o = zo + min(x0 + 64 * u + lane, W - 1);
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) v[u][r] = ld<A>(rp[r], o);
+                    }
+#pragma unroll
+                    for (int u = 0; u < SU; ++u) {
+                        const int xx = x0 + 64 * u + lane;
+                        if (xx < W1) {
+                            C4 c4;
+                            c4.x = rin[0] ? v[u][0] : (A)0;
+                            c4.y = rin[1] ? v[u][1] : (A)0;
+                            c4.z = rin[2] ? v[u][2] : (A)0;
+                            c4.w = rin[3] ? v[u][3] : (A)0;
+                            dst[xx] = c4;
+                        }
+                    }
                 }
             }
             __syncthreads();
@@ -309,7 +378,7 @@ __global__ __launch_bounds__(NT) void k_job_fp(const TjJob* __restrict__ jobs, c
                         d = (A)v;
                         v2 = v * v;
                     }
-                    ((A*)J.diff)[o] = d;
+                    st((A*)J.diff, o, d);
                 }
                 e2[k][r] = wave_sum(v2);
             }
@@ -327,7 +396,7 @@ __global__ __launch_bounds__(NT) void k_job_fp(const TjJob* __restrict__ jobs, c
                 double s = red[0][tid];
 #pragma unroll
                 for (int w = 1; w < NT / 64; ++w) s += red[w][tid];
-                J.err_part[((size_t)p * ph + j) * J.nct + ct] = s;
+                gp(J.err_part)[((size_t)p * ph + j) * J.nct + ct] = s;
             }
         }
     }
@@ -373,7 +442,7 @@ __global__ __launch_bounds__(NT) void k_job_bp(const TjJob* __restrict__ jobs, c
 #pragma unroll
             for (int r = 0; r < R; ++r)
                 acc[kk][r] = (active && kk < nk && r < nr)
-                                 ? (MASTER ? (A)J.master[(size_t)(j0 + r) * plane + (size_t)(k0 + kk) * W + i]
+                                 ? (MASTER ? ld<A>(J.master, (size_t)(j0 + r) * plane + (size_t)(k0 + kk) * W + i)
                                            : ld<A>(vol, (size_t)(j0 + r) * plane + (size_t)(k0 + kk) * W + i))
                                  : (A)0;
         const A pwm1 = (A)(pw - 1);
@@ -407,18 +476,27 @@ __global__ __launch_bounds__(NT) void k_job_bp(const TjJob* __restrict__ jobs, c
                 const int jj = s_row[g][r], x0 = s_x0[g];
                 const A* row = diff + ((size_t)s_img[g] * ph + max(jj, 0)) * pw;
                 C2* dst = &lds[g][r][0];
-                for (int xx = lane; xx < XW; xx += 64) {
-                    const int xs = x0 + xx;
-                    C2 v;
-                    v.x = (A)0;
-                    v.y = (A)0;
-                    if (jj >= 0 && xs < pw && xx < XW - 1) {
-                        const A ll = row[xs];
-                        const A lr = row[min(xs + 1, pw - 1)];
-                        v.x = ll;
-                        v.y = lr - ll;
+                // SU columns per lane per batch, loads first (addresses clamped
+                // into the row; cells that take no value store 0)
+                for (int c0 = 0; c0 < XW; c0 += 64 * SU) {
+                    A ll[SU], lr[SU];
+#pragma unroll
+                    for (int u = 0; u < SU; ++u) {
+                        const int xs = x0 + c0 + 64 * u + lane;
+                        ll[u] = ld<A>(row, min(xs, pw - 1));
+                        lr[u] = ld<A>(row, min(xs + 1, pw - 1));
                     }
-                    dst[xx] = v;
+#pragma unroll
+                    for (int u = 0; u < SU; ++u) {
+                        const int xx = c0 + 64 * u + lane;
+                        if (xx < XW) {
+                            const bool take = jj >= 0 && x0 + xx < pw && xx < XW - 1;
+                            C2 v;
+                            v.x = take ? ll[u] : (A)0;
+                            v.y = take ? lr[u] - ll[u] : (A)0;
+                            dst[xx] = v;
+                        }
+                    }
                 }
             }
             __syncthreads();
@@ -457,7 +535,7 @@ __global__ __launch_bounds__(NT) void k_job_bp(const TjJob* __restrict__ jobs, c
                         A a = acc[kk][r];
                         if (isnan(a) || isinf(a)) a = (A)0;
                         const size_t o = (size_t)(j0 + r) * plane + (size_t)(k0 + kk) * W + i;
-                        if (MASTER) J.master[o] = (float)a;
+                        if (MASTER) st(J.master, o, (float)a);
                         st(vol, o, a);
                     }
         }

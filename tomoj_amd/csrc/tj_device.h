@@ -49,6 +49,7 @@ struct TjBpRow {
 #define TJ_ACCURACY ((double)1e-10f)
 
 #define TJ_BLOCK 256
+#define TJ_SU 4  // columns per lane per LDS staging batch (loads in flight)
 
 // Stores the message tj_last_error() returns on the calling thread.
 void tj_record_error(const char* msg);

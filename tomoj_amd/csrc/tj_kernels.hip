@@ -563,15 +563,26 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_bp_lds(
                 const bool rok = (s_rflag[gg][r] & 1) != 0;
                 const float* img = images + (size_t)s_img[gg] * npx + (size_t)s_ryb[gg][r] * g.pw;
                 double2* dst = &lds[gg][r][0];
-                for (int xx = lane; xx < XW; xx += 64) {
-                    const int xs = x0s + xx;
-                    double2 v = make_double2(0.0, 0.0);
-                    if (rok && xs < g.pw && xx < XW - 1) {
-                        const double ll = img[xs];
-                        const double lr = img[min(xs + 1, g.pw - 1)];
-                        v = make_double2(ll, lr - ll);
+                // TJ_SU columns per lane per batch, loads first (addresses
+                // clamped into the row; cells that take no value store 0)
+                for (int x0 = 0; x0 < XW; x0 += 64 * TJ_SU) {
+                    float ll[TJ_SU], lr[TJ_SU];
+#pragma unroll
+                    for (int u = 0; u < TJ_SU; ++u) {
+                        const int xs = x0s + x0 + 64 * u + lane;
+                        ll[u] = img[min(xs, g.pw - 1)];
+                        lr[u] = img[min(xs + 1, g.pw - 1)];
                     }
-                    dst[xx] = v;
+#pragma unroll
+                    for (int u = 0; u < TJ_SU; ++u) {
+                        const int xx = x0 + 64 * u + lane;
+                        const int xs = x0s + xx;
+                        if (xx < XW) {
+                            double2 v = make_double2(0.0, 0.0);
+                            if (rok && xs < g.pw && xx < XW - 1) v = make_double2((double)ll[u], (double)lr[u] - (double)ll[u]);
+                            dst[xx] = v;
+                        }
+                    }
                 }
             }
         }
@@ -729,13 +740,29 @@ __device__ __forceinline__ void fp_lds_chunks(const TjGeomDev& g, const float* _
         __syncthreads();
         // wave w stages slices w, w+4, ...; lanes take consecutive columns
         // (32-bit offsets within a plane: the plane D*W is far below 2^31)
+        // SU columns per lane per batch: 4*SU loads in flight before the first
+        // LDS write.  Rows outside the volume read row 0 (a valid address) and
+        // store 0; columns past W read column W-1.
         for (int zz = wave; zz < zn; zz += TJ_BLOCK / 64) {
             const int zo = (z0 + zz) * g.vw;
             float4* dst = lds4 + zz * W1;
-            for (int xx = lane; xx < W1; xx += 64) {
-                const int o = zo + min(xx, g.vw - 1);
-                dst[xx] = make_float4(rin0 ? r0[o] : 0.0f, rin1 ? r1[o] : 0.0f, rin2 ? r2[o] : 0.0f,
-                                      rin3 ? r3[o] : 0.0f);
+            for (int x0 = 0; x0 < W1; x0 += 64 * TJ_SU) {
+                float v[TJ_SU][4];
+#pragma unroll
+                for (int u = 0; u < TJ_SU; ++u) {
+                    const int o = zo + min(x0 + 64 * u + lane, g.vw - 1);
+                    v[u][0] = r0[o];
+                    v[u][1] = r1[o];
+                    v[u][2] = r2[o];
+                    v[u][3] = r3[o];
+                }
+#pragma unroll
+                for (int u = 0; u < TJ_SU; ++u) {
+                    const int xx = x0 + 64 * u + lane;
+                    if (xx < W1)
+                        dst[xx] = make_float4(rin0 ? v[u][0] : 0.0f, rin1 ? v[u][1] : 0.0f, rin2 ? v[u][2] : 0.0f,
+                                              rin3 ? v[u][3] : 0.0f);
+                }
             }
         }
         __syncthreads();
