@@ -9,10 +9,14 @@ OS-SART iteration (every projection forward-projected, differenced and
 backprojected once, 10 volume updates).  Inputs are resident in HBM before the
 timed region.  Extras on the same line: voxel-ray GUPS, the per-kernel HBM
 roofline from HIP events, the literal 512^3 SIRT iteration rate, the weighted
-backprojection pass, and TomoJ's CPU algorithm (the C restatement in oracle/)
-timed on the host cores for one full iteration of the same workload.
+backprojection pass, the precision form (16-bit storage modes against the FP64
+twin, at 512^3), the batched form (64 uneven subtomograms in one graph launch),
+the serving form (a Poisson stream, p50/p99 latency), and TomoJ's CPU
+algorithm (the C restatement in oracle/) timed on the host cores for one full
+iteration of the same workload.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-forms]
+       python bench.py --precision-study W H D N   (one precision study only)
 """
 import argparse
 import json
@@ -106,6 +110,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="also run a plain device copy of BYTES (HBM counter calibration)")
+    ap.add_argument("--no-forms", action="store_true", help="skip the precision/batched/serving forms")
+    ap.add_argument("--precision-study", type=int, nargs=4, metavar=("W", "H", "D", "N"),
+                    help="run only the precision study (30 + 2 iterations, 80 ellipsoids, 2%% noise)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -114,6 +121,13 @@ def main():
 
     import tomoj_amd as tj
     from tomoj_amd import workloads as WL
+
+    if args.precision_study:
+        from tomoj_amd import forms
+        W, H, D, N = args.precision_study
+        res = forms.precision_study(W, H, D, N, iterations=30, refine=2, ellipsoids=80, noise=0.02)
+        print(json.dumps({"precision_study": res}), flush=True)
+        return
 
     w = WL.OSSART_512
     log(f"[bench] workload {w.name}: {w.W}x{w.H}x{w.D}, {w.n} projections, update {w.update}")
@@ -205,6 +219,26 @@ def main():
                                "roofline_frac": WL.algorithmic_bytes_per_iteration(c) * k / dtc / 1e9 / HBM_PEAK_GBS}
         log(f"[bench] SIRT 512^3: {k / dtc:.2f} it/s")
         Q.close()
+
+    if not args.no_forms:
+        from tomoj_amd import forms
+        t0 = time.perf_counter()
+        out["precision_512cube"] = forms.precision_study(512, 512, 512, 121, iterations=10, refine=2,
+                                                         ellipsoids=40, noise=0.01)
+        modes = out["precision_512cube"]["modes"]
+        log(f"[bench] precision 512^3 ({time.perf_counter() - t0:.1f} s): plain "
+            f"{out['precision_512cube']['plain_f32_it_s']:.1f} it/s; " +
+            ", ".join(f"{k} {v['it_s']:.1f} it/s x{v['vs_plain']:.2f} rel-L2 {v.get('rel_l2_vs_f64', float('nan')):.1e}"
+                      for k, v in modes.items()))
+        out["batched"] = {}
+        for st, master in (("f32", False), ("bf16", False), ("bf16", True)):
+            key = st + ("+master" if master else "")
+            out["batched"][key] = forms.batched_run(64, 15, st, master)
+            log(f"[bench] batched 64 jobs {key}: {out['batched'][key]['ms']:.1f} ms for 15 iterations, "
+                f"{out['batched'][key]['gups']:.0f} GUPS")
+        out["serving"] = forms.serving_run(24, 20.0)
+        log(f"[bench] serving 24 requests at 20/s: p50 {out['serving']['p50_s']:.2f} s, "
+            f"p99 {out['serving']['p99_s']:.2f} s, {out['serving']['gups']:.0f} GUPS")
 
     if args.copy_probe > 0:
         cms = tj._lib.copy_probe(args.copy_probe, repeats=3)
