@@ -179,6 +179,13 @@ int tj_apply_weighting(tj_ctx *ctx, float *image);
 int tj_start_iteration(tj_ctx *ctx);
 int tj_end_iteration(tj_ctx *ctx);
 
+/* FISTA wrapper: ReconstructionParameters.getProjector returns
+ * new FistaProjector3D(proj) when fista is set (ReconstructionParameters.java:331).
+ * With `on`, tj_start_iteration keeps the volume and tj_end_iteration applies
+ * FistaProjector3D.endOfIteration (FistaProjector3D.java:43-67); tj_run_ossart
+ * does both around each iteration.  Resets t to 0. */
+int tj_set_fista(tj_ctx *ctx, int on);
+
 /* Projector.getCompletion (Projector.java:122-124); may be called from any thread. */
 double tj_get_completion(tj_ctx *ctx);
 

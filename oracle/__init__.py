@@ -100,6 +100,7 @@ def _L():
             "or_gaussian_noise": (None, [u64, l, P]),
             "or_phantom_volume": (None, [P, i, i, i, i, d, d, d, P]),
             "or_num_procs": (i, []),
+            "or_fista_step": (None, [P, P, l, P]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
@@ -239,6 +240,16 @@ def ossart(g: Geom, vol, projs, eulers, iterations, lam, update, first_index=0,
     if rc != 0:
         raise MemoryError("oracle OS-SART allocation failed")
     return err
+
+
+def fista_step(vol, prev, t: float) -> float:
+    """FistaProjector3D.endOfIteration (FistaProjector3D.java:43-59) on ``vol``
+    (in place) against ``prev`` (the volume at startOfIteration); returns the new t."""
+    assert vol.dtype == np.float32 and vol.flags["C_CONTIGUOUS"]
+    prev = _c(prev, np.float32)
+    tt = np.array([t], np.float64)
+    _L().or_fista_step(_p(vol), _p(prev), vol.size, _p(tt))
+    return float(tt[0])
 
 
 def weighting_mask(sx, sy, tilts_deg, diameter, axis_deg=None) -> np.ndarray:

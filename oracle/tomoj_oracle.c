@@ -976,3 +976,25 @@ void or_phantom_volume(const double *ell, int k, int W, int H, int D, double vcx
 }
 
 int or_num_procs(void) { return omp_get_num_procs(); }
+
+/* ---------------------------------------------------------------------------
+ * FISTA extrapolation, FistaProjector3D.endOfIteration
+ * (tomography/projectors/FistaProjector3D.java:43-59): t <- (1 + sqrt(1 + 4t^2))/2,
+ * factor = (t_prev - 1)/t, and per voxel val = factor * (x - prev) with the
+ * difference of the two floats taken in float (getPixel returns float), then
+ * x += (float)val unless val is NaN (TomoReconstruction2.addToPixel :365-370).
+ * *t carries the state (0 before the first iteration, FistaProjector3D.java:8).
+ * ------------------------------------------------------------------------- */
+void or_fista_step(float *vol, const float *prev, long n, double *t)
+{
+    const double t_1 = *t;
+    const double tn = (1 + sqrt(1 + 4 * t_1 * t_1)) / 2;
+    const double factor = (t_1 - 1) / tn;
+    *t = tn;
+    for (long i = 0; i < n; ++i) {
+        const float d = vol[i] - prev[i];
+        const double val = factor * (double)d;
+        if (val != val) continue;
+        vol[i] = vol[i] + (float)val;
+    }
+}

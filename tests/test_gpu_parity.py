@@ -388,3 +388,34 @@ def test_kernel_variants_bit_exact(tj, oracle, tuning):
     ref_err = oracle.ossart(ogeom(oracle, P), ref, exps, E, 2, 1.0, 11, first_index=3)
     assert same(got, ref), int((got != ref).sum())
     assert np.allclose(err[:, 0], ref_err[:, 0], rtol=1e-12)
+
+
+def test_fista_bit_exact(tj, oracle):
+    """The FISTA wrapper (FistaProjector3D) around OS-SART, through the fused
+    loop and through the SPI hooks, against the oracle's iteration-by-iteration
+    restatement."""
+    from tomoj_amd import reconstruction as R
+    W, H, D = 40, 9, 18
+    tilts = np.arange(-60, 61, 10.0)
+    E = oracle.tilt_eulers(tilts)
+    ell = oracle.ellipsoids(17, 5, W, H, D)
+    exps = oracle.phantom_project(ell, E, W, H).astype(np.float32)
+    iters, update = 4, 5
+    ref = np.zeros((D, H, W), np.float32)
+    g = ogeom(oracle, make(tj, (W, H, D)))
+    t = 0.0
+    for _ in range(iters):
+        prev = ref.copy()
+        oracle.ossart(g, ref, exps, E, 1, 1.0, update, first_index=2)
+        t = oracle.fista_step(ref, prev, t)
+    with make(tj, (W, H, D)) as P:
+        P.load_tilt_series(exps, E)
+        P.setFista(True)
+        P.run_ossart(iters, 1.0, update, first_index=2)
+        fused = P.get_volume()
+        P.reset_volume()
+        P.setFista(True)
+        R.ossart(P, E, iters, 1.0, update, first_index=2)
+        spi = P.get_volume()
+    assert same(fused, ref)
+    assert same(spi, ref)

@@ -372,3 +372,33 @@ def test_weighting_mask_special_cases(oracle):
 def test_noise_statistics(oracle):
     z = oracle.gaussian_noise(12345, 200000)
     assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
+
+
+def test_fista_step_closed_form(oracle):
+    """FistaProjector3D.endOfIteration: t runs 0 -> 1 -> golden ratio -> ...,
+    so the first step restores the start-of-iteration volume (factor -1), the
+    second leaves the volume alone (factor 0), the third extrapolates by
+    (phi - 1) / t3; NaN differences are skipped."""
+    rng = np.random.default_rng(4)
+    prev = rng.standard_normal(64).astype(np.float32)
+    vol = rng.standard_normal(64).astype(np.float32)
+    v = vol.copy()
+    t = oracle.fista_step(v, prev, 0.0)
+    assert t == 1.0 and np.allclose(v, prev, rtol=1e-6, atol=1e-6)
+    for k in range(64):      # x + (float)(-1 * (float)(x - p))
+        assert v[k] == np.float32(vol[k] + np.float32(-float(np.float32(vol[k] - prev[k]))))
+    v = vol.copy()
+    t2 = oracle.fista_step(v, prev, 1.0)
+    assert t2 == (1 + 5 ** 0.5) / 2 and np.array_equal(v, vol)
+    v = vol.copy()
+    t3 = oracle.fista_step(v, prev, t2)
+    assert t3 == (1 + (1 + 4 * t2 * t2) ** 0.5) / 2
+    f = (t2 - 1) / t3
+    for k in range(64):      # float difference, double scale, float add
+        d = np.float32(vol[k] - prev[k])
+        assert v[k] == np.float32(vol[k] + np.float32(f * float(d)))
+    v = vol.copy()
+    p2 = prev.copy()
+    p2[5] = np.nan
+    oracle.fista_step(v, p2, t2)
+    assert v[5] == vol[5]

@@ -101,6 +101,7 @@ _SIGS = {
     "tj_apply_weighting": (C.c_int, [_P, _P]),
     "tj_start_iteration": (C.c_int, [_P]),
     "tj_end_iteration": (C.c_int, [_P]),
+    "tj_set_fista": (C.c_int, [_P, C.c_int]),
     "tj_get_completion": (C.c_double, [_P]),
     "tj_interrupt": (C.c_int, [_P]),
     "tj_run_ossart": (C.c_int, [_P, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, _P]),

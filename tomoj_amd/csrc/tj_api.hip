@@ -34,6 +34,7 @@ hipError_t plane_flags(hipStream_t, const float*, size_t, int, int, uint8_t*);
 hipError_t phantom_project(hipStream_t, const double*, int, const TjProj*, int, int, int, double, double, double*);
 hipError_t add_noise(hipStream_t, const double*, size_t, uint64_t, double, double*, int, float*);
 hipError_t copy(hipStream_t, const void*, void*, size_t);
+hipError_t fista(hipStream_t, float*, const float*, size_t, double);
 hipError_t weighting_init(hipStream_t, int, int, double, const double*, const double*, int, float*);
 hipError_t ramp_init(hipStream_t, int, int, int, double, double, float*);
 hipError_t weighting_apply(hipStream_t, int, int, int, const float*, const int*, const float*, float*, void*, size_t);
@@ -125,6 +126,11 @@ struct tj_ctx {
     void* wspace = nullptr;
     size_t wspace_size = 0;
 
+    // FISTA wrapper (FistaProjector3D): volume at the start of the iteration, t
+    int fista = 0;
+    double fista_t = 0.0;
+    float* prev = nullptr;      // [vh][vd][vw]
+
     // kernel selection and shapes (tj_set_tuning)
     tj_tuning tune{0, 2, 4, 4, 0, 4, 8, 3};
 
@@ -140,6 +146,7 @@ struct tj_ctx {
 
 static void ctx_free(tj_ctx* c)
 {
+    dfree(c->prev);
     dfree(c->vol);
     dfree(c->pflag);
     dfree(c->norm_tab);
@@ -896,8 +903,65 @@ int tj_get_tuning(tj_ctx* c, tj_tuning* t)
     return TJ_OK;
 }
 
-int tj_start_iteration(tj_ctx* c) { return c ? TJ_OK : fail(TJ_EINVAL, "null context"); }
-int tj_end_iteration(tj_ctx* c) { return c ? TJ_OK : fail(TJ_EINVAL, "null context"); }
+// FistaProjector3D.startOfIteration (:62-67): keep the volume of the start of
+// the iteration.
+static int fista_start(tj_ctx* c)
+{
+    if (!c->fista) return TJ_OK;
+    TJ_HIP(hipMemcpyAsync(c->prev, c->vol, c->nvox * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    return TJ_OK;
+}
+
+// FistaProjector3D.endOfIteration (:43-59): t and the extrapolation factor on
+// the host in double, the voxel update on the device; the plane flags are
+// refreshed because an extrapolated voxel may overflow.
+static int fista_end(tj_ctx* c)
+{
+    if (!c->fista) return TJ_OK;
+    const double t_1 = c->fista_t;
+    c->fista_t = (1 + std::sqrt(1 + 4 * t_1 * t_1)) / 2;
+    const double factor = (t_1 - 1) / c->fista_t;
+    TJ_HIP(tjk::fista(c->stream, c->vol, c->prev, c->nvox, factor));
+    TJ_HIP(tjk::plane_flags(c->stream, c->vol, c->plane, 0, c->g.vh, c->pflag));
+    std::vector<uint8_t> hf(c->g.vh + 1);
+    TJ_HIP(hipMemcpyAsync(hf.data(), c->pflag, c->g.vh + 1, hipMemcpyDeviceToHost, c->stream));
+    TJ_HIP(hipStreamSynchronize(c->stream));
+    c->any_nonfinite = false;
+    for (uint8_t f : hf) c->any_nonfinite |= (f != 0);
+    return TJ_OK;
+}
+
+int tj_set_fista(tj_ctx* c, int on)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_set_fista: null context");
+    TJ_HIP(hipSetDevice(c->device));
+    if (on && !c->prev) {
+        if (dalloc(&c->prev, c->nvox) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(TJ_ENOMEM, "tj_set_fista: cannot allocate the previous-iteration volume");
+        }
+    }
+    c->fista = on ? 1 : 0;
+    c->fista_t = 0.0;  // FistaProjector3D.java:8
+    return TJ_OK;
+}
+
+int tj_start_iteration(tj_ctx* c)
+{
+    if (!c) return fail(TJ_EINVAL, "null context");
+    TJ_HIP(hipSetDevice(c->device));
+    int rc = fista_start(c);
+    if (rc) return rc;
+    TJ_HIP(hipStreamSynchronize(c->stream));
+    return TJ_OK;
+}
+
+int tj_end_iteration(tj_ctx* c)
+{
+    if (!c) return fail(TJ_EINVAL, "null context");
+    TJ_HIP(hipSetDevice(c->device));
+    return fista_end(c);
+}
 
 double tj_get_completion(tj_ctx* c) { return c ? c->completion.load() : 0.0; }
 
@@ -960,6 +1024,9 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
     std::vector<std::pair<hipEvent_t, hipEvent_t>> fpev, resev, bpev;
     for (int it = 0; it < iterations; ++it) {
         if (c->interrupted.load()) return fail(TJ_ECANCELLED, "tj_run_ossart: interrupted");
+        // projector.startOfIteration() / endOfIteration() around each
+        // osartIteration (TomoReconstruction2.java:593-595)
+        if ((rc = fista_start(c))) return rc;
         int diffIndex = 0;
         for (int t = 0; t < n; ++t) {
             ++diffIndex;
@@ -1026,6 +1093,7 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
             e5[3] = std::sqrt(errmax / wh);
             e5[4] = imax;
         }
+        if ((rc = fista_end(c))) return rc;
         c->completion.store(c->completion.load() + completion_inc);
     }
     if (c->timing_on) {

@@ -899,6 +899,23 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_fp_lds(
 }
 
 // ---------------------------------------------------------------------------
+// FISTA extrapolation of FistaProjector3D.endOfIteration
+// (FistaProjector3D.java:43-59) for every voxel: the float difference with the
+// volume of the start of the iteration, scaled in double, added back as a
+// float unless it is NaN (TomoReconstruction2.addToPixel :365-370).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TJ_BLOCK) void k_fista(float* __restrict__ vol, const float* __restrict__ prev,
+                                                    size_t n, double factor)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+        const float x = vol[k];
+        const float d = x - prev[k];
+        const double val = factor * (double)d;
+        if (!isnan(val)) vol[k] = x + (float)val;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Volume layout conversion between the host ImageStack layout [z][y][x] and the
 // device layout [y][z][x], for rows [y0, y0+nrows).
 // ---------------------------------------------------------------------------
@@ -1194,6 +1211,14 @@ hipError_t bp(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp, 
     default: TJ_BP_LAUNCH(4) break;
     }
 #undef TJ_BP_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t fista(hipStream_t st, float* vol, const float* prev, size_t n, double factor)
+{
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    const int blocks = (int)std::min<size_t>((n + TJ_BLOCK - 1) / TJ_BLOCK, 8192);
+    k_fista<<<blocks, TJ_BLOCK, 0, st>>>(vol, prev, n, factor);
     return hipGetLastError();
 }
 

@@ -174,6 +174,11 @@ class VoxelProjector3DHIP:
     def endOfIteration(self) -> None:
         call("tj_end_iteration", self._ctx)
 
+    def setFista(self, on: bool = True) -> None:
+        """Wraps the projector like FistaProjector3D (FistaProjector3D.java:5-72):
+        the iteration hooks (and the fused loop) apply FISTA's extrapolation."""
+        call("tj_set_fista", self._ctx, int(bool(on)))
+
     def getCompletion(self) -> float:
         return _lib.load().tj_get_completion(self._ctx)
 
