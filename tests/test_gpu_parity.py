@@ -419,3 +419,56 @@ def test_fista_bit_exact(tj, oracle):
         spi = P.get_volume()
     assert same(fused, ref)
     assert same(spi, ref)
+
+
+def test_parameters_driver_matches_oracle(tj, oracle):
+    """ReconstructionParameters + ReconstructionThreadManager (the CPU-manager
+    path) on the HIP projector: OS-SART and BP bit for bit, WBP within the FP
+    parity bound, FISTA through the factory."""
+    from tomoj_amd import parameters as RP
+    W, H, D = 36, 10, 15
+    tilts = list(np.arange(-60, 61, 12.0))
+    E = oracle.tilt_eulers(tilts)
+    ell = oracle.ellipsoids(23, 5, W, H, D)
+    exps = oracle.phantom_project(ell, E, W, H).astype(np.float32)
+    ts = RP.TiltSeriesData(exps, tilts)
+    g = oracle.make_geom(W, H, D, nthreads=8)
+
+    p = RP.ReconstructionParameters.createOSSARTParameters(W, H, D, 3, 1.0, 4)
+    m = RP.ReconstructionThreadManager(ts)
+    err = m.doRec(p, first_index=1)
+    ref = np.zeros((D, H, W), np.float32)
+    ref_err = oracle.ossart(g, ref, exps, E, 3, 1.0, 4, first_index=1)
+    assert same(m.getRec2(), ref)
+    assert np.allclose(err[:, 0], ref_err[:, 0], rtol=1e-12)
+    m.close()
+
+    b = RP.ReconstructionParameters.createWBPParameters(W, H, D, float("nan"))
+    m = RP.ReconstructionThreadManager(ts)
+    m.doRec(b)
+    ref = np.zeros((D, H, W), np.float32)
+    oracle.wbp(g, ref, exps, E)
+    assert same(m.getRec2(), ref)
+    m.close()
+
+    wp = RP.ReconstructionParameters.createWBPParameters(W, H, D, 0.5)
+    m = RP.ReconstructionThreadManager(ts)
+    m.doRec(wp)
+    mask = oracle.weighting_mask(W, H, tilts, 0.5 * W)
+    ref = np.zeros((D, H, W), np.float32)
+    oracle.wbp(g, ref, exps, E, mask=mask)
+    assert rel_l2(m.getRec2(), ref) <= 1e-5
+    m.close()
+
+    f = RP.ReconstructionParameters.createOSSARTParameters(W, H, D, 3, 1.0, 0)
+    f.setFista(True)
+    m = RP.ReconstructionThreadManager(ts)
+    m.doRec(f)
+    ref = np.zeros((D, H, W), np.float32)
+    t = 0.0
+    for _ in range(3):
+        prev = ref.copy()
+        oracle.ossart(g, ref, exps, E, 1, 1.0, 0)
+        t = oracle.fista_step(ref, prev, t)
+    assert same(m.getRec2(), ref)
+    m.close()

@@ -10,6 +10,8 @@ from ._lib import TjError, TjCancelled, load as load_library
 from .projector import VoxelProjector3DHIP, phantom_ellipsoids
 from .reconstruction import tilt_euler, tilt_eulers, order_shuffle, ossart, wbp
 from . import workloads
+from .parameters import ReconstructionParameters, ReconstructionThreadManager, TiltSeriesData
 
 __all__ = ["VoxelProjector3DHIP", "TjError", "TjCancelled", "load_library", "phantom_ellipsoids",
-           "tilt_euler", "tilt_eulers", "order_shuffle", "ossart", "wbp", "workloads"]
+           "tilt_euler", "tilt_eulers", "order_shuffle", "ossart", "wbp", "workloads",
+           "ReconstructionParameters", "ReconstructionThreadManager", "TiltSeriesData"]

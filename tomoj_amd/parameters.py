@@ -1,0 +1,178 @@
+"""Host mirror of TomoJ's reconstruction parameters, projector factory and
+CPU-manager driver, over the HIP projector.
+
+``ReconstructionParameters`` follows
+src/TomoJ_Base/src/main/java/fr/curie/tomoj/tomography/ReconstructionParameters.java:12-411
+(field names, defaults and the two factories), ``getProjector`` its
+:313-333, and ``ReconstructionThreadManager`` the CPU path of
+ReconstructionThreadManager.java (createRec2 :169-196, doCPUrec :218-255).
+The projections are TomoJ's ``ts.getPixels(i)`` (already normalised/aligned
+by the caller) and the Euler matrices ``getEulerMatrix(ts, i)``.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from .projector import VoxelProjector3DHIP
+from .reconstruction import tilt_eulers
+
+BP = 0
+WBP = 1
+OSSART = 2
+
+
+@dataclass
+class TiltSeriesData:
+    """What the projector path reads from a TiltSeries: the projections
+    (n x ph x pw, float32) and the tilt geometry."""
+    images: np.ndarray
+    tilts: Sequence[float]
+    tilt_axis_deg: float = 0.0
+    axis_vertical: bool = True
+
+    @property
+    def width(self) -> int:
+        return int(self.images.shape[2])
+
+    @property
+    def height(self) -> int:
+        return int(self.images.shape[1])
+
+    def eulers(self) -> np.ndarray:
+        return tilt_eulers(self.tilts, self.tilt_axis_deg, self.axis_vertical)
+
+    def tilt_axes(self) -> np.ndarray:
+        return np.full(len(self.tilts), 0.0 if self.axis_vertical else float(self.tilt_axis_deg))
+
+
+class ReconstructionParameters:
+    """ReconstructionParameters.java:12-411 (reconstruction type, iterations,
+    relaxation, update size, weighting radius, constraints)."""
+
+    def __init__(self, width: int, height: int, depth: int):
+        self.width, self.height, self.depth = int(width), int(height), int(depth)
+        self.recNbRaysPerPixels = 1
+        self.reconstructionType = BP
+        self.nbIterations = 0
+        self.relaxationCoefficient = 0.0
+        self.updateNb = 0
+        self.positivityConstraint = False
+        self.fista = False
+        self.weightingRadius = 0.0
+        self.rescaleData = True            # field default (:35)
+        self.longObjectCompensation = False
+        self.reconstructionCenterModifiers: Optional[Sequence[float]] = None
+        self._tempNbCall = 0
+
+    # factories (:279-311)
+    @staticmethod
+    def createOSSARTParameters(width, height, depth, nbIterations, relaxationCoefficient, updateNb,
+                               reconstructionCenterModifiers=None) -> "ReconstructionParameters":
+        p = ReconstructionParameters(width, height, depth)
+        p.reconstructionType = OSSART
+        p.nbIterations = int(nbIterations)
+        p.relaxationCoefficient = float(relaxationCoefficient)
+        p.updateNb = int(updateNb)
+        p.weightingRadius = float("nan")
+        p.reconstructionCenterModifiers = reconstructionCenterModifiers
+        return p
+
+    @staticmethod
+    def createWBPParameters(width, height, depth, weightingRadius) -> "ReconstructionParameters":
+        p = ReconstructionParameters(width, height, depth)
+        wr = float(weightingRadius)
+        p.reconstructionType = BP if (math.isnan(wr) or wr <= 0) else WBP
+        p.weightingRadius = wr
+        p.updateNb = 0
+        p.nbIterations = 0
+        p.relaxationCoefficient = float("nan")
+        return p
+
+    def getRelaxationCoefficient(self) -> float:
+        """A negative coefficient means the 1/k schedule (:88-91)."""
+        if self.relaxationCoefficient < 0:
+            self._tempNbCall += 1
+            return 1.0 / self._tempNbCall
+        return self.relaxationCoefficient
+
+    def resetTmpNbCall(self) -> None:
+        self._tempNbCall = 0
+
+    def setFista(self, fista: bool) -> None:
+        self.fista = bool(fista)
+
+    def isFista(self) -> bool:
+        return self.fista
+
+    def volume_centers(self) -> List[float]:
+        """TomoReconstruction2 centres (n-1)/2 (:63-71) shifted by the centre
+        modifiers (ReconstructionThreadManager.java:179-182)."""
+        c = [(self.width - 1) / 2.0, (self.height - 1) / 2.0, (self.depth - 1) / 2.0]
+        if self.reconstructionCenterModifiers is not None:
+            c = [c[k] + float(self.reconstructionCenterModifiers[k]) for k in range(3)]
+        return c
+
+    def getProjector(self, ts: TiltSeriesData, device: int = 0) -> VoxelProjector3DHIP:
+        """:313-333: a voxel projector with the weighting filter for WBP, the
+        positivity constraint for OS-SART, the scale ts.W / rec.W when
+        rescaleData, long-object compensation, and the FISTA wrapper."""
+        scale = ts.width / float(self.width) if self.rescaleData else 1.0
+        P = VoxelProjector3DHIP((self.width, self.height, self.depth), (ts.width, ts.height), device=device,
+                                scale=scale,
+                                positivity=(self.reconstructionType == OSSART and self.positivityConstraint),
+                                long_object_compensation=self.longObjectCompensation,
+                                vol_center=self.volume_centers())
+        P.load_tilt_series(ts.images, ts.eulers())
+        if self.reconstructionType == WBP:
+            P.setWeightingFilter(self.weightingRadius * ts.width, ts.tilts, ts.tilt_axes())
+        if self.fista:
+            P.setFista(True)
+        return P
+
+
+class ReconstructionThreadManager:
+    """The CPU-manager driver (ReconstructionThreadManager.java:79-255) with the
+    projector on the GPU: one zero-initialised reconstruction, WBP/BP or
+    OS-SART over all of its rows."""
+
+    def __init__(self, ts: TiltSeriesData, device: int = 0):
+        self.ts = ts
+        self.device = device
+        self.projector: Optional[VoxelProjector3DHIP] = None
+        self.params: Optional[ReconstructionParameters] = None
+
+    def createRec2(self, params: ReconstructionParameters) -> None:
+        if self.projector is not None:
+            self.projector.close()
+        self.params = params
+        self.projector = params.getProjector(self.ts, self.device)
+
+    def doRec(self, params: ReconstructionParameters, first_index: int = 0) -> Optional[np.ndarray]:
+        """doCPUrec (:218-255): WBP/BP return None; OS-SART returns the
+        per-iteration err1 rows.  OSSART reads the relaxation field directly
+        (TomoReconstruction2.java:645), so the 1/k schedule never applies.
+        TomoJ draws the first image of each iteration at random
+        (TomoReconstruction2.java:755); here it is `first_index`."""
+        if self.projector is None or params is not self.params:
+            self.createRec2(params)
+        P = self.projector
+        if params.reconstructionType in (BP, WBP):
+            P.run_wbp()
+            return None
+        if params.reconstructionType == OSSART:
+            return P.run_ossart(params.nbIterations, params.relaxationCoefficient, params.updateNb,
+                                first_index=first_index)
+        raise ValueError(f"reconstruction type {params.reconstructionType} is not on this path")
+
+    def getRec2(self) -> np.ndarray:
+        """The reconstruction in ImageStack layout (D, H, W)."""
+        return self.projector.get_volume()
+
+    def close(self) -> None:
+        if self.projector is not None:
+            self.projector.close()
+            self.projector = None
