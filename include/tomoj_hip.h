@@ -231,6 +231,15 @@ void tj_phantom_ellipsoids(uint64_t seed, int k, int vol_w, int vol_h, int vol_d
 int tj_synth_projections(tj_ctx *ctx, const double *ell, int k, const double *euler, int n,
                          double noise_rel, uint64_t noise_seed);
 
+/* Projection supply with the identity alignment: what TiltSeries.getPixels(i)
+ * returns with normalisation on (TomoJ's default, TiltSeries.java:128),
+ * AffineAlignment.applyTransformOnImage (AffineAlignment.java:136-156):
+ * every resident projection i becomes (float)((x - mean_i) / stdDev_i) with the
+ * statistics of its ROI (TiltSeries.getImageStatistics :809-835, columns
+ * round(W cos(tilt_deg[i])) wide, at least min_size).  stats (may be NULL)
+ * receives n x {mean, stdDev}. */
+int tj_normalize_projections(tj_ctx *ctx, const double *tilt_deg, int min_size, double *stats);
+
 /* Copies resident projection `index` to the host. */
 int tj_get_loaded_projection(tj_ctx *ctx, int index, float *out);
 

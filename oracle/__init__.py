@@ -101,6 +101,9 @@ def _L():
             "or_phantom_volume": (None, [P, i, i, i, i, d, d, d, P]),
             "or_num_procs": (i, []),
             "or_fista_step": (None, [P, P, l, P]),
+            "or_roi_rect": (None, [i, d, i, P, P]),
+            "or_roi_stats": (None, [P, i, i, i, i, P, P]),
+            "or_normalize": (None, [P, l, d, d]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
@@ -250,6 +253,32 @@ def fista_step(vol, prev, t: float) -> float:
     tt = np.array([t], np.float64)
     _L().or_fista_step(_p(vol), _p(prev), vol.size, _p(tt))
     return float(tt[0])
+
+
+def roi_rect(width, tilt_deg, min_size=0):
+    """Columns [x0, x1) of TiltSeries.getImageStatistics' ROI (TiltSeries.java:809-835)
+    with the identity alignment."""
+    a, b = C.c_int(), C.c_int()
+    _L().or_roi_rect(int(width), float(tilt_deg), int(min_size), C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def normalized_projections(images, tilts_deg, min_size=0):
+    """TiltSeries.getPixels(i) with normalisation and the identity alignment
+    (AffineAlignment.java:136-156): (float)((x - mean) / stdDev) with the ROI
+    statistics; returns (images, means, stddevs)."""
+    imgs = np.array(images, np.float32, copy=True)
+    n, h, w = imgs.shape
+    means, sds = np.zeros(n), np.zeros(n)
+    for k in range(n):
+        x0, x1 = roi_rect(w, tilts_deg[k], min_size)
+        m, sd = C.c_double(), C.c_double()
+        _L().or_roi_stats(_p(np.ascontiguousarray(imgs[k])), w, h, x0, x1, C.byref(m), C.byref(sd))
+        means[k], sds[k] = m.value, sd.value
+        row = np.ascontiguousarray(imgs[k])
+        _L().or_normalize(_p(row), row.size, m.value, sd.value)
+        imgs[k] = row
+    return imgs, means, sds
 
 
 def weighting_mask(sx, sy, tilts_deg, diameter, axis_deg=None) -> np.ndarray:

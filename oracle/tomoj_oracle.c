@@ -998,3 +998,50 @@ void or_fista_step(float *vol, const float *prev, long n, double *t)
         vol[i] = vol[i] + (float)val;
     }
 }
+
+/* ---------------------------------------------------------------------------
+ * Projection supply with the identity alignment: TiltSeries.getPixels(i)
+ * (TiltSeries.java:678-687) -> AffineAlignment.applyTransformOnImage
+ * (AffineAlignment.java:136-156) normalises the image by the statistics of a
+ * tilt-dependent ROI (TiltSeries.getImageStatistics :809-835): a rectangle of
+ * roisx = max(round(W cos(tilt)), minSize) columns centred by integer
+ * division, all rows.  Statistics as ImageJ 1.53a FloatStatistics (third
+ * party, restated): double sums over the ROI row by row, mean = sum/n,
+ * stdDev = sqrt(((n*sum2 - sum*sum)/n)/(n-1)) (0 when not positive).
+ * ------------------------------------------------------------------------- */
+void or_roi_rect(int width, double tilt_deg, int min_size, int *x0, int *x1)
+{
+    const double rad = tilt_deg / 180.0 * M_PI;
+    long r = (long)floor(width * cos(rad) + 0.5); /* Math.round */
+    int roisx = (int)(r > min_size ? r : min_size);
+    const double cx = (width - 1.0) / 2;
+    const double p0 = (double)((width - roisx) / 2) - cx; /* int division, then - cx */
+    *x0 = (int)(p0 + cx);
+    *x1 = (int)((p0 + roisx) + cx);
+}
+
+void or_roi_stats(const float *img, int width, int height, int x0, int x1, double *mean, double *sd)
+{
+    double n = 0, sum = 0, sum2 = 0;
+    if (x0 < 0) x0 = 0;
+    if (x1 > width) x1 = width;
+    for (int y = 0; y < height; ++y)
+        for (int x = x0; x < x1; ++x) {
+            const double v = img[(long)y * width + x];
+            sum += v;
+            sum2 += v * v;
+            n += 1;
+        }
+    *mean = n > 0 ? sum / n : 0.0;
+    double s = 0.0;
+    if (n > 0) {
+        s = (n * sum2 - sum * sum) / n;
+        s = (s > 0.0) ? sqrt(s / (n - 1.0)) : 0.0;
+    }
+    *sd = s;
+}
+
+void or_normalize(float *img, long npx, double mean, double sd)
+{
+    for (long i = 0; i < npx; ++i) img[i] = (float)((img[i] - mean) / sd);
+}

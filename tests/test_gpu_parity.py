@@ -472,3 +472,22 @@ def test_parameters_driver_matches_oracle(tj, oracle):
         t = oracle.fista_step(ref, prev, t)
     assert same(m.getRec2(), ref)
     m.close()
+
+
+@pytest.mark.parametrize("min_size", [0, 30])
+def test_normalised_projection_supply(tj, oracle, min_size):
+    """TiltSeries.getPixels with normalisation (identity alignment): ROI
+    statistics reduced in a different order than the sequential Java loop, so
+    the means and deviations agree to 1e-12 and the images to 1 ulp."""
+    rng = np.random.default_rng(12)
+    W, H = 37, 19
+    tilts = [-61.0, -20.5, 0.0, 33.0, 58.0]
+    imgs = (rng.standard_normal((5, H, W)) * 2.5 + 7).astype(np.float32)
+    ref, means, sds = oracle.normalized_projections(imgs, tilts, min_size)
+    with make(tj, (W, H, 8)) as P:
+        P.load_tilt_series(imgs, oracle.tilt_eulers(tilts))
+        stats = P.normalize_projections(tilts, min_size)
+        got = np.stack([P.get_loaded_projection(k) for k in range(5)])
+    assert np.allclose(stats[:, 0], means, rtol=1e-12) and np.allclose(stats[:, 1], sds, rtol=1e-12)
+    ulps = np.abs(got.view(np.int32).astype(np.int64) - ref.view(np.int32).astype(np.int64))
+    assert ulps.max() <= 1 and (ulps == 0).mean() > 0.99

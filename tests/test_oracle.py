@@ -402,3 +402,23 @@ def test_fista_step_closed_form(oracle):
     p2[5] = np.nan
     oracle.fista_step(v, p2, t2)
     assert v[5] == vol[5]
+
+
+def test_normalisation_roi_and_statistics(oracle):
+    """Projection supply with the identity alignment: the ROI is round(W cos t)
+    columns centred by integer division (at least min_size), the statistics
+    are the sample mean and standard deviation (n - 1) of the ROI."""
+    assert oracle.roi_rect(512, 0.0) == (0, 512)
+    assert oracle.roi_rect(512, 60.0) == (128, 384)              # round(256) = 256 columns
+    assert oracle.roi_rect(100, 45.0) == (14, 85)                # round(70.71) = 71, (100-71)/2 = 14
+    assert oracle.roi_rect(100, 89.0, min_size=20) == (40, 60)
+    rng = np.random.default_rng(9)
+    imgs = rng.standard_normal((3, 12, 40)).astype(np.float32) * 3 + 5
+    tilts = [-50.0, 0.0, 30.0]
+    out, means, sds = oracle.normalized_projections(imgs, tilts)
+    for k, t in enumerate(tilts):
+        x0, x1 = oracle.roi_rect(40, t)
+        roi = imgs[k][:, x0:x1].astype(np.float64)
+        assert np.isclose(means[k], roi.mean(), rtol=1e-13)
+        assert np.isclose(sds[k], roi.std(ddof=1), rtol=1e-12)
+        assert np.array_equal(out[k], ((imgs[k].astype(np.float64) - means[k]) / sds[k]).astype(np.float32))

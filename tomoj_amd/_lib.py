@@ -110,6 +110,7 @@ _SIGS = {
     "tj_phantom_ellipsoids": (None, [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
     "tj_synth_projections": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, C.c_double, C.c_uint64]),
     "tj_get_loaded_projection": (C.c_int, [_P, C.c_int, _P]),
+    "tj_normalize_projections": (C.c_int, [_P, _P, C.c_int, _P]),
     "tj_get_timing": (C.c_int, [_P, C.POINTER(tj_timing)]),
     "tj_enable_timing": (C.c_int, [_P, C.c_int]),
     "tj_set_tuning": (C.c_int, [_P, C.POINTER(tj_tuning)]),

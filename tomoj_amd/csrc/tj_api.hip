@@ -35,6 +35,7 @@ hipError_t phantom_project(hipStream_t, const double*, int, const TjProj*, int, 
 hipError_t add_noise(hipStream_t, const double*, size_t, uint64_t, double, double*, int, float*);
 hipError_t copy(hipStream_t, const void*, void*, size_t);
 hipError_t fista(hipStream_t, float*, const float*, size_t, double);
+hipError_t normalize(hipStream_t, float*, int, int, int, const int*, double*);
 hipError_t weighting_init(hipStream_t, int, int, double, const double*, const double*, int, float*);
 hipError_t ramp_init(hipStream_t, int, int, int, double, double, float*);
 hipError_t weighting_apply(hipStream_t, int, int, int, const float*, const int*, const float*, float*, void*, size_t);
@@ -602,6 +603,43 @@ int tj_load_projections(tj_ctx* c, const float* stack, int n, const double* eule
     int rc = alloc_projections(c, n, euler);
     if (rc) return rc;
     TJ_HIP(hipMemcpy(c->exps, stack, sizeof(float) * c->npx * n, hipMemcpyHostToDevice));
+    return TJ_OK;
+}
+
+// TiltSeries.getImageStatistics ROI columns (TiltSeries.java:813-828) with the
+// identity alignment: Math.round(W cos(tilt)) (at least min_size) columns,
+// offset by integer division; the double round trip (x - cx) + cx of the Java
+// code is kept.
+static void roi_columns(int width, double tilt_deg, int min_size, int* x0, int* x1)
+{
+    const double rad = tilt_deg / 180.0 * M_PI;
+    const long r = (long)std::floor(width * std::cos(rad) + 0.5);
+    const int roisx = (int)(r > min_size ? r : min_size);
+    const double cx = (width - 1.0) / 2;
+    const double p0 = (double)((width - roisx) / 2) - cx;
+    *x0 = (int)(p0 + cx);
+    *x1 = (int)((p0 + roisx) + cx);
+}
+
+int tj_normalize_projections(tj_ctx* c, const double* tilt_deg, int min_size, double* stats)
+{
+    if (!c || !tilt_deg) return fail(TJ_EINVAL, "tj_normalize_projections: null argument");
+    if (c->nproj <= 0) return fail(TJ_ESTATE, "tj_normalize_projections: no resident projections");
+    TJ_HIP(hipSetDevice(c->device));
+    std::vector<int> roi(2 * (size_t)c->nproj);
+    for (int p = 0; p < c->nproj; ++p) roi_columns(c->g.pw, tilt_deg[p], min_size, &roi[2 * p], &roi[2 * p + 1]);
+    int* droi = nullptr;
+    double* dstats = nullptr;
+    hipError_t e = dalloc(&droi, roi.size());
+    if (e == hipSuccess) e = dalloc(&dstats, roi.size());
+    if (e == hipSuccess) e = hipMemcpyAsync(droi, roi.data(), sizeof(int) * roi.size(), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = tjk::normalize(c->stream, c->exps, c->g.pw, c->g.ph, c->nproj, droi, dstats);
+    if (e == hipSuccess && stats)
+        e = hipMemcpyAsync(stats, dstats, sizeof(double) * roi.size(), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(droi);
+    dfree(dstats);
+    if (e != hipSuccess) return fail(TJ_EDEVICE, "tj_normalize_projections: %s", hipGetErrorString(e));
     return TJ_OK;
 }
 

@@ -899,6 +899,54 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_fp_lds(
 }
 
 // ---------------------------------------------------------------------------
+// Projection supply with the identity alignment (TiltSeries.getPixels ->
+// AffineAlignment.applyTransformOnImage, AffineAlignment.java:136-156): each
+// image is normalised by the mean and sample standard deviation of its
+// tilt-dependent ROI (TiltSeries.getImageStatistics :809-835; columns
+// [x0, x1), all rows; ImageJ FloatStatistics: double sums, stdDev =
+// sqrt(((n*sum2 - sum^2)/n)/(n-1))).  One block per image, fixed reduction tree.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(TJ_BLOCK) void k_roi_stats(const float* __restrict__ imgs, int pw, int ph,
+                                                        const int* __restrict__ roi, double* __restrict__ stats)
+{
+    __shared__ double sh[TJ_BLOCK];
+    const int p = blockIdx.x;
+    const int x0 = max(roi[2 * p], 0), x1 = min(roi[2 * p + 1], pw);
+    const int rw = max(x1 - x0, 0);
+    const size_t cnt = (size_t)rw * ph;
+    const float* img = imgs + (size_t)p * pw * ph;
+    double sum = 0.0, sum2 = 0.0;
+    for (size_t k = threadIdx.x; k < cnt; k += TJ_BLOCK) {
+        const int y = (int)(k / rw), x = x0 + (int)(k - (size_t)y * rw);
+        const double v = img[(size_t)y * pw + x];
+        sum += v;
+        sum2 += v * v;
+    }
+    sum = block_reduce(sum, sh, AddD());
+    sum2 = block_reduce(sum2, sh, AddD());
+    if (threadIdx.x == 0) {
+        const double n = (double)cnt;
+        double sd = 0.0;
+        if (n > 0) {
+            sd = (n * sum2 - sum * sum) / n;
+            sd = (sd > 0.0) ? sqrt(sd / (n - 1.0)) : 0.0;
+        }
+        stats[2 * p] = n > 0 ? sum / n : 0.0;
+        stats[2 * p + 1] = sd;
+    }
+}
+
+__global__ __launch_bounds__(TJ_BLOCK) void k_normalize(float* __restrict__ imgs, size_t npx,
+                                                        const double* __restrict__ stats)
+{
+    const int p = blockIdx.y;
+    const double mean = stats[2 * p], sd = stats[2 * p + 1];
+    float* img = imgs + (size_t)p * npx;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < npx; k += (size_t)gridDim.x * blockDim.x)
+        img[k] = (float)((img[k] - mean) / sd);
+}
+
+// ---------------------------------------------------------------------------
 // FISTA extrapolation of FistaProjector3D.endOfIteration
 // (FistaProjector3D.java:43-59) for every voxel: the float difference with the
 // volume of the start of the iteration, scaled in double, added back as a
@@ -1211,6 +1259,17 @@ hipError_t bp(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp, 
     default: TJ_BP_LAUNCH(4) break;
     }
 #undef TJ_BP_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t normalize(hipStream_t st, float* imgs, int pw, int ph, int n, const int* roi, double* stats)
+{
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    if (n <= 0) return hipSuccess;
+    const size_t npx = (size_t)pw * ph;
+    k_roi_stats<<<n, TJ_BLOCK, 0, st>>>(imgs, pw, ph, roi, stats);
+    k_normalize<<<dim3((unsigned)std::min<size_t>((npx + TJ_BLOCK - 1) / TJ_BLOCK, 1024), n), TJ_BLOCK, 0, st>>>(
+        imgs, npx, stats);
     return hipGetLastError();
 }
 

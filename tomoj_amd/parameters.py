@@ -6,8 +6,10 @@ src/TomoJ_Base/src/main/java/fr/curie/tomoj/tomography/ReconstructionParameters.
 (field names, defaults and the two factories), ``getProjector`` its
 :313-333, and ``ReconstructionThreadManager`` the CPU path of
 ReconstructionThreadManager.java (createRec2 :169-196, doCPUrec :218-255).
-The projections are TomoJ's ``ts.getPixels(i)`` (already normalised/aligned
-by the caller) and the Euler matrices ``getEulerMatrix(ts, i)``.
+The projections are the tilt series' images: with ``normalize`` they are
+normalised on the device as ``ts.getPixels(i)`` does for unaligned images
+(identity alignment); otherwise they are used as given (TomoJ's
+-noNormalisation).  The Euler matrices are ``getEulerMatrix(ts, i)``.
 """
 from __future__ import annotations
 
@@ -33,6 +35,8 @@ class TiltSeriesData:
     tilts: Sequence[float]
     tilt_axis_deg: float = 0.0
     axis_vertical: bool = True
+    normalize: bool = False            # TomoJ's default is True (TiltSeries.java:128)
+    normalization_min_size: int = 0
 
     @property
     def width(self) -> int:
@@ -127,6 +131,9 @@ class ReconstructionParameters:
                                 long_object_compensation=self.longObjectCompensation,
                                 vol_center=self.volume_centers())
         P.load_tilt_series(ts.images, ts.eulers())
+        if ts.normalize:
+            # getPixels(i) with normalisation, for unaligned images (identity alignment)
+            P.normalize_projections(ts.tilts, ts.normalization_min_size)
         if self.reconstructionType == WBP:
             P.setWeightingFilter(self.weightingRadius * ts.width, ts.tilts, ts.tilt_axes())
         if self.fista:

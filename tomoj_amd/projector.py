@@ -102,6 +102,16 @@ class VoxelProjector3DHIP:
              eulers.shape[0], float(noise_rel), C.c_uint64(int(noise_seed)))
         self.n_loaded = eulers.shape[0]
 
+    def normalize_projections(self, tilts_deg, min_size: int = 0) -> np.ndarray:
+        """TiltSeries.getPixels with normalisation and the identity alignment,
+        applied to the resident projections; returns n x (mean, stdDev)."""
+        t = _f64(tilts_deg)
+        if t.shape[0] != self.n_loaded:
+            raise ValueError(f"{t.shape[0]} tilt angles for {self.n_loaded} projections")
+        stats = np.zeros((self.n_loaded, 2), np.float64)
+        call("tj_normalize_projections", self._ctx, _ptr(t), int(min_size), _ptr(stats))
+        return stats
+
     def get_loaded_projection(self, index: int) -> np.ndarray:
         out = np.empty((self.ph, self.pw), np.float32)
         call("tj_get_loaded_projection", self._ctx, int(index), _ptr(out))
