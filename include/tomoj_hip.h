@@ -320,6 +320,8 @@ typedef struct {
     int bp_zchunk;      /* slices per back tile: 4, 8 or 16 (16 with 2 rows)         */
     int bp_group;       /* images per LDS pass (derived from the shape)              */
     int blocks_per_cu;  /* persistent blocks per compute unit: 1..16                 */
+    int fp_packed;      /* BF16/FP16 storage: forward projection on packed 16-bit
+                           pairs with v_dot2 (8 rows per tile) instead of FP32    */
 } tj_batch_tuning;
 
 #define TJ_RUN_GRAPH 1  /* tj_batch_run: capture the iterations in one HIP graph */

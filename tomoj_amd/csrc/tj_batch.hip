@@ -403,6 +403,264 @@ __global__ __launch_bounds__(NT) void k_job_fp(const TjJob* __restrict__ jobs, c
 }
 
 // ---------------------------------------------------------------------------
+// Forward projection for 16-bit storage on packed pairs.  Each LDS cell holds,
+// for 8 volume rows, the two neighbours (v[x], v[x+1]) of a column as the two
+// 16-bit halves of a word, so a row sample is one v_dot2 of that pair with the
+// weights (1 - xf, xf) accumulated straight into the FP32 ray sum.  The ray
+// sums stay FP32.
+// ---------------------------------------------------------------------------
+typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 b2_t __attribute__((ext_vector_type(2)));
+
+template <typename T> struct Pair;
+// Weights and pairs travel as 32-bit words and are reinterpreted only at the
+// dot instruction: selects and constants of __bf16 vectors were miscompiled
+// (a half of the vector dropped) where plain 32-bit values are not.
+//
+// The fraction is rounded to a multiple of 2^-11 (FP16) or 2^-8 (BF16): then
+// both weights are exact in the 16-bit format and sum to exactly 1, so the
+// sample's error is (q1 - q0) * (w1 - xf), bounded by the local difference
+// times 2^-12 or 2^-9, and a sample of a constant region is exact.
+template <> struct Pair<_Float16> {
+    __device__ static __forceinline__ uint32_t weights(float xf)
+    {
+        const float w1 = rintf(xf * 2048.0f) * (1.0f / 2048.0f);
+        // both values are exact in FP16, so the truncating conversion is exact
+        return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(1.0f - w1, w1));
+    }
+    __device__ static __forceinline__ uint32_t nearest() { return 0x00003C00u; }  // (1, 0)
+    __device__ static __forceinline__ float dot(uint32_t p, uint32_t w, float acc)
+    {
+        return __builtin_amdgcn_fdot2(__builtin_bit_cast(h2_t, p), __builtin_bit_cast(h2_t, w), acc, false);
+    }
+};
+template <> struct Pair<Bf16> {
+    __device__ static __forceinline__ uint32_t weights(float xf)
+    {
+        const float w1 = rintf(xf * 256.0f) * (1.0f / 256.0f);
+        // exact in BF16: the high halves of the FP32 words are the BF16 values
+        return __builtin_amdgcn_perm(__float_as_uint(w1), __float_as_uint(1.0f - w1), 0x07060302u);
+    }
+    // (1, 0).  Materialised in a register on purpose: as a literal the compiler
+    // encodes it as the inline constant 1.0, which the BF16 dot reads as the
+    // FP32 pattern 0x3F800000, i.e. the pair (0, 1).
+    __device__ static __forceinline__ uint32_t nearest()
+    {
+        uint32_t v;
+        asm volatile("v_mov_b32 %0, 0x3f80" : "=v"(v));
+        return v;
+    }
+    __device__ static __forceinline__ float dot(uint32_t p, uint32_t w, float acc)
+    {
+        return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(b2_t, p), __builtin_bit_cast(b2_t, w), acc,
+                                              false);
+    }
+};
+
+template <typename T, int G>
+__device__ __forceinline__ void fpp_accumulate(const uint4 (&c0)[G], const uint4 (&c1)[G], const uint32_t (&w)[G],
+                                               float (&rs)[G][8])
+{
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        rs[k][0] = Pair<T>::dot(c0[k].x, w[k], rs[k][0]);
+        rs[k][1] = Pair<T>::dot(c0[k].y, w[k], rs[k][1]);
+        rs[k][2] = Pair<T>::dot(c0[k].z, w[k], rs[k][2]);
+        rs[k][3] = Pair<T>::dot(c0[k].w, w[k], rs[k][3]);
+        rs[k][4] = Pair<T>::dot(c1[k].x, w[k], rs[k][4]);
+        rs[k][5] = Pair<T>::dot(c1[k].y, w[k], rs[k][5]);
+        rs[k][6] = Pair<T>::dot(c1[k].z, w[k], rs[k][6]);
+        rs[k][7] = Pair<T>::dot(c1[k].w, w[k], rs[k][7]);
+    }
+}
+
+// Slice z with FP64 positions (the nearest-value samples): zl = 0 on the last
+// slice (every row nearest), ymask bit r set for a last-row row r.
+template <typename T, int G>
+__device__ __forceinline__ void fpp_slice_f64(const uint4* cell, int z, int zo, int zero, bool last, double wd,
+                                              unsigned ymask, const double (&bd)[G], const double (&td)[G],
+                                              const int (&zlo)[G], const int (&zhi)[G], float (&rs)[G][8])
+{
+    int a[G];
+    uint32_t w[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        const double xv = fma((double)z, td[k], bd[k]);
+        const double xs = fmin(fmax(xv, 0.0), wd);
+        const int xb = (int)xs;
+        w[k] = last ? Pair<T>::nearest() : Pair<T>::weights((float)(xs - (double)xb));
+        a[k] = ((z >= zlo[k]) && (z <= zhi[k])) ? zo + xb : zero;
+    }
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        const uint4 c0 = cell[2 * a[k]], c1 = cell[2 * a[k] + 1];
+        const uint32_t v[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+            rs[k][r] = Pair<T>::dot(v[r], ((ymask >> r) & 1u) ? Pair<T>::nearest() : w[k], rs[k][r]);
+    }
+}
+
+template <typename T, typename TE, int G>
+__global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, const int4* __restrict__ tiles,
+                                                int ntiles, int lds_cells)
+{
+    constexpr int RB = 8;  // volume rows per tile
+    constexpr int PSU = 2; // columns per lane per staging batch
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint4* cell = reinterpret_cast<uint4*>(smem);  // (z, x): cell[2(zz W + x)] rows 0-3, [+1] rows 4-7
+    __shared__ double red[NT / 64][G * RB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int4 tl = tiles[t];
+        if (tl.x < 0) continue;
+        const TjJob& J = jobs[tl.x];
+        const uint16_t* vol = (const uint16_t*)J.vol;
+        const int W = J.W, D = J.D, pw = J.pw, ph = J.ph, n = J.n;
+        const int zc = min(D, (lds_cells - 2) / (2 * W));
+        const int zero = zc * W;
+        const int p0 = tl.y, j0 = tl.z * RB, ct = tl.w;
+        const int i = ct * NT + tid;
+        const bool col = i < pw;
+        const double ic = col ? i : pw - 1;
+        const size_t plane = (size_t)D * W;
+        bool rin[RB];
+        const uint16_t* rp[RB];
+        unsigned ymask = 0;  // bit r: row r is the last volume row (nearest value in x)
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            const int y = j0 + r + J.dy;
+            rin[r] = (j0 + r < ph) && y >= 0 && y < J.H;
+            rp[r] = vol + (size_t)(rin[r] ? y : 0) * plane;
+            ymask |= (y == J.H - 1) ? (1u << r) : 0u;
+        }
+        float base[G], tn[G], rs[G][RB];
+        double bd[G], td[G];
+        int zlo[G], zhi[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const double* q = J.prm + PRM * min(p0 + k, n - 1);
+            bd[k] = ic * q[0] + q[1];
+            td[k] = q[2];
+            base[k] = (float)bd[k];
+            tn[k] = (float)td[k];
+            line_interval(bd[k], td[k], q[7], -TJ_ACCURACY, (double)W, D, zlo[k], zhi[k]);
+#pragma unroll
+            for (int r = 0; r < RB; ++r) rs[k][r] = 0.0f;
+        }
+        const float wm1 = (float)(W - 1);
+        const double wd = (double)(W - 1);
+        for (int z0 = 0; z0 < D; z0 += zc) {
+            const int zn = min(zc, D - z0);
+            __syncthreads();
+            if (tid < 2) cell[2 * zero + tid] = make_uint4(0u, 0u, 0u, 0u);
+            // wave w stages slices w, w + NT/64, ...; the pair of column x is
+            // (x, min(x+1, W-1)); rows outside the volume read row 0 and store 0
+            for (int zz = wave; zz < zn; zz += NT / 64) {
+                const size_t zo = (size_t)(z0 + zz) * W;
+                uint4* dst = cell + 2 * (size_t)zz * W;
+                for (int x0 = 0; x0 < W; x0 += 64 * PSU) {
+                    uint32_t v[PSU][RB];
+#pragma unroll
+                    for (int u = 0; u < PSU; ++u) {
+                        const int x = x0 + 64 * u + lane;
+                        const size_t oa = zo + min(x, W - 1), ob = zo + min(x + 1, W - 1);
+#pragma unroll
+                        for (int r = 0; r < RB; ++r)
+                            v[u][r] = (uint32_t)gp(rp[r])[oa] | ((uint32_t)gp(rp[r])[ob] << 16);
+                    }
+#pragma unroll
+                    for (int u = 0; u < PSU; ++u) {
+                        const int x = x0 + 64 * u + lane;
+                        if (x < W) {
+                            dst[2 * x] = make_uint4(rin[0] ? v[u][0] : 0u, rin[1] ? v[u][1] : 0u,
+                                                    rin[2] ? v[u][2] : 0u, rin[3] ? v[u][3] : 0u);
+                            dst[2 * x + 1] = make_uint4(rin[4] ? v[u][4] : 0u, rin[5] ? v[u][5] : 0u,
+                                                        rin[6] ? v[u][6] : 0u, rin[7] ? v[u][7] : 0u);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            const int zlast = (z0 + zn == D) ? zn - 1 : zn;
+            const int zend = min(zlast, zn);
+            if (ymask) {
+                for (int zz = 0; zz < zend; ++zz)
+                    fpp_slice_f64<T, G>(cell, z0 + zz, zz * W, zero, false, wd, ymask, bd, td, zlo, zhi, rs);
+            } else {
+                for (int zz = 0; zz < zend; ++zz) {
+                    const int z = z0 + zz;
+                    const int zo = zz * W;
+                    const float zf = (float)z;
+                    int a[G];
+                    uint32_t w[G];
+#pragma unroll
+                    for (int k = 0; k < G; ++k) {
+                        const float xs = fminf(fmaxf(fmaf(zf, tn[k], base[k]), 0.0f), wm1);
+                        const int xb = (int)xs;
+                        w[k] = Pair<T>::weights(__builtin_amdgcn_fractf(xs));
+                        a[k] = ((z >= zlo[k]) && (z <= zhi[k])) ? zo + xb : zero;
+                    }
+                    uint4 c0[G], c1[G];
+#pragma unroll
+                    for (int k = 0; k < G; ++k) {
+                        c0[k] = cell[2 * a[k]];
+                        c1[k] = cell[2 * a[k] + 1];
+                    }
+                    fpp_accumulate<T, G>(c0, c1, w, rs);
+                }
+            }
+            if (zlast < zn)
+                fpp_slice_f64<T, G>(cell, z0 + zlast, zlast * W, zero, true, wd, ymask, bd, td, zlo, zhi, rs);
+        }
+        // projection, norm and difference per (projection, row); error partials
+        double e2[G][RB];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int p = p0 + k;
+            const double ninc = J.prm[PRM * min(p, n - 1) + 3];
+            const int cnt = zhi[k] - zlo[k] + 1;
+            const double nrm = (cnt > 0) ? (double)(cnt + (zhi[k] == D - 1 ? 0 : 1)) * ninc : 0.0;
+#pragma unroll
+            for (int r = 0; r < RB; ++r) {
+                const int j = j0 + r;
+                double v2 = 0.0;
+                if (p < n && col && j < ph) {
+                    const size_t o = ((size_t)p * ph + j) * pw + i;
+                    const float norm = rin[r] ? (float)nrm : 0.0f;
+                    const float th = (norm > 0.0f) ? (float)((double)rs[k][r] * ninc) : NAN;
+                    const float ex = ld<float>((const TE*)J.exps, o);
+                    float d = 0.0f;
+                    if (!isnan(ex) && !isnan(th)) {
+                        const double v = (double)((ex - th) / norm) * J.factor;
+                        d = (float)v;
+                        v2 = v * v;
+                    }
+                    st((float*)J.diff, o, d);
+                }
+                e2[k][r] = wave_sum(v2);
+            }
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < G; ++k)
+#pragma unroll
+                for (int r = 0; r < RB; ++r) red[wave][k * RB + r] = e2[k][r];
+        }
+        __syncthreads();
+        if (tid < G * RB) {
+            const int p = p0 + tid / RB, j = j0 + (tid % RB);
+            if (p < n && j < ph) {
+                double s = red[0][tid];
+#pragma unroll
+                for (int w = 1; w < NT / 64; ++w) s += red[w][tid];
+                gp(J.err_part)[((size_t)p * ph + j) * J.nct + ct] = s;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Backprojection of all N differences of a job (SIRT), added to each voxel in
 // the job's projection order (orderShuffleART), as osartIteration adds them.  Tile (job, 256-column
 // tile, R volume rows, KC slices): each thread keeps the KC*R sums of its voxel
@@ -678,7 +936,7 @@ struct tj_batch {
     int next_id = 1;
     long epoch = 0;
     std::map<int, BJob*> jobs;
-    tj_batch_tuning tune{4, 64, 4, 8, 4, 4};
+    tj_batch_tuning tune{4, 64, 4, 8, 4, 4, 1};
     RunPlan plan;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
@@ -798,6 +1056,17 @@ static hipError_t allow_lds2()
     return e;
 }
 
+template <typename T, typename TE>
+static hipError_t allow_lds_p()
+{
+    hipError_t e = hipFuncSetAttribute((const void*)tjb::k_job_fpp<T, TE, 2>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, kMaxFpLds);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)tjb::k_job_fpp<T, TE, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                kMaxFpLds);
+    return e;
+}
+
 static hipError_t allow_fp_lds()
 {
     hipError_t e = allow_lds2<float, float>();
@@ -806,6 +1075,10 @@ static hipError_t allow_fp_lds()
     if (e == hipSuccess) e = allow_lds2<_Float16, _Float16>();
     if (e == hipSuccess) e = allow_lds2<_Float16, float>();
     if (e == hipSuccess) e = allow_lds<double, double, double, 2>(kMaxFpLds);
+    if (e == hipSuccess) e = allow_lds_p<Bf16, Bf16>();
+    if (e == hipSuccess) e = allow_lds_p<Bf16, float>();
+    if (e == hipSuccess) e = allow_lds_p<_Float16, _Float16>();
+    if (e == hipSuccess) e = allow_lds_p<_Float16, float>();
     return e;
 }
 
@@ -824,11 +1097,38 @@ static int fp_rays_of(const tj_batch_tuning& tu, int storage)
     return tu.fp_rays <= 2 ? 2 : 4;
 }
 
+// packed-pair forward kernel for 16-bit storage (8 rows per tile)
+template <typename T, typename TE>
+static hipError_t launch_fpp_s(hipStream_t st, const tj_batch_tuning& tu, const TjJob* jobs, const int4* tiles,
+                               int ntiles, int grid, int max_w)
+{
+    int lds = std::min(std::max(tu.fp_lds_kb, 16), 128) * 1024;
+    lds = std::max(lds, (2 * max_w + 2) * (int)sizeof(uint4));
+    const int cells = lds / (int)sizeof(uint4);
+    if (tu.fp_rays <= 2) tjb::k_job_fpp<T, TE, 2><<<grid, tjb::NT, lds, st>>>(jobs, tiles, ntiles, cells);
+    else tjb::k_job_fpp<T, TE, 4><<<grid, tjb::NT, lds, st>>>(jobs, tiles, ntiles, cells);
+    return hipGetLastError();
+}
+
+static bool use_packed(const tj_batch_tuning& tu, int storage)
+{
+    return tu.fp_packed && (storage == TJ_STORAGE_BF16 || storage == TJ_STORAGE_F16);
+}
+
+static int fp_rows_of(const tj_batch_tuning& tu, int storage) { return use_packed(tu, storage) ? 8 : 4; }
+
 static hipError_t launch_fp(hipStream_t st, int storage, int pstorage, const tj_batch_tuning& tu,
                             const TjJob* jobs, const int4* tiles, int ntiles, int grid, int max_w)
 {
     (void)hipGetLastError();
     const bool pf = pstorage == TJ_STORAGE_F32;
+    if (use_packed(tu, storage)) {
+        if (storage == TJ_STORAGE_BF16)
+            return pf ? launch_fpp_s<Bf16, float>(st, tu, jobs, tiles, ntiles, grid, max_w)
+                      : launch_fpp_s<Bf16, Bf16>(st, tu, jobs, tiles, ntiles, grid, max_w);
+        return pf ? launch_fpp_s<_Float16, float>(st, tu, jobs, tiles, ntiles, grid, max_w)
+                  : launch_fpp_s<_Float16, _Float16>(st, tu, jobs, tiles, ntiles, grid, max_w);
+    }
     switch (storage) {
     case TJ_STORAGE_F32: return launch_fp_s<float, float>(st, tu, jobs, tiles, ntiles, grid, max_w);
     case TJ_STORAGE_BF16:
@@ -925,12 +1225,12 @@ static std::vector<int4> interleave(std::vector<Group>& groups)
     return out;
 }
 
-static std::vector<int4> fp_tiles(const std::vector<BJob*>& js, const std::vector<int>& idx, int G)
+static std::vector<int4> fp_tiles(const std::vector<BJob*>& js, const std::vector<int>& idx, int G, int RB)
 {
     std::vector<Group> groups;
     for (size_t m = 0; m < js.size(); ++m) {
         const tj_job_desc& d = js[m]->d;
-        const int nct = fp_col_tiles(d.proj_w), nrb = (d.proj_h + 3) / 4;
+        const int nct = fp_col_tiles(d.proj_w), nrb = (d.proj_h + RB - 1) / RB;
         for (int rb = 0; rb < nrb; ++rb) {
             Group g;
             g.cost = (double)d.vol_d * ((double)d.vol_w + (double)tjb::NT * G) * ((d.n_proj + G - 1) / G) * nct;
@@ -1042,7 +1342,7 @@ static int build_plan(tj_batch* b, const std::vector<int>& ids)
                     fidx.push_back(sidx[m]);
                 }
             if (fj.empty()) continue;
-            std::vector<int4> ft = fp_tiles(fj, fidx, fp_rays_of(b->tune, s));
+            std::vector<int4> ft = fp_tiles(fj, fidx, fp_rays_of(b->tune, s), fp_rows_of(b->tune, s));
             int4* dft = nullptr;
             if (int rc = upload(ft, &dft)) return rc;
             int max_w = 0;
@@ -1565,7 +1865,7 @@ int tj_batch_set_tuning(tj_batch* b, const tj_batch_tuning* t)
 {
     if (!b || !t) return bfail(TJ_EINVAL, "null argument");
     if (t->fp_rays < 1 || t->fp_lds_kb < 16 || t->fp_lds_kb > 128 || t->bp_rows < 1 || t->bp_zchunk < 1 ||
-        t->bp_group < 1 || t->blocks_per_cu < 1 || t->blocks_per_cu > 16)
+        t->bp_group < 1 || t->blocks_per_cu < 1 || t->blocks_per_cu > 16 || t->fp_packed < 0 || t->fp_packed > 1)
         return bfail(TJ_EINVAL, "tj_batch_set_tuning: value out of range");
     b->tune = *t;
     return TJ_OK;
@@ -1642,11 +1942,18 @@ int tj_batch_autotune(tj_batch* b, const int* ids, int nids, tj_batch_tuning* be
     const tj_batch_tuning start = b->tune;
     // forward: rays per tile x LDS per tile x persistent blocks per CU
     const int fr[] = {2, 4}, fl[] = {32, 64}, bpc[] = {2, 4, 8};
+    bool any_half = false;
+    for (int id : v) {
+        BJob* j = find_job(b, id);
+        if (j && (j->d.storage == TJ_STORAGE_BF16 || j->d.storage == TJ_STORAGE_F16)) any_half = true;
+    }
     tj_batch_tuning cur = start, bestf = start;
     double tf = 1e300;
+    for (int pk = any_half ? 0 : 1; pk <= 1; ++pk)
     for (int r : fr)
         for (int l : fl)
             for (int c : bpc) {
+                cur.fp_packed = any_half ? pk : start.fp_packed;
                 cur.fp_rays = r;
                 cur.fp_lds_kb = l;
                 cur.blocks_per_cu = c;
@@ -1684,6 +1991,7 @@ int tj_batch_autotune(tj_batch* b, const int* ids, int nids, tj_batch_tuning* be
     tj_batch_tuning fin = bestb;
     fin.fp_rays = bestf.fp_rays;
     fin.fp_lds_kb = bestf.fp_lds_kb;
+    fin.fp_packed = bestf.fp_packed;
     double tot_best = 1e300;
     tj_batch_tuning pick = fin;
     for (int c : {bestf.blocks_per_cu, bestb.blocks_per_cu}) {
