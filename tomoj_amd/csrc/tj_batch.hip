@@ -54,6 +54,7 @@ struct Bf16 {
 // global (address space 1) pointers so the compiler emits global rather than
 // flat instructions (flat loads also count against the LDS wait counter).
 #define TJ_GLOBAL __attribute__((address_space(1)))
+#define TJ_LDS __attribute__((address_space(3)))
 template <typename T>
 __device__ __forceinline__ const TJ_GLOBAL T* gp(const T* p) { return (const TJ_GLOBAL T*)p; }
 template <typename T>
@@ -474,26 +475,28 @@ __device__ __forceinline__ void fpp_accumulate(const uint4 (&c0)[G], const uint4
     }
 }
 
-// Slice z with FP64 positions (the nearest-value samples): zl = 0 on the last
-// slice (every row nearest), ymask bit r set for a last-row row r.
+// Slice z with FP64 positions (the nearest-value samples): `last` on the last
+// slice (every row nearest), ymask bit r set for a last-row row r.  Cell c of
+// the slice (offset zo) holds column x = c - 1; rays outside read cell 0 (zero).
 template <typename T, int G>
-__device__ __forceinline__ void fpp_slice_f64(const uint4* cell, int z, int zo, int zero, bool last, double wd,
-                                              unsigned ymask, const double (&bd)[G], const double (&td)[G],
+__device__ __forceinline__ void fpp_slice_f64(const uint4* cell, int hp, int z, int zo, bool last, double wd,
+                                              unsigned ymask, const double* prm, int p0, int n, double ic,
                                               const int (&zlo)[G], const int (&zhi)[G], float (&rs)[G][8])
 {
     int a[G];
     uint32_t w[G];
 #pragma unroll
     for (int k = 0; k < G; ++k) {
-        const double xv = fma((double)z, td[k], bd[k]);
+        const double* q = prm + PRM * min(p0 + k, n - 1);
+        const double xv = fma((double)z, q[2], ic * q[0] + q[1]);
         const double xs = fmin(fmax(xv, 0.0), wd);
         const int xb = (int)xs;
         w[k] = last ? Pair<T>::nearest() : Pair<T>::weights((float)(xs - (double)xb));
-        a[k] = ((z >= zlo[k]) && (z <= zhi[k])) ? zo + xb : zero;
+        a[k] = ((z >= zlo[k]) && (z <= zhi[k])) ? zo + xb + 1 : zo;
     }
 #pragma unroll
     for (int k = 0; k < G; ++k) {
-        const uint4 c0 = cell[2 * a[k]], c1 = cell[2 * a[k] + 1];
+        const uint4 c0 = cell[a[k]], c1 = cell[hp + a[k]];
         const uint32_t v[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
 #pragma unroll
         for (int r = 0; r < 8; ++r)
@@ -501,14 +504,30 @@ __device__ __forceinline__ void fpp_slice_f64(const uint4* cell, int z, int zo, 
     }
 }
 
+// Forward projection for 16-bit storage on packed pairs (BF16/FP16 forms).
+// Tile (job, G projections, 8 detector rows, 256-column tile).  Per slice the
+// LDS holds W + 2 cells of 32 bytes: cell c is column x = c - 1 of the 8 volume
+// rows as pairs (v[x], v[x+1]) -- (v[W-1], v[W-1]) on the last column, getPixel's
+// nearest value -- and cells 0 and W + 1 are zero.  An interpolated sample
+// clamps its shifted position x + 1 to [0, W + 1] and reads the cell under it,
+// so rays outside the volume read zeros and the inner loop has no compare:
+//   xs = med3(fma(z, tan, base + 1), 0, W + 1), c = (int)xs, w = (1 - f, f)
+// 10 VALU per ray and slice for 8 row samples, one v_dot2 each.  Inside/outside
+// of a sample is decided by the FP32 position here (the ray norms still come
+// from the exact FP64 interval); the last slice and the last volume row, where
+// getPixel takes the nearest value, use FP64 positions as TomoJ computes them.
+// Staging reads 32-bit words (two columns) when W is even, so a pair cell is
+// a word or one byte permutation of two.
 template <typename T, typename TE, int G>
 __global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, const int4* __restrict__ tiles,
                                                 int ntiles, int lds_cells)
 {
     constexpr int RB = 8;  // volume rows per tile
-    constexpr int PSU = 2; // columns per lane per staging batch
+    constexpr int PF = 3;  // staging items (column pairs of one slice) per thread and slab
     extern __shared__ __align__(16) unsigned char smem[];
-    uint4* cell = reinterpret_cast<uint4*>(smem);  // (z, x): cell[2(zz W + x)] rows 0-3, [+1] rows 4-7
+    // cell[zz WC + c]: rows 0-3 of cell c of slice zz; cell[hp + zz WC + c]: rows 4-7
+    // (two planes, so consecutive cells are consecutive 16-byte words)
+    uint4* cell = reinterpret_cast<uint4*>(smem);
     __shared__ double red[NT / 64][G * RB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -517,8 +536,8 @@ __global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, 
         const TjJob& J = jobs[tl.x];
         const uint16_t* vol = (const uint16_t*)J.vol;
         const int W = J.W, D = J.D, pw = J.pw, ph = J.ph, n = J.n;
-        const int zc = min(D, (lds_cells - 2) / (2 * W));
-        const int zero = zc * W;
+        const int WC = W + 2;
+        const int zcl = min(D, lds_cells / (2 * WC));  // slices the LDS holds
         const int p0 = tl.y, j0 = tl.z * RB, ct = tl.w;
         const int i = ct * NT + tid;
         const bool col = i < pw;
@@ -527,58 +546,103 @@ __global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, 
         bool rin[RB];
         const uint16_t* rp[RB];
         unsigned ymask = 0;  // bit r: row r is the last volume row (nearest value in x)
+        bool allin = true;
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
             const int y = j0 + r + J.dy;
             rin[r] = (j0 + r < ph) && y >= 0 && y < J.H;
             rp[r] = vol + (size_t)(rin[r] ? y : 0) * plane;
             ymask |= (y == J.H - 1) ? (1u << r) : 0u;
+            allin = allin && rin[r];
         }
-        float base[G], tn[G], rs[G][RB];
-        double bd[G], td[G];
+        const int half = W >> 1;
+        // word staging (W even, every row inside, a slab within PF items per
+        // thread) is software-pipelined: the next slab's words are loaded into
+        // registers while the current slab is projected
+        const bool words = allin && (W & 1) == 0 && half <= PF * NT;
+        const int zc = words ? min(zcl, (PF * NT) / half) : zcl;
+        const int hp = zc * WC;  // offset of the rows 4-7 plane
+        float base1[G], tn[G], rs[G][RB];
         int zlo[G], zhi[G];
 #pragma unroll
         for (int k = 0; k < G; ++k) {
             const double* q = J.prm + PRM * min(p0 + k, n - 1);
-            bd[k] = ic * q[0] + q[1];
-            td[k] = q[2];
-            base[k] = (float)bd[k];
-            tn[k] = (float)td[k];
-            line_interval(bd[k], td[k], q[7], -TJ_ACCURACY, (double)W, D, zlo[k], zhi[k]);
+            const double bd = ic * q[0] + q[1];
+            base1[k] = (float)(bd + 1.0);
+            tn[k] = (float)q[2];
+            line_interval(bd, q[2], q[7], -TJ_ACCURACY, (double)W, D, zlo[k], zhi[k]);
 #pragma unroll
             for (int r = 0; r < RB; ++r) rs[k][r] = 0.0f;
         }
-        const float wm1 = (float)(W - 1);
+        const float wc1 = (float)(W + 1);
         const double wd = (double)(W - 1);
+        // item q of this thread: m = tid + q NT = (slice zz, column pair x, x + 1)
+        int izz[PF], ix[PF];
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+            const int m = tid + q * NT;
+            izz[q] = words ? m / half : 0;
+            ix[q] = words ? 2 * (m - izz[q] * half) : 0;
+        }
+        uint32_t wa[PF][RB], wb[PF][RB];
+        // cell x + 1 is the word at x; cell x + 2 is (v[x+1], v[x+2]) from the
+        // words at x and x + 2, or (v[W-1], v[W-1]) on the last pair
+        auto fetch = [&](int zs, int znn) {
+#pragma unroll
+            for (int q = 0; q < PF; ++q) {
+                const bool ok = izz[q] < znn;
+                const size_t o0 = (size_t)(zs + (ok ? izz[q] : 0)) * W + ix[q];
+                const size_t o1 = (ix[q] + 2 >= W) ? o0 : o0 + 2;
+#pragma unroll
+                for (int r = 0; r < RB; ++r) {
+                    wa[q][r] = *(const TJ_GLOBAL uint32_t*)(gp(rp[r]) + o0);
+                    wb[q][r] = *(const TJ_GLOBAL uint32_t*)(gp(rp[r]) + o1);
+                }
+            }
+        };
+        if (words) fetch(0, min(zc, D));
         for (int z0 = 0; z0 < D; z0 += zc) {
             const int zn = min(zc, D - z0);
             __syncthreads();
-            if (tid < 2) cell[2 * zero + tid] = make_uint4(0u, 0u, 0u, 0u);
-            // wave w stages slices w, w + NT/64, ...; the pair of column x is
-            // (x, min(x+1, W-1)); rows outside the volume read row 0 and store 0
-            for (int zz = wave; zz < zn; zz += NT / 64) {
-                const size_t zo = (size_t)(z0 + zz) * W;
-                uint4* dst = cell + 2 * (size_t)zz * W;
-                for (int x0 = 0; x0 < W; x0 += 64 * PSU) {
-                    uint32_t v[PSU][RB];
+            for (int q = tid; q < zn; q += NT) {
+                uint4* d = cell + (size_t)q * WC;
+                const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+                d[0] = zero4;
+                d[hp] = zero4;
+                d[W + 1] = zero4;
+                d[hp + W + 1] = zero4;
+            }
+            if (words) {
 #pragma unroll
-                    for (int u = 0; u < PSU; ++u) {
-                        const int x = x0 + 64 * u + lane;
-                        const size_t oa = zo + min(x, W - 1), ob = zo + min(x + 1, W - 1);
-#pragma unroll
-                        for (int r = 0; r < RB; ++r)
-                            v[u][r] = (uint32_t)gp(rp[r])[oa] | ((uint32_t)gp(rp[r])[ob] << 16);
+                for (int q = 0; q < PF; ++q) {
+                    if (izz[q] < zn) {
+                        const uint32_t sel = (ix[q] + 2 >= W) ? 0x03020302u : 0x05040302u;
+                        uint4* d = cell + ((size_t)izz[q] * WC + ix[q] + 1);
+                        d[0] = make_uint4(wa[q][0], wa[q][1], wa[q][2], wa[q][3]);
+                        d[hp] = make_uint4(wa[q][4], wa[q][5], wa[q][6], wa[q][7]);
+                        d[1] = make_uint4(__builtin_amdgcn_perm(wb[q][0], wa[q][0], sel),
+                                          __builtin_amdgcn_perm(wb[q][1], wa[q][1], sel),
+                                          __builtin_amdgcn_perm(wb[q][2], wa[q][2], sel),
+                                          __builtin_amdgcn_perm(wb[q][3], wa[q][3], sel));
+                        d[hp + 1] = make_uint4(__builtin_amdgcn_perm(wb[q][4], wa[q][4], sel),
+                                          __builtin_amdgcn_perm(wb[q][5], wa[q][5], sel),
+                                          __builtin_amdgcn_perm(wb[q][6], wa[q][6], sel),
+                                          __builtin_amdgcn_perm(wb[q][7], wa[q][7], sel));
                     }
+                }
+                if (z0 + zn < D) fetch(z0 + zn, min(zc, D - z0 - zn));
+            } else {
+                // any width, rows outside the volume (they read row 0 and store 0)
+                for (int m = tid; m < zn * W; m += NT) {
+                    const int zz = m / W, x = m - zz * W;
+                    const size_t oa = (size_t)(z0 + zz) * W + x, ob = (size_t)(z0 + zz) * W + min(x + 1, W - 1);
+                    uint32_t v[RB];
 #pragma unroll
-                    for (int u = 0; u < PSU; ++u) {
-                        const int x = x0 + 64 * u + lane;
-                        if (x < W) {
-                            dst[2 * x] = make_uint4(rin[0] ? v[u][0] : 0u, rin[1] ? v[u][1] : 0u,
-                                                    rin[2] ? v[u][2] : 0u, rin[3] ? v[u][3] : 0u);
-                            dst[2 * x + 1] = make_uint4(rin[4] ? v[u][4] : 0u, rin[5] ? v[u][5] : 0u,
-                                                        rin[6] ? v[u][6] : 0u, rin[7] ? v[u][7] : 0u);
-                        }
-                    }
+                    for (int r = 0; r < RB; ++r)
+                        v[r] = rin[r] ? ((uint32_t)gp(rp[r])[oa] | ((uint32_t)gp(rp[r])[ob] << 16)) : 0u;
+                    uint4* d = cell + ((size_t)zz * WC + x + 1);
+                    d[0] = make_uint4(v[0], v[1], v[2], v[3]);
+                    d[hp] = make_uint4(v[4], v[5], v[6], v[7]);
                 }
             }
             __syncthreads();
@@ -586,32 +650,30 @@ __global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, 
             const int zend = min(zlast, zn);
             if (ymask) {
                 for (int zz = 0; zz < zend; ++zz)
-                    fpp_slice_f64<T, G>(cell, z0 + zz, zz * W, zero, false, wd, ymask, bd, td, zlo, zhi, rs);
+                    fpp_slice_f64<T, G>(cell, hp, z0 + zz, zz * WC, false, wd, ymask, J.prm, p0, n, ic, zlo, zhi, rs);
             } else {
                 for (int zz = 0; zz < zend; ++zz) {
-                    const int z = z0 + zz;
-                    const int zo = zz * W;
-                    const float zf = (float)z;
+                    const float zf = (float)(z0 + zz);
+                    const int zo = zz * WC;
                     int a[G];
                     uint32_t w[G];
 #pragma unroll
                     for (int k = 0; k < G; ++k) {
-                        const float xs = fminf(fmaxf(fmaf(zf, tn[k], base[k]), 0.0f), wm1);
-                        const int xb = (int)xs;
+                        const float xs = __builtin_amdgcn_fmed3f(fmaf(zf, tn[k], base1[k]), 0.0f, wc1);
+                        a[k] = zo + (int)xs;
                         w[k] = Pair<T>::weights(__builtin_amdgcn_fractf(xs));
-                        a[k] = ((z >= zlo[k]) && (z <= zhi[k])) ? zo + xb : zero;
                     }
                     uint4 c0[G], c1[G];
 #pragma unroll
                     for (int k = 0; k < G; ++k) {
-                        c0[k] = cell[2 * a[k]];
-                        c1[k] = cell[2 * a[k] + 1];
+                        c0[k] = cell[a[k]];
+                        c1[k] = cell[hp + a[k]];
                     }
                     fpp_accumulate<T, G>(c0, c1, w, rs);
                 }
             }
             if (zlast < zn)
-                fpp_slice_f64<T, G>(cell, z0 + zlast, zlast * W, zero, true, wd, ymask, bd, td, zlo, zhi, rs);
+                fpp_slice_f64<T, G>(cell, hp, z0 + zlast, zlast * WC, true, wd, ymask, J.prm, p0, n, ic, zlo, zhi, rs);
         }
         // projection, norm and difference per (projection, row); error partials
         double e2[G][RB];
@@ -674,7 +736,13 @@ __global__ __launch_bounds__(NT) void k_job_bp(const TjJob* __restrict__ jobs, c
     using C2 = typename V2<A>::t;
     constexpr bool POS = (MODE & 1) != 0;     // positivity after each image
     constexpr bool MASTER = (MODE & 2) != 0;  // FP32 master volume beside the stored one
+    // FP32 cell positions (16-bit storage forms): LDS cell li of an image row
+    // holds column x = x0 + li - 1, zero outside [0, pw); a voxel clamps its
+    // shifted position xx + 1 to [0, pw + 1] and reads the cell under it, so
+    // the slice loop has no compare (inside/outside decided in FP32).
+    constexpr bool FAST = (MODE & 4) != 0;
     constexpr int XW = NT + KC + 8;  // reachable columns + a zero cell at XW-1
+    static_assert(KC % 4 == 0, "slices per tile come in fours");
     __shared__ C2 lds[G][R][XW];
     __shared__ double s_c[G], s_ms[G], s_b2[G], s_ims[G];
     __shared__ int s_x0[G], s_img[G];
@@ -717,7 +785,7 @@ __global__ __launch_bounds__(NT) void k_job_bp(const TjJob* __restrict__ jobs, c
                 const double ka = k0, kb = k0 + KC - 1;
                 const double m = fmin(fmin((double)i0 * c + ka * ms, (double)i0 * c + kb * ms),
                                       fmin((double)ilast * c + ka * ms, (double)ilast * c + kb * ms)) + b2;
-                s_x0[tid] = max(0, (int)floor(m) - 1);
+                s_x0[tid] = FAST ? min(max(0, (int)floor(m)), pw + 1) : max(0, (int)floor(m) - 1);
                 s_c[tid] = c;
                 s_ms[tid] = ms;
                 s_b2[tid] = b2;
@@ -736,19 +804,21 @@ __global__ __launch_bounds__(NT) void k_job_bp(const TjJob* __restrict__ jobs, c
                 C2* dst = &lds[g][r][0];
                 // SU columns per lane per batch, loads first (addresses clamped
                 // into the row; cells that take no value store 0)
+                const int xsh = FAST ? x0 - 1 : x0;  // column of cell 0
                 for (int c0 = 0; c0 < XW; c0 += 64 * SU) {
                     A ll[SU], lr[SU];
 #pragma unroll
                     for (int u = 0; u < SU; ++u) {
-                        const int xs = x0 + c0 + 64 * u + lane;
-                        ll[u] = ld<A>(row, min(xs, pw - 1));
-                        lr[u] = ld<A>(row, min(xs + 1, pw - 1));
+                        const int xs = xsh + c0 + 64 * u + lane;
+                        ll[u] = ld<A>(row, min(max(xs, 0), pw - 1));
+                        lr[u] = ld<A>(row, min(max(xs + 1, 0), pw - 1));
                     }
 #pragma unroll
                     for (int u = 0; u < SU; ++u) {
                         const int xx = c0 + 64 * u + lane;
                         if (xx < XW) {
-                            const bool take = jj >= 0 && x0 + xx < pw && xx < XW - 1;
+                            const int xs = xsh + xx;
+                            const bool take = jj >= 0 && xs >= 0 && xs < pw && (FAST || xx < XW - 1);
                             C2 v;
                             v.x = take ? ll[u] : (A)0;
                             v.y = take ? lr[u] - ll[u] : (A)0;
@@ -758,7 +828,45 @@ __global__ __launch_bounds__(NT) void k_job_bp(const TjJob* __restrict__ jobs, c
                 }
             }
             __syncthreads();
-            for (int g = 0; g < gn; ++g) {
+            for (int g = 0; g < gn && FAST; ++g) {
+                const float ms = (float)s_ms[g];
+                const float xe1 = (float)(ie * s_c[g] + s_b2[g] + 1.0);
+                const float cmax = (float)(pw + 1);
+                // byte address of shifted cell 0 of row 0 (LDS addresses are 32-bit;
+                // uniform, so it lives in a scalar register and a cell address is
+                // one shift-add)
+                const uint32_t cg = __builtin_amdgcn_readfirstlane(
+                    (uint32_t)(size_t)(const TJ_LDS C2*)&lds[g][0][0] - (uint32_t)sizeof(C2) * (uint32_t)s_x0[g]);
+                // 4 slices at a time: positions, then all LDS reads, then the sums
+#pragma unroll
+                for (int k4 = 0; k4 < KC; k4 += 4) {
+                    int c[4];
+                    float xf[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const float xs =
+                            __builtin_amdgcn_fmed3f(fmaf((float)(k0 + k4 + u), ms, xe1), 0.0f, cmax);
+                        c[u] = (int)xs;
+                        xf[u] = __builtin_amdgcn_fractf(xs);
+                    }
+                    typedef A av2 __attribute__((ext_vector_type(2)));
+                    av2 q[4][R];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int r = 0; r < R; ++r)
+                            q[u][r] = ((const TJ_LDS av2*)(cg + (uint32_t)c[u] * (uint32_t)sizeof(C2)))[r * XW];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            A a = acc[k4 + u][r] + fma((A)xf[u], q[u][r].y, q[u][r].x);
+                            if (POS) a = (a < (A)0) ? (A)0 : a;
+                            acc[k4 + u][r] = a;
+                        }
+                }
+            }
+            for (int g = 0; g < gn && !FAST; ++g) {
                 const A ms = (A)s_ms[g];
                 const double xed = ie * s_c[g] + s_b2[g];
                 const A xe = (A)xed;
@@ -1091,28 +1199,31 @@ static hipError_t launch_fp_s(hipStream_t st, const tj_batch_tuning& tu, const T
     return launch_fp_t<T, TE, float, 4>(st, jobs, tiles, ntiles, grid, lds, max_w);
 }
 
+static bool use_packed(const tj_batch_tuning& tu, int storage)
+{
+    return tu.fp_packed && (storage == TJ_STORAGE_BF16 || storage == TJ_STORAGE_F16);
+}
+
+// projections per forward tile: 2 or 4
 static int fp_rays_of(const tj_batch_tuning& tu, int storage)
 {
     if (storage == TJ_STORAGE_F64) return 2;
     return tu.fp_rays <= 2 ? 2 : 4;
 }
 
-// packed-pair forward kernel for 16-bit storage (8 rows per tile)
+// packed-pair forward kernel for 16-bit storage (8 rows per tile); one slice
+// of the widest job needs 2 (W + 2) cells
 template <typename T, typename TE>
 static hipError_t launch_fpp_s(hipStream_t st, const tj_batch_tuning& tu, const TjJob* jobs, const int4* tiles,
                                int ntiles, int grid, int max_w)
 {
     int lds = std::min(std::max(tu.fp_lds_kb, 16), 128) * 1024;
-    lds = std::max(lds, (2 * max_w + 2) * (int)sizeof(uint4));
+    lds = std::max(lds, 2 * (max_w + 2) * (int)sizeof(uint4));
     const int cells = lds / (int)sizeof(uint4);
-    if (tu.fp_rays <= 2) tjb::k_job_fpp<T, TE, 2><<<grid, tjb::NT, lds, st>>>(jobs, tiles, ntiles, cells);
+    if (fp_rays_of(tu, TJ_STORAGE_BF16) == 2)
+        tjb::k_job_fpp<T, TE, 2><<<grid, tjb::NT, lds, st>>>(jobs, tiles, ntiles, cells);
     else tjb::k_job_fpp<T, TE, 4><<<grid, tjb::NT, lds, st>>>(jobs, tiles, ntiles, cells);
     return hipGetLastError();
-}
-
-static bool use_packed(const tj_batch_tuning& tu, int storage)
-{
-    return tu.fp_packed && (storage == TJ_STORAGE_BF16 || storage == TJ_STORAGE_F16);
 }
 
 static int fp_rows_of(const tj_batch_tuning& tu, int storage) { return use_packed(tu, storage) ? 8 : 4; }
@@ -1145,11 +1256,14 @@ static hipError_t launch_fp(hipStream_t st, int storage, int pstorage, const tj_
 struct BpShape {
     int r, kc, g;
 };
+// (rows, slices) from the tuning: rows 2, 4 or 8; slices 4, 8 or 16
 static BpShape bp_shape_of(const tj_batch_tuning& tu, int storage)
 {
     if (storage == TJ_STORAGE_F64) return {2, 8, 2};
     if (tu.bp_rows <= 2) return tu.bp_zchunk >= 16 ? BpShape{2, 16, 2} : BpShape{2, 8, 4};
-    return tu.bp_zchunk <= 4 ? BpShape{4, 4, 4} : BpShape{4, 8, 4};
+    if (tu.bp_rows >= 8) return tu.bp_zchunk >= 16 ? BpShape{8, 16, 2} : BpShape{8, 8, 2};
+    if (tu.bp_zchunk <= 4) return {4, 4, 4};
+    return tu.bp_zchunk >= 16 ? BpShape{4, 16, 4} : BpShape{4, 8, 4};
 }
 
 template <typename T, int MODE>
@@ -1158,11 +1272,15 @@ static hipError_t launch_bp_s(hipStream_t st, const BpShape& s, const TjJob* job
 {
     if (s.r == 2 && s.kc == 16) tjb::k_job_bp<T, float, 2, 16, 2, MODE><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
     else if (s.r == 2) tjb::k_job_bp<T, float, 2, 8, 4, MODE><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
+    else if (s.r == 8 && s.kc == 16) tjb::k_job_bp<T, float, 8, 16, 2, MODE><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
+    else if (s.r == 8) tjb::k_job_bp<T, float, 8, 8, 2, MODE><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
     else if (s.kc == 4) tjb::k_job_bp<T, float, 4, 4, 4, MODE><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
+    else if (s.kc == 16) tjb::k_job_bp<T, float, 4, 16, 4, MODE><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
     else tjb::k_job_bp<T, float, 4, 8, 4, MODE><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
     return hipGetLastError();
 }
 
+// 16-bit storage always takes the FP32-cell-position form (mode bit 2)
 template <typename T>
 static hipError_t launch_bp_m(hipStream_t st, int mode, const BpShape& s, const TjJob* jobs, const int4* tiles,
                               int ntiles, int grid)
@@ -1170,8 +1288,10 @@ static hipError_t launch_bp_m(hipStream_t st, int mode, const BpShape& s, const 
     switch (mode) {
     case 0: return launch_bp_s<T, 0>(st, s, jobs, tiles, ntiles, grid);
     case 1: return launch_bp_s<T, 1>(st, s, jobs, tiles, ntiles, grid);
-    case 2: return launch_bp_s<T, 2>(st, s, jobs, tiles, ntiles, grid);
-    default: return launch_bp_s<T, 3>(st, s, jobs, tiles, ntiles, grid);
+    case 4: return launch_bp_s<T, 4>(st, s, jobs, tiles, ntiles, grid);
+    case 5: return launch_bp_s<T, 5>(st, s, jobs, tiles, ntiles, grid);
+    case 6: return launch_bp_s<T, 6>(st, s, jobs, tiles, ntiles, grid);
+    default: return launch_bp_s<T, 7>(st, s, jobs, tiles, ntiles, grid);
     }
 }
 
@@ -1183,8 +1303,8 @@ static hipError_t launch_bp(hipStream_t st, int storage, int mode, const tj_batc
     const BpShape s = bp_shape_of(tu, storage);
     switch (storage) {
     case TJ_STORAGE_F32: return launch_bp_m<float>(st, mode & 1, s, jobs, tiles, ntiles, grid);
-    case TJ_STORAGE_BF16: return launch_bp_m<Bf16>(st, mode, s, jobs, tiles, ntiles, grid);
-    case TJ_STORAGE_F16: return launch_bp_m<_Float16>(st, mode, s, jobs, tiles, ntiles, grid);
+    case TJ_STORAGE_BF16: return launch_bp_m<Bf16>(st, mode | 4, s, jobs, tiles, ntiles, grid);
+    case TJ_STORAGE_F16: return launch_bp_m<_Float16>(st, mode | 4, s, jobs, tiles, ntiles, grid);
     default:
         if (mode & 1) tjb::k_job_bp<double, double, 2, 8, 2, 1><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
         else tjb::k_job_bp<double, double, 2, 8, 2, 0><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
@@ -1941,7 +2061,7 @@ int tj_batch_autotune(tj_batch* b, const int* ids, int nids, tj_batch_tuning* be
     std::vector<int> v(ids, ids + nids);
     const tj_batch_tuning start = b->tune;
     // forward: rays per tile x LDS per tile x persistent blocks per CU
-    const int fr[] = {2, 4}, fl[] = {32, 64}, bpc[] = {2, 4, 8};
+    const int fr[] = {2, 4}, fl[] = {32, 64, 96}, bpc[] = {2, 4, 8};
     bool any_half = false;
     for (int id : v) {
         BJob* j = find_job(b, id);
@@ -1967,7 +2087,7 @@ int tj_batch_autotune(tj_batch* b, const int* ids, int nids, tj_batch_tuning* be
                 }
             }
     // backprojection: (rows, slices) x persistent blocks per CU, forward fixed
-    const int shapes[][2] = {{4, 8}, {2, 8}, {4, 4}, {2, 16}};
+    const int shapes[][2] = {{4, 8}, {2, 8}, {4, 4}, {2, 16}, {4, 16}, {8, 8}, {8, 16}};
     cur = bestf;
     tj_batch_tuning bestb = bestf;
     double tb = 1e300;
@@ -1975,7 +2095,7 @@ int tj_batch_autotune(tj_batch* b, const int* ids, int nids, tj_batch_tuning* be
         for (int c : bpc) {
             cur.bp_rows = sh[0];
             cur.bp_zchunk = sh[1];
-            cur.bp_group = (sh[1] >= 16) ? 2 : 4;
+            cur.bp_group = (sh[1] >= 16 || sh[0] >= 8) ? 2 : 4;
             cur.blocks_per_cu = c;
             b->tune = cur;
             if (int rc = build_plan(b, v)) return rc;
