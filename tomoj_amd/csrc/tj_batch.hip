@@ -103,6 +103,7 @@ struct TjJob {
     double* err_part;
     const double* prm;   // [n][PRM]: a, b, tan, normInc, e00, e02, b2, 1/tan, 1/e02
     float* master;       // FP32 master volume [y][z][x] when the stored volume is 16-bit (or null)
+    uint32_t* pairs;     // 16-bit storage: differences as FP16 pairs (d[x], d[x+1] - d[x]), [n][ph][pw]
     const int* order;    // [n] order in which the backprojection adds the images
     double* err_hist;    // [hist_cap] sum of diff^2 per iteration
     int* iter;           // iterations done (device counter)
@@ -739,14 +740,26 @@ __global__ __launch_bounds__(NT) void k_job_bp(const TjJob* __restrict__ jobs, c
     // FP32 cell positions (16-bit storage forms): LDS cell li of an image row
     // holds column x = x0 + li - 1, zero outside [0, pw); a voxel clamps its
     // shifted position xx + 1 to [0, pw + 1] and reads the cell under it, so
-    // the slice loop has no compare (inside/outside decided in FP32).
+    // the slice loop has no compare (inside/outside decided in FP32).  The
+    // cells are the FP16 pairs (d[x], d[x+1] - d[x]) of the job's pair image,
+    // so an update is one v_dot2 with the weights (1, f) into the FP32 sum, and
+    // the next group's cells are loaded into registers while the current group
+    // is backprojected.
     constexpr bool FAST = (MODE & 4) != 0;
     constexpr int XW = NT + KC + 8;  // reachable columns + a zero cell at XW-1
     static_assert(KC % 4 == 0, "slices per tile come in fours");
-    __shared__ C2 lds[G][R][XW];
-    __shared__ double s_c[G], s_ms[G], s_b2[G], s_ims[G];
-    __shared__ int s_x0[G], s_img[G];
-    __shared__ int s_row[G][R];
+    constexpr int RPW = (G * R + 3) / 4;  // staged rows per wave
+    constexpr int NB = (XW + 63) / 64;    // 64-cell batches per row
+    constexpr int XWP = NB * 64;          // FP16-pair row stride (whole batches)
+    // FP32/FP64 pairs [G][R][XW]; or, for the FP16-pair form, two buffers of
+    // [G][R][XWP] words filled by direct buffer-to-LDS loads
+    constexpr size_t LDS_BYTES = FAST ? (size_t)2 * G * R * XWP * 4 : (size_t)G * R * XW * sizeof(C2);
+    __shared__ __align__(16) unsigned char lraw[LDS_BYTES];
+    C2(*lds)[R][XW] = reinterpret_cast<C2(*)[R][XW]>(lraw);
+    uint32_t(*ldsf)[G][R][XWP] = reinterpret_cast<uint32_t(*)[G][R][XWP]>(lraw);
+    __shared__ double s_c[2][G], s_ms[2][G], s_b2[2][G], s_ims[2][G];
+    __shared__ int s_x0[2][G], s_img[2][G];
+    __shared__ int s_row[2][G][R];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int4 tl = tiles[t];
@@ -771,13 +784,11 @@ __global__ __launch_bounds__(NT) void k_job_bp(const TjJob* __restrict__ jobs, c
                                  ? (MASTER ? ld<A>(J.master, (size_t)(j0 + r) * plane + (size_t)(k0 + kk) * W + i)
                                            : ld<A>(vol, (size_t)(j0 + r) * plane + (size_t)(k0 + kk) * W + i))
                                  : (A)0;
-        const A pwm1 = (A)(pw - 1);
-        for (int p0 = 0; p0 < n; p0 += G) {
-            const int gn = min(G, n - p0);
-            __syncthreads();
+        // constants of the images p0 .. p0 + gn - 1 into parameter buffer `buf`
+        auto params = [&](int p0, int gn, int buf) {
             if (tid < gn) {
                 const int pj = J.order[p0 + tid];
-                s_img[tid] = pj;
+                s_img[buf][tid] = pj;
                 const double* q = J.prm + PRM * pj;
                 const double c = q[4], ms = q[5], b2 = q[6];
                 // smallest xx of the tile (xx is linear in i and k: a corner);
@@ -785,109 +796,155 @@ __global__ __launch_bounds__(NT) void k_job_bp(const TjJob* __restrict__ jobs, c
                 const double ka = k0, kb = k0 + KC - 1;
                 const double m = fmin(fmin((double)i0 * c + ka * ms, (double)i0 * c + kb * ms),
                                       fmin((double)ilast * c + ka * ms, (double)ilast * c + kb * ms)) + b2;
-                s_x0[tid] = FAST ? min(max(0, (int)floor(m)), pw + 1) : max(0, (int)floor(m) - 1);
-                s_c[tid] = c;
-                s_ms[tid] = ms;
-                s_b2[tid] = b2;
-                s_ims[tid] = q[8];
+                s_x0[buf][tid] = FAST ? min(max(0, (int)floor(m)), pw + 1) : max(0, (int)floor(m) - 1);
+                s_c[buf][tid] = c;
+                s_ms[buf][tid] = ms;
+                s_b2[buf][tid] = b2;
+                s_ims[buf][tid] = q[8];
             }
             if (tid < G * R) {
                 const int g = tid / R, r = tid % R;
                 const int jj = j0 + r + J.dyb;
-                s_row[g][r] = (g < gn && r < nr && jj >= 0 && jj < ph) ? jj : -1;
+                s_row[buf][g][r] = (g < gn && r < nr && jj >= 0 && jj < ph) ? jj : -1;
             }
-            __syncthreads();
-            for (int gr = wave; gr < gn * R; gr += NT / 64) {
-                const int g = gr / R, r = gr - (gr / R) * R;
-                const int jj = s_row[g][r], x0 = s_x0[g];
-                const A* row = diff + ((size_t)s_img[g] * ph + max(jj, 0)) * pw;
-                C2* dst = &lds[g][r][0];
-                // SU columns per lane per batch, loads first (addresses clamped
-                // into the row; cells that take no value store 0)
-                const int xsh = FAST ? x0 - 1 : x0;  // column of cell 0
-                for (int c0 = 0; c0 < XW; c0 += 64 * SU) {
-                    A ll[SU], lr[SU];
+        };
+        if constexpr (FAST) {
+            // wave w stages rows gr = w, w + 4, ... of the group: batch b of a
+            // row is cells 64 b + lane, loaded from the pair image straight into
+            // LDS buffer `buf` by range-checked buffer loads (cells off the row,
+            // or rows outside the detector, read 0)
+            const int wv = __builtin_amdgcn_readfirstlane(wave);
+            auto fetch = [&](int buf, int gn) {
 #pragma unroll
-                    for (int u = 0; u < SU; ++u) {
-                        const int xs = xsh + c0 + 64 * u + lane;
-                        ll[u] = ld<A>(row, min(max(xs, 0), pw - 1));
-                        lr[u] = ld<A>(row, min(max(xs + 1, 0), pw - 1));
+                for (int w = 0; w < RPW; ++w) {
+                    const int gr = wv + 4 * w;
+                    if (gr < G * R) {
+                        const int g = gr / R, r = gr % R;
+                        const int jj = __builtin_amdgcn_readfirstlane(s_row[buf][g][r]);
+                        const int xsh = __builtin_amdgcn_readfirstlane(s_x0[buf][g]) - 1;
+                        const int img = __builtin_amdgcn_readfirstlane(s_img[buf][g]);
+                        const bool okr = gr < gn * R && jj >= 0;
+                        const uint32_t* prow = J.pairs + ((size_t)(okr ? img : 0) * ph + (okr ? jj : 0)) * pw;
+                        const __amdgpu_buffer_rsrc_t rs =
+                            __builtin_amdgcn_make_buffer_rsrc((void*)prow, 0, okr ? pw * 4 : 0, 0x00020000);
+#pragma unroll
+                        for (int bb = 0; bb < NB; ++bb)
+                            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                                rs, (TJ_LDS void*)&ldsf[buf][g][r][64 * bb], 4,
+                                (uint32_t)(xsh + 64 * bb + lane) * 4u, 0, 0, 0);
                     }
+                }
+            };
+            __syncthreads();  // the previous tile is done with the parameters and cells
+            params(0, min(G, n), 0);
+            __syncthreads();
+            fetch(0, min(G, n));
+            int buf = 0;
+            for (int p0 = 0; p0 < n; p0 += G, buf ^= 1) {
+                const int gn = min(G, n - p0);
+                __builtin_amdgcn_s_waitcnt(0);  // this thread's loads into buffer `buf` are done
+                __syncthreads();  // all of them; the other buffer and parameters are free
+                if (p0 + G < n) params(p0 + G, min(G, n - p0 - G), buf ^ 1);
+                __syncthreads();
+                if (p0 + G < n) fetch(buf ^ 1, min(G, n - p0 - G));
+                for (int g = 0; g < gn; ++g) {
+                    const float ms = (float)s_ms[buf][g];
+                    const float xe1 = (float)(ie * s_c[buf][g] + s_b2[buf][g] + 1.0);
+                    const float cmax = (float)(pw + 1);
+                    // byte address of shifted cell 0 of row 0 (LDS addresses are
+                    // 32-bit; uniform, so it lives in a scalar register and a cell
+                    // address is one shift-add)
+                    const uint32_t cg = __builtin_amdgcn_readfirstlane(
+                        (uint32_t)(size_t)(const TJ_LDS uint32_t*)&ldsf[buf][g][0][0] - 4u * (uint32_t)s_x0[buf][g]);
+                    // 4 slices at a time: positions, then all LDS reads, then the sums
 #pragma unroll
-                    for (int u = 0; u < SU; ++u) {
-                        const int xx = c0 + 64 * u + lane;
-                        if (xx < XW) {
-                            const int xs = xsh + xx;
-                            const bool take = jj >= 0 && xs >= 0 && xs < pw && (FAST || xx < XW - 1);
-                            C2 v;
-                            v.x = take ? ll[u] : (A)0;
-                            v.y = take ? lr[u] - ll[u] : (A)0;
-                            dst[xx] = v;
+                    for (int k4 = 0; k4 < KC; k4 += 4) {
+                        int c[4];
+                        h2_t wt[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const float xs =
+                                __builtin_amdgcn_fmed3f(fmaf((float)(k0 + k4 + u), ms, xe1), 0.0f, cmax);
+                            c[u] = (int)xs;
+                            wt[u] = h2_t{(_Float16)1.0f, (_Float16)__builtin_amdgcn_fractf(xs)};
                         }
+                        uint32_t q[4][R];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+#pragma unroll
+                            for (int r = 0; r < R; ++r)
+                                q[u][r] = ((const TJ_LDS uint32_t*)(cg + 4u * (uint32_t)c[u]))[r * XWP];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+#pragma unroll
+                            for (int r = 0; r < R; ++r) {
+                                A a = __builtin_amdgcn_fdot2(__builtin_bit_cast(h2_t, q[u][r]), wt[u],
+                                                             acc[k4 + u][r], false);
+                                if (POS) a = (a < (A)0) ? (A)0 : a;
+                                acc[k4 + u][r] = a;
+                            }
                     }
                 }
             }
-            __syncthreads();
-            for (int g = 0; g < gn && FAST; ++g) {
-                const float ms = (float)s_ms[g];
-                const float xe1 = (float)(ie * s_c[g] + s_b2[g] + 1.0);
-                const float cmax = (float)(pw + 1);
-                // byte address of shifted cell 0 of row 0 (LDS addresses are 32-bit;
-                // uniform, so it lives in a scalar register and a cell address is
-                // one shift-add)
-                const uint32_t cg = __builtin_amdgcn_readfirstlane(
-                    (uint32_t)(size_t)(const TJ_LDS C2*)&lds[g][0][0] - (uint32_t)sizeof(C2) * (uint32_t)s_x0[g]);
-                // 4 slices at a time: positions, then all LDS reads, then the sums
+        } else {
+            const A pwm1 = (A)(pw - 1);
+            for (int p0 = 0; p0 < n; p0 += G) {
+                const int gn = min(G, n - p0);
+                __syncthreads();
+                params(p0, gn, 0);
+                __syncthreads();
+                for (int gr = wave; gr < gn * R; gr += NT / 64) {
+                    const int g = gr / R, r = gr - (gr / R) * R;
+                    const int jj = s_row[0][g][r], x0 = s_x0[0][g];
+                    const A* row = diff + ((size_t)s_img[0][g] * ph + max(jj, 0)) * pw;
+                    C2* dst = &lds[g][r][0];
+                    // SU columns per lane per batch, loads first (addresses clamped
+                    // into the row; cells that take no value store 0)
+                    for (int c0 = 0; c0 < XW; c0 += 64 * SU) {
+                        A ll[SU], lr[SU];
 #pragma unroll
-                for (int k4 = 0; k4 < KC; k4 += 4) {
-                    int c[4];
-                    float xf[4];
+                        for (int u = 0; u < SU; ++u) {
+                            const int xs = x0 + c0 + 64 * u + lane;
+                            ll[u] = ld<A>(row, min(xs, pw - 1));
+                            lr[u] = ld<A>(row, min(xs + 1, pw - 1));
+                        }
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const float xs =
-                            __builtin_amdgcn_fmed3f(fmaf((float)(k0 + k4 + u), ms, xe1), 0.0f, cmax);
-                        c[u] = (int)xs;
-                        xf[u] = __builtin_amdgcn_fractf(xs);
+                        for (int u = 0; u < SU; ++u) {
+                            const int xx = c0 + 64 * u + lane;
+                            if (xx < XW) {
+                                const bool take = jj >= 0 && x0 + xx < pw && xx < XW - 1;
+                                C2 cv;
+                                cv.x = take ? ll[u] : (A)0;
+                                cv.y = take ? lr[u] - ll[u] : (A)0;
+                                dst[xx] = cv;
+                            }
+                        }
                     }
-                    typedef A av2 __attribute__((ext_vector_type(2)));
-                    av2 q[4][R];
+                }
+                __syncthreads();
+                for (int g = 0; g < gn; ++g) {
+                    const A ms = (A)s_ms[0][g];
+                    const double xed = ie * s_c[0][g] + s_b2[0][g];
+                    const A xe = (A)xed;
+                    const int x0 = s_x0[0][g];
+                    // slices whose xx lies in [0, pw) (VoxelProjector3D.java:285)
+                    int klo, khi;
+                    line_interval(xed, s_ms[0][g], s_ims[0][g], 0.0, (double)pw, D, klo, khi);
 #pragma unroll
-                    for (int u = 0; u < 4; ++u)
-#pragma unroll
-                        for (int r = 0; r < R; ++r)
-                            q[u][r] = ((const TJ_LDS av2*)(cg + (uint32_t)c[u] * (uint32_t)sizeof(C2)))[r * XW];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u)
+                    for (int kk = 0; kk < KC; ++kk) {
+                        const A xx = fma((A)(k0 + kk), ms, xe);
+                        const bool vx = (k0 + kk >= klo) && (k0 + kk <= khi);
+                        const A xs = vx ? fmin(fmax(xx, (A)0), pwm1) : (A)0;
+                        const int xb = (int)xs;
+                        const A xf = xs - (A)xb;
+                        const int li = vx ? xb - x0 : XW - 1;
 #pragma unroll
                         for (int r = 0; r < R; ++r) {
-                            A a = acc[k4 + u][r] + fma((A)xf[u], q[u][r].y, q[u][r].x);
+                            const C2 q = lds[g][r][li];
+                            A a = acc[kk][r] + fma(xf, q.y, q.x);
                             if (POS) a = (a < (A)0) ? (A)0 : a;
-                            acc[k4 + u][r] = a;
+                            acc[kk][r] = a;
                         }
-                }
-            }
-            for (int g = 0; g < gn && !FAST; ++g) {
-                const A ms = (A)s_ms[g];
-                const double xed = ie * s_c[g] + s_b2[g];
-                const A xe = (A)xed;
-                const int x0 = s_x0[g];
-                // slices whose xx lies in [0, pw) (VoxelProjector3D.java:285)
-                int klo, khi;
-                line_interval(xed, s_ms[g], s_ims[g], 0.0, (double)pw, D, klo, khi);
-#pragma unroll
-                for (int kk = 0; kk < KC; ++kk) {
-                    const A xx = fma((A)(k0 + kk), ms, xe);
-                    const bool vx = (k0 + kk >= klo) && (k0 + kk <= khi);
-                    const A xs = vx ? fmin(fmax(xx, (A)0), pwm1) : (A)0;
-                    const int xb = (int)xs;
-                    const A xf = xs - (A)xb;
-                    const int li = vx ? xb - x0 : XW - 1;
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        const C2 q = lds[g][r][li];
-                        A a = acc[kk][r] + fma(xf, q.y, q.x);
-                        if (POS) a = (a < (A)0) ? (A)0 : a;
-                        acc[kk][r] = a;
                     }
                 }
             }
@@ -910,22 +967,52 @@ __global__ __launch_bounds__(NT) void k_job_bp(const TjJob* __restrict__ jobs, c
 
 // Sum of squared differences of each job of a run, in a fixed order; appended
 // to the job's error history.  One block per job.
-__global__ __launch_bounds__(NT) void k_job_err(const TjJob* __restrict__ jobs, int njobs)
+constexpr int NTE = 1024;  // threads of the error reduction (one block per job)
+__global__ __launch_bounds__(NTE) void k_job_err(const TjJob* __restrict__ jobs, int njobs)
 {
-    __shared__ double sh[NT / 64];
+    __shared__ double sh[NTE / 64];
     const TjJob& J = jobs[blockIdx.x];
     const size_t m = (size_t)J.n * J.ph * J.nct;
-    double s = 0.0;
-    for (size_t k = threadIdx.x; k < m; k += NT) s += J.err_part[k];
+    // 8 independent partial sums per thread, so 8 loads are in flight
+    double p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    size_t k = threadIdx.x;
+    for (; k + 7 * NTE < m; k += 8 * NTE) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) p[u] += gp(J.err_part)[k + (size_t)u * NTE];
+    }
+    for (int u = 0; k < m; k += NTE, ++u) p[u] += gp(J.err_part)[k];
+    double s = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
     s = wave_sum(s);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
         double tot = sh[0];
-        for (int w = 1; w < NT / 64; ++w) tot += sh[w];
+        for (int w = 1; w < NTE / 64; ++w) tot += sh[w];
         const int it = *J.iter;
         J.err_hist[it % J.hist_cap] = tot;
         *J.iter = it + 1;
+    }
+}
+
+// Differences of the 16-bit forms as FP16 pairs (d[x], d[x+1] - d[x]) for the
+// backprojection's cells; (d[pw-1], 0) on the last column (ImageJ's clamped
+// edge).  Rounded to nearest; magnitudes beyond the FP16 range saturate.
+// Blocks walk detector rows (all images) of job blockIdx.y.
+__global__ __launch_bounds__(NT) void k_job_pairs(const TjJob* __restrict__ jobs)
+{
+    const TjJob& J = jobs[blockIdx.y];
+    const float* diff = (const float*)J.diff;
+    const int pw = J.pw;
+    const size_t rows = (size_t)J.n * J.ph;
+    for (size_t row = blockIdx.x; row < rows; row += gridDim.x) {
+        const size_t o = row * pw;
+        for (int x = threadIdx.x; x < pw; x += NT) {
+            const float d0 = gp(diff)[o + x];
+            const float d1 = (x + 1 < pw) ? gp(diff)[o + x + 1] : d0;
+            const float lim = 65504.0f;
+            const h2_t h = {(_Float16)fminf(fmaxf(d0, -lim), lim), (_Float16)fminf(fmaxf(d1 - d0, -lim), lim)};
+            gp(J.pairs)[o + x] = __builtin_bit_cast(uint32_t, h);
+        }
     }
 }
 
@@ -1005,6 +1092,7 @@ struct BJob {
     tj_job_desc d{};
     void* vol = nullptr;
     float* master = nullptr;  // FP32 master volume (16-bit storage with fp32_master)
+    uint32_t* pairs = nullptr;  // FP16 difference pairs (16-bit storage)
     void* exps = nullptr;
     void* diff = nullptr;
     double* err_part = nullptr;
@@ -1027,7 +1115,7 @@ struct RunPlan {
     std::vector<int4*> dtiles;  // owned tile tables
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
-    // launches of one iteration: kind 0 forward, 1 error, 2 back
+    // launches of one iteration: kind 0 forward, 1 error, 3 difference pairs, 2 back
     struct L {
         int kind, storage, pos, ntiles, grid;  // pos: back mode bits (back), widest volume (forward)
         int4* tiles;
@@ -1068,7 +1156,7 @@ static void free_plan(RunPlan& p)
 
 static void free_job(BJob* j)
 {
-    void* ptrs[] = {j->vol, j->master, j->exps, j->diff, j->err_part, j->prm, j->order, j->err_hist, j->iter};
+    void* ptrs[] = {j->vol, j->master, j->pairs, j->exps, j->diff, j->err_part, j->prm, j->order, j->err_hist, j->iter};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete j;
@@ -1080,8 +1168,10 @@ static size_t job_bytes(const tj_job_desc* d)
     const size_t nvox = (size_t)d->vol_w * d->vol_h * d->vol_d;
     const size_t npx = (size_t)d->proj_w * d->proj_h * d->n_proj;
     const size_t parts = (size_t)d->n_proj * d->proj_h * fp_col_tiles(d->proj_w);
-    const bool master = d->fp32_master && (d->storage == TJ_STORAGE_BF16 || d->storage == TJ_STORAGE_F16);
-    return nvox * (s + (master ? 4 : 0)) + npx * (sp + a) + parts * 8 + (size_t)d->n_proj * (tjb::PRM * 8 + 4) +
+    const bool half = d->storage == TJ_STORAGE_BF16 || d->storage == TJ_STORAGE_F16;
+    const bool master = d->fp32_master && half;
+    return nvox * (s + (master ? 4 : 0)) + npx * (sp + a + (half ? 4 : 0)) + parts * 8 +
+           (size_t)d->n_proj * (tjb::PRM * 8 + 4) +
            (size_t)std::max(d->max_iterations, 1) * 8 + 256;
 }
 
@@ -1393,6 +1483,7 @@ static TjJob dev_desc(const BJob* j)
     TjJob t{};
     t.vol = j->vol;
     t.master = j->master;
+    t.pairs = j->pairs;
     t.exps = j->exps;
     t.diff = j->diff;
     t.err_part = j->err_part;
@@ -1486,6 +1577,16 @@ static int build_plan(tj_batch* b, const std::vector<int>& ids)
     }
     p.launches = fwd;
     p.launches.push_back({1, 0, 0, 0, (int)js.size(), nullptr, 0, (int)js.size(), 0});
+    {
+        // 16-bit jobs are contiguous in the descriptor table (sorted by storage)
+        int h0 = -1, hn = 0;
+        for (size_t m = 0; m < js.size(); ++m)
+            if (js[m]->pairs) {
+                if (h0 < 0) h0 = (int)m;
+                ++hn;
+            }
+        if (hn > 0) p.launches.push_back({3, 0, 0, 0, 0, nullptr, h0, hn, 0});
+    }
     p.launches.insert(p.launches.end(), back.begin(), back.end());
     p.ids = ids;
     p.tune = b->tune;
@@ -1500,7 +1601,11 @@ static hipError_t enqueue_launch(tj_batch* b, const RunPlan::L& l)
         return launch_fp(b->stream, l.storage, l.pstorage, p.tune, p.djobs, l.tiles, l.ntiles, l.grid, l.pos);
     if (l.kind == 2) return launch_bp(b->stream, l.storage, l.pos, p.tune, p.djobs, l.tiles, l.ntiles, l.grid);
     (void)hipGetLastError();
-    tjb::k_job_err<<<l.njobs, tjb::NT, 0, b->stream>>>(p.djobs + l.job0, l.njobs);
+    if (l.kind == 3) {
+        tjb::k_job_pairs<<<dim3(2048, l.njobs), tjb::NT, 0, b->stream>>>(p.djobs + l.job0);
+        return hipGetLastError();
+    }
+    tjb::k_job_err<<<l.njobs, tjb::NTE, 0, b->stream>>>(p.djobs + l.job0, l.njobs);
     return hipGetLastError();
 }
 
@@ -1636,6 +1741,8 @@ int tj_batch_add_job(tj_batch* b, const tj_job_desc* d, const double* euler, int
     }
     if (e == hipSuccess) e = hipMalloc((void**)&j->exps, npx * sp);
     if (e == hipSuccess) e = hipMalloc(&j->diff, npx * a);
+    if (e == hipSuccess && (d->storage == TJ_STORAGE_BF16 || d->storage == TJ_STORAGE_F16))
+        e = hipMalloc((void**)&j->pairs, npx * 4);
     if (e == hipSuccess) e = hipMalloc((void**)&j->err_part, parts * 8);
     if (e == hipSuccess) e = hipMalloc((void**)&j->prm, j->hprm.size() * 8);
     if (e == hipSuccess) e = hipMalloc((void**)&j->err_hist, (size_t)d->max_iterations * 8);
@@ -1858,7 +1965,10 @@ int tj_batch_convert(tj_batch* b, int id, int storage)
     const size_t nvox = (size_t)nd.vol_w * nd.vol_h * nd.vol_d;
     const size_t npx = (size_t)nd.proj_w * nd.proj_h * nd.n_proj;
     void *vol = nullptr, *exps = nullptr, *diff = nullptr;
+    uint32_t* pairs = nullptr;
+    const bool half_new = storage == TJ_STORAGE_BF16 || storage == TJ_STORAGE_F16;
     hipError_t e = hipMalloc(&vol, nvox * storage_bytes(storage));
+    if (e == hipSuccess && half_new && !j->pairs) e = hipMalloc((void**)&pairs, npx * 4);
     const int ps_old = proj_storage(j->d), ps_new = proj_storage(nd);
     if (e == hipSuccess) e = hipMalloc(&exps, npx * storage_bytes(ps_new));
     if (e == hipSuccess && arith_bytes(storage) != arith_bytes(j->d.storage))
@@ -1874,6 +1984,7 @@ int tj_batch_convert(tj_batch* b, int id, int storage)
         if (vol) (void)hipFree(vol);
         if (exps) (void)hipFree(exps);
         if (diff) (void)hipFree(diff);
+        if (pairs) (void)hipFree(pairs);
         return bfail(TJ_EDEVICE, "tj_batch_convert: %s", hipGetErrorString(e));
     }
     (void)hipFree(j->vol);
@@ -1883,6 +1994,11 @@ int tj_batch_convert(tj_batch* b, int id, int storage)
     if (diff) {
         (void)hipFree(j->diff);
         j->diff = diff;
+    }
+    if (pairs) j->pairs = pairs;
+    if (!half_new && j->pairs) {
+        (void)hipFree(j->pairs);
+        j->pairs = nullptr;
     }
     // a 16-bit target keeps the master when the job asked for one
     if (j->master && !(storage == TJ_STORAGE_BF16 || storage == TJ_STORAGE_F16)) {
