@@ -423,7 +423,21 @@ template <typename T> struct Pair;
 // both weights are exact in the 16-bit format and sum to exactly 1, so the
 // sample's error is (q1 - q0) * (w1 - xf), bounded by the local difference
 // times 2^-12 or 2^-9, and a sample of a constant region is exact.
+// magic(W): adding it to a position in [0, W + 1] rounds the position to the
+// weight grid (2^-11 for FP16, 2^-8 for BF16) in one FP32 addition, after which
+// the integer part is the cell (plus the magic) and the fraction w1 is exact;
+// of(w1) is the pair (1 - w1, w1).
 template <> struct Pair<_Float16> {
+    __device__ static __forceinline__ float magic(int W)
+    {
+        int m = 4096;
+        while (m < W + 2) m <<= 1;
+        return (float)m;
+    }
+    __device__ static __forceinline__ uint32_t of(float w1)
+    {
+        return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(1.0f - w1, w1));
+    }
     __device__ static __forceinline__ uint32_t weights(float xf)
     {
         const float w1 = rintf(xf * 2048.0f) * (1.0f / 2048.0f);
@@ -437,6 +451,11 @@ template <> struct Pair<_Float16> {
     }
 };
 template <> struct Pair<Bf16> {
+    __device__ static __forceinline__ float magic(int) { return 32768.0f; }
+    __device__ static __forceinline__ uint32_t of(float w1)
+    {
+        return __builtin_amdgcn_perm(__float_as_uint(w1), __float_as_uint(1.0f - w1), 0x07060302u);
+    }
     __device__ static __forceinline__ uint32_t weights(float xf)
     {
         const float w1 = rintf(xf * 256.0f) * (1.0f / 256.0f);
@@ -479,8 +498,13 @@ __device__ __forceinline__ void fpp_accumulate(const uint4 (&c0)[G], const uint4
 // Slice z with FP64 positions (the nearest-value samples): `last` on the last
 // slice (every row nearest), ymask bit r set for a last-row row r.  Cell c of
 // the slice (offset zo) holds column x = c - 1; rays outside read cell 0 (zero).
+// LDS index of cell c in a slice: even cells first, odd cells from HB on (HB
+// is 4 mod 8 cells, so two neighbouring cells lie 64 bytes apart in the bank
+// space and rays 1 or 2 cells apart read distinct banks)
+__device__ __forceinline__ int fpp_cidx(int c, int hb) { return (c >> 1) + (c & 1) * hb; }
+
 template <typename T, int G>
-__device__ __forceinline__ void fpp_slice_f64(const uint4* cell, int hp, int z, int zo, bool last, double wd,
+__device__ __forceinline__ void fpp_slice_f64(const uint4* cell, int hp, int hb, int z, int zo, bool last, double wd,
                                               unsigned ymask, const double* prm, int p0, int n, double ic,
                                               const int (&zlo)[G], const int (&zhi)[G], float (&rs)[G][8])
 {
@@ -493,7 +517,7 @@ __device__ __forceinline__ void fpp_slice_f64(const uint4* cell, int hp, int z, 
         const double xs = fmin(fmax(xv, 0.0), wd);
         const int xb = (int)xs;
         w[k] = last ? Pair<T>::nearest() : Pair<T>::weights((float)(xs - (double)xb));
-        a[k] = ((z >= zlo[k]) && (z <= zhi[k])) ? zo + xb + 1 : zo;
+        a[k] = zo + fpp_cidx(((z >= zlo[k]) && (z <= zhi[k])) ? xb + 1 : 0, hb);
     }
 #pragma unroll
     for (int k = 0; k < G; ++k) {
@@ -537,8 +561,10 @@ __global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, 
         const TjJob& J = jobs[tl.x];
         const uint16_t* vol = (const uint16_t*)J.vol;
         const int W = J.W, D = J.D, pw = J.pw, ph = J.ph, n = J.n;
-        const int WC = W + 2;
-        const int zcl = min(D, lds_cells / (2 * WC));  // slices the LDS holds
+        const int WC = W + 2;                              // cells of a slice
+        const int hb = (((WC + 1) / 2 + 3) & ~7) + 4;       // first odd cell: >= even count, 4 mod 8
+        const int ss = hb + WC / 2;                        // slice stride (cells)
+        const int zcl = min(D, lds_cells / (2 * ss));      // slices the LDS holds
         const int p0 = tl.y, j0 = tl.z * RB, ct = tl.w;
         const int i = ct * NT + tid;
         const bool col = i < pw;
@@ -562,7 +588,7 @@ __global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, 
         // registers while the current slab is projected
         const bool words = allin && (W & 1) == 0 && half <= PF * NT;
         const int zc = words ? min(zcl, (PF * NT) / half) : zcl;
-        const int hp = zc * WC;  // offset of the rows 4-7 plane
+        const int hp = zc * ss;  // offset of the rows 4-7 plane
         float base1[G], tn[G], rs[G][RB];
         int zlo[G], zhi[G];
 #pragma unroll
@@ -585,6 +611,12 @@ __global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, 
             izz[q] = words ? m / half : 0;
             ix[q] = words ? 2 * (m - izz[q] * half) : 0;
         }
+        // the volume rows as buffers (uniform, so the loads take a scalar base
+        // and a 32-bit offset)
+        __amdgpu_buffer_rsrc_t rsc[RB];
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+            rsc[r] = __builtin_amdgcn_make_buffer_rsrc((void*)rp[r], 0, (int)(plane * 2), 0x00020000);
         uint32_t wa[PF][RB], wb[PF][RB];
         // cell x + 1 is the word at x; cell x + 2 is (v[x+1], v[x+2]) from the
         // words at x and x + 2, or (v[W-1], v[W-1]) on the last pair
@@ -592,12 +624,12 @@ __global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, 
 #pragma unroll
             for (int q = 0; q < PF; ++q) {
                 const bool ok = izz[q] < znn;
-                const size_t o0 = (size_t)(zs + (ok ? izz[q] : 0)) * W + ix[q];
-                const size_t o1 = (ix[q] + 2 >= W) ? o0 : o0 + 2;
+                const uint32_t o0 = (uint32_t)((zs + (ok ? izz[q] : 0)) * W + ix[q]) * 2u;
+                const uint32_t o1 = (ix[q] + 2 >= W) ? o0 : o0 + 4u;
 #pragma unroll
                 for (int r = 0; r < RB; ++r) {
-                    wa[q][r] = *(const TJ_GLOBAL uint32_t*)(gp(rp[r]) + o0);
-                    wb[q][r] = *(const TJ_GLOBAL uint32_t*)(gp(rp[r]) + o1);
+                    wa[q][r] = __builtin_amdgcn_raw_buffer_load_b32(rsc[r], o0, 0, 0);
+                    wb[q][r] = __builtin_amdgcn_raw_buffer_load_b32(rsc[r], o1, 0, 0);
                 }
             }
         };
@@ -606,29 +638,31 @@ __global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, 
             const int zn = min(zc, D - z0);
             __syncthreads();
             for (int q = tid; q < zn; q += NT) {
-                uint4* d = cell + (size_t)q * WC;
+                uint4* d = cell + (size_t)q * ss;
                 const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
-                d[0] = zero4;
-                d[hp] = zero4;
-                d[W + 1] = zero4;
-                d[hp + W + 1] = zero4;
+                d[fpp_cidx(0, hb)] = zero4;
+                d[hp + fpp_cidx(0, hb)] = zero4;
+                d[fpp_cidx(W + 1, hb)] = zero4;
+                d[hp + fpp_cidx(W + 1, hb)] = zero4;
             }
             if (words) {
 #pragma unroll
                 for (int q = 0; q < PF; ++q) {
                     if (izz[q] < zn) {
                         const uint32_t sel = (ix[q] + 2 >= W) ? 0x03020302u : 0x05040302u;
-                        uint4* d = cell + ((size_t)izz[q] * WC + ix[q] + 1);
-                        d[0] = make_uint4(wa[q][0], wa[q][1], wa[q][2], wa[q][3]);
-                        d[hp] = make_uint4(wa[q][4], wa[q][5], wa[q][6], wa[q][7]);
-                        d[1] = make_uint4(__builtin_amdgcn_perm(wb[q][0], wa[q][0], sel),
-                                          __builtin_amdgcn_perm(wb[q][1], wa[q][1], sel),
-                                          __builtin_amdgcn_perm(wb[q][2], wa[q][2], sel),
-                                          __builtin_amdgcn_perm(wb[q][3], wa[q][3], sel));
-                        d[hp + 1] = make_uint4(__builtin_amdgcn_perm(wb[q][4], wa[q][4], sel),
-                                          __builtin_amdgcn_perm(wb[q][5], wa[q][5], sel),
-                                          __builtin_amdgcn_perm(wb[q][6], wa[q][6], sel),
-                                          __builtin_amdgcn_perm(wb[q][7], wa[q][7], sel));
+                        // cell x + 1 (odd) and cell x + 2 (even)
+                        uint4* d1 = cell + ((size_t)izz[q] * ss + hb + (ix[q] >> 1));
+                        uint4* d2 = cell + ((size_t)izz[q] * ss + (ix[q] >> 1) + 1);
+                        d1[0] = make_uint4(wa[q][0], wa[q][1], wa[q][2], wa[q][3]);
+                        d1[hp] = make_uint4(wa[q][4], wa[q][5], wa[q][6], wa[q][7]);
+                        d2[0] = make_uint4(__builtin_amdgcn_perm(wb[q][0], wa[q][0], sel),
+                                           __builtin_amdgcn_perm(wb[q][1], wa[q][1], sel),
+                                           __builtin_amdgcn_perm(wb[q][2], wa[q][2], sel),
+                                           __builtin_amdgcn_perm(wb[q][3], wa[q][3], sel));
+                        d2[hp] = make_uint4(__builtin_amdgcn_perm(wb[q][4], wa[q][4], sel),
+                                            __builtin_amdgcn_perm(wb[q][5], wa[q][5], sel),
+                                            __builtin_amdgcn_perm(wb[q][6], wa[q][6], sel),
+                                            __builtin_amdgcn_perm(wb[q][7], wa[q][7], sel));
                     }
                 }
                 if (z0 + zn < D) fetch(z0 + zn, min(zc, D - z0 - zn));
@@ -641,7 +675,7 @@ __global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, 
 #pragma unroll
                     for (int r = 0; r < RB; ++r)
                         v[r] = rin[r] ? ((uint32_t)gp(rp[r])[oa] | ((uint32_t)gp(rp[r])[ob] << 16)) : 0u;
-                    uint4* d = cell + ((size_t)zz * WC + x + 1);
+                    uint4* d = cell + ((size_t)zz * ss + fpp_cidx(x + 1, hb));
                     d[0] = make_uint4(v[0], v[1], v[2], v[3]);
                     d[hp] = make_uint4(v[4], v[5], v[6], v[7]);
                 }
@@ -649,32 +683,60 @@ __global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, 
             __syncthreads();
             const int zlast = (z0 + zn == D) ? zn - 1 : zn;
             const int zend = min(zlast, zn);
-            if (ymask) {
-                for (int zz = 0; zz < zend; ++zz)
-                    fpp_slice_f64<T, G>(cell, hp, z0 + zz, zz * WC, false, wd, ymask, J.prm, p0, n, ic, zlo, zhi, rs);
-            } else {
+            // interpolated slices: FP32 positions; a last-row row (ymask) takes
+            // getPixel's nearest value, the weights (1, 0)
+            // LDS byte address of cell index 0 of the slab (uniform)
+            const uint32_t sb = __builtin_amdgcn_readfirstlane((uint32_t)(size_t)(const TJ_LDS uint4*)cell);
+            const float mg = Pair<T>::magic(W);
+            const int mgh = (int)mg >> 1;
+            auto slices = [&](auto nearest_rows) {
                 for (int zz = 0; zz < zend; ++zz) {
                     const float zf = (float)(z0 + zz);
-                    const int zo = zz * WC;
-                    int a[G];
-                    uint32_t w[G];
+                    // byte address of index zz*ss - mg/2: cell c + mg then sits at
+                    // ((c + mg) >> 1) + ((c + mg) & 1) hb
+                    const uint32_t sbz = sb + 16u * (uint32_t)(zz * ss - mgh);
+                    uint32_t a[G], w[G];
 #pragma unroll
                     for (int k = 0; k < G; ++k) {
                         const float xs = __builtin_amdgcn_fmed3f(fmaf(zf, tn[k], base1[k]), 0.0f, wc1);
-                        a[k] = zo + (int)xs;
-                        w[k] = Pair<T>::weights(__builtin_amdgcn_fractf(xs));
+                        const float xm = xs + mg;  // rounded to the weight grid
+                        const uint32_t cm = (uint32_t)xm;
+                        uint32_t odd = cm & 1u;
+                        asm("" : "+v"(odd));  // keeps (odd * hb) a 24-bit multiply-add, not a select
+                        uint32_t idx = __umul24(odd, (uint32_t)hb) + (cm >> 1);
+                        asm("" : "+v"(idx));
+                        a[k] = sbz + (idx << 4);
+                        w[k] = Pair<T>::of(__builtin_amdgcn_fractf(xm));
                     }
                     uint4 c0[G], c1[G];
 #pragma unroll
                     for (int k = 0; k < G; ++k) {
-                        c0[k] = cell[a[k]];
-                        c1[k] = cell[hp + a[k]];
+                        typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+                        const u4v t0 = *(const TJ_LDS u4v*)(a[k]);
+                        const u4v t1 = *(const TJ_LDS u4v*)(a[k] + 16u * (uint32_t)hp);
+                        c0[k] = make_uint4(t0.x, t0.y, t0.z, t0.w);
+                        c1[k] = make_uint4(t1.x, t1.y, t1.z, t1.w);
                     }
-                    fpp_accumulate<T, G>(c0, c1, w, rs);
+                    if constexpr (decltype(nearest_rows)::value) {
+                        const uint32_t wn = Pair<T>::nearest();
+#pragma unroll
+                        for (int k = 0; k < G; ++k) {
+                            const uint32_t v[8] = {c0[k].x, c0[k].y, c0[k].z, c0[k].w,
+                                                   c1[k].x, c1[k].y, c1[k].z, c1[k].w};
+#pragma unroll
+                            for (int r = 0; r < 8; ++r)
+                                rs[k][r] = Pair<T>::dot(v[r], ((ymask >> r) & 1u) ? wn : w[k], rs[k][r]);
+                        }
+                    } else {
+                        fpp_accumulate<T, G>(c0, c1, w, rs);
+                    }
                 }
-            }
+            };
+            if (ymask) slices(std::true_type{});
+            else slices(std::false_type{});
             if (zlast < zn)
-                fpp_slice_f64<T, G>(cell, hp, z0 + zlast, zlast * WC, true, wd, ymask, J.prm, p0, n, ic, zlo, zhi, rs);
+                fpp_slice_f64<T, G>(cell, hp, hb, z0 + zlast, zlast * ss, true, wd, ymask, J.prm, p0, n, ic, zlo,
+                                    zhi, rs);
         }
         // projection, norm and difference per (projection, row); error partials
         double e2[G][RB];
@@ -1302,13 +1364,13 @@ static int fp_rays_of(const tj_batch_tuning& tu, int storage)
 }
 
 // packed-pair forward kernel for 16-bit storage (8 rows per tile); one slice
-// of the widest job needs 2 (W + 2) cells
+// of the widest job needs 2 (W + 2) cells plus the padding of the odd half
 template <typename T, typename TE>
 static hipError_t launch_fpp_s(hipStream_t st, const tj_batch_tuning& tu, const TjJob* jobs, const int4* tiles,
                                int ntiles, int grid, int max_w)
 {
     int lds = std::min(std::max(tu.fp_lds_kb, 16), 128) * 1024;
-    lds = std::max(lds, 2 * (max_w + 2) * (int)sizeof(uint4));
+    lds = std::max(lds, 2 * (max_w + 16) * (int)sizeof(uint4));
     const int cells = lds / (int)sizeof(uint4);
     if (fp_rays_of(tu, TJ_STORAGE_BF16) == 2)
         tjb::k_job_fpp<T, TE, 2><<<grid, tjb::NT, lds, st>>>(jobs, tiles, ntiles, cells);
