@@ -113,6 +113,10 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip the precision/batched/serving forms")
     ap.add_argument("--precision-study", type=int, nargs=4, metavar=("W", "H", "D", "N"),
                     help="run only the precision study (30 + 2 iterations, 80 ellipsoids, 2%% noise)")
+    ap.add_argument("--modes", nargs="+", default=None,
+                    help="precision study: only these storage modes (e.g. bf16+master+p32)")
+    ap.add_argument("--no-reference", action="store_true", help="precision study: skip the FP64 twin")
+    ap.add_argument("--no-tune", action="store_true", help="precision study: default tile shapes, no autotuning")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -125,7 +129,13 @@ def main():
     if args.precision_study:
         from tomoj_amd import forms
         W, H, D, N = args.precision_study
-        res = forms.precision_study(W, H, D, N, iterations=30, refine=2, ellipsoids=80, noise=0.02)
+        modes = forms.PRECISION_MODES
+        if args.modes:
+            modes = [m for m in modes if m[0] in args.modes]
+            if not modes:
+                ap.error("--modes: none of " + ", ".join(m[0] for m in forms.PRECISION_MODES))
+        res = forms.precision_study(W, H, D, N, iterations=30, refine=2, ellipsoids=80, noise=0.02, modes=modes,
+                                    reference=not args.no_reference, tune=not args.no_tune)
         print(json.dumps({"precision_study": res}), flush=True)
         return
 

@@ -1194,7 +1194,7 @@ struct tj_batch {
     int next_id = 1;
     long epoch = 0;
     std::map<int, BJob*> jobs;
-    tj_batch_tuning tune{4, 64, 4, 8, 4, 4, 1};
+    tj_batch_tuning tune{4, 64, 4, 16, 2, 8, 1};  // the autotuner's pick at 512^3 BF16
     RunPlan plan;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
