@@ -150,6 +150,69 @@ def test_half_storage_with_refinement(batch, oracle, mode):
     assert e_ref <= bound, e_ref
 
 
+# 16-bit forms on shapes that take the other code paths of their kernels:
+# (W, H, D, pw, ph, storage, positivity)
+AWKWARD = [
+    (45, 13, 22, 45, 13, "bf16", False),  # odd width (element staging), rows past the volume in the last tile
+    (64, 14, 30, 64, 10, "f16", True),    # detector shorter than the volume, positivity after each image
+    (50, 16, 17, 40, 16, "bf16", True),   # detector narrower than the volume, positivity
+    (96, 24, 40, 96, 24, "f16", False),   # word staging, several row tiles
+]
+
+
+@pytest.mark.parametrize("case", AWKWARD, ids=[f"{c[0]}x{c[1]}x{c[2]}-{c[3]}x{c[4]}-{c[5]}{'-pos' if c[6] else ''}"
+                                              for c in AWKWARD])
+def test_half_storage_awkward_shapes(batch, oracle, case):
+    """BF16/FP16 storage with FP32 master volume and projections, 10 iterations
+    then 2 FP32 iterations, against 12 FP64 oracle iterations: rel-L2 <= 1e-3."""
+    W, H, D, pw, ph, storage, pos = case
+    tilts = list(np.arange(-60, 61, 5.0))
+    E = oracle.tilt_eulers(tilts)
+    ell = oracle.ellipsoids(17, 6, pw, ph, D)
+    exps = oracle.phantom_project(ell, E, pw, ph).astype(np.float32)
+    if pos:
+        exps -= np.float32(0.5 * exps.mean())
+    with batch.JobEngine(GIB) as eng:
+        jid = eng.add(batch.JobSpec(W, H, D, tilts, storage=storage, positivity=pos, fp32_master=True,
+                                    fp32_projections=True, pw=pw, ph=ph))
+        eng.set_projections(jid, exps)
+        eng.run([jid], 10)
+        eng.convert(jid, "f32")
+        eng.set_projections(jid, exps)
+        eng.run([jid], 2)
+        got = eng.get_volume(jid)
+    g = oracle.make_geom(W, H, D, pw, ph, positivity=pos, nthreads=8)
+    ref = np.zeros((D, H, W), np.float64)
+    oracle.ossart(g, ref, exps, E, 12, 1.0, len(E), first_index=0, f64=True)
+    e = rel_l2(got, ref)
+    print(f"{case}: rel-L2 {e:.2e}")
+    assert e <= 1e-3, e
+    if pos:
+        assert (got >= 0).all()
+
+
+def test_half_tile_shapes_do_not_change_results(batch, oracle):
+    """The 16-bit kernels' tile and slab shapes partition the work without
+    changing any voxel's arithmetic: bit-identical volumes."""
+    W, H, D = 70, 21, 27
+    tilts, E, exps = phantom_case(oracle, W, H, D, tilts=np.arange(-60, 61, 10.0))
+    shapes = [dict(fp_rays=2, fp_lds_kb=16, bp_rows=2, bp_zchunk=8, blocks_per_cu=1),
+              dict(fp_rays=4, fp_lds_kb=64, bp_rows=4, bp_zchunk=16, blocks_per_cu=8),
+              dict(fp_rays=4, fp_lds_kb=128, bp_rows=8, bp_zchunk=8, blocks_per_cu=2),
+              dict(fp_rays=2, fp_lds_kb=32, bp_rows=8, bp_zchunk=16, blocks_per_cu=4),
+              dict(fp_rays=4, fp_lds_kb=48, bp_rows=4, bp_zchunk=4, blocks_per_cu=3)]
+    results = []
+    for sh in shapes:
+        with batch.JobEngine(GIB) as eng:
+            eng.set_tuning(fp_packed=1, **sh)
+            jid = eng.add(batch.JobSpec(W, H, D, tilts, storage="bf16", fp32_master=True, fp32_projections=True))
+            eng.set_projections(jid, exps)
+            eng.run([jid], 3)
+            results.append(eng.get_volume(jid))
+    for r in results[1:]:
+        assert np.array_equal(results[0], r)
+
+
 def test_result_same_alone_and_in_mix(batch, oracle):
     """A job's volume is bit-identical alone and in a mixed launch (other sizes
     and storage types), with and without graph capture."""
