@@ -240,6 +240,30 @@ int tj_synth_projections(tj_ctx *ctx, const double *ell, int k, const double *eu
  * receives n x {mean, stdDev}. */
 int tj_normalize_projections(tj_ctx *ctx, const double *tilt_deg, int min_size, double *stats);
 
+/* Fill of aligned pixels that map outside the original (TiltSeries.FILL_*,
+ * TiltSeries.java:48-56). */
+#define TJ_FILL_NONE 0
+#define TJ_FILL_AVG 1
+#define TJ_FILL_NAN 2
+
+/* Projection supply with an affine alignment: what TiltSeries.getPixels(i)
+ * returns through AffineAlignment.applyTransformOnImage (AffineAlignment.java:
+ * 136-254) for every resident projection, in place.  transforms: n x 6, the
+ * forward transform T_i of AffineAlignment.getTransform(i) in
+ * java.awt.geom.AffineTransform.getMatrix order (m00, m10, m01, m11, m02, m12);
+ * an identity T_i takes the identity branch.  Otherwise pixel (i, j) samples the
+ * original bilinearly at T_i^-1 (i - W/2, j - H/2) + (W/2, H/2) (the last column
+ * and row nearest), is normalised with the image's statistics when `normalize`,
+ * and pixels mapping outside take `fill`.  The statistics are those of
+ * TiltSeries.getImageStatistics(i) (:809-835): pass them in stats_in (n x
+ * {mean, stdDev}, e.g. from ImageJ), or NULL to compute them on the device from
+ * the ROI rectangle mapped by T_i (only for transforms without rotation or
+ * shear, m01 = m10 = 0; TJ_EUNSUP otherwise).  tilt_deg is needed only then.
+ * stats_out (may be NULL) receives the statistics used.  TJ_EINVAL for a
+ * non-invertible transform. */
+int tj_align_projections(tj_ctx *ctx, const double *transforms, const double *tilt_deg, int min_size,
+                         int normalize, int fill, const double *stats_in, double *stats_out);
+
 /* Copies resident projection `index` to the host. */
 int tj_get_loaded_projection(tj_ctx *ctx, int index, float *out);
 

@@ -104,6 +104,10 @@ def _L():
             "or_roi_rect": (None, [i, d, i, P, P]),
             "or_roi_stats": (None, [P, i, i, i, i, P, P]),
             "or_normalize": (None, [P, l, d, d]),
+            "or_affine_inverse": (i, [P, P]),
+            "or_affine_resample": (None, [P, i, i, P, i, i, d, d, P]),
+            "or_roi_rect_affine": (None, [i, i, d, i, P, P]),
+            "or_rect_stats": (None, [P, i, i, P, P, P]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
@@ -279,6 +283,61 @@ def normalized_projections(images, tilts_deg, min_size=0):
         _L().or_normalize(_p(row), row.size, m.value, sd.value)
         imgs[k] = row
     return imgs, means, sds
+
+
+def affine_inverse(m):
+    """java.awt.geom.AffineTransform.createInverse of (m00, m10, m01, m11, m02, m12);
+    None when not invertible."""
+    a = _c(m, np.float64)
+    r = np.zeros(6, np.float64)
+    return r if _L().or_affine_inverse(_p(a), _p(r)) else None
+
+
+def roi_rect_affine(width, height, tilt_deg, transform, min_size=0):
+    """[x0, x1, y0, y1] of getImageStatistics' ROI mapped by a rotation-free transform."""
+    a = _c(transform, np.float64)
+    r = np.zeros(4, np.int32)
+    _L().or_roi_rect_affine(int(width), int(height), float(tilt_deg), int(min_size), _p(a), _p(r))
+    return [int(v) for v in r]
+
+
+def rect_stats(img, rect):
+    img = _c(img, np.float32)
+    r = _c(rect, np.int32)
+    m, sd = C.c_double(), C.c_double()
+    _L().or_rect_stats(_p(img), img.shape[1], img.shape[0], _p(r), C.byref(m), C.byref(sd))
+    return m.value, sd.value
+
+
+def aligned_projections(images, transforms, tilts_deg=None, min_size=0, normalize=True, fill=1, stats=None):
+    """TiltSeries.getPixels(i) through AffineAlignment.applyTransformOnImage
+    (AffineAlignment.java:136-254) for every image; stats n x (mean, stdDev) or
+    None (the rectangle ROI of rotation-free transforms).  Returns (images, stats)."""
+    imgs = np.array(images, np.float32, copy=True)
+    n, h, w = imgs.shape
+    T = _c(transforms, np.float64).reshape(n, 6)
+    st = np.zeros((n, 2))
+    for k in range(n):
+        src = np.ascontiguousarray(imgs[k])
+        need = normalize or fill == 1
+        if need:
+            if stats is not None:
+                st[k] = stats[k]
+            else:
+                st[k] = rect_stats(src, roi_rect_affine(w, h, tilts_deg[k], T[k], min_size))
+        if np.array_equal(T[k], [1.0, 0.0, 0.0, 1.0, 0.0, 0.0]):
+            if normalize:
+                _L().or_normalize(_p(src), src.size, st[k][0], st[k][1])
+            imgs[k] = src
+            continue
+        inv = affine_inverse(T[k])
+        if inv is None:
+            raise ValueError(f"transform {k} is not invertible")
+        out = np.empty_like(src)
+        _L().or_affine_resample(_p(src), w, h, _p(inv), int(bool(normalize)), int(fill), float(st[k][0]),
+                                float(st[k][1]) if need else 1.0, _p(out))
+        imgs[k] = out
+    return imgs, st
 
 
 def weighting_mask(sx, sy, tilts_deg, diameter, axis_deg=None) -> np.ndarray:

@@ -1045,3 +1045,142 @@ void or_normalize(float *img, long npx, double mean, double sd)
 {
     for (long i = 0; i < npx; ++i) img[i] = (float)((img[i] - mean) / sd);
 }
+
+/* ---------------------------------------------------------------------------
+ * Aligned projection supply: AffineAlignment.applyTransformOnImage for a
+ * non-identity transform (tomography/align/AffineAlignment.java:157-254) and
+ * the ROI of TiltSeries.getImageStatistics (tomography/TiltSeries.java:809-835).
+ * Matrices are 6 doubles in java.awt.geom.AffineTransform.getMatrix order:
+ * m00, m10, m01, m11, m02, m12.
+ * ------------------------------------------------------------------------- */
+
+/* java.awt.geom.AffineTransform.createInverse: the inverse per transform state
+ * (identity / translation, scale (+translation), pure shear (+translation),
+ * general).  Returns 0 for a non-invertible transform. */
+int or_affine_inverse(const double *m, double *r)
+{
+    double a = m[0], b = m[1], c = m[2], d = m[3], e = m[4], f = m[5];
+    int tr = (e != 0.0 || f != 0.0);
+    if (c == 0.0 && b == 0.0) {
+        if (a == 1.0 && d == 1.0) {
+            r[0] = 1.0; r[1] = 0.0; r[2] = 0.0; r[3] = 1.0;
+            r[4] = tr ? -e : 0.0;
+            r[5] = tr ? -f : 0.0;
+            return 1;
+        }
+        if (a == 0.0 || d == 0.0) return 0;
+        r[0] = 1.0 / a; r[1] = 0.0; r[2] = 0.0; r[3] = 1.0 / d;
+        r[4] = tr ? -e / a : 0.0;
+        r[5] = tr ? -f / d : 0.0;
+        return 1;
+    }
+    if (a == 0.0 && d == 0.0) {
+        if (c == 0.0 || b == 0.0) return 0;
+        r[0] = 0.0; r[1] = 1.0 / c; r[2] = 1.0 / b; r[3] = 0.0;
+        r[4] = tr ? -f / b : 0.0;
+        r[5] = tr ? -e / c : 0.0;
+        return 1;
+    }
+    {
+        double det = a * d - c * b;
+        if (fabs(det) <= 4.9406564584124654e-324) return 0;
+        r[0] = d / det; r[1] = -b / det; r[2] = -c / det; r[3] = a / det;
+        r[4] = tr ? (c * f - d * e) / det : 0.0;
+        r[5] = tr ? (b * e - a * f) / det : 0.0;
+    }
+    return 1;
+}
+
+/* One aligned image (AffineAlignment.java:157-254): inv is the inverse
+ * transform, centres w/2, h/2; fill 0 none, 1 average, 2 NaN. */
+void or_affine_resample(const float *src, int w, int h, const double *inv, int normalize, int fill, double mean,
+                        double sd, float *dst)
+{
+    double cx = w / 2.0, cy = h / 2.0;
+    for (int j = 0; j < h; ++j) {
+        double yc = j - cy;
+        for (int i = 0; i < w; ++i) {
+            double xc = i - cx;
+            double xx = (inv[0] * xc + inv[2] * yc + inv[4]) + cx;
+            double yy = (inv[1] * xc + inv[3] * yc + inv[5]) + cy;
+            int ix0 = (int)xx, iy0 = (int)yy;
+            double dx0 = xx - ix0, dy0 = yy - iy0;
+            float pix;
+            if (ix0 >= 0 && ix0 < w && iy0 >= 0 && iy0 < h) {
+                if (ix0 == w - 1 || iy0 == h - 1) {
+                    pix = src[ix0 + (long)iy0 * w];
+                } else {
+                    double fac4 = dx0 * dy0;
+                    long pos = ix0 + (long)iy0 * w;
+                    double v1 = src[pos], v3 = src[pos + w], v2 = src[pos + 1], v4 = src[pos + 1 + w];
+                    pix = (float)(v1 * (1 - dx0 - dy0 + fac4) + v2 * (dx0 - fac4) + v3 * (dy0 - fac4) + v4 * fac4);
+                }
+                if (normalize) pix = (float)((pix - mean) / sd);
+            } else if (fill == 1) {
+                pix = normalize ? 0.0f : (float)mean;
+            } else if (fill == 2) {
+                pix = NAN;
+            } else {
+                pix = 0.0f;
+            }
+            dst[(long)j * w + i] = pix;
+        }
+    }
+}
+
+/* The ROI polygon of getImageStatistics mapped by T, for a transform without
+ * rotation or shear: the rectangle [x0, x1) x [y0, y1) of its int corners. */
+void or_roi_rect_affine(int width, int height, double tilt_deg, int min_size, const double *m, int *r4)
+{
+    int x0, x1;
+    double rad = tilt_deg / 180.0 * M_PI;
+    long r = (long)floor(width * cos(rad) + 0.5);
+    int roisx = (int)(r > min_size ? r : min_size);
+    double cx = (width - 1.0) / 2, cy = (height - 1.0) / 2;
+    double pts[8];
+    int xs[4], ys[4];
+    pts[0] = (double)((width - roisx) / 2) - cx;
+    pts[1] = -cy;
+    pts[2] = pts[0] + roisx;
+    pts[3] = pts[1];
+    pts[4] = pts[2];
+    pts[5] = height - cy;
+    pts[6] = pts[0];
+    pts[7] = pts[5];
+    for (int k = 0; k < 4; ++k) {
+        xs[k] = (int)((m[0] * pts[2 * k] + m[2] * pts[2 * k + 1] + m[4]) + cx);
+        ys[k] = (int)((m[1] * pts[2 * k] + m[3] * pts[2 * k + 1] + m[5]) + cy);
+    }
+    x0 = xs[0]; x1 = xs[0];
+    r4[2] = ys[0]; r4[3] = ys[0];
+    for (int k = 1; k < 4; ++k) {
+        if (xs[k] < x0) x0 = xs[k];
+        if (xs[k] > x1) x1 = xs[k];
+        if (ys[k] < r4[2]) r4[2] = ys[k];
+        if (ys[k] > r4[3]) r4[3] = ys[k];
+    }
+    r4[0] = x0;
+    r4[1] = x1;
+}
+
+/* Mean and sample standard deviation over the rectangle [x0, x1) x [y0, y1)
+ * (clipped), as or_roi_stats. */
+void or_rect_stats(const float *img, int width, int height, const int *r4, double *mean, double *sd)
+{
+    double n = 0, sum = 0, sum2 = 0, s = 0.0;
+    int x0 = r4[0] < 0 ? 0 : r4[0], x1 = r4[1] > width ? width : r4[1];
+    int y0 = r4[2] < 0 ? 0 : r4[2], y1 = r4[3] > height ? height : r4[3];
+    for (int y = y0; y < y1; ++y)
+        for (int x = x0; x < x1; ++x) {
+            double v = img[(long)y * width + x];
+            sum += v;
+            sum2 += v * v;
+            n += 1;
+        }
+    *mean = n > 0 ? sum / n : 0.0;
+    if (n > 0) {
+        s = (n * sum2 - sum * sum) / n;
+        s = (s > 0.0) ? sqrt(s / (n - 1.0)) : 0.0;
+    }
+    *sd = s;
+}

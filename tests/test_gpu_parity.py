@@ -491,3 +491,59 @@ def test_normalised_projection_supply(tj, oracle, min_size):
     assert np.allclose(stats[:, 0], means, rtol=1e-12) and np.allclose(stats[:, 1], sds, rtol=1e-12)
     ulps = np.abs(got.view(np.int32).astype(np.int64) - ref.view(np.int32).astype(np.int64))
     assert ulps.max() <= 1 and (ulps == 0).mean() > 0.99
+
+
+AFFINE_CASES = [
+    # (transforms kind, normalize, fill)
+    ("rotation", True, 1),
+    ("rotation", False, 2),
+    ("rotation", False, 1),
+    ("translation", True, 1),
+    ("mixed", False, 0),
+]
+
+
+@pytest.mark.parametrize("case", AFFINE_CASES, ids=[f"{c[0]}-{'norm' if c[1] else 'raw'}-fill{c[2]}"
+                                                    for c in AFFINE_CASES])
+def test_aligned_projection_supply(tj, oracle, case):
+    """TiltSeries.getPixels through AffineAlignment.applyTransformOnImage on the
+    device: bit-identical to the oracle for rotated/scaled/translated images
+    (statistics given), and for translations with the statistics computed on
+    the device (checked to 1e-12, then the images against the oracle run with
+    the device's statistics)."""
+    kind, normalize, fill = case
+    rng = np.random.default_rng(21)
+    W, H, n = 41, 23, 5
+    tilts = [-50.0, -12.0, 0.0, 25.0, 61.0]
+    imgs = (rng.standard_normal((n, H, W)) * 2 + 6).astype(np.float32)
+    T = np.zeros((n, 6))
+    for k in range(n):
+        if kind == "translation" or (kind == "mixed" and k % 2):
+            T[k] = [1, 0, 0, 1, rng.uniform(-4, 4), rng.uniform(-3, 3)]
+        else:
+            a, s = rng.uniform(-0.3, 0.3), rng.uniform(0.9, 1.1)
+            T[k] = [s * np.cos(a), s * np.sin(a), -s * np.sin(a), s * np.cos(a), rng.uniform(-3, 3),
+                    rng.uniform(-2, 2)]
+    T[2] = [1, 0, 0, 1, 0, 0]                         # the identity branch
+    given = None if kind == "translation" else (rng.uniform(1, 3, (n, 2)) * [3, 1])
+    with make(tj, (W, H, 8)) as P:
+        P.load_tilt_series(imgs, oracle.tilt_eulers(tilts))
+        st = P.align_projections(T, tilts, 0, normalize, fill, given)
+        got = np.stack([P.get_loaded_projection(k) for k in range(n)])
+    if given is None:
+        _, ref_st = oracle.aligned_projections(imgs, T, tilts, 0, normalize, fill)
+        assert np.allclose(st, ref_st, rtol=1e-12)
+    ref, _ = oracle.aligned_projections(imgs, T, tilts, 0, normalize, fill, stats=st if given is None else given)
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    assert np.array_equal(np.nan_to_num(got), np.nan_to_num(ref))
+
+
+def test_aligned_supply_rejects_rotation_without_statistics(tj, oracle):
+    W, H = 16, 8
+    imgs = np.ones((2, H, W), np.float32)
+    with make(tj, (W, H, 4)) as P:
+        P.load_tilt_series(imgs, oracle.tilt_eulers([0.0, 10.0]))
+        with pytest.raises(tj.TjError):
+            P.align_projections([[1, 0, 0, 1, 0, 0], [0.9, 0.1, -0.1, 0.9, 0, 0]], [0.0, 10.0], 0, True, 1)
+        with pytest.raises(tj.TjError):
+            P.align_projections([[1, 0, 0, 1, 0, 0], [0, 0, 0, 0, 1, 1]], [0.0, 10.0], 0, False, 0)

@@ -422,3 +422,64 @@ def test_normalisation_roi_and_statistics(oracle):
         assert np.isclose(means[k], roi.mean(), rtol=1e-13)
         assert np.isclose(sds[k], roi.std(ddof=1), rtol=1e-12)
         assert np.array_equal(out[k], ((imgs[k].astype(np.float64) - means[k]) / sds[k]).astype(np.float32))
+
+
+def test_affine_inverse_states(oracle):
+    """createInverse per transform state: translation and scale invert entry by
+    entry (exact), the general state through the determinant (T Tinv = I)."""
+    assert np.array_equal(oracle.affine_inverse([1, 0, 0, 1, 3.5, -2]), [1, 0, 0, 1, -3.5, 2])
+    assert np.array_equal(oracle.affine_inverse([2, 0, 0, 4, 1, 2]), [0.5, 0, 0, 0.25, -0.5, -0.5])
+    assert np.array_equal(oracle.affine_inverse([0, 1, -1, 0, 0, 0]), [0, -1, 1, 0, 0, 0])   # 90 degrees
+    assert oracle.affine_inverse([0, 0, 0, 0, 1, 1]) is None
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        m = rng.standard_normal(6)
+        r = oracle.affine_inverse(m)
+        A = np.array([[m[0], m[2], m[4]], [m[1], m[3], m[5]], [0, 0, 1]])
+        B = np.array([[r[0], r[2], r[4]], [r[1], r[3], r[5]], [0, 0, 1]])
+        assert np.allclose(A @ B, np.eye(3), atol=1e-12)
+
+
+def test_affine_resample_closed_forms(oracle):
+    """applyTransformOnImage: an integer translation shifts the image exactly,
+    a 90-degree rotation of a square image permutes it, a 2x scale samples
+    midpoints (exact bilinear of a linear image), and pixels that map outside
+    take the fill (mean, 0 when normalised, NaN, or 0)."""
+    rng = np.random.default_rng(5)
+    h, w = 10, 14
+    img = (rng.standard_normal((h, w)) * 3 + 8).astype(np.float32)
+    out, _ = oracle.aligned_projections(img[None], [[1, 0, 0, 1, 3, -2]], normalize=False, fill=0)
+    want = np.zeros_like(img)
+    want[0:h - 2, 3:] = img[2:, :w - 3]
+    assert np.array_equal(out[0], want)
+    # rotation by 90 degrees: dst[j, i] = src[n - i, j] (i >= 1), NaN-filled column 0
+    n = 12
+    sq = (rng.standard_normal((n, n)) + 1).astype(np.float32)
+    out, _ = oracle.aligned_projections(sq[None], [[0, 1, -1, 0, 0, 0]], normalize=False, fill=2)
+    assert np.isnan(out[0][:, 0]).all()
+    for i in range(1, n):
+        assert np.array_equal(out[0][:, i], sq[n - i, :])
+    # 2x scale of a linear image: bilinear samples are exact up to rounding
+    lin = (0.25 * np.arange(w)[None, :] + 0.5 * np.arange(h)[:, None] + 1).astype(np.float32)
+    out, _ = oracle.aligned_projections(lin[None], [[2, 0, 0, 2, 0, 0]], normalize=False, fill=0)
+    jj, ii = np.mgrid[0:h, 0:w]
+    xx, yy = 0.5 * (ii - w / 2) + w / 2, 0.5 * (jj - h / 2) + h / 2
+    inner = (xx < w - 1) & (yy < h - 1)
+    assert np.allclose(out[0][inner], (0.25 * xx + 0.5 * yy + 1)[inner], rtol=1e-6)
+    # fills and normalisation with given statistics
+    st = np.array([[2.0, 4.0]])
+    for fill, normalize, val in [(1, False, 2.0), (1, True, 0.0), (0, False, 0.0)]:
+        out, _ = oracle.aligned_projections(img[None], [[1, 0, 0, 1, 5, 0]], normalize=normalize, fill=fill,
+                                            stats=st)
+        assert (out[0][:, :5] == np.float32(val)).all()
+    out, _ = oracle.aligned_projections(img[None], [[1, 0, 0, 1, 5, 0]], normalize=True, fill=1, stats=st)
+    assert np.array_equal(out[0][:, 5:], ((img[:, :w - 5].astype(np.float64) - 2.0) / 4.0).astype(np.float32))
+
+
+def test_affine_roi_rectangle(oracle):
+    """getImageStatistics' polygon under a rotation-free transform: the identity
+    gives the normalisation ROI over every row; a translation moves it."""
+    for w, t in [(512, 0.0), (100, 45.0), (37, -61.0)]:
+        x0, x1 = oracle.roi_rect(w, t)
+        assert oracle.roi_rect_affine(w, 20, t, [1, 0, 0, 1, 0, 0]) == [x0, x1, 0, 20]
+        assert oracle.roi_rect_affine(w, 20, t, [1, 0, 0, 1, 4, -3]) == [x0 + 4, x1 + 4, -3, 17]

@@ -19,6 +19,10 @@ TJ_ESTATE = -4
 TJ_EUNSUP = -5
 TJ_ECANCELLED = -6
 
+TJ_FILL_NONE = 0
+TJ_FILL_AVG = 1
+TJ_FILL_NAN = 2
+
 TJ_STORAGE_F32 = 0
 TJ_STORAGE_BF16 = 1
 TJ_STORAGE_F16 = 2
@@ -112,6 +116,7 @@ _SIGS = {
     "tj_synth_projections": (C.c_int, [_P, _P, C.c_int, _P, C.c_int, C.c_double, C.c_uint64]),
     "tj_get_loaded_projection": (C.c_int, [_P, C.c_int, _P]),
     "tj_normalize_projections": (C.c_int, [_P, _P, C.c_int, _P]),
+    "tj_align_projections": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P]),
     "tj_get_timing": (C.c_int, [_P, C.POINTER(tj_timing)]),
     "tj_enable_timing": (C.c_int, [_P, C.c_int]),
     "tj_set_tuning": (C.c_int, [_P, C.POINTER(tj_tuning)]),

@@ -36,6 +36,9 @@ hipError_t add_noise(hipStream_t, const double*, size_t, uint64_t, double, doubl
 hipError_t copy(hipStream_t, const void*, void*, size_t);
 hipError_t fista(hipStream_t, float*, const float*, size_t, double);
 hipError_t normalize(hipStream_t, float*, int, int, int, const int*, double*);
+hipError_t roi_stats(hipStream_t, const float*, int, int, int, const int*, double*);
+hipError_t affine(hipStream_t, const float*, float*, int, int, int, const double*, const int*, const double*, int,
+                  int);
 hipError_t weighting_init(hipStream_t, int, int, double, const double*, const double*, int, float*);
 hipError_t ramp_init(hipStream_t, int, int, int, double, double, float*);
 hipError_t weighting_apply(hipStream_t, int, int, int, const float*, const int*, const float*, float*, void*, size_t);
@@ -626,20 +629,157 @@ int tj_normalize_projections(tj_ctx* c, const double* tilt_deg, int min_size, do
     if (!c || !tilt_deg) return fail(TJ_EINVAL, "tj_normalize_projections: null argument");
     if (c->nproj <= 0) return fail(TJ_ESTATE, "tj_normalize_projections: no resident projections");
     TJ_HIP(hipSetDevice(c->device));
-    std::vector<int> roi(2 * (size_t)c->nproj);
-    for (int p = 0; p < c->nproj; ++p) roi_columns(c->g.pw, tilt_deg[p], min_size, &roi[2 * p], &roi[2 * p + 1]);
+    std::vector<int> roi(4 * (size_t)c->nproj);
+    for (int p = 0; p < c->nproj; ++p) {
+        roi_columns(c->g.pw, tilt_deg[p], min_size, &roi[4 * p], &roi[4 * p + 1]);
+        roi[4 * p + 2] = 0;
+        roi[4 * p + 3] = c->g.ph;
+    }
     int* droi = nullptr;
     double* dstats = nullptr;
     hipError_t e = dalloc(&droi, roi.size());
-    if (e == hipSuccess) e = dalloc(&dstats, roi.size());
+    if (e == hipSuccess) e = dalloc(&dstats, 2 * (size_t)c->nproj);
     if (e == hipSuccess) e = hipMemcpyAsync(droi, roi.data(), sizeof(int) * roi.size(), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = tjk::normalize(c->stream, c->exps, c->g.pw, c->g.ph, c->nproj, droi, dstats);
     if (e == hipSuccess && stats)
-        e = hipMemcpyAsync(stats, dstats, sizeof(double) * roi.size(), hipMemcpyDeviceToHost, c->stream);
+        e = hipMemcpyAsync(stats, dstats, sizeof(double) * 2 * c->nproj, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     dfree(droi);
     dfree(dstats);
     if (e != hipSuccess) return fail(TJ_EDEVICE, "tj_normalize_projections: %s", hipGetErrorString(e));
+    return TJ_OK;
+}
+
+// java.awt.geom.AffineTransform.createInverse by the transform's state (the
+// same partition of the matrix as AffineTransform.updateState): translation,
+// scale and pure-shear states invert entry by entry, the general state by the
+// determinant.  m: m00, m10, m01, m11, m02, m12 (AffineTransform.getMatrix
+// order).  Returns false for a non-invertible transform.
+static bool affine_inverse(const double* m, double* r)
+{
+    const double m00 = m[0], m10 = m[1], m01 = m[2], m11 = m[3], m02 = m[4], m12 = m[5];
+    const bool trans = !(m02 == 0.0 && m12 == 0.0);
+    auto set = [&](double a, double b, double c2, double d, double e, double f) {
+        r[0] = a; r[1] = b; r[2] = c2; r[3] = d; r[4] = e; r[5] = f;
+    };
+    if (m01 == 0.0 && m10 == 0.0) {
+        if (m00 == 1.0 && m11 == 1.0) {
+            if (trans) set(1.0, 0.0, 0.0, 1.0, -m02, -m12);
+            else set(1.0, 0.0, 0.0, 1.0, 0.0, 0.0);
+            return true;
+        }
+        if (m00 == 0.0 || m11 == 0.0) return false;
+        if (trans) set(1.0 / m00, 0.0, 0.0, 1.0 / m11, -m02 / m00, -m12 / m11);
+        else set(1.0 / m00, 0.0, 0.0, 1.0 / m11, 0.0, 0.0);
+        return true;
+    }
+    if (m00 == 0.0 && m11 == 0.0) {
+        if (m01 == 0.0 || m10 == 0.0) return false;
+        if (trans) set(0.0, 1.0 / m01, 1.0 / m10, 0.0, -m12 / m10, -m02 / m01);
+        else set(0.0, 1.0 / m01, 1.0 / m10, 0.0, 0.0, 0.0);
+        return true;
+    }
+    const double det = m00 * m11 - m01 * m10;
+    if (std::fabs(det) <= 4.9406564584124654e-324) return false;  // Double.MIN_VALUE
+    if (trans)
+        set(m11 / det, -m10 / det, -m01 / det, m00 / det, (m01 * m12 - m11 * m02) / det,
+            (m10 * m02 - m00 * m12) / det);
+    else
+        set(m11 / det, -m10 / det, -m01 / det, m00 / det, 0.0, 0.0);
+    return true;
+}
+
+// TiltSeries.getImageStatistics' polygon (TiltSeries.java:809-835): the ROI
+// rectangle of the identity case, its corners mapped by T and cast to int.  For
+// a transform without rotation or shear the polygon is the axis-aligned
+// rectangle [min x, max x) x [min y, max y).
+static void roi_rect_affine(int width, int height, double tilt_deg, int min_size, const double* m, int* r4)
+{
+    const double rad = tilt_deg / 180.0 * M_PI;
+    const long rr = (long)std::floor(width * std::cos(rad) + 0.5);
+    const int roisx = (int)(rr > min_size ? rr : min_size);
+    const double cx = (width - 1.0) / 2, cy = (height - 1.0) / 2;
+    double pts[8];
+    pts[0] = (double)((width - roisx) / 2) - cx;  // integer division, then - cx
+    pts[1] = -cy;
+    pts[2] = pts[0] + roisx;
+    pts[3] = pts[1];
+    pts[4] = pts[2];
+    pts[5] = height - cy;
+    pts[6] = pts[0];
+    pts[7] = pts[5];
+    int xs[4], ys[4];
+    for (int k = 0; k < 4; ++k) {
+        const double x = pts[2 * k], y = pts[2 * k + 1];
+        xs[k] = (int)((m[0] * x + m[2] * y + m[4]) + cx);
+        ys[k] = (int)((m[1] * x + m[3] * y + m[5]) + cy);
+    }
+    r4[0] = std::min(std::min(xs[0], xs[1]), std::min(xs[2], xs[3]));
+    r4[1] = std::max(std::max(xs[0], xs[1]), std::max(xs[2], xs[3]));
+    r4[2] = std::min(std::min(ys[0], ys[1]), std::min(ys[2], ys[3]));
+    r4[3] = std::max(std::max(ys[0], ys[1]), std::max(ys[2], ys[3]));
+}
+
+int tj_align_projections(tj_ctx* c, const double* transforms, const double* tilt_deg, int min_size, int normalize,
+                         int fill, const double* stats_in, double* stats_out)
+{
+    if (!c || !transforms) return fail(TJ_EINVAL, "tj_align_projections: null argument");
+    if (c->nproj <= 0) return fail(TJ_ESTATE, "tj_align_projections: no resident projections");
+    if (fill < TJ_FILL_NONE || fill > TJ_FILL_NAN) return fail(TJ_EINVAL, "tj_align_projections: fill %d", fill);
+    const int n = c->nproj, w = c->g.pw, h = c->g.ph;
+    const bool need_stats = normalize || fill == TJ_FILL_AVG;
+    std::vector<double> inv(6 * (size_t)n);
+    std::vector<int> ident(n), roi(4 * (size_t)n);
+    for (int p = 0; p < n; ++p) {
+        const double* m = transforms + 6 * (size_t)p;
+        ident[p] = (m[0] == 1.0 && m[1] == 0.0 && m[2] == 0.0 && m[3] == 1.0 && m[4] == 0.0 && m[5] == 0.0);
+        if (!affine_inverse(m, &inv[6 * (size_t)p]))
+            return fail(TJ_EINVAL, "tj_align_projections: transform %d is not invertible", p);
+        if (need_stats && !stats_in) {
+            if (!tilt_deg) return fail(TJ_EINVAL, "tj_align_projections: tilt angles needed for the statistics");
+            if (m[1] != 0.0 || m[2] != 0.0)
+                return fail(TJ_EUNSUP,
+                            "tj_align_projections: transform %d rotates or shears the ROI polygon; pass its "
+                            "statistics (ImageJ polygon ROI) in stats_in",
+                            p);
+            roi_rect_affine(w, h, tilt_deg[p], min_size, m, &roi[4 * (size_t)p]);
+        }
+    }
+    TJ_HIP(hipSetDevice(c->device));
+    double *dinv = nullptr, *dstats = nullptr;
+    int *dident = nullptr, *droi = nullptr;
+    float* orig = nullptr;
+    hipError_t e = dalloc(&dinv, inv.size());
+    if (e == hipSuccess) e = dalloc(&dident, ident.size());
+    if (e == hipSuccess) e = dalloc(&dstats, 2 * (size_t)n);
+    if (e == hipSuccess) e = dalloc(&orig, c->npx * n);
+    if (e == hipSuccess) e = hipMemcpyAsync(dinv, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(dident, ident.data(), sizeof(int) * ident.size(), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(orig, c->exps, sizeof(float) * c->npx * n, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess && need_stats) {
+        if (stats_in) {
+            e = hipMemcpyAsync(dstats, stats_in, sizeof(double) * 2 * n, hipMemcpyHostToDevice, c->stream);
+        } else {
+            e = dalloc(&droi, roi.size());
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(droi, roi.data(), sizeof(int) * roi.size(), hipMemcpyHostToDevice, c->stream);
+            if (e == hipSuccess) e = tjk::roi_stats(c->stream, orig, w, h, n, droi, dstats);
+        }
+    }
+    if (e == hipSuccess)
+        e = tjk::affine(c->stream, orig, c->exps, w, h, n, dinv, dident, need_stats ? dstats : nullptr,
+                        normalize ? 1 : 0, fill);
+    if (e == hipSuccess && stats_out && need_stats)
+        e = hipMemcpyAsync(stats_out, dstats, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(dinv);
+    dfree(dident);
+    dfree(dstats);
+    dfree(droi);
+    dfree(orig);
+    if (e != hipSuccess) return fail(TJ_EDEVICE, "tj_align_projections: %s", hipGetErrorString(e));
     return TJ_OK;
 }
 

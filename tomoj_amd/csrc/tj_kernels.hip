@@ -906,18 +906,20 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_fp_lds(
 // [x0, x1), all rows; ImageJ FloatStatistics: double sums, stdDev =
 // sqrt(((n*sum2 - sum^2)/n)/(n-1))).  One block per image, fixed reduction tree.
 // ---------------------------------------------------------------------------
+// roi: 4 ints per image, the pixel rectangle [x0, x1) x [y0, y1) (clipped)
 __global__ __launch_bounds__(TJ_BLOCK) void k_roi_stats(const float* __restrict__ imgs, int pw, int ph,
                                                         const int* __restrict__ roi, double* __restrict__ stats)
 {
     __shared__ double sh[TJ_BLOCK];
     const int p = blockIdx.x;
-    const int x0 = max(roi[2 * p], 0), x1 = min(roi[2 * p + 1], pw);
+    const int x0 = max(roi[4 * p], 0), x1 = min(roi[4 * p + 1], pw);
+    const int y0 = max(roi[4 * p + 2], 0), y1 = min(roi[4 * p + 3], ph);
     const int rw = max(x1 - x0, 0);
-    const size_t cnt = (size_t)rw * ph;
+    const size_t cnt = (size_t)rw * max(y1 - y0, 0);
     const float* img = imgs + (size_t)p * pw * ph;
     double sum = 0.0, sum2 = 0.0;
     for (size_t k = threadIdx.x; k < cnt; k += TJ_BLOCK) {
-        const int y = (int)(k / rw), x = x0 + (int)(k - (size_t)y * rw);
+        const int y = y0 + (int)(k / rw), x = x0 + (int)(k - (size_t)(y - y0) * rw);
         const double v = img[(size_t)y * pw + x];
         sum += v;
         sum2 += v * v;
@@ -944,6 +946,67 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_normalize(float* __restrict__ imgs
     float* img = imgs + (size_t)p * npx;
     for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < npx; k += (size_t)gridDim.x * blockDim.x)
         img[k] = (float)((img[k] - mean) / sd);
+}
+
+// ---------------------------------------------------------------------------
+// Aligned projection supply: AffineAlignment.applyTransformOnImage for a
+// non-identity transform (AffineAlignment.java:157-254).  Pixel (i, j) of image
+// p samples the original at Tinv (i - w/2, j - h/2) + (w/2, h/2), evaluated as
+// java.awt.geom.AffineTransform.transform does (m00 x + m01 y + m02, in FP64,
+// no fused multiply-add); (int) truncation toward zero, so positions in
+// (-1, 0) sample column or row 0 with a negative fraction as Java does; the
+// last column and row take the nearest value; the bilinear weights are
+// Java's (1 - dx - dy + dx dy, dx - dx dy, dy - dx dy, dx dy) in FP64.
+// Normalisation (float)((pix - mean) / stdDev) after the cast; pixels that
+// map outside take the fill: mean (FILL_AVG, not normalised), 0 (FILL_AVG
+// normalised, FILL_NONE) or NaN (FILL_NaN).  inv: 6 doubles per image in the
+// order m00, m10, m01, m11, m02, m12.
+// ---------------------------------------------------------------------------
+// ident[p] != 0: the identity branch (AffineAlignment.java:148-156), a copy or
+// (float)((x - mean) / stdDev) of the pixel itself.
+__global__ __launch_bounds__(TJ_BLOCK) void k_affine(const float* __restrict__ src, float* __restrict__ dst,
+                                                     int w, int h, const double* __restrict__ inv,
+                                                     const int* __restrict__ ident,
+                                                     const double* __restrict__ stats, int normalize, int fill)
+{
+    const int i = blockIdx.x * TJ_BLOCK + threadIdx.x;
+    const int j = blockIdx.y, p = blockIdx.z;
+    if (i >= w) return;
+    if (ident[p]) {
+        const size_t o = (size_t)p * w * h + (size_t)j * w + i;
+        dst[o] = normalize ? (float)((src[o] - stats[2 * p]) / stats[2 * p + 1]) : src[o];
+        return;
+    }
+    const double* m = inv + 6 * p;
+    const double cx = w / 2.0, cy = h / 2.0;
+    const double x = i - cx, y = j - cy;
+    const double xx = (m[0] * x + m[2] * y + m[4]) + cx;
+    const double yy = (m[1] * x + m[3] * y + m[5]) + cy;
+    const int ix0 = (int)xx, iy0 = (int)yy;
+    const double dx0 = xx - ix0, dy0 = yy - iy0;
+    const float* img = src + (size_t)p * w * h;
+    const double mean = stats ? stats[2 * p] : 0.0, sd = stats ? stats[2 * p + 1] : 1.0;
+    float out;
+    if (ix0 >= 0 && ix0 < w && iy0 >= 0 && iy0 < h) {
+        float pix;
+        if (ix0 == w - 1 || iy0 == h - 1) {
+            pix = img[ix0 + (size_t)iy0 * w];
+        } else {
+            const double fac4 = dx0 * dy0;
+            const size_t pos = ix0 + (size_t)iy0 * w;
+            const double v1 = img[pos], v3 = img[pos + w], v2 = img[pos + 1], v4 = img[pos + 1 + w];
+            pix = (float)(v1 * (1 - dx0 - dy0 + fac4) + v2 * (dx0 - fac4) + v3 * (dy0 - fac4) + v4 * fac4);
+        }
+        if (normalize) pix = (float)((pix - mean) / sd);
+        out = pix;
+    } else if (fill == 1) {
+        out = normalize ? 0.0f : (float)mean;
+    } else if (fill == 2) {
+        out = NAN;
+    } else {
+        out = 0.0f;
+    }
+    dst[(size_t)p * w * h + (size_t)j * w + i] = out;
 }
 
 // ---------------------------------------------------------------------------
@@ -1259,6 +1322,24 @@ hipError_t bp(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp, 
     default: TJ_BP_LAUNCH(4) break;
     }
 #undef TJ_BP_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t roi_stats(hipStream_t st, const float* imgs, int pw, int ph, int n, const int* roi, double* stats)
+{
+    (void)hipGetLastError();
+    if (n <= 0) return hipSuccess;
+    k_roi_stats<<<n, TJ_BLOCK, 0, st>>>(imgs, pw, ph, roi, stats);
+    return hipGetLastError();
+}
+
+hipError_t affine(hipStream_t st, const float* src, float* dst, int w, int h, int n, const double* inv,
+                  const int* ident, const double* stats, int normalize, int fill)
+{
+    (void)hipGetLastError();
+    if (n <= 0) return hipSuccess;
+    k_affine<<<dim3((w + TJ_BLOCK - 1) / TJ_BLOCK, h, n), TJ_BLOCK, 0, st>>>(src, dst, w, h, inv, ident, stats,
+                                                                              normalize, fill);
     return hipGetLastError();
 }
 

@@ -37,6 +37,11 @@ class TiltSeriesData:
     axis_vertical: bool = True
     normalize: bool = False            # TomoJ's default is True (TiltSeries.java:128)
     normalization_min_size: int = 0
+    # affine alignment: n x 6 forward transforms (m00, m10, m01, m11, m02, m12) of
+    # AffineAlignment.getTransform(i), None for the identity alignment
+    transforms: Optional[np.ndarray] = None
+    fill: int = 1                      # TiltSeries.FILL_AVG (TiltSeries.java:130)
+    stats: Optional[np.ndarray] = None  # n x (mean, stdDev) of the ImageJ ROI, when rotated
 
     @property
     def width(self) -> int:
@@ -131,7 +136,11 @@ class ReconstructionParameters:
                                 long_object_compensation=self.longObjectCompensation,
                                 vol_center=self.volume_centers())
         P.load_tilt_series(ts.images, ts.eulers())
-        if ts.normalize:
+        if ts.transforms is not None:
+            # getPixels(i) through the affine alignment (AffineAlignment.applyTransformOnImage)
+            P.align_projections(ts.transforms, ts.tilts, ts.normalization_min_size, ts.normalize, ts.fill,
+                                ts.stats)
+        elif ts.normalize:
             # getPixels(i) with normalisation, for unaligned images (identity alignment)
             P.normalize_projections(ts.tilts, ts.normalization_min_size)
         if self.reconstructionType == WBP:

@@ -112,6 +112,24 @@ class VoxelProjector3DHIP:
         call("tj_normalize_projections", self._ctx, _ptr(t), int(min_size), _ptr(stats))
         return stats
 
+    def align_projections(self, transforms, tilts_deg=None, min_size: int = 0, normalize: bool = True,
+                          fill: int = 1, stats=None) -> np.ndarray:
+        """TiltSeries.getPixels through AffineAlignment.applyTransformOnImage
+        (AffineAlignment.java:136-254) applied to the resident projections:
+        `transforms` n x 6 forward transforms (m00, m10, m01, m11, m02, m12);
+        `fill` TJ_FILL_NONE/AVG/NAN; `stats` n x (mean, stdDev) of the ImageJ
+        ROI, or None to compute them for rotation-free transforms.  Returns the
+        statistics used (zeros when neither normalisation nor FILL_AVG needs them)."""
+        T = _f64(transforms).reshape(-1, 6)
+        if T.shape[0] != self.n_loaded:
+            raise ValueError(f"{T.shape[0]} transforms for {self.n_loaded} projections")
+        t = None if tilts_deg is None else _f64(tilts_deg)
+        sin = None if stats is None else _f64(stats).reshape(-1, 2)
+        out = np.zeros((self.n_loaded, 2), np.float64)
+        call("tj_align_projections", self._ctx, _ptr(T), _ptr(t) if t is not None else None, int(min_size),
+             1 if normalize else 0, int(fill), _ptr(sin) if sin is not None else None, _ptr(out))
+        return out
+
     def get_loaded_projection(self, index: int) -> np.ndarray:
         out = np.empty((self.ph, self.pw), np.float32)
         call("tj_get_loaded_projection", self._ctx, int(index), _ptr(out))
