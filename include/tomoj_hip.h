@@ -38,6 +38,7 @@ extern "C" {
 #define TJ_ESTATE -4     /* call not valid in the current state                  */
 #define TJ_EUNSUP -5     /* geometry or option not supported                     */
 #define TJ_ECANCELLED -6 /* tj_interrupt() was called during the loop             */
+#define TJ_EIO -7        /* a file could not be read or written                   */
 
 /* Storage type of the resident volume and projections.  The projector context
  * (tj_open) stores FP32 only.  The job engine (tj_batch_*) stores any of them:
@@ -263,6 +264,33 @@ int tj_normalize_projections(tj_ctx *ctx, const double *tilt_deg, int min_size, 
  * non-invertible transform. */
 int tj_align_projections(tj_ctx *ctx, const double *transforms, const double *tilt_deg, int min_size,
                          int normalize, int fill, const double *stats_in, double *stats_out);
+
+/* ---------------------------------------------------------------------------
+ * MRC files (InputOutput/MRC_Reader.java, MRC_Writer.java): the container
+ * TomoJ reads tilt series from and writes reconstructions to.  Host only.
+ * ------------------------------------------------------------------------- */
+typedef struct tj_mrc_info {
+    int nx, ny, nz;    /* columns, rows, sections (images)                      */
+    int mode;          /* 0 uint8, 1 int16, 2 float32, 6 uint16                  */
+    int little_endian; /* byte order found by MRC_Reader's rule (:176-215)       */
+    int extra_bytes;   /* extended header size (word 23)                         */
+    int fei;           /* extended header of 131072 bytes: FEI tilt angles       */
+    float amin, amax, amean; /* header statistics (words 19-21)                  */
+} tj_mrc_info;
+
+/* Header of an MRC file, as MRC_Reader.getMRCFileInfo reads it
+ * (MRC_Reader.java:176-233).  TJ_EUNSUP for other data modes, TJ_EIO when the
+ * file cannot be read. */
+int tj_mrc_read_info(const char *path, tj_mrc_info *info);
+
+/* The data as float, nz x ny x nx (ImageStack layout); tilts (may be NULL)
+ * receives nz FEI tilt angles (MRC_Reader.readTiltAngles :235-256), or NaN
+ * when the file has no FEI extended header. */
+int tj_mrc_read(const char *path, float *out, float *tilts);
+
+/* A float volume or stack nz x ny x nx as MRC_Writer writes a float image
+ * (MRC_Writer.java:60-160): little-endian, mode 2, header statistics. */
+int tj_mrc_write(const char *path, const float *data, int nx, int ny, int nz);
 
 /* Copies resident projection `index` to the host. */
 int tj_get_loaded_projection(tj_ctx *ctx, int index, float *out);

@@ -18,6 +18,7 @@ TJ_EDEVICE = -3
 TJ_ESTATE = -4
 TJ_EUNSUP = -5
 TJ_ECANCELLED = -6
+TJ_EIO = -7
 
 TJ_FILL_NONE = 0
 TJ_FILL_AVG = 1
@@ -117,6 +118,9 @@ _SIGS = {
     "tj_get_loaded_projection": (C.c_int, [_P, C.c_int, _P]),
     "tj_normalize_projections": (C.c_int, [_P, _P, C.c_int, _P]),
     "tj_align_projections": (C.c_int, [_P, _P, _P, C.c_int, C.c_int, C.c_int, _P, _P]),
+    "tj_mrc_read_info": (C.c_int, [C.c_char_p, _P]),
+    "tj_mrc_read": (C.c_int, [C.c_char_p, _P, _P]),
+    "tj_mrc_write": (C.c_int, [C.c_char_p, _P, C.c_int, C.c_int, C.c_int]),
     "tj_get_timing": (C.c_int, [_P, C.POINTER(tj_timing)]),
     "tj_enable_timing": (C.c_int, [_P, C.c_int]),
     "tj_set_tuning": (C.c_int, [_P, C.POINTER(tj_tuning)]),

@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtomoj_hip.so")
-SOURCES = ["tj_kernels.hip", "tj_fft.hip", "tj_api.hip", "tj_batch.hip"]
+SOURCES = ["tj_kernels.hip", "tj_fft.hip", "tj_api.hip", "tj_batch.hip", "tj_mrc.cpp"]
 ARCH = os.environ.get("TOMOJ_OFFLOAD_ARCH", "gfx950")
 
 
@@ -47,7 +47,7 @@ def build(verbose: bool = False) -> str:
     os.makedirs(obj_dir, exist_ok=True)
     procs = []
     for src in SOURCES:
-        obj = os.path.join(obj_dir, src.replace(".hip", ".o"))
+        obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
         cmd = [cc, *compile_flags(), "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
