@@ -27,7 +27,7 @@ hipError_t residual(hipStream_t, int, int, int, const float*, const int*, const 
                     float*, double*, uint8_t*, double*);
 hipError_t row_flags(hipStream_t, int, int, int, const float*, const int*, uint8_t*);
 hipError_t bp(hipStream_t, const TjGeomDev&, float*, const TjProj*, const int*, int, const float*, const int*,
-              const uint8_t*, TjBpRow*, int, int, int, int, int, int, int);
+              const uint8_t*, TjBpRow*, int, int, int, int, int, int, int, double2*);
 hipError_t zyx_to_yzx(hipStream_t, const float*, float*, int, int, int, int);
 hipError_t yzx_to_zyx(hipStream_t, const float*, float*, int, int, int, int);
 hipError_t plane_flags(hipStream_t, const float*, size_t, int, int, uint8_t*);
@@ -135,6 +135,10 @@ struct tj_ctx {
     double fista_t = 0.0;
     float* prev = nullptr;      // [vh][vd][vw]
 
+    // FP64 pair images of the backprojected images (LDS backprojection)
+    double2* bpairs = nullptr;  // [bpairs_cap][ph][pw]
+    int bpairs_cap = 0;
+
     // kernel selection and shapes (tj_set_tuning)
     tj_tuning tune{0, 2, 4, 4, 0, 4, 8, 3};
 
@@ -150,6 +154,7 @@ struct tj_ctx {
 
 static void ctx_free(tj_ctx* c)
 {
+    dfree(c->bpairs);
     dfree(c->prev);
     dfree(c->vol);
     dfree(c->pflag);
@@ -270,6 +275,17 @@ static int pick_bp_kernel(const tj_ctx* c, const std::vector<int>& idx)
     return bp_lds_ok(c, idx) ? 3 : 2;
 }
 
+// Room for the pair images of n backprojected images (grows, never shrinks).
+static int ensure_bpairs(tj_ctx* c, int n)
+{
+    if (n <= c->bpairs_cap) return TJ_OK;
+    dfree(c->bpairs);
+    c->bpairs_cap = 0;
+    TJ_HIP(dalloc(&c->bpairs, (size_t)n * c->npx));
+    c->bpairs_cap = n;
+    return TJ_OK;
+}
+
 static int ensure_iota(tj_ctx* c, int n)
 {
     if (n <= c->iota_n) return TJ_OK;
@@ -376,9 +392,11 @@ static int do_backproject(tj_ctx* c, const int* slots, const std::vector<int>& h
                           const int* image_index, int start_y, int end_y)
 {
     const int kernel = pick_bp_kernel(c, hslots);
+    if (kernel == 3)
+        if (int rc = ensure_bpairs(c, (int)hslots.size())) return rc;
     TJ_HIP(tjk::bp(c->stream, c->g, c->vol, c->pp, slots, (int)hslots.size(), images, image_index, c->rowflag,
                    c->bprows, start_y, end_y, kernel, c->tune.bp_rows, c->tune.bp_zchunk, c->tune.bp_group,
-                   c->any_nonfinite ? 0 : 1));
+                   c->any_nonfinite ? 0 : 1, c->bpairs));
     return TJ_OK;
 }
 
@@ -1230,9 +1248,11 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
                 if (c->timing_on) TJ_HIP(hipEventRecord(b, c->stream));
                 if ((rc = do_residual(c, slots, diffIndex, (double)factorimg, c->err + off))) return rc;
                 if (c->timing_on) TJ_HIP(hipEventRecord(r, c->stream));
+                if (bk == 3)
+                    if ((rc = ensure_bpairs(c, diffIndex))) return rc;
                 TJ_HIP(tjk::bp(c->stream, c->g, c->vol, c->pp, slots, diffIndex, c->sdiff, c->iota, c->rowflag,
                                c->bprows, start_y, end_y, bk, c->tune.bp_rows, c->tune.bp_zchunk, c->tune.bp_group,
-                               c->any_nonfinite ? 0 : 1));
+                               c->any_nonfinite ? 0 : 1, c->bpairs));
                 if (c->timing_on) {
                     TJ_HIP(hipEventRecord(d, c->stream));
                     fpev.emplace_back(a, b);

@@ -461,6 +461,26 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_bp_generic(
 // vertical axis with scale 1).  Same arithmetic as the kernels above.
 // ---------------------------------------------------------------------------
 
+// FP64 pair image of the images a backprojection adds: pairs[s][y][x] =
+// ((double)ll, (double)lr - (double)ll) with ll = image(s)[y][x], lr its right
+// neighbour (the last column repeated) -- the two operands of ImageJ's
+// interpolation along x, in the cells' order, so the backprojection stages
+// them with plain buffer-to-LDS copies.
+__global__ __launch_bounds__(TJ_BLOCK) void k_bp_pairs(const float* __restrict__ images,
+                                                       const int* __restrict__ image_index, int pw, int ph,
+                                                       double2* __restrict__ pairs)
+{
+    const int s = blockIdx.y;
+    const float* img = images + (size_t)image_index[s] * pw * ph;
+    double2* out = pairs + (size_t)s * pw * ph;
+    const size_t npx = (size_t)pw * ph;
+    for (size_t k = (size_t)blockIdx.x * TJ_BLOCK + threadIdx.x; k < npx; k += (size_t)gridDim.x * TJ_BLOCK) {
+        const int x = (int)(k % pw);
+        const double ll = img[k], lr = img[x + 1 < pw ? k + 1 : k];
+        out[k] = make_double2(ll, lr - ll);
+    }
+}
+
 // Backprojection: a 256-thread block owns 256 voxel columns x, R volume rows and
 // KC slices, and keeps their KC*R sums in registers.  For each group of G added
 // images it stages, per (image, row), the detector columns its voxels can reach
@@ -476,9 +496,10 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_bp_lds(
     TjGeomDev g, float* __restrict__ vol, const TjProj* __restrict__ pp,
     const int* __restrict__ slot_proj, int nslot, const float* __restrict__ images,
     const int* __restrict__ image_index, const TjBpRow* __restrict__ rows, int start_y, int end_y,
-    int finite_volume)
+    int finite_volume, const double2* __restrict__ pairs)
 {
     constexpr int XW = TJ_BLOCK + KC + 8;  // reachable columns + a zero cell at XW-1
+    constexpr int NB = (XW + 63) / 64;     // 64-cell copies per staged row
     __shared__ double2 lds[G][R][XW];
     // per-slot constants of the current group, staged once per pass
     __shared__ double s_e0[G], s_e2[G];
@@ -553,39 +574,34 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_bp_lds(
             s_ryf[gg][r] = yf;
         }
         __syncthreads();
-        // wave w stages image rows w, w+4, ... of the group; lanes take
-        // consecutive columns, so a row's constants are uniform in the wave
+        // wave w stages image rows w, w+4, ... of the group straight from the
+        // pair image into LDS: each 64-cell copy is one range-checked 16-byte
+        // buffer load per lane (cells past the row, rows that take no value
+        // and the zero cell XW-1 read 0); the last copy of a row is shifted
+        // back to end at XW, rewriting cells of the previous one with the
+        // same values, so no copy writes past the row
         {
             const int lane = tid & 63;
-            for (int gr = tid >> 6; gr < gn * R; gr += TJ_BLOCK / 64) {
+            const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+            for (int gr = wv; gr < gn * R; gr += TJ_BLOCK / 64) {
                 const int gg = gr / R, r = gr - (gr / R) * R;
-                const int x0s = s_x0[gg];
-                const bool rok = (s_rflag[gg][r] & 1) != 0;
-                const float* img = images + (size_t)s_img[gg] * npx + (size_t)s_ryb[gg][r] * g.pw;
-                double2* dst = &lds[gg][r][0];
-                // TJ_SU columns per lane per batch, loads first (addresses
-                // clamped into the row; cells that take no value store 0)
-                for (int x0 = 0; x0 < XW; x0 += 64 * TJ_SU) {
-                    float ll[TJ_SU], lr[TJ_SU];
+                const int x0s = __builtin_amdgcn_readfirstlane(s_x0[gg]);
+                const bool rok = (__builtin_amdgcn_readfirstlane(s_rflag[gg][r]) & 1) != 0;
+                const int ryb = __builtin_amdgcn_readfirstlane(s_ryb[gg][r]);
+                const double2* prow = pairs + ((size_t)(s0 + gg) * g.ph + (rok ? ryb : 0)) * g.pw;
+                const __amdgpu_buffer_rsrc_t rs =
+                    __builtin_amdgcn_make_buffer_rsrc((void*)prow, 0, rok ? g.pw * 16 : 0, 0x00020000);
 #pragma unroll
-                    for (int u = 0; u < TJ_SU; ++u) {
-                        const int xs = x0s + x0 + 64 * u + lane;
-                        ll[u] = img[min(xs, g.pw - 1)];
-                        lr[u] = img[min(xs + 1, g.pw - 1)];
-                    }
-#pragma unroll
-                    for (int u = 0; u < TJ_SU; ++u) {
-                        const int xx = x0 + 64 * u + lane;
-                        const int xs = x0s + xx;
-                        if (xx < XW) {
-                            double2 v = make_double2(0.0, 0.0);
-                            if (rok && xs < g.pw && xx < XW - 1) v = make_double2((double)ll[u], (double)lr[u] - (double)ll[u]);
-                            dst[xx] = v;
-                        }
-                    }
+                for (int b = 0; b < NB; ++b) {
+                    const int c0 = min(64 * b, XW - 64);
+                    const int xx = c0 + lane;
+                    const uint32_t voff = (xx < XW - 1) ? (uint32_t)(x0s + xx) * 16u : 0xFFFFFFF0u;
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)&lds[gg][r][c0],
+                                                             16, voff, 0, 0, 0);
                 }
             }
         }
+        __builtin_amdgcn_s_waitcnt(0);  // this thread's copies have landed
         __syncthreads();
         for (int gg = 0; gg < gn; ++gg) {
             const double e2 = s_e2[gg];
@@ -1259,22 +1275,26 @@ hipError_t row_flags(hipStream_t st, int pw, int ph, int nslot, const float* ima
 template <int R, int KC, int G>
 static hipError_t launch_bp_lds(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp,
                                 const int* slot_proj, int nslot, const float* images, const int* image_index,
-                                const TjBpRow* rows, int start_y, int end_y, int finite_volume)
+                                const TjBpRow* rows, int start_y, int end_y, int finite_volume, double2* pairs)
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    const size_t npx = (size_t)g.pw * g.ph;
+    k_bp_pairs<<<dim3((unsigned)std::min<size_t>((npx + TJ_BLOCK - 1) / TJ_BLOCK, 1024), nslot), TJ_BLOCK, 0, st>>>(
+        images, image_index, g.pw, g.ph, pairs);
     const int ny = end_y - start_y;
     dim3 grid((g.vw + TJ_BLOCK - 1) / TJ_BLOCK, (ny + R - 1) / R, (g.vd + KC - 1) / KC);
     if (g.positivity)
-        k_bp_lds<R, KC, G, true><<<grid, TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume);
+        k_bp_lds<R, KC, G, true><<<grid, TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume, pairs);
     else
-        k_bp_lds<R, KC, G, false><<<grid, TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume);
+        k_bp_lds<R, KC, G, false><<<grid, TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume, pairs);
     return hipGetLastError();
 }
 
 // kernel: 1 generic, 2 axis (global loads), 3 axis with LDS staging (scale 1)
 hipError_t bp(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp, const int* slot_proj, int nslot,
               const float* images, const int* image_index, const uint8_t* rowflag, TjBpRow* rows, int start_y,
-              int end_y, int kernel, int rows_per_thread, int kchunk, int group, int finite_volume)
+              int end_y, int kernel, int rows_per_thread, int kchunk, int group, int finite_volume,
+              double2* pairs)
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nslot <= 0 || end_y <= start_y) return hipSuccess;
@@ -1289,7 +1309,7 @@ hipError_t bp(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp, 
     }
     k_bp_rows<<<dim3((g.vh + 255) / 256, nslot), 256, 0, st>>>(g, pp, slot_proj, nslot, rowflag, rows);
     if (kernel == 3) {
-#define TJ_BPL(RR, KK, GG) return launch_bp_lds<RR, KK, GG>(st, g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume)
+#define TJ_BPL(RR, KK, GG) return launch_bp_lds<RR, KK, GG>(st, g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume, pairs)
         if (rows_per_thread == 2) {
             if (kchunk >= 16) { if (group >= 4) TJ_BPL(2, 16, 4); TJ_BPL(2, 16, 2); }
             if (group >= 8) TJ_BPL(2, 8, 8);
