@@ -508,11 +508,33 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_bp_lds(
     __shared__ int s_rflag[G][R];  // bit0 valid, bit1 shortcut
     __shared__ int s_ryb[G][R], s_ry1[G][R];
     const int tid = threadIdx.x;
-    const int i0 = blockIdx.x * TJ_BLOCK;
+    // XCD-aware block order (1-D grid of nbx * nby * nbz blocks): blocks are
+    // dealt to the 8 XCDs round-robin, so block b runs on XCD b % 8; all blocks
+    // of one row group (the ones that stage the same image rows) are put on one
+    // XCD, consecutive there, so its L2 serves their copies.  With a row-group
+    // count that is not a multiple of 8 the order is plain x, y, z.
+    const int nbx = (g.vw + TJ_BLOCK - 1) / TJ_BLOCK, nby = (end_y - start_y + R - 1) / R;
+    const int nbz = (g.vd + KC - 1) / KC;
+    int bx, by, bz;
+    {
+        const int b = blockIdx.x;
+        if ((nby & 7) == 0) {
+            const int per = nbx * nbz, k = b >> 3;
+            by = (b & 7) + 8 * (k / per);
+            const int rr = k - (k / per) * per;
+            bz = rr / nbx;
+            bx = rr - bz * nbx;
+        } else {
+            bx = b % nbx;
+            by = (b / nbx) % nby;
+            bz = b / (nbx * nby);
+        }
+    }
+    const int i0 = bx * TJ_BLOCK;
     const int i = i0 + tid;
-    const int j0 = start_y + blockIdx.y * R;
+    const int j0 = start_y + by * R;
     const int nr = min(R, end_y - j0);
-    const int k0 = blockIdx.z * KC;
+    const int k0 = bz * KC;
     const int nk = min(KC, g.vd - k0);
     const int ilast = min(i0 + TJ_BLOCK - 1, g.vw - 1);
     const bool active = i < g.vw;
@@ -1282,7 +1304,7 @@ static hipError_t launch_bp_lds(hipStream_t st, const TjGeomDev& g, float* vol, 
     k_bp_pairs<<<dim3((unsigned)std::min<size_t>((npx + TJ_BLOCK - 1) / TJ_BLOCK, 1024), nslot), TJ_BLOCK, 0, st>>>(
         images, image_index, g.pw, g.ph, pairs);
     const int ny = end_y - start_y;
-    dim3 grid((g.vw + TJ_BLOCK - 1) / TJ_BLOCK, (ny + R - 1) / R, (g.vd + KC - 1) / KC);
+    const unsigned grid = (unsigned)(((g.vw + TJ_BLOCK - 1) / TJ_BLOCK) * ((ny + R - 1) / R) * ((g.vd + KC - 1) / KC));
     if (g.positivity)
         k_bp_lds<R, KC, G, true><<<grid, TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume, pairs);
     else
