@@ -170,8 +170,8 @@ def main():
         "kernel": kname,
         "bytes_per_launch": kernels[dom]["bytes_per_launch"],
         "avg_launch_ms": kernels[dom]["avg_ms"],
-        "limiter": "VALU: FP64 positions and ImageJ's FP64 bilinear weights per voxel update "
-                   "(SQ_INSTS_VALU in profiles/); HBM traffic is below the slab-streaming model",
+        "limiter": "VALU and LDS: FP64 positions and ImageJ's FP64 bilinear weights per voxel update "
+                   "(SQ_INSTS_VALU in profiles/); measured HBM traffic is within 8% of the algorithmic bytes",
     }
     alg = WL.algorithmic_bytes_per_iteration(w)
     out = {
