@@ -7,6 +7,7 @@
 // kernel (paths relative to src/TomoJ_Base/src/main/java/fr/curie/tomoj/).
 #include "tj_device.h"
 #include <math.h>
+#include <type_traits>
 
 namespace tj {
 
@@ -754,7 +755,9 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_norm_table(TjGeomDev g, const TjPr
 // zero cell.  Requires a finite volume and rows whose y is an integer (the
 // host checks both; otherwise k_fp_axis runs).  Blocks are numbered so the
 // blocks sharing one row group run on one XCD back to back and share its L2.
-template <int G, bool NEARY>
+// NG: the rays of the thread that belong to projections of the launch (the
+// last group of a launch can hold fewer than G); only they are marched.
+template <int G, int NG, bool NEARY>
 __device__ __forceinline__ void fp_lds_chunks(const TjGeomDev& g, const float* __restrict__ vol, float4* lds4,
                                               int zc, int rowbase, int ns, const float (&ym)[4], double (&xc)[G],
                                               const double (&dirx)[G], float4 (&rs)[G])
@@ -811,7 +814,7 @@ __device__ __forceinline__ void fp_lds_chunks(const TjGeomDev& g, const float* _
             const float zl = (zz == zlast) ? 0.0f : 1.0f;
             const int zo = zz * W1;
 #pragma unroll
-            for (int k = 0; k < G; ++k) {
+            for (int k = 0; k < NG; ++k) {
                 xc[k] = xc[k] + dirx[k];
                 const double xv = xc[k] + g.vcx;
                 const bool in = (xv >= nacc) && (xv < bx);
@@ -904,10 +907,20 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_fp_lds(
     }
     if (have_row) {
         if (tid < 2) lds4[zc * (g.vw + 1) + tid] = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (neary)
-            fp_lds_chunks<G, true>(g, vol, lds4, zc, rowbase, ns, ym, xc, dirx, rs);
-        else
-            fp_lds_chunks<G, false>(g, vol, lds4, zc, rowbase, ns, ym, xc, dirx, rs);
+        auto run = [&](auto ng) {
+            constexpr int NG = decltype(ng)::value;
+            if (neary)
+                fp_lds_chunks<G, NG, true>(g, vol, lds4, zc, rowbase, ns, ym, xc, dirx, rs);
+            else
+                fp_lds_chunks<G, NG, false>(g, vol, lds4, zc, rowbase, ns, ym, xc, dirx, rs);
+        };
+        // the smallest supported count of marched rays that covers ns
+        if (ns >= G) run(std::integral_constant<int, G>{});
+        else if (ns == 1) run(std::integral_constant<int, 1>{});
+        else if (ns == 2) run(std::integral_constant<int, (G > 2 ? 2 : G)>{});
+        else if (ns == 3) run(std::integral_constant<int, (G > 3 ? 3 : G)>{});
+        else if (ns == 4) run(std::integral_constant<int, (G > 4 ? 4 : G)>{});
+        else run(std::integral_constant<int, G>{});
     }
     if (!col) return;
 #pragma unroll
