@@ -194,13 +194,15 @@ def test_half_storage_awkward_shapes(batch, oracle, case):
 def test_half_tile_shapes_do_not_change_results(batch, oracle):
     """The 16-bit kernels' tile and slab shapes partition the work without
     changing any voxel's arithmetic: bit-identical volumes."""
-    W, H, D = 70, 21, 27
+    W, H, D = 300, 21, 27   # two 256-column tiles (one shared block with fp_cols=2)
     tilts, E, exps = phantom_case(oracle, W, H, D, tilts=np.arange(-60, 61, 10.0))
     shapes = [dict(fp_rays=2, fp_lds_kb=16, bp_rows=2, bp_zchunk=8, blocks_per_cu=1),
               dict(fp_rays=4, fp_lds_kb=64, bp_rows=4, bp_zchunk=16, blocks_per_cu=8),
               dict(fp_rays=4, fp_lds_kb=128, bp_rows=8, bp_zchunk=8, blocks_per_cu=2),
               dict(fp_rays=2, fp_lds_kb=32, bp_rows=8, bp_zchunk=16, blocks_per_cu=4),
-              dict(fp_rays=4, fp_lds_kb=48, bp_rows=4, bp_zchunk=4, blocks_per_cu=3)]
+              dict(fp_rays=4, fp_lds_kb=48, bp_rows=4, bp_zchunk=4, blocks_per_cu=3),
+              dict(fp_rays=4, fp_lds_kb=64, fp_cols=2, bp_rows=4, bp_zchunk=16, blocks_per_cu=4),
+              dict(fp_rays=2, fp_lds_kb=32, fp_cols=2, bp_rows=8, bp_zchunk=8, blocks_per_cu=2)]
     results = []
     for sh in shapes:
         with batch.JobEngine(GIB) as eng:

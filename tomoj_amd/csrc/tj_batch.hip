@@ -543,8 +543,10 @@ __device__ __forceinline__ void fpp_slice_f64(const uint4* cell, int hp, int hb,
 // getPixel takes the nearest value, use FP64 positions as TomoJ computes them.
 // Staging reads 32-bit words (two columns) when W is even, so a pair cell is
 // a word or one byte permutation of two.
-template <typename T, typename TE, int G>
-__global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, const int4* __restrict__ tiles,
+// CT: 256-column tiles per block (a block of CT * 256 threads stages the slab
+// once for all of them)
+template <typename T, typename TE, int G, int CT>
+__global__ __launch_bounds__(NT * CT) void k_job_fpp(const TjJob* __restrict__ jobs, const int4* __restrict__ tiles,
                                                 int ntiles, int lds_cells)
 {
     constexpr int RB = 8;  // volume rows per tile
@@ -553,7 +555,8 @@ __global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, 
     // cell[zz WC + c]: rows 0-3 of cell c of slice zz; cell[hp + zz WC + c]: rows 4-7
     // (two planes, so consecutive cells are consecutive 16-byte words)
     uint4* cell = reinterpret_cast<uint4*>(smem);
-    __shared__ double red[NT / 64][G * RB];
+    constexpr int BT = NT * CT;  // threads per block
+    __shared__ double red[BT / 64][G * RB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int4 tl = tiles[t];
@@ -565,8 +568,8 @@ __global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, 
         const int hb = (((WC + 1) / 2 + 3) & ~7) + 4;       // first odd cell: >= even count, 4 mod 8
         const int ss = hb + WC / 2;                        // slice stride (cells)
         const int zcl = min(D, lds_cells / (2 * ss));      // slices the LDS holds
-        const int p0 = tl.y, j0 = tl.z * RB, ct = tl.w;
-        const int i = ct * NT + tid;
+        const int p0 = tl.y, j0 = tl.z * RB;
+        const int i = tl.w * BT + tid;
         const bool col = i < pw;
         const double ic = col ? i : pw - 1;
         const size_t plane = (size_t)D * W;
@@ -586,8 +589,8 @@ __global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, 
         // word staging (W even, every row inside, a slab within PF items per
         // thread) is software-pipelined: the next slab's words are loaded into
         // registers while the current slab is projected
-        const bool words = allin && (W & 1) == 0 && half <= PF * NT;
-        const int zc = words ? min(zcl, (PF * NT) / half) : zcl;
+        const bool words = allin && (W & 1) == 0 && half <= PF * BT;
+        const int zc = words ? min(zcl, (PF * BT) / half) : zcl;
         const int hp = zc * ss;  // offset of the rows 4-7 plane
         float base1[G], tn[G], rs[G][RB];
         int zlo[G], zhi[G];
@@ -603,11 +606,11 @@ __global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, 
         }
         const float wc1 = (float)(W + 1);
         const double wd = (double)(W - 1);
-        // item q of this thread: m = tid + q NT = (slice zz, column pair x, x + 1)
+        // item q of this thread: m = tid + q BT = (slice zz, column pair x, x + 1)
         int izz[PF], ix[PF];
 #pragma unroll
         for (int q = 0; q < PF; ++q) {
-            const int m = tid + q * NT;
+            const int m = tid + q * BT;
             izz[q] = words ? m / half : 0;
             ix[q] = words ? 2 * (m - izz[q] * half) : 0;
         }
@@ -637,7 +640,7 @@ __global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, 
         for (int z0 = 0; z0 < D; z0 += zc) {
             const int zn = min(zc, D - z0);
             __syncthreads();
-            for (int q = tid; q < zn; q += NT) {
+            for (int q = tid; q < zn; q += BT) {
                 uint4* d = cell + (size_t)q * ss;
                 const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
                 d[fpp_cidx(0, hb)] = zero4;
@@ -668,7 +671,7 @@ __global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, 
                 if (z0 + zn < D) fetch(z0 + zn, min(zc, D - z0 - zn));
             } else {
                 // any width, rows outside the volume (they read row 0 and store 0)
-                for (int m = tid; m < zn * W; m += NT) {
+                for (int m = tid; m < zn * W; m += BT) {
                     const int zz = m / W, x = m - zz * W;
                     const size_t oa = (size_t)(z0 + zz) * W + x, ob = (size_t)(z0 + zz) * W + min(x + 1, W - 1);
                     uint32_t v[RB];
@@ -773,13 +776,15 @@ __global__ __launch_bounds__(NT) void k_job_fpp(const TjJob* __restrict__ jobs, 
                 for (int r = 0; r < RB; ++r) red[wave][k * RB + r] = e2[k][r];
         }
         __syncthreads();
-        if (tid < G * RB) {
-            const int p = p0 + tid / RB, j = j0 + (tid % RB);
-            if (p < n && j < ph) {
-                double s = red[0][tid];
+        // per 256-column tile: the sum of its 4 waves in order
+        if (tid < CT * G * RB) {
+            const int h = tid / (G * RB), e = tid - h * (G * RB);
+            const int p = p0 + e / RB, j = j0 + (e % RB), cth = tl.w * CT + h;
+            if (p < n && j < ph && cth < J.nct) {
+                double s = red[4 * h][e];
 #pragma unroll
-                for (int w = 1; w < NT / 64; ++w) s += red[w][tid];
-                gp(J.err_part)[((size_t)p * ph + j) * J.nct + ct] = s;
+                for (int w = 1; w < NT / 64; ++w) s += red[4 * h + w][e];
+                gp(J.err_part)[((size_t)p * ph + j) * J.nct + cth] = s;
             }
         }
     }
@@ -1194,7 +1199,7 @@ struct tj_batch {
     int next_id = 1;
     long epoch = 0;
     std::map<int, BJob*> jobs;
-    tj_batch_tuning tune{4, 64, 4, 16, 2, 8, 1};  // the autotuner's pick at 512^3 BF16
+    tj_batch_tuning tune{4, 128, 4, 16, 2, 8, 1, 2};  // the autotuner's pick at 512^3 BF16
     RunPlan plan;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
@@ -1319,11 +1324,11 @@ static hipError_t allow_lds2()
 template <typename T, typename TE>
 static hipError_t allow_lds_p()
 {
-    hipError_t e = hipFuncSetAttribute((const void*)tjb::k_job_fpp<T, TE, 2>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, kMaxFpLds);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)tjb::k_job_fpp<T, TE, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                kMaxFpLds);
+    const void* fns[] = {(const void*)tjb::k_job_fpp<T, TE, 2, 1>, (const void*)tjb::k_job_fpp<T, TE, 4, 1>,
+                         (const void*)tjb::k_job_fpp<T, TE, 2, 2>, (const void*)tjb::k_job_fpp<T, TE, 4, 2>};
+    hipError_t e = hipSuccess;
+    for (const void* f : fns)
+        if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxFpLds);
     return e;
 }
 
@@ -1356,6 +1361,12 @@ static bool use_packed(const tj_batch_tuning& tu, int storage)
     return tu.fp_packed && (storage == TJ_STORAGE_BF16 || storage == TJ_STORAGE_F16);
 }
 
+// 256-column tiles per packed forward block: 1 or 2
+static int fp_cols_of(const tj_batch_tuning& tu, int storage)
+{
+    return (use_packed(tu, storage) && tu.fp_cols >= 2) ? 2 : 1;
+}
+
 // projections per forward tile: 2 or 4
 static int fp_rays_of(const tj_batch_tuning& tu, int storage)
 {
@@ -1372,9 +1383,14 @@ static hipError_t launch_fpp_s(hipStream_t st, const tj_batch_tuning& tu, const 
     int lds = std::min(std::max(tu.fp_lds_kb, 16), 128) * 1024;
     lds = std::max(lds, 2 * (max_w + 16) * (int)sizeof(uint4));
     const int cells = lds / (int)sizeof(uint4);
-    if (fp_rays_of(tu, TJ_STORAGE_BF16) == 2)
-        tjb::k_job_fpp<T, TE, 2><<<grid, tjb::NT, lds, st>>>(jobs, tiles, ntiles, cells);
-    else tjb::k_job_fpp<T, TE, 4><<<grid, tjb::NT, lds, st>>>(jobs, tiles, ntiles, cells);
+    const bool two = fp_cols_of(tu, TJ_STORAGE_BF16) == 2, g2 = fp_rays_of(tu, TJ_STORAGE_BF16) == 2;
+    if (two) {
+        if (g2) tjb::k_job_fpp<T, TE, 2, 2><<<grid, 2 * tjb::NT, lds, st>>>(jobs, tiles, ntiles, cells);
+        else tjb::k_job_fpp<T, TE, 4, 2><<<grid, 2 * tjb::NT, lds, st>>>(jobs, tiles, ntiles, cells);
+    } else {
+        if (g2) tjb::k_job_fpp<T, TE, 2, 1><<<grid, tjb::NT, lds, st>>>(jobs, tiles, ntiles, cells);
+        else tjb::k_job_fpp<T, TE, 4, 1><<<grid, tjb::NT, lds, st>>>(jobs, tiles, ntiles, cells);
+    }
     return hipGetLastError();
 }
 
@@ -1497,15 +1513,17 @@ static std::vector<int4> interleave(std::vector<Group>& groups)
     return out;
 }
 
-static std::vector<int4> fp_tiles(const std::vector<BJob*>& js, const std::vector<int>& idx, int G, int RB)
+// CT: 256-column tiles per block (the tile's column index counts blocks)
+static std::vector<int4> fp_tiles(const std::vector<BJob*>& js, const std::vector<int>& idx, int G, int RB,
+                                  int CT)
 {
     std::vector<Group> groups;
     for (size_t m = 0; m < js.size(); ++m) {
         const tj_job_desc& d = js[m]->d;
-        const int nct = fp_col_tiles(d.proj_w), nrb = (d.proj_h + RB - 1) / RB;
+        const int nct = (fp_col_tiles(d.proj_w) + CT - 1) / CT, nrb = (d.proj_h + RB - 1) / RB;
         for (int rb = 0; rb < nrb; ++rb) {
             Group g;
-            g.cost = (double)d.vol_d * ((double)d.vol_w + (double)tjb::NT * G) * ((d.n_proj + G - 1) / G) * nct;
+            g.cost = (double)d.vol_d * ((double)d.vol_w + (double)tjb::NT * CT * G) * ((d.n_proj + G - 1) / G) * nct;
             for (int p0 = 0; p0 < d.n_proj; p0 += G)
                 for (int ct = 0; ct < nct; ++ct) g.tiles.push_back(make_int4(idx[m], p0, rb, ct));
             groups.push_back(std::move(g));
@@ -1615,7 +1633,8 @@ static int build_plan(tj_batch* b, const std::vector<int>& ids)
                     fidx.push_back(sidx[m]);
                 }
             if (fj.empty()) continue;
-            std::vector<int4> ft = fp_tiles(fj, fidx, fp_rays_of(b->tune, s), fp_rows_of(b->tune, s));
+            std::vector<int4> ft =
+                fp_tiles(fj, fidx, fp_rays_of(b->tune, s), fp_rows_of(b->tune, s), fp_cols_of(b->tune, s));
             int4* dft = nullptr;
             if (int rc = upload(ft, &dft)) return rc;
             int max_w = 0;
@@ -2163,7 +2182,8 @@ int tj_batch_set_tuning(tj_batch* b, const tj_batch_tuning* t)
 {
     if (!b || !t) return bfail(TJ_EINVAL, "null argument");
     if (t->fp_rays < 1 || t->fp_lds_kb < 16 || t->fp_lds_kb > 128 || t->bp_rows < 1 || t->bp_zchunk < 1 ||
-        t->bp_group < 1 || t->blocks_per_cu < 1 || t->blocks_per_cu > 16 || t->fp_packed < 0 || t->fp_packed > 1)
+        t->bp_group < 1 || t->blocks_per_cu < 1 || t->blocks_per_cu > 16 || t->fp_packed < 0 || t->fp_packed > 1 ||
+        t->fp_cols < 0 || t->fp_cols > 2)
         return bfail(TJ_EINVAL, "tj_batch_set_tuning: value out of range");
     b->tune = *t;
     return TJ_OK;
@@ -2239,7 +2259,7 @@ int tj_batch_autotune(tj_batch* b, const int* ids, int nids, tj_batch_tuning* be
     std::vector<int> v(ids, ids + nids);
     const tj_batch_tuning start = b->tune;
     // forward: rays per tile x LDS per tile x persistent blocks per CU
-    const int fr[] = {2, 4}, fl[] = {32, 64, 96}, bpc[] = {2, 4, 8};
+    const int fr[] = {2, 4}, fl[] = {32, 64, 96, 128}, bpc[] = {2, 4, 8};
     bool any_half = false;
     for (int id : v) {
         BJob* j = find_job(b, id);
@@ -2248,10 +2268,12 @@ int tj_batch_autotune(tj_batch* b, const int* ids, int nids, tj_batch_tuning* be
     tj_batch_tuning cur = start, bestf = start;
     double tf = 1e300;
     for (int pk = any_half ? 0 : 1; pk <= 1; ++pk)
+    for (int cols = 1; cols <= ((any_half && pk) ? 2 : 1); ++cols)
     for (int r : fr)
         for (int l : fl)
             for (int c : bpc) {
                 cur.fp_packed = any_half ? pk : start.fp_packed;
+                cur.fp_cols = (any_half && pk) ? cols : start.fp_cols;
                 cur.fp_rays = r;
                 cur.fp_lds_kb = l;
                 cur.blocks_per_cu = c;
@@ -2290,6 +2312,7 @@ int tj_batch_autotune(tj_batch* b, const int* ids, int nids, tj_batch_tuning* be
     fin.fp_rays = bestf.fp_rays;
     fin.fp_lds_kb = bestf.fp_lds_kb;
     fin.fp_packed = bestf.fp_packed;
+    fin.fp_cols = bestf.fp_cols;
     double tot_best = 1e300;
     tj_batch_tuning pick = fin;
     for (int c : {bestf.blocks_per_cu, bestb.blocks_per_cu}) {
