@@ -187,6 +187,24 @@ int tj_end_iteration(tj_ctx *ctx);
  * does both around each iteration.  Resets t to 0. */
 int tj_set_fista(tj_ctx *ctx, int on);
 
+/* TVM regularisation: the projector AdvancedReconstructionParameters.getProjector
+ * builds for the TVM type (TomoJ_AdvancedReconstruction/.../tomography/
+ * AdvancedReconstructionParameters.java:179-193) is TVMProjector3D
+ * (.../projectors/TVMProjector3D.java:24-157), whose endOfIteration denoises the
+ * whole volume with TotalVariationDenoising_3D.denoise3D(stack, theta, g, dt,
+ * iterations) (Filters/.../TotalVariationDenoising_3D.java:84-202), dt being
+ * (float) regularizationLambda (:152).  With `iterations` > 0, tj_end_iteration
+ * and every iteration of tj_run_ossart end with that step (before the FISTA
+ * extrapolation when both are set: FistaProjector3D wraps TVMProjector3D);
+ * 0 turns it off.  theta and g are cast to float as setTvmTheta / setTvmG do
+ * (:64-74).  Holds 16 bytes per voxel of device memory while on. */
+int tj_set_tvm(tj_ctx *ctx, int iterations, double theta, double g, double lambda);
+
+/* TotalVariationDenoising_3D.denoise3D(stack, theta, g, dt, iterations)
+ * (TotalVariationDenoising_3D.java:84-202) of the resident volume, once: the
+ * plugin's run (:66-73) with the volume as the stack. */
+int tj_tv_denoise(tj_ctx *ctx, float theta, float g, float dt, int iterations);
+
 /* Projector.getCompletion (Projector.java:122-124); may be called from any thread. */
 double tj_get_completion(tj_ctx *ctx);
 

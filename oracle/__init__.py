@@ -108,6 +108,8 @@ def _L():
             "or_affine_resample": (None, [P, i, i, P, i, i, d, d, P]),
             "or_roi_rect_affine": (None, [i, i, d, i, P, P]),
             "or_rect_stats": (None, [P, i, i, P, P, P]),
+            "or_tv_denoise3d": (None, [P, i, i, i, C.c_float, C.c_float, C.c_float, i]),
+            "or_tv_denoise2d": (None, [P, i, i, C.c_float, C.c_float, C.c_float, i]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
@@ -385,6 +387,26 @@ def wbp(g: Geom, vol, projs, eulers, mask=None, update=0, start_y=0, end_y=None,
         _L().or_wbp(C.byref(g), _p(vol), _p(projs), _p(eulers), n,
                     None if m is None else _p(m), int(update), start_y, end_y)
     return vol
+
+
+def tv_denoise3d(vol, theta, g, dt, iterations) -> np.ndarray:
+    """TotalVariationDenoising_3D.denoise3D (TotalVariationDenoising_3D.java:84-202) of a
+    (D, H, W) float32 volume; returns the denoised copy.  theta, g and dt are floats
+    as in Java."""
+    out = np.array(vol, np.float32, order="C", copy=True)
+    D, H, W = out.shape
+    _L().or_tv_denoise3d(_p(out), W, H, D, float(np.float32(theta)), float(np.float32(g)),
+                         float(np.float32(dt)), int(iterations))
+    return out
+
+
+def tv_denoise2d(img, theta, g, dt, iterations) -> np.ndarray:
+    """TotalVariationDenoising_3D.denoise2D (:206-260) of an (H, W) float32 image."""
+    out = np.array(img, np.float32, order="C", copy=True)
+    h, w = out.shape
+    _L().or_tv_denoise2d(_p(out), w, h, float(np.float32(theta)), float(np.float32(g)),
+                         float(np.float32(dt)), int(iterations))
+    return out
 
 
 # --------------------------------------------------------------------------

@@ -21,6 +21,8 @@
  *        OSSART :548-634, osartIteration :636-733, orderShuffleART :735-781
  *   tomography/filters/FFTWeighting.java          createWeightingMask :31-65, weighting :87-112
  *   src/Utility/.../fr/curie/utils/MatrixUtils.java eulerAngles2Matrix :226-251
+ *   src/Filters/.../fr/curie/filters/TotalVariationDenoising_3D.java denoise3D :84-202,
+ *        denoise2D :206-260 (the step TVMProjector3D.endOfIteration runs)
  *   third party: ImageJ 1.53a ImageProcessor.getInterpolatedValue (not in the
  *   reference tree; restated from its published source, see ij_interpolated_value).
  *
@@ -1183,4 +1185,129 @@ void or_rect_stats(const float *img, int width, int height, const int *r4, doubl
         s = (s > 0.0) ? sqrt(s / (n - 1.0)) : 0.0;
     }
     *sd = s;
+}
+
+/* ---------------------------------------------------------------------------
+ * Total-variation denoising, TotalVariationDenoising_3D.denoise3D
+ * (src/Filters/src/main/java/fr/curie/filters/TotalVariationDenoising_3D.java:84-202):
+ * the end-of-iteration step of TVMProjector3D (src/TomoJ_AdvancedReconstruction/
+ * src/main/java/fr/curie/tomoj/tomography/projectors/TVMProjector3D.java:135-157).
+ * Volume f in ImageStack layout [z][y][x], replaced by the result.  u starts as
+ * a copy of f and p (three floats per voxel, interleaved x, y, z as pstack
+ * holds them) at zero; each step runs updateP over every voxel, then updateU.
+ * Float arithmetic in Java's order, nothing fused:
+ *   updateP (:115-156): forward differences duy, dux, duz, 0 on the last row,
+ *     column or slice; dd1 = 1 + dt/theta/g * |(float)sqrt(duy^2 + dux^2 + duz^2)|;
+ *     p_c = (p_c - dt/theta * du_c) / dd1.
+ *   updateU (:158-202): div = p_x(x) at x = 0, -p_x(x) at x = W-1 (the voxel's
+ *     own component, not its left neighbour's), p_x(x) - p_x(x-1) between;
+ *     then the same for y and z added on; u = f - theta * div.
+ * ------------------------------------------------------------------------- */
+void or_tv_denoise3d(float *f, int W, int H, int D, float theta, float g, float dt, int iterations)
+{
+    const long plane = (long)W * H, n = plane * D;
+    float *u = (float *)malloc(sizeof(float) * n);
+    float *p = (float *)calloc((size_t)n * 3, sizeof(float));
+    if (!u || !p) {
+        free(u);
+        free(p);
+        return;
+    }
+    memcpy(u, f, sizeof(float) * n);
+    for (int it = 0; it < iterations; ++it) {
+        /* updateP: slice z reads slice min(z+1, D-1) for the z difference */
+        for (int z = 0; z < D; ++z) {
+            const float *us = u + z * plane;
+            const float *u1 = u + (z + 1 < D ? z + 1 : D - 1) * plane;
+            float *ps = p + 3 * z * plane;
+            for (int y = 0; y < H; ++y)
+                for (int x = 0; x < W; ++x) {
+                    const long i = (long)y * W + x;
+                    const float duy = (y < H - 1) ? us[i + W] - us[i] : 0.0f;
+                    const float dux = (x < W - 1) ? us[i + 1] - us[i] : 0.0f;
+                    const float duz = (z < D - 1) ? u1[i] - us[i] : 0.0f;
+                    const float dd1 = 1 + dt / theta / g * fabsf((float)sqrt((double)(duy * duy + dux * dux + duz * duz)));
+                    ps[3 * i] = (ps[3 * i] - dt / theta * dux) / dd1;
+                    ps[3 * i + 1] = (ps[3 * i + 1] - dt / theta * duy) / dd1;
+                    ps[3 * i + 2] = (ps[3 * i + 2] - dt / theta * duz) / dd1;
+                }
+        }
+        /* updateU: slice z reads p of slice max(z-1, 0) for the z divergence */
+        for (int z = 0; z < D; ++z) {
+            const float *ps = p + 3 * z * plane;
+            const float *p1 = p + 3 * (z > 0 ? z - 1 : 0) * plane;
+            const float *fs = f + z * plane;
+            float *us = u + z * plane;
+            for (int y = 0; y < H; ++y)
+                for (int x = 0; x < W; ++x) {
+                    const long i = (long)y * W + x;
+                    float div;
+                    if (x == 0) div = ps[3 * i];
+                    else if (x == W - 1) div = -ps[3 * i];
+                    else div = ps[3 * i] - ps[3 * (i - 1)];
+                    if (y == 0) div += ps[3 * i + 1];
+                    else if (y == H - 1) div += -ps[3 * i + 1];
+                    else div += ps[3 * i + 1] - ps[3 * (i - W) + 1];
+                    if (z == 0) div += ps[3 * i + 2];
+                    else if (z == D - 1) div += -ps[3 * i + 2];
+                    else div += ps[3 * i + 2] - p1[3 * i + 2];
+                    us[i] = fs[i] - theta * div;
+                }
+        }
+    }
+    memcpy(f, u, sizeof(float) * n);
+    free(u);
+    free(p);
+}
+
+/* The 2-D form, TotalVariationDenoising_3D.denoise2D (:206-260), on one W x H
+ * image in place: forward differences of the whole image first (du holds the
+ * y difference in its first half and the x difference in its second, and keeps
+ * the previous step's value where no difference is formed), then p, then the
+ * divergence, then u.  A volume whose slices are all equal has duz = 0 and
+ * p_z = 0 throughout, so denoise3D reduces to this on each slice. */
+void or_tv_denoise2d(float *img, int w, int h, float theta, float g, float dt, int iterations)
+{
+    const long n = (long)w * h;
+    float *u = (float *)malloc(sizeof(float) * n);
+    float *p = (float *)calloc((size_t)n * 2, sizeof(float));
+    float *du = (float *)calloc((size_t)n * 2, sizeof(float));
+    float *divp = (float *)calloc((size_t)n, sizeof(float));
+    if (!u || !p || !du || !divp) {
+        free(u);
+        free(p);
+        free(du);
+        free(divp);
+        return;
+    }
+    memcpy(u, img, sizeof(float) * n);
+    for (int it = 0; it < iterations; ++it) {
+        for (int j = 0; j < h; ++j)
+            for (int i = 0; i < w; ++i) {
+                if (i < w - 1) du[i + (long)w * (j + h)] = u[i + 1 + (long)w * j] - u[i + (long)w * j];
+                if (j < h - 1) du[i + (long)w * j] = u[i + (long)w * (j + 1)] - u[i + (long)w * j];
+            }
+        for (int j = 0; j < h; ++j)
+            for (int i = 0; i < w; ++i) {
+                const float du1 = du[i + (long)w * j], du2 = du[i + (long)w * (j + h)];
+                const float dd1 = 1 + dt / theta / g * fabsf((float)sqrt((double)(du1 * du1 + du2 * du2)));
+                p[i + (long)w * j] = (p[i + (long)w * j] - dt / theta * du[i + (long)w * j]) / dd1;
+                p[i + (long)w * (j + h)] = (p[i + (long)w * (j + h)] - dt / theta * du[i + (long)w * (j + h)]) / dd1;
+            }
+        for (int j = 0; j < h; ++j)
+            for (int i = 0; i < w; ++i) {
+                if (i == 0) divp[(long)w * j] = p[(long)w * (j + h)];
+                else if (i == w - 1) divp[w - 1 + (long)w * j] = -p[w - 1 + (long)w * (j + h)];
+                else divp[i + (long)w * j] = p[i + (long)w * (j + h)] - p[i - 1 + (long)w * (j + h)];
+                if (j == 0) divp[i] += p[i];
+                else if (j == h - 1) divp[i + (long)w * (h - 1)] += -p[i + (long)w * (h - 1)];
+                else divp[i + (long)w * j] += p[i + (long)w * j] - p[i + (long)w * (j - 1)];
+            }
+        for (long k = 0; k < n; ++k) u[k] = img[k] - theta * divp[k];
+    }
+    memcpy(img, u, sizeof(float) * n);
+    free(u);
+    free(p);
+    free(du);
+    free(divp);
 }

@@ -547,3 +547,123 @@ def test_aligned_supply_rejects_rotation_without_statistics(tj, oracle):
             P.align_projections([[1, 0, 0, 1, 0, 0], [0.9, 0.1, -0.1, 0.9, 0, 0]], [0.0, 10.0], 0, True, 1)
         with pytest.raises(tj.TjError):
             P.align_projections([[1, 0, 0, 1, 0, 0], [0, 0, 0, 0, 1, 1]], [0.0, 10.0], 0, False, 0)
+
+
+# --------------------------------------------------------------------------
+# TVM regularisation (TVMProjector3D: TV denoising at the end of each iteration)
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("shape,theta,g,dt,n", [
+    ((37, 5, 70), 25.0, 1.0, 0.1, 5),     # two slice runs of the kernel (64 + 6)
+    ((300, 3, 9), 12.5, 2.0, 0.25, 3),    # two column tiles, the second ragged
+    ((1, 4, 3), 25.0, 1.0, 0.1, 2),       # W = 1: the first-column rule only
+    ((6, 1, 1), 25.0, 1.0, 0.1, 4),       # H = D = 1
+    ((16, 7, 5), 8.0, 0.5, 0.05, 0),      # no step: unchanged
+])
+def test_tv_denoise_bit_exact(tj, oracle, shape, theta, g, dt, n):
+    W, H, D = shape
+    print(f"tv_denoise {shape} theta={theta} g={g} dt={dt} n={n}", flush=True)
+    rng = np.random.default_rng(W * 131 + H * 17 + D)
+    vol = rng.standard_normal((D, H, W)).astype(np.float32)
+    ref = oracle.tv_denoise3d(vol, theta, g, dt, n)
+    with make(tj, (W, H, D)) as P:
+        P.set_volume(vol)
+        P.tv_denoise(theta, g, dt, n)
+        got = P.get_volume()
+        P.tv_denoise(theta, g, dt, n)   # buffers reused: p restarts at zero
+        again = P.get_volume()
+    assert same(got, ref)
+    assert same(again, oracle.tv_denoise3d(ref, theta, g, dt, n))
+
+
+def test_tvm_reconstruction_bit_exact(tj, oracle):
+    """The TVM type end to end (AdvancedReconstructionParameters.createTVMParameters
+    -> getProjector -> ReconstructionThreadManager.doRec, i.e.
+    TomoReconstruction2.regularization) against the oracle's SIRT iteration
+    followed by denoise3D with dt = 0.1, with and without the FISTA wrapper, and
+    through the SPI hooks."""
+    from tomoj_amd import parameters as RP
+    from tomoj_amd import reconstruction as R
+    W, H, D = 34, 7, 12
+    tilts = list(np.arange(-60, 61, 10.0))
+    E = oracle.tilt_eulers(tilts)
+    ell = oracle.ellipsoids(41, 5, W, H, D)
+    exps = (oracle.phantom_project(ell, E, W, H) + 0.05 * oracle.gaussian_noise(9, len(tilts) * W * H)
+            .reshape(len(tilts), H, W)).astype(np.float32)
+    ts = RP.TiltSeriesData(exps, tilts)
+    g = oracle.make_geom(W, H, D, nthreads=8)
+    iters, relax, theta, tg, tn = 3, 0.8, 20.0, 1.5, 4
+    for fista in (False, True):
+        print(f"tvm fista={fista}", flush=True)
+        ref = np.zeros((D, H, W), np.float32)
+        ref_err = []
+        t = 0.0
+        for _ in range(iters):
+            prev = ref.copy()
+            ref_err.append(oracle.ossart(g, ref, exps, E, 1, relax, 0, first_index=0)[0])
+            ref[...] = oracle.tv_denoise3d(ref, theta, tg, 0.1, tn)
+            if fista:
+                t = oracle.fista_step(ref, prev, t)
+        ref_err = np.array(ref_err)
+        p = RP.AdvancedReconstructionParameters.createTVMParameters(W, H, D, iters, relax, theta, tg, 0.3, tn)
+        p.setFista(fista)
+        m = RP.ReconstructionThreadManager(ts)
+        try:
+            err = m.doRec(p)
+            got = m.getRec2()
+            assert same(got, ref)
+            assert np.allclose(err[:, [0, 1, 3]], ref_err[:, [0, 1, 3]], rtol=1e-12)
+            assert np.array_equal(err[:, [2, 4]], ref_err[:, [2, 4]])
+            P = m.projector
+            P.reset_volume()
+            if fista:
+                P.setFista(True)
+            R.ossart(P, E, iters, relax, 0, first_index=0)
+            assert same(P.get_volume(), ref)
+        finally:
+            m.close()
+
+
+def test_tvm_setting_off_restores_plain_loop(tj, oracle):
+    """setTvmIteration(0) turns the hook off: the loop is plain SIRT again."""
+    W, H, D = 20, 4, 8
+    tilts = np.arange(-60, 61, 20.0)
+    E = oracle.tilt_eulers(tilts)
+    exps = oracle.phantom_project(oracle.ellipsoids(3, 3, W, H, D), E, W, H).astype(np.float32)
+    with tj.TVMProjector3DHIP((W, H, D)) as P:
+        P.load_tilt_series(exps, E)
+        P.setTvmIteration(0)
+        P.run_ossart(2, 1.0, 0)
+        got = P.get_volume()
+    ref = np.zeros((D, H, W), np.float32)
+    oracle.ossart(oracle.make_geom(W, H, D, nthreads=8), ref, exps, E, 2, 1.0, 0)
+    assert same(got, ref)
+
+
+def test_tv_denoise_nonfinite_flags(tj, oracle):
+    """Denoising spreads a non-finite voxel to its neighbours (other rows y
+    among them); the plane flags the last updateU pass sets must send the
+    next forward projection down the full 4-tap path on exactly those planes,
+    and a finite volume afterwards must clear them again."""
+    W, H, D = 20, 8, 9
+    rng = np.random.default_rng(12)
+    vol = rng.random((D, H, W), dtype=np.float32)
+    vol[4, 3, 7] = np.inf
+    E = oracle.tilt_eulers([-40.0, 0.0, 15.0])
+    ref = oracle.tv_denoise3d(vol, 25.0, 1.0, 0.1, 2)
+    assert not np.isfinite(ref[:, 2, :]).all() and np.isfinite(ref[:, 6, :]).all()
+    fine = rng.random((D, H, W), dtype=np.float32)
+    with make(tj, (W, H, D)) as P:
+        P.load_tilt_series(np.zeros((3, H, W), np.float32), E)
+        for v, r in ((vol, ref), (fine, oracle.tv_denoise3d(fine, 25.0, 1.0, 0.1, 2))):
+            P.set_volume(v)
+            P.tv_denoise(25.0, 1.0, 0.1, 2)
+            assert same(P.get_volume(), r)
+            P.clearAllProjections()
+            for k in range(3):
+                P.addProjection(k)
+            P.project()
+            ref_p, ref_n = oracle.project(ogeom(oracle, P), r, E)
+            for k in range(3):
+                assert same(P.getProjection(k), ref_p[k])
+                assert same(P.getNormImage(k), ref_n[k])

@@ -483,3 +483,114 @@ def test_affine_roi_rectangle(oracle):
         x0, x1 = oracle.roi_rect(w, t)
         assert oracle.roi_rect_affine(w, 20, t, [1, 0, 0, 1, 0, 0]) == [x0, x1, 0, 20]
         assert oracle.roi_rect_affine(w, 20, t, [1, 0, 0, 1, 4, -3]) == [x0 + 4, x1 + 4, -3, 17]
+
+
+# --------------------------------------------------------------------------
+# total-variation denoising (TotalVariationDenoising_3D.denoise3D)
+# --------------------------------------------------------------------------
+
+def _tv_line_f64(vals, theta, g, dt, iterations):
+    """One voxel-thick line worked out in double: the gradient has one
+    non-zero component, so dd1 = 1 + dt/theta/g |du|; the last voxel keeps
+    its value because TomoJ's divergence there is minus the voxel's own p,
+    which stays 0 (its forward difference is 0)."""
+    f = np.array(vals, np.float64)
+    u = f.copy()
+    p = np.zeros_like(f)
+    for _ in range(iterations):
+        du = np.append(np.diff(u), 0.0)
+        p = (p - dt / theta * du) / (1 + dt / theta / g * np.abs(du))
+        div = np.empty_like(p)
+        div[0] = p[0]
+        div[1:-1] = p[1:-1] - p[:-2]
+        div[-1] = -p[-1]
+        u = f - theta * div
+    return u
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_tv_line_closed_form(oracle, axis):
+    """A 3-voxel line along z, y or x: after one step the first voxel moves by
+    dt (b-a)/dd1 towards its neighbour and the last one does not move; after
+    three steps the line agrees with the double-precision working of the same
+    recurrence.  Pins the axis each difference runs along, the sign of the
+    p update, g's place in dd1, f (not u) as the base of the u update, and
+    the boundary rule of TomoJ's divergence."""
+    vals = [1.0, 3.0, 2.5]
+    shape = [1, 1, 1]
+    shape[axis] = 3
+    vol = np.array(vals, np.float32).reshape(shape)
+    theta, g, dt = 25.0, 2.0, 0.1
+    one = oracle.tv_denoise3d(vol, theta, g, dt, 1).ravel()
+    dd1 = 1 + dt / theta / g * 2.0
+    assert one[0] == pytest.approx(1.0 + dt * 2.0 / dd1, rel=1e-6)
+    assert one[2] == np.float32(2.5)
+    three = oracle.tv_denoise3d(vol, theta, g, dt, 3).ravel()
+    assert np.allclose(three, _tv_line_f64(vals, theta, g, dt, 3), rtol=1e-6)
+    assert three[2] == np.float32(2.5)
+
+
+def test_tv_constant_volume_is_fixed(oracle):
+    vol = np.full((5, 4, 6), 0.75, np.float32)
+    assert np.array_equal(oracle.tv_denoise3d(vol, 25, 1, 0.25, 7), vol)
+
+
+def test_tv_slice_constant_volume_reduces_to_2d(oracle):
+    """Equal slices: duz = 0 and p_z = 0 everywhere, so denoise3D must equal the
+    separately structured 2-D routine (denoise2D) on each slice, bit for bit."""
+    rng = np.random.default_rng(5)
+    img = rng.standard_normal((11, 13)).astype(np.float32)
+    vol = np.repeat(img[None], 4, axis=0)
+    for theta, g, dt, n in [(25.0, 1.0, 0.25, 5), (12.0, 3.0, 0.1, 2)]:
+        out = oracle.tv_denoise3d(vol, theta, g, dt, n)
+        ref = oracle.tv_denoise2d(img, theta, g, dt, n)
+        for z in range(4):
+            assert np.array_equal(out[z], ref)
+
+
+def test_tv_random_volume_double_precision(oracle):
+    """A random volume against the same recurrence written with whole-array
+    differences in double precision."""
+    rng = np.random.default_rng(8)
+    f = rng.standard_normal((6, 7, 9)).astype(np.float32)
+    theta, g, dt, n = 20.0, 1.5, 0.2, 3
+    out = oracle.tv_denoise3d(f, theta, g, dt, n)
+    F = f.astype(np.float64)
+    u = F.copy()
+    p = np.zeros((3,) + F.shape)
+    for _ in range(n):
+        d = [np.zeros_like(u) for _ in range(3)]
+        d[0][:, :, :-1] = np.diff(u, axis=2)   # x
+        d[1][:, :-1, :] = np.diff(u, axis=1)   # y
+        d[2][:-1, :, :] = np.diff(u, axis=0)   # z
+        dd1 = 1 + dt / theta / g * np.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2)
+        p = (p - dt / theta * np.stack(d)) / dd1
+        div = np.zeros_like(u)
+        for c, ax in ((0, 2), (1, 1), (2, 0)):
+            q = np.moveaxis(p[c], ax, 0)
+            dv = np.empty_like(q)
+            dv[0] = q[0]
+            dv[1:-1] = q[1:-1] - q[:-2]
+            dv[-1] = -q[-1]
+            div += np.moveaxis(dv, 0, ax)
+        u = F - theta * div
+    assert np.abs(out - u).max() <= 1e-5 * np.abs(u).max()
+
+
+def test_tv_lowers_total_variation(oracle):
+    """Denoising a noisy smooth volume with TVMProjector3D's effective step
+    (dt = regularizationLambda = 0.1) lowers its total variation and brings it
+    closer to the clean volume (a sign error in the p update raises the
+    variation)."""
+    rng = np.random.default_rng(2)
+    z, y, x = np.mgrid[0:12, 0:14, 0:16]
+    clean = np.sin(x / 4.0) + np.cos(y / 5.0) + 0.1 * z
+    noisy = (clean + 0.3 * rng.standard_normal(clean.shape)).astype(np.float32)
+
+    def tv(v):
+        v = v.astype(np.float64)
+        return sum(np.abs(np.diff(v, axis=a)).sum() for a in range(3))
+
+    out = oracle.tv_denoise3d(noisy, 25.0, 1.0, 0.1, 5)
+    assert tv(out) < 0.5 * tv(noisy)
+    assert np.abs(out - clean).mean() < np.abs(noisy - clean).mean()

@@ -35,6 +35,7 @@ hipError_t phantom_project(hipStream_t, const double*, int, const TjProj*, int, 
 hipError_t add_noise(hipStream_t, const double*, size_t, uint64_t, double, double*, int, float*);
 hipError_t copy(hipStream_t, const void*, void*, size_t);
 hipError_t fista(hipStream_t, float*, const float*, size_t, double);
+hipError_t tv_denoise(hipStream_t, float*, float*, float*, int, int, int, float, float, float, int, uint8_t*);
 hipError_t normalize(hipStream_t, float*, int, int, int, const int*, double*);
 hipError_t roi_stats(hipStream_t, const float*, int, int, int, const int*, double*);
 hipError_t affine(hipStream_t, const float*, float*, int, int, int, const double*, const int*, const double*, int,
@@ -135,6 +136,12 @@ struct tj_ctx {
     double fista_t = 0.0;
     float* prev = nullptr;      // [vh][vd][vw]
 
+    // TVMProjector3D end-of-iteration denoising (TVMProjector3D.java:26-32)
+    int tvm_iter = 0;           // tvmIteration; 0 = hook off
+    float tvm_theta = 25.0f, tvm_g = 1.0f, tvm_dt = 0.1f;
+    float* tv_f = nullptr;      // [vh][vd][vw] the volume entering the denoising
+    float* tv_p = nullptr;      // [3][vh][vd][vw] px, py, pz
+
     // FP64 pair images of the backprojected images (LDS backprojection)
     double2* bpairs = nullptr;  // [bpairs_cap][ph][pw]
     int bpairs_cap = 0;
@@ -156,6 +163,8 @@ static void ctx_free(tj_ctx* c)
 {
     dfree(c->bpairs);
     dfree(c->prev);
+    dfree(c->tv_f);
+    dfree(c->tv_p);
     dfree(c->vol);
     dfree(c->pflag);
     dfree(c->norm_tab);
@@ -1108,9 +1117,22 @@ static int fista_start(tj_ctx* c)
     return TJ_OK;
 }
 
+// Non-finite flags of every volume plane, after a whole-volume update that may
+// overflow (FISTA extrapolation, TV denoising); `computed`: the update's own
+// kernel has set them already.
+static int refresh_plane_flags(tj_ctx* c, bool computed = false)
+{
+    if (!computed) TJ_HIP(tjk::plane_flags(c->stream, c->vol, c->plane, 0, c->g.vh, c->pflag));
+    std::vector<uint8_t> hf(c->g.vh + 1);
+    TJ_HIP(hipMemcpyAsync(hf.data(), c->pflag, c->g.vh + 1, hipMemcpyDeviceToHost, c->stream));
+    TJ_HIP(hipStreamSynchronize(c->stream));
+    c->any_nonfinite = false;
+    for (uint8_t f : hf) c->any_nonfinite |= (f != 0);
+    return TJ_OK;
+}
+
 // FistaProjector3D.endOfIteration (:43-59): t and the extrapolation factor on
-// the host in double, the voxel update on the device; the plane flags are
-// refreshed because an extrapolated voxel may overflow.
+// the host in double, the voxel update on the device.
 static int fista_end(tj_ctx* c)
 {
     if (!c->fista) return TJ_OK;
@@ -1118,13 +1140,75 @@ static int fista_end(tj_ctx* c)
     c->fista_t = (1 + std::sqrt(1 + 4 * t_1 * t_1)) / 2;
     const double factor = (t_1 - 1) / c->fista_t;
     TJ_HIP(tjk::fista(c->stream, c->vol, c->prev, c->nvox, factor));
-    TJ_HIP(tjk::plane_flags(c->stream, c->vol, c->plane, 0, c->g.vh, c->pflag));
-    std::vector<uint8_t> hf(c->g.vh + 1);
-    TJ_HIP(hipMemcpyAsync(hf.data(), c->pflag, c->g.vh + 1, hipMemcpyDeviceToHost, c->stream));
-    TJ_HIP(hipStreamSynchronize(c->stream));
-    c->any_nonfinite = false;
-    for (uint8_t f : hf) c->any_nonfinite |= (f != 0);
+    return refresh_plane_flags(c);
+}
+
+static int ensure_tv(tj_ctx* c)
+{
+    if (c->tv_f && c->tv_p) return TJ_OK;
+    if (!c->tv_f && dalloc(&c->tv_f, c->nvox) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(TJ_ENOMEM, "TV denoising: cannot allocate the input copy (%zu voxels)", c->nvox);
+    }
+    if (!c->tv_p && dalloc(&c->tv_p, 3 * c->nvox) != hipSuccess) {
+        (void)hipGetLastError();
+        dfree(c->tv_f);
+        return fail(TJ_ENOMEM, "TV denoising: cannot allocate the dual field (%zu voxels x 3)", c->nvox);
+    }
     return TJ_OK;
+}
+
+// TotalVariationDenoising_3D.denoise3D over the whole volume (every row, as
+// TVMProjector3D passes rec's whole stack).
+static int tv_run(tj_ctx* c, float theta, float g, float dt, int iterations)
+{
+    if (iterations <= 0) return TJ_OK;
+    int rc = ensure_tv(c);
+    if (rc) return rc;
+    TJ_HIP(tjk::tv_denoise(c->stream, c->vol, c->tv_f, c->tv_p, c->g.vw, c->g.vh, c->g.vd, theta, g, dt,
+                           iterations, c->pflag));
+    return refresh_plane_flags(c, true);
+}
+
+// TVMProjector3D.endOfIteration (:135-157): dt is reset to the regularisation
+// lambda before every denoising, so the tvmDt setting never reaches it.
+static int tvm_end(tj_ctx* c)
+{
+    if (c->tvm_iter <= 0) return TJ_OK;
+    return tv_run(c, c->tvm_theta, c->tvm_g, c->tvm_dt, c->tvm_iter);
+}
+
+// With both set, FistaProjector3D wraps the TVM projector: its endOfIteration
+// runs the wrapped projector's first (FistaProjector3D.java:44).
+static int end_hooks(tj_ctx* c)
+{
+    int rc = tvm_end(c);
+    if (rc) return rc;
+    return fista_end(c);
+}
+
+int tj_set_tvm(tj_ctx* c, int iterations, double theta, double g, double lambda)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_set_tvm: null context");
+    if (iterations < 0) return fail(TJ_EINVAL, "tj_set_tvm: negative iteration count %d", iterations);
+    TJ_HIP(hipSetDevice(c->device));
+    if (iterations > 0) {
+        int rc = ensure_tv(c);
+        if (rc) return rc;
+    }
+    c->tvm_iter = iterations;
+    c->tvm_theta = (float)theta;  // setTvmTheta / setTvmG cast to float (:64-74)
+    c->tvm_g = (float)g;
+    c->tvm_dt = (float)lambda;    // dt = (float) regularizationLambda (:152)
+    return TJ_OK;
+}
+
+int tj_tv_denoise(tj_ctx* c, float theta, float g, float dt, int iterations)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_tv_denoise: null context");
+    if (iterations < 0) return fail(TJ_EINVAL, "tj_tv_denoise: negative iteration count %d", iterations);
+    TJ_HIP(hipSetDevice(c->device));
+    return tv_run(c, theta, g, dt, iterations);
 }
 
 int tj_set_fista(tj_ctx* c, int on)
@@ -1156,7 +1240,7 @@ int tj_end_iteration(tj_ctx* c)
 {
     if (!c) return fail(TJ_EINVAL, "null context");
     TJ_HIP(hipSetDevice(c->device));
-    return fista_end(c);
+    return end_hooks(c);
 }
 
 double tj_get_completion(tj_ctx* c) { return c ? c->completion.load() : 0.0; }
@@ -1291,7 +1375,7 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
             e5[3] = std::sqrt(errmax / wh);
             e5[4] = imax;
         }
-        if ((rc = fista_end(c))) return rc;
+        if ((rc = end_hooks(c))) return rc;
         c->completion.store(c->completion.load() + completion_inc);
     }
     if (c->timing_on) {

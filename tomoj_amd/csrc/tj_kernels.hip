@@ -1078,6 +1078,101 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_fista(float* __restrict__ vol, con
 }
 
 // ---------------------------------------------------------------------------
+// Total-variation denoising, TotalVariationDenoising_3D.denoise3D
+// (Filters/.../TotalVariationDenoising_3D.java:84-202), on the device layout
+// [y][z][x] with p held as three planes px, py, pz of the volume's size.  A
+// block owns 256 columns x of one row y over a run of TJ_TV_ZR slices and
+// marches z, carrying the next slice's u (updateP) or the previous slice's pz
+// (updateU) in a register.  Blocks are ordered so that each XCD sweeps the rows
+// y of its share in turn: the neighbouring row a block reads (y+1 in updateP,
+// y-1 in updateU) is the row the XCD's previous blocks read, still in its L2.
+// Float arithmetic in Java's order; this file is built without contraction and
+// with correctly rounded division and square root.
+// ---------------------------------------------------------------------------
+#define TJ_TV_ZR 64
+
+struct TvTile {
+    int x, y, z0, z1;
+};
+
+__device__ inline bool tv_tile(int W, int H, int D, int nreal, TvTile& t)
+{
+    const int nb = gridDim.x;  // a multiple of 8
+    const int b = blockIdx.x;
+    const int k = (b & 7) * (nb >> 3) + (b >> 3);
+    if (k >= nreal) return false;
+    const int nxt = (W + TJ_BLOCK - 1) / TJ_BLOCK;
+    const int r = k / nxt;
+    t.x = (k - r * nxt) * TJ_BLOCK + (int)threadIdx.x;
+    t.y = r % H;
+    t.z0 = (r / H) * TJ_TV_ZR;
+    t.z1 = min(t.z0 + TJ_TV_ZR, D);
+    return t.x < W;
+}
+
+// updateP (:115-156): a = dt/theta, b = dt/theta/g, both rounded to float as
+// Java forms them.
+__global__ __launch_bounds__(TJ_BLOCK) void k_tv_p(const float* __restrict__ u, float* __restrict__ px,
+                                                   float* __restrict__ py, float* __restrict__ pz, int W, int H,
+                                                   int D, int nreal, float a, float b)
+{
+    TvTile t;
+    if (!tv_tile(W, H, D, nreal, t)) return;
+    const size_t plane = (size_t)D * W;
+    size_t k = ((size_t)t.y * D + t.z0) * W + t.x;
+    const bool hx = t.x < W - 1, hy = t.y < H - 1;
+    float uc = u[k];
+#pragma unroll 4
+    for (int z = t.z0; z < t.z1; ++z, k += W) {
+        const bool hz = z < D - 1;
+        const float un = hz ? u[k + W] : 0.0f;
+        const float duy = hy ? u[k + plane] - uc : 0.0f;
+        const float dux = hx ? u[k + 1] - uc : 0.0f;
+        const float duz = hz ? un - uc : 0.0f;
+        const float dd1 = 1.0f + b * fabsf(sqrtf(duy * duy + dux * dux + duz * duz));
+        px[k] = (px[k] - a * dux) / dd1;
+        py[k] = (py[k] - a * duy) / dd1;
+        pz[k] = (pz[k] - a * duz) / dd1;
+        uc = un;
+    }
+}
+
+// updateU (:158-202): TomoJ's divergence takes the voxel's own component with
+// a minus sign on the last column, row and slice.  With pflag, a non-finite
+// result sets its row's plane flag (the caller clears the flags first).
+__global__ __launch_bounds__(TJ_BLOCK) void k_tv_u(const float* __restrict__ f, float* __restrict__ u,
+                                                   const float* __restrict__ px, const float* __restrict__ py,
+                                                   const float* __restrict__ pz, int W, int H, int D, int nreal,
+                                                   float theta, uint8_t* __restrict__ pflag)
+{
+    TvTile t;
+    if (!tv_tile(W, H, D, nreal, t)) return;
+    const size_t plane = (size_t)D * W;
+    size_t k = ((size_t)t.y * D + t.z0) * W + t.x;
+    float pzp = t.z0 > 0 ? pz[k - W] : 0.0f;
+    bool bad = false;
+#pragma unroll 4
+    for (int z = t.z0; z < t.z1; ++z, k += W) {
+        const float pxc = px[k], pyc = py[k], pzc = pz[k];
+        float div;
+        if (t.x == 0) div = pxc;
+        else if (t.x == W - 1) div = -pxc;
+        else div = pxc - px[k - 1];
+        if (t.y == 0) div = div + pyc;
+        else if (t.y == H - 1) div = div + (-pyc);
+        else div = div + (pyc - py[k - plane]);
+        if (z == 0) div = div + pzc;
+        else if (z == D - 1) div = div + (-pzc);
+        else div = div + (pzc - pzp);
+        const float un = f[k] - theta * div;
+        u[k] = un;
+        bad |= !isfinite(un);
+        pzp = pzc;
+    }
+    if (pflag && bad) pflag[t.y] = 1;
+}
+
+// ---------------------------------------------------------------------------
 // Volume layout conversion between the host ImageStack layout [z][y][x] and the
 // device layout [y][z][x], for rows [y0, y0+nrows).
 // ---------------------------------------------------------------------------
@@ -1099,17 +1194,22 @@ __global__ void k_yzx_to_zyx(const float* __restrict__ vol, float* __restrict__ 
     dst[((size_t)z * nrows + yy) * W + x] = vol[((size_t)(y0 + yy) * D + z) * W + x];
 }
 
-// Non-finite flag per volume plane y.
+// Non-finite flag per volume plane y: blockIdx.y picks the plane, blockIdx.x
+// one of gridDim.x slices of it; a slice holding a non-finite value sets the
+// flag (cleared beforehand).
 __global__ __launch_bounds__(TJ_BLOCK) void k_plane_flags(const float* __restrict__ vol, size_t plane,
                                                           int y0, uint8_t* __restrict__ pflag)
 {
     __shared__ unsigned sh[TJ_BLOCK];
-    const int y = y0 + blockIdx.x;
+    const int y = y0 + blockIdx.y;
     const float* p = vol + (size_t)y * plane;
+    const size_t per = (plane + gridDim.x - 1) / gridDim.x;
+    const size_t k0 = per * blockIdx.x, k1 = min(plane, k0 + per);
     unsigned bad = 0;
-    for (size_t k = threadIdx.x; k < plane; k += blockDim.x) bad |= isfinite(p[k]) ? 0u : 1u;
+#pragma unroll 4
+    for (size_t k = k0 + threadIdx.x; k < k1; k += blockDim.x) bad |= isfinite(p[k]) ? 0u : 1u;
     bad = block_reduce(bad, sh, OrU());
-    if (threadIdx.x == 0) pflag[y] = (uint8_t)bad;
+    if (threadIdx.x == 0 && bad) pflag[y] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -1417,6 +1517,32 @@ hipError_t fista(hipStream_t st, float* vol, const float* prev, size_t n, double
     return hipGetLastError();
 }
 
+// denoise3D (:84-113): f keeps the input, u (the volume) starts equal to it,
+// p starts at zero.
+hipError_t tv_denoise(hipStream_t st, float* vol, float* f, float* p, int W, int H, int D, float theta, float g,
+                      float dt, int iterations, uint8_t* pflag)
+{
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    const size_t n = (size_t)W * H * D;
+    if (n == 0 || iterations <= 0) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(f, vol, n * sizeof(float), hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemsetAsync(p, 0, 3 * n * sizeof(float), st);
+    if (e != hipSuccess) return e;
+    const float a = dt / theta;
+    const float b = dt / theta / g;
+    const int nxt = (W + TJ_BLOCK - 1) / TJ_BLOCK;
+    const long nreal = (long)nxt * H * ((D + TJ_TV_ZR - 1) / TJ_TV_ZR);
+    if (nreal > 0x7ffffff0L) return hipErrorInvalidValue;
+    const int nb = (int)((nreal + 7) & ~7L);
+    if (pflag && (e = hipMemsetAsync(pflag, 0, H, st)) != hipSuccess) return e;
+    for (int it = 0; it < iterations; ++it) {
+        k_tv_p<<<nb, TJ_BLOCK, 0, st>>>(vol, p, p + n, p + 2 * n, W, H, D, (int)nreal, a, b);
+        k_tv_u<<<nb, TJ_BLOCK, 0, st>>>(f, vol, p, p + n, p + 2 * n, W, H, D, (int)nreal, theta,
+                                        it == iterations - 1 ? pflag : nullptr);
+    }
+    return hipGetLastError();
+}
+
 hipError_t copy(hipStream_t st, const void* src, void* dst, size_t bytes)
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
@@ -1444,7 +1570,10 @@ hipError_t plane_flags(hipStream_t st, const float* vol, size_t plane, int y0, i
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nrows <= 0) return hipSuccess;
-    k_plane_flags<<<nrows, TJ_BLOCK, 0, st>>>(vol, plane, y0, pflag);
+    hipError_t e = hipMemsetAsync(pflag + y0, 0, nrows, st);
+    if (e != hipSuccess) return e;
+    const unsigned split = (unsigned)std::min<size_t>(16, (plane + 4095) / 4096);
+    k_plane_flags<<<dim3(split, nrows), TJ_BLOCK, 0, st>>>(vol, plane, y0, pflag);
     return hipGetLastError();
 }
 

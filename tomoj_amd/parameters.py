@@ -19,12 +19,13 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from .projector import VoxelProjector3DHIP
+from .projector import TVMProjector3DHIP, VoxelProjector3DHIP
 from .reconstruction import tilt_eulers
 
 BP = 0
 WBP = 1
 OSSART = 2
+TVM = 3        # AdvancedReconstructionParameters.java:10
 
 
 @dataclass
@@ -129,12 +130,14 @@ class ReconstructionParameters:
         """:313-333: a voxel projector with the weighting filter for WBP, the
         positivity constraint for OS-SART, the scale ts.W / rec.W when
         rescaleData, long-object compensation, and the FISTA wrapper."""
+        return self._projector(VoxelProjector3DHIP, ts, device,
+                               self.reconstructionType == OSSART and self.positivityConstraint)
+
+    def _projector(self, cls, ts: TiltSeriesData, device: int, positivity: bool):
         scale = ts.width / float(self.width) if self.rescaleData else 1.0
-        P = VoxelProjector3DHIP((self.width, self.height, self.depth), (ts.width, ts.height), device=device,
-                                scale=scale,
-                                positivity=(self.reconstructionType == OSSART and self.positivityConstraint),
-                                long_object_compensation=self.longObjectCompensation,
-                                vol_center=self.volume_centers())
+        P = cls((self.width, self.height, self.depth), (ts.width, ts.height), device=device,
+                scale=scale, positivity=positivity, long_object_compensation=self.longObjectCompensation,
+                vol_center=self.volume_centers())
         P.load_tilt_series(ts.images, ts.eulers())
         if ts.transforms is not None:
             # getPixels(i) through the affine alignment (AffineAlignment.applyTransformOnImage)
@@ -147,6 +150,59 @@ class ReconstructionParameters:
             P.setWeightingFilter(self.weightingRadius * ts.width, ts.tilts, ts.tilt_axes())
         if self.fista:
             P.setFista(True)
+        return P
+
+
+class AdvancedReconstructionParameters(ReconstructionParameters):
+    """AdvancedReconstructionParameters (TomoJ_AdvancedReconstruction/.../tomography/
+    AdvancedReconstructionParameters.java:8-283) for the TVM type: the TV settings
+    and their defaults (:24-27), the two createTVMParameters factories (:112-133)
+    and getProjector's TVM branch (:179-193)."""
+
+    def __init__(self, width: int, height: int, depth: int):
+        super().__init__(width, height, depth)
+        self.regularizationWeight = 0.0
+        self.tvmTheta = 25.0
+        self.tvmG = 1.0
+        self.tvmDt = 0.1
+        self.tvmIterations = 5
+
+    @staticmethod
+    def createTVMParameters(width, height, depth, nbIterations, *args) -> "AdvancedReconstructionParameters":
+        """(w, h, d, nbIterations, regularizationWeight) sets the relaxation to 1
+        (:112-120); (w, h, d, nbIterations, relaxationCoefficient, tvmTheta, tvmG,
+        tvmDt, tvmIterations) sets the TV settings (:122-133).  Both: update 0
+        (every projection per subset, SIRT), no weighting."""
+        p = AdvancedReconstructionParameters(width, height, depth)
+        p.reconstructionType = TVM
+        p.nbIterations = int(nbIterations)
+        p.weightingRadius = float("nan")
+        p.updateNb = 0
+        if len(args) == 1:
+            p.relaxationCoefficient = 1.0
+            p.regularizationWeight = float(args[0])
+        elif len(args) == 5:
+            p.relaxationCoefficient = float(args[0])
+            p.tvmTheta, p.tvmG, p.tvmDt = (float(v) for v in args[1:4])
+            p.tvmIterations = int(args[4])
+        else:
+            raise TypeError("createTVMParameters takes 5 or 9 arguments")
+        return p
+
+    def getProjector(self, ts: TiltSeriesData, device: int = 0):
+        """The TVM branch of getProjector (:179-193): TVMProjector3D with the
+        relaxation as regularisation alpha, the positivity and long-object
+        settings, the TV settings, the scale when rescaleData, and the FISTA
+        wrapper; TomoJ leaves its regularisation lambda at 0.1 (the call that
+        would set it is commented out, :182).  Other types go to the base class."""
+        if self.reconstructionType != TVM:
+            return super().getProjector(ts, device)
+        P = self._projector(TVMProjector3DHIP, ts, device, self.positivityConstraint)
+        P.setRegularizationAlpha(self.relaxationCoefficient)
+        P.setTvmTheta(self.tvmTheta)
+        P.setTvmG(self.tvmG)
+        P.setTvmDt(self.tvmDt)
+        P.setTvmIteration(self.tvmIterations)
         return P
 
 
@@ -179,7 +235,9 @@ class ReconstructionThreadManager:
         if params.reconstructionType in (BP, WBP):
             P.run_wbp()
             return None
-        if params.reconstructionType == OSSART:
+        if params.reconstructionType in (OSSART, TVM):
+            # TVM goes through TomoReconstruction2.regularization (:783-878), the
+            # OS-SART loop with the projector's own iteration hooks.
             return P.run_ossart(params.nbIterations, params.relaxationCoefficient, params.updateNb,
                                 first_index=first_index)
         raise ValueError(f"reconstruction type {params.reconstructionType} is not on this path")

@@ -207,6 +207,18 @@ class VoxelProjector3DHIP:
         the iteration hooks (and the fused loop) apply FISTA's extrapolation."""
         call("tj_set_fista", self._ctx, int(bool(on)))
 
+    def setTvm(self, iterations: int, theta: float = 25.0, g: float = 1.0, lam: float = 0.1) -> None:
+        """Turns the projector into TVMProjector3D (TVMProjector3D.java:24-157):
+        every endOfIteration (and every iteration of the fused loop) ends with
+        TotalVariationDenoising_3D.denoise3D(volume, theta, g, (float) lam,
+        iterations).  iterations = 0 turns it off."""
+        call("tj_set_tvm", self._ctx, int(iterations), float(theta), float(g), float(lam))
+
+    def tv_denoise(self, theta: float, g: float, dt: float, iterations: int) -> None:
+        """TotalVariationDenoising_3D.denoise3D (TotalVariationDenoising_3D.java:84-202)
+        of the resident volume, once."""
+        call("tj_tv_denoise", self._ctx, float(theta), float(g), float(dt), int(iterations))
+
     def getCompletion(self) -> float:
         return _lib.load().tj_get_completion(self._ctx)
 
@@ -270,6 +282,50 @@ class VoxelProjector3DHIP:
         t = tj_timing()
         call("tj_get_timing", self._ctx, C.byref(t))
         return {k: getattr(t, k) for k, _ in tj_timing._fields_}
+
+
+class TVMProjector3DHIP(VoxelProjector3DHIP):
+    """TVMProjector3D (TomoJ_AdvancedReconstruction/.../projectors/TVMProjector3D.java:24-157)
+    on the GPU: the voxel projector whose endOfIteration denoises the volume by
+    total variation.  The setters keep TomoJ's names and defaults (:26-32);
+    as in TomoJ, endOfIteration sets dt to the regularisation lambda (:152), so
+    setTvmDt has no effect on the result, and regularisationAlpha is kept but
+    not used."""
+
+    def __init__(self, *args, **kw):
+        super().__init__(*args, **kw)
+        self.regularizationLambda = 0.1
+        self.regularizationAlpha = 0.1
+        self.theta = 25.0
+        self.dt = 0.05
+        self.g = 1.0
+        self.tvmIteration = 5
+        self._push()
+
+    def _push(self) -> None:
+        self.setTvm(self.tvmIteration, self.theta, self.g, self.regularizationLambda)
+
+    def setRegularizationLambda(self, v: float) -> None:
+        self.regularizationLambda = float(v)
+        self._push()
+
+    def setRegularizationAlpha(self, v: float) -> None:
+        self.regularizationAlpha = float(v)
+
+    def setTvmTheta(self, v: float) -> None:
+        self.theta = float(np.float32(v))
+        self._push()
+
+    def setTvmDt(self, v: float) -> None:
+        self.dt = float(np.float32(v))
+
+    def setTvmG(self, v: float) -> None:
+        self.g = float(np.float32(v))
+        self._push()
+
+    def setTvmIteration(self, n: int) -> None:
+        self.tvmIteration = int(n)
+        self._push()
 
 
 def phantom_ellipsoids(seed: int, k: int, W: int, H: int, D: int) -> np.ndarray:
