@@ -200,6 +200,24 @@ int tj_set_fista(tj_ctx *ctx, int on);
  * (:64-74).  Holds 16 bytes per voxel of device memory while on. */
 int tj_set_tvm(tj_ctx *ctx, int iterations, double theta, double g, double lambda);
 
+/* Bayesian regularisation: the projector getProjector builds for the BAYESIAN
+ * type (AdvancedReconstructionParameters.java:194-203) is
+ * BayesianVoxelProjector3D (TomoJ_AdvancedReconstruction/.../projectors/
+ * BayesianVoxelProjector3D.java:25-204).  With `on`: tj_start_iteration (and
+ * each iteration of tj_run_ossart) sets a device distance volume to
+ * -lambda x the volume convolved with the [1,-2,1]^3 stencil, zero outside
+ * (startOfIteration / convolve, :134-198); every backprojection of
+ * differences adds into the distance volume instead of the reconstruction,
+ * and a backprojection of the original images (no differences, tj_run_wbp)
+ * adds nothing (:75-110); tj_end_iteration multiplies the distance volume by
+ * (float) alpha, adds it to the volume and applies the positivity constraint
+ * when the geometry has it (:121-132).  alpha = regularizationAlpha (the
+ * relaxation coefficient), lambda = regularizationWeight.  TomoJ never wraps
+ * this projector in FISTA; the hooks still compose if asked.  Refused while
+ * the TVM hook is on (and tj_set_tvm is refused while this one is).  Holds 4
+ * bytes per voxel of device memory while on. */
+int tj_set_bayesian(tj_ctx *ctx, int on, double alpha, double lambda);
+
 /* TotalVariationDenoising_3D.denoise3D(stack, theta, g, dt, iterations)
  * (TotalVariationDenoising_3D.java:84-202) of the resident volume, once: the
  * plugin's run (:66-73) with the volume as the stack. */

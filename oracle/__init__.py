@@ -14,6 +14,7 @@ layout ``[z][y][x]`` (ImageStack: one slice per z, x fastest); projections are
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -110,6 +111,8 @@ def _L():
             "or_rect_stats": (None, [P, i, i, P, P, P]),
             "or_tv_denoise3d": (None, [P, i, i, i, C.c_float, C.c_float, C.c_float, i]),
             "or_tv_denoise2d": (None, [P, i, i, C.c_float, C.c_float, C.c_float, i]),
+            "or_bayes_convolve": (None, [P, P, i, i, i, d]),
+            "or_bayes_end": (None, [P, P, l, d, i]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
@@ -407,6 +410,44 @@ def tv_denoise2d(img, theta, g, dt, iterations) -> np.ndarray:
     _L().or_tv_denoise2d(_p(out), w, h, float(np.float32(theta)), float(np.float32(g)),
                          float(np.float32(dt)), int(iterations))
     return out
+
+
+def bayes_convolve(vol, factor) -> np.ndarray:
+    """BayesianVoxelProjector3D.convolve(rec, kernel, dist, factor) (:175-198) of a
+    (D, H, W) float32 volume; returns the distance volume."""
+    vol = _c(vol, np.float32)
+    D, H, W = vol.shape
+    dist = np.empty_like(vol)
+    _L().or_bayes_convolve(_p(vol), _p(dist), W, H, D, float(factor))
+    return dist
+
+
+def bayes_iteration(g: Geom, vol, projs, eulers, alpha, lam, first_index=0):
+    """One iteration of TomoReconstruction2.regularization (:783-878) with
+    BayesianVoxelProjector3D and update = N (the Bayesian factory's): the
+    distance volume is -lam times the convolved volume, every projection's
+    difference (factor (float)(alpha/N)) is backprojected into it, then
+    vol += (float)alpha * dist and the positivity step of ``g``.  ``vol`` is
+    updated in place; returns the err row of osartIteration."""
+    assert vol.dtype == np.float32 and vol.flags["C_CONTIGUOUS"]
+    eulers = _c(eulers, np.float64).reshape(-1, 9)
+    n = eulers.shape[0]
+    order = order_shuffle(eulers, first_index)
+    dist = bayes_convolve(vol, -float(lam))
+    factor = float(np.float32(alpha / n))
+    diffs, errs, _, _ = project_and_difference(g, vol, _c(projs, np.float32)[order], eulers[order], factor)
+    backproject(g, dist, diffs, eulers[order])
+    _L().or_bayes_end(_p(vol), _p(dist), vol.size, float(alpha), int(g.positivity))
+    wh = float(g.vw * g.vh)
+    som, emin, emax, imin, imax = 0.0, 3.4028234663852886e38, 0.0, 0, 0
+    for t in range(n):
+        e = errs[t]
+        som += e
+        if e > emax:
+            emax, imax = e, int(order[t])
+        elif e < emin:
+            emin, imin = e, int(order[t])
+    return [math.sqrt(som / (n * wh)), math.sqrt(emin / wh), imin, math.sqrt(emax / wh), imax]
 
 
 # --------------------------------------------------------------------------

@@ -1311,3 +1311,62 @@ void or_tv_denoise2d(float *img, int w, int h, float theta, float g, float dt, i
     free(du);
     free(divp);
 }
+
+/* ---------------------------------------------------------------------------
+ * Bayesian regularisation, BayesianVoxelProjector3D (src/TomoJ_AdvancedReconstruction/
+ * src/main/java/fr/curie/tomoj/tomography/projectors/BayesianVoxelProjector3D.java).
+ * Volumes in ImageStack layout [z][y][x].
+ *
+ * or_bayes_convolve: startOfIteration (:134-137) -> convolve(rec, kernel,
+ * distanceVolume, -lambda) (:175-198).  For each voxel the 27 values of
+ * TomoReconstruction2.getCubicNeightborhoodWithClamp(x, y, z, 1) (TomoReconstruction2.java:317-339;
+ * despite its name it reads getPixel(x+i, y+j, z+k), 0 outside the volume, :241-246),
+ * i over x outermost and k over z innermost, each times its kernel entry (:31-33,
+ * the separable [1,-2,1] stencil) in float, summed in double in that order;
+ * dist = (float)(total * factor) (putPixel :351-353).
+ * ------------------------------------------------------------------------- */
+static const float or_bayes_kernel[27] = {1, -2, 1, -2, 4, -2, 1, -2, 1,
+                                          -2, 4, -2, 4, -8, 4, -2, 4, -2,
+                                          1, -2, 1, -2, 4, -2, 1, -2, 1};
+
+static float or_vox(const float *v, int W, int H, int D, int x, int y, int z)
+{
+    if (x < 0 || x >= W || y < 0 || y >= H || z < 0 || z >= D) return 0.0f;
+    return v[((long)z * H + y) * W + x];
+}
+
+void or_bayes_convolve(const float *vol, float *dist, int W, int H, int D, double factor)
+{
+    for (int z = 0; z < D; ++z)
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                double total = 0;
+                int idx = 0;
+                for (int i = -1; i <= 1; ++i)
+                    for (int j = -1; j <= 1; ++j)
+                        for (int k = -1; k <= 1; ++k) {
+                            const float prod = or_vox(vol, W, H, D, x + i, y + j, z + k) * or_bayes_kernel[idx++];
+                            total += prod;
+                        }
+                dist[((long)z * H + y) * W + x] = (float)(total * factor);
+            }
+}
+
+/* or_bayes_end: endOfIteration (:121-132): mult(distanceVolume, alpha) (:159-173,
+ * ImageJ FloatProcessor.multiply: the factor rounded to float, then a float
+ * product), rec.operation(distanceVolume, ADD) (TomoReconstruction2.java:386-403,
+ * FloatBlitter ADD: a float sum), then with positivity
+ * rec.applyPositivityConstraint() (:405-412, FloatBlitter MAX against 0: a value
+ * is replaced only where 0 > value, so NaN and -0 stay).  ImageJ 1.53a's
+ * FloatProcessor / FloatBlitter are third party, restated from their published
+ * source. */
+void or_bayes_end(float *vol, float *dist, long n, double alpha, int positivity)
+{
+    const float c = (float)alpha;
+    for (long i = 0; i < n; ++i) {
+        dist[i] = dist[i] * c;
+        float v = dist[i] + vol[i];
+        if (positivity && 0.0f > v) v = 0.0f;
+        vol[i] = v;
+    }
+}

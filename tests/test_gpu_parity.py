@@ -667,3 +667,58 @@ def test_tv_denoise_nonfinite_flags(tj, oracle):
             for k in range(3):
                 assert same(P.getProjection(k), ref_p[k])
                 assert same(P.getNormImage(k), ref_n[k])
+
+
+# --------------------------------------------------------------------------
+# Bayesian projector (BayesianVoxelProjector3D)
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("positivity", [False, True])
+def test_bayesian_reconstruction_bit_exact(tj, oracle, positivity):
+    """The BAYESIAN type end to end (createBayesianParameters -> getProjector ->
+    doRec, i.e. TomoReconstruction2.regularization) against the oracle's
+    iteration, and the same through the SPI hooks; 70 slices so the
+    convolution's slice runs split."""
+    from tomoj_amd import parameters as RP
+    from tomoj_amd import reconstruction as R
+    W, H, D = 30, 6, 70
+    tilts = list(np.arange(-60, 61, 12.0))
+    E = oracle.tilt_eulers(tilts)
+    ell = oracle.ellipsoids(43, 5, W, H, D)
+    exps = (oracle.phantom_project(ell, E, W, H) + 0.05 * oracle.gaussian_noise(4, len(tilts) * W * H)
+            .reshape(len(tilts), H, W)).astype(np.float32)
+    ts = RP.TiltSeriesData(exps, tilts)
+    g = oracle.make_geom(W, H, D, positivity=positivity, nthreads=8)
+    iters, alpha, weight = 3, 0.9, 0.02
+    print(f"bayesian positivity={positivity}", flush=True)
+    ref = np.zeros((D, H, W), np.float32)
+    ref_err = np.array([oracle.bayes_iteration(g, ref, exps, E, alpha, weight) for _ in range(iters)])
+    p = RP.AdvancedReconstructionParameters.createBayesianParameters(W, H, D, iters, alpha, weight)
+    p.positivityConstraint = positivity
+    p.setFista(True)   # ignored: TomoJ never wraps this projector
+    m = RP.ReconstructionThreadManager(ts)
+    try:
+        err = m.doRec(p)
+        assert same(m.getRec2(), ref)
+        assert np.allclose(err[:, [0, 1, 3]], ref_err[:, [0, 1, 3]], rtol=1e-12)
+        assert np.array_equal(err[:, [2, 4]], ref_err[:, [2, 4]])
+        P = m.projector
+        P.reset_volume()
+        R.ossart(P, E, iters, alpha, 0, first_index=0)
+        assert same(P.get_volume(), ref)
+        before = P.get_volume()
+        P.run_wbp()   # no differences: the Bayesian backprojection adds nothing
+        assert same(P.get_volume(), before)
+    finally:
+        m.close()
+
+
+def test_bayesian_and_tvm_hooks_exclusive(tj):
+    with make(tj, (8, 2, 4)) as P:
+        P.setTvm(2)
+        with pytest.raises(tj.TjError):
+            P.setBayesian(True)
+        P.setTvm(0)
+        P.setBayesian(True)
+        with pytest.raises(tj.TjError):
+            P.setTvm(2)

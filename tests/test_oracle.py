@@ -594,3 +594,82 @@ def test_tv_lowers_total_variation(oracle):
     out = oracle.tv_denoise3d(noisy, 25.0, 1.0, 0.1, 5)
     assert tv(out) < 0.5 * tv(noisy)
     assert np.abs(out - clean).mean() < np.abs(noisy - clean).mean()
+
+
+# --------------------------------------------------------------------------
+# Bayesian projector (BayesianVoxelProjector3D)
+# --------------------------------------------------------------------------
+
+def test_bayes_convolution_closed_forms(oracle):
+    """The [1,-2,1]^3 stencil with zero outside the volume: a constant volume
+    is separable, so the result is c times the product of the per-axis sums
+    (0 inside, -1 on a face, -2 across a 1-voxel axis) and -lambda times that
+    survives only at corners; a delta reproduces the stencil around it; a
+    volume that does not vary along one axis gives 0 away from the borders."""
+    lam = 0.25
+    c = np.float32(3.0)
+    vol = np.full((5, 4, 6), c, np.float32)
+    d = oracle.bayes_convolve(vol, -lam)
+    corner = np.zeros_like(d, bool)
+    corner[np.ix_([0, 4], [0, 3], [0, 5])] = True
+    assert np.all(d[corner] == np.float32(lam * c))
+    assert np.all(d[~corner] == 0)
+    flat = np.full((1, 3, 3), c, np.float32)   # D = 1: the z sum is -2
+    assert oracle.bayes_convolve(flat, -lam)[0, 0, 0] == np.float32(-lam * c * -2 * -1 * -1)
+    delta = np.zeros((5, 5, 5), np.float32)
+    delta[2, 2, 2] = 1.0
+    a = np.array([1.0, -2.0, 1.0])
+    stencil = np.einsum("i,j,k->ijk", a, a, a)
+    assert np.array_equal(oracle.bayes_convolve(delta, -lam)[1:4, 1:4, 1:4], (-lam * stencil).astype(np.float32))
+    rng = np.random.default_rng(4)
+    xz = rng.standard_normal((6, 1, 7)).astype(np.float32)
+    assert np.all(oracle.bayes_convolve(np.repeat(xz, 5, axis=1), -lam)[:, 1:-1, :] == 0)
+
+
+def test_bayes_convolution_random_volume(oracle):
+    """A random volume against a zero-padded separable convolution in double."""
+    rng = np.random.default_rng(6)
+    vol = rng.standard_normal((7, 8, 9)).astype(np.float32)
+    got = oracle.bayes_convolve(vol, -0.3)
+    pad = np.pad(vol.astype(np.float64), 1)
+    ref = np.zeros(vol.shape)
+    a = (1.0, -2.0, 1.0)
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                ref += a[i] * a[j] * a[k] * pad[k:k + 7, j:j + 8, i:i + 9]
+    assert np.allclose(got, (-0.3 * ref).astype(np.float32), rtol=1e-6, atol=1e-6)
+
+
+def test_bayes_iteration_without_prior_is_sirt(oracle):
+    """With lambda = 0 and alpha = 1 the distance volume holds only the
+    backprojected differences: from a zero volume the first iteration equals
+    a SIRT iteration bit for bit (the same sums in the same order), the next
+    ones to float rounding."""
+    W, H, D = 24, 5, 10
+    E = oracle.tilt_eulers(np.arange(-60, 61, 15.0))
+    exps = oracle.phantom_project(oracle.ellipsoids(5, 4, W, H, D), E, W, H).astype(np.float32)
+    g = oracle.make_geom(W, H, D, nthreads=4)
+    a = np.zeros((D, H, W), np.float32)
+    b = np.zeros((D, H, W), np.float32)
+    ea = oracle.bayes_iteration(g, a, exps, E, 1.0, 0.0)
+    eb = oracle.ossart(g, b, exps, E, 1, 1.0, 0)[0]
+    assert np.array_equal(a, b)
+    assert np.allclose(ea, eb, rtol=1e-12)
+    for _ in range(2):
+        oracle.bayes_iteration(g, a, exps, E, 1.0, 0.0)
+        oracle.ossart(g, b, exps, E, 1, 1.0, 0)
+    assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max()
+
+
+def test_bayes_end_step(oracle):
+    """endOfIteration: rec += (float)alpha * dist, then MAX against 0 with
+    positivity (NaN stays NaN)."""
+    a = np.array([1.0, -3.0, np.nan], np.float32)
+    d = np.array([2.0, 1.0, 1.0], np.float32)
+    oracle._L().or_bayes_end(oracle._p(a), oracle._p(d), 3, 0.5, 1)
+    assert a[0] == 2.0 and a[1] == 0.0 and np.isnan(a[2])
+    assert np.array_equal(d, [1.0, 0.5, 0.5])
+    b = np.array([-3.0], np.float32)
+    oracle._L().or_bayes_end(oracle._p(b), oracle._p(np.array([1.0], np.float32)), 1, 0.5, 0)
+    assert b[0] == -2.5

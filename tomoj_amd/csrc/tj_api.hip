@@ -36,6 +36,8 @@ hipError_t add_noise(hipStream_t, const double*, size_t, uint64_t, double, doubl
 hipError_t copy(hipStream_t, const void*, void*, size_t);
 hipError_t fista(hipStream_t, float*, const float*, size_t, double);
 hipError_t tv_denoise(hipStream_t, float*, float*, float*, int, int, int, float, float, float, int, uint8_t*);
+hipError_t bayes_conv(hipStream_t, const float*, float*, int, int, int, double);
+hipError_t bayes_end(hipStream_t, float*, float*, size_t, int, float, int, uint8_t*);
 hipError_t normalize(hipStream_t, float*, int, int, int, const int*, double*);
 hipError_t roi_stats(hipStream_t, const float*, int, int, int, const int*, double*);
 hipError_t affine(hipStream_t, const float*, float*, int, int, int, const double*, const int*, const double*, int,
@@ -142,6 +144,13 @@ struct tj_ctx {
     float* tv_f = nullptr;      // [vh][vd][vw] the volume entering the denoising
     float* tv_p = nullptr;      // [3][vh][vd][vw] px, py, pz
 
+    // BayesianVoxelProjector3D (BayesianVoxelProjector3D.java:25-30): its
+    // backprojection goes into the distance volume, folded into the volume at
+    // the end of the iteration
+    int bayes = 0;
+    double bayes_alpha = 0.1, bayes_lambda = 0.1;
+    float* dist = nullptr;      // [vh][vd][vw]
+
     // FP64 pair images of the backprojected images (LDS backprojection)
     double2* bpairs = nullptr;  // [bpairs_cap][ph][pw]
     int bpairs_cap = 0;
@@ -165,6 +174,7 @@ static void ctx_free(tj_ctx* c)
     dfree(c->prev);
     dfree(c->tv_f);
     dfree(c->tv_p);
+    dfree(c->dist);
     dfree(c->vol);
     dfree(c->pflag);
     dfree(c->norm_tab);
@@ -397,15 +407,21 @@ static int do_residual(tj_ctx* c, const int* slots, int nslot, double factor, do
     return TJ_OK;
 }
 
+// The volume the backprojection adds into: the reconstruction, or the
+// Bayesian projector's distance volume (BayesianVoxelProjector3D.backproject
+// :75-110), whose finiteness is not tracked (the checked update path).
+static float* bp_target(tj_ctx* c) { return c->bayes ? c->dist : c->vol; }
+static int bp_finite(tj_ctx* c) { return (c->bayes || c->any_nonfinite) ? 0 : 1; }
+
 static int do_backproject(tj_ctx* c, const int* slots, const std::vector<int>& hslots, const float* images,
                           const int* image_index, int start_y, int end_y)
 {
     const int kernel = pick_bp_kernel(c, hslots);
     if (kernel == 3)
         if (int rc = ensure_bpairs(c, (int)hslots.size())) return rc;
-    TJ_HIP(tjk::bp(c->stream, c->g, c->vol, c->pp, slots, (int)hslots.size(), images, image_index, c->rowflag,
-                   c->bprows, start_y, end_y, kernel, c->tune.bp_rows, c->tune.bp_zchunk, c->tune.bp_group,
-                   c->any_nonfinite ? 0 : 1, c->bpairs));
+    TJ_HIP(tjk::bp(c->stream, c->g, bp_target(c), c->pp, slots, (int)hslots.size(), images, image_index,
+                   c->rowflag, c->bprows, start_y, end_y, kernel, c->tune.bp_rows, c->tune.bp_zchunk,
+                   c->tune.bp_group, bp_finite(c), c->bpairs));
     return TJ_OK;
 }
 
@@ -1024,6 +1040,8 @@ int tj_backproject(tj_ctx* c, int start_y, int end_y)
         images = c->sdiff;
         index = c->iota;
     } else {
+        // BayesianVoxelProjector3D.backproject only adds differences (:90)
+        if (c->bayes) return TJ_OK;
         if ((rc = prepare_originals(c, c->slot_proj, c->added, &images, &index))) return rc;
     }
     if ((rc = do_backproject(c, c->slot_proj, c->added, images, index, start_y, end_y))) return rc;
@@ -1178,19 +1196,72 @@ static int tvm_end(tj_ctx* c)
     return tv_run(c, c->tvm_theta, c->tvm_g, c->tvm_dt, c->tvm_iter);
 }
 
-// With both set, FistaProjector3D wraps the TVM projector: its endOfIteration
-// runs the wrapped projector's first (FistaProjector3D.java:44).
+static int ensure_dist(tj_ctx* c)
+{
+    if (c->dist) return TJ_OK;
+    if (dalloc(&c->dist, c->nvox) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(TJ_ENOMEM, "Bayesian projector: cannot allocate the distance volume (%zu voxels)", c->nvox);
+    }
+    return TJ_OK;
+}
+
+// BayesianVoxelProjector3D.startOfIteration (:134-137): the distance volume
+// restarts as -lambda times the convolved volume.
+static int bayes_start(tj_ctx* c)
+{
+    if (!c->bayes) return TJ_OK;
+    TJ_HIP(tjk::bayes_conv(c->stream, c->vol, c->dist, c->g.vw, c->g.vh, c->g.vd, -c->bayes_lambda));
+    return TJ_OK;
+}
+
+// BayesianVoxelProjector3D.endOfIteration (:121-132).
+static int bayes_end(tj_ctx* c)
+{
+    if (!c->bayes) return TJ_OK;
+    TJ_HIP(tjk::bayes_end(c->stream, c->vol, c->dist, c->plane, c->g.vh, (float)c->bayes_alpha,
+                          c->g.positivity, c->pflag));
+    return refresh_plane_flags(c, true);
+}
+
+// FistaProjector3D wraps the working projector: its startOfIteration keeps the
+// volume, then runs the wrapped one's (:62-67); its endOfIteration runs the
+// wrapped one's first (:44).  TVM and Bayesian are never both on.
+static int start_hooks(tj_ctx* c)
+{
+    int rc = fista_start(c);
+    if (rc) return rc;
+    return bayes_start(c);
+}
+
 static int end_hooks(tj_ctx* c)
 {
     int rc = tvm_end(c);
     if (rc) return rc;
+    if ((rc = bayes_end(c))) return rc;
     return fista_end(c);
+}
+
+int tj_set_bayesian(tj_ctx* c, int on, double alpha, double lambda)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_set_bayesian: null context");
+    if (on && c->tvm_iter > 0) return fail(TJ_ESTATE, "tj_set_bayesian: the TVM hook is on");
+    TJ_HIP(hipSetDevice(c->device));
+    if (on) {
+        int rc = ensure_dist(c);
+        if (rc) return rc;
+    }
+    c->bayes = on ? 1 : 0;
+    c->bayes_alpha = alpha;
+    c->bayes_lambda = lambda;
+    return TJ_OK;
 }
 
 int tj_set_tvm(tj_ctx* c, int iterations, double theta, double g, double lambda)
 {
     if (!c) return fail(TJ_EINVAL, "tj_set_tvm: null context");
     if (iterations < 0) return fail(TJ_EINVAL, "tj_set_tvm: negative iteration count %d", iterations);
+    if (iterations > 0 && c->bayes) return fail(TJ_ESTATE, "tj_set_tvm: the Bayesian hook is on");
     TJ_HIP(hipSetDevice(c->device));
     if (iterations > 0) {
         int rc = ensure_tv(c);
@@ -1230,7 +1301,7 @@ int tj_start_iteration(tj_ctx* c)
 {
     if (!c) return fail(TJ_EINVAL, "null context");
     TJ_HIP(hipSetDevice(c->device));
-    int rc = fista_start(c);
+    int rc = start_hooks(c);
     if (rc) return rc;
     TJ_HIP(hipStreamSynchronize(c->stream));
     return TJ_OK;
@@ -1306,7 +1377,7 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
         if (c->interrupted.load()) return fail(TJ_ECANCELLED, "tj_run_ossart: interrupted");
         // projector.startOfIteration() / endOfIteration() around each
         // osartIteration (TomoReconstruction2.java:593-595)
-        if ((rc = fista_start(c))) return rc;
+        if ((rc = start_hooks(c))) return rc;
         int diffIndex = 0;
         for (int t = 0; t < n; ++t) {
             ++diffIndex;
@@ -1334,9 +1405,9 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
                 if (c->timing_on) TJ_HIP(hipEventRecord(r, c->stream));
                 if (bk == 3)
                     if ((rc = ensure_bpairs(c, diffIndex))) return rc;
-                TJ_HIP(tjk::bp(c->stream, c->g, c->vol, c->pp, slots, diffIndex, c->sdiff, c->iota, c->rowflag,
-                               c->bprows, start_y, end_y, bk, c->tune.bp_rows, c->tune.bp_zchunk, c->tune.bp_group,
-                               c->any_nonfinite ? 0 : 1, c->bpairs));
+                TJ_HIP(tjk::bp(c->stream, c->g, bp_target(c), c->pp, slots, diffIndex, c->sdiff, c->iota,
+                               c->rowflag, c->bprows, start_y, end_y, bk, c->tune.bp_rows, c->tune.bp_zchunk,
+                               c->tune.bp_group, bp_finite(c), c->bpairs));
                 if (c->timing_on) {
                     TJ_HIP(hipEventRecord(d, c->stream));
                     fpev.emplace_back(a, b);
@@ -1417,6 +1488,8 @@ int tj_run_wbp(tj_ctx* c, int start_y, int end_y)
     if (rc) return rc;
     const int n = c->nproj;
     if (n <= 0) return fail(TJ_ESTATE, "tj_run_wbp: no resident projections");
+    // the Bayesian projector's backproject adds nothing without differences (:90)
+    if (c->bayes) return TJ_OK;
     TJ_HIP(hipSetDevice(c->device));
     if ((rc = ensure_slots(c, n))) return rc;
     if ((rc = ensure_iota(c, n))) return rc;

@@ -1173,6 +1173,87 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_tv_u(const float* __restrict__ f, 
 }
 
 // ---------------------------------------------------------------------------
+// Bayesian regularisation (BayesianVoxelProjector3D): the distance volume of
+// startOfIteration, convolve(rec, kernel, dist, -lambda) (:175-198), and the
+// update of endOfIteration (:121-132).  Same tiles and block order as the TV
+// kernels: a thread marches z down one column (x, y) and keeps the 3 x 3 x 3
+// neighbourhood in registers, loading the 9 values of slice z+1 per step.
+// ---------------------------------------------------------------------------
+__device__ inline float bayes_vox(const float* __restrict__ v, int W, int H, int D, int x, int y, int z)
+{
+    // TomoReconstruction2.getPixel(int, int, int) (:241-246): 0 outside
+    if (x < 0 || x >= W || y < 0 || y >= H || z < 0 || z >= D) return 0.0f;
+    return v[((size_t)y * D + z) * W + x];
+}
+
+// dist = (float)(total * factor), total the double sum of the 27 float
+// products value * kernel, x offset outermost and z offset innermost (the order
+// of getCubicNeightborhoodWithClamp, TomoReconstruction2.java:317-339, which
+// reads the unclamped neighbours).  The kernel is [1,-2,1] along each axis.
+__global__ __launch_bounds__(TJ_BLOCK) void k_bayes_conv(const float* __restrict__ vol, float* __restrict__ dist,
+                                                         int W, int H, int D, int nreal, double factor)
+{
+    TvTile t;
+    if (!tv_tile(W, H, D, nreal, t)) return;
+    constexpr float a[3] = {1.0f, -2.0f, 1.0f};
+    float w[3][3][3];  // [i][j][k]: x offset, y offset, z offset (z-1, z, z+1)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            w[i][j][0] = bayes_vox(vol, W, H, D, t.x + i - 1, t.y + j - 1, t.z0 - 1);
+            w[i][j][1] = bayes_vox(vol, W, H, D, t.x + i - 1, t.y + j - 1, t.z0);
+        }
+    size_t k = ((size_t)t.y * D + t.z0) * W + t.x;
+    for (int z = t.z0; z < t.z1; ++z, k += W) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) w[i][j][2] = bayes_vox(vol, W, H, D, t.x + i - 1, t.y + j - 1, z + 1);
+        double total = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) {
+                    const float prod = w[i][j][kk] * (a[i] * a[j] * a[kk]);
+                    total += (double)prod;
+                }
+        dist[k] = (float)(total * factor);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                w[i][j][0] = w[i][j][1];
+                w[i][j][1] = w[i][j][2];
+            }
+    }
+}
+
+// endOfIteration: dist *= (float) alpha (ImageJ FloatProcessor.multiply), rec +=
+// dist (FloatBlitter ADD), then with positivity FloatBlitter MAX against 0
+// (applyPositivityConstraint, TomoReconstruction2.java:405-412).  A
+// non-finite result sets its row's plane flag (cleared by the caller).
+__global__ __launch_bounds__(TJ_BLOCK) void k_bayes_end(float* __restrict__ vol, float* __restrict__ dist,
+                                                        size_t plane, int H, float alpha, int positivity,
+                                                        uint8_t* __restrict__ pflag)
+{
+    const int y = blockIdx.y;
+    bool bad = false;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < plane; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t k = (size_t)y * plane + q;
+        const float d = dist[k] * alpha;
+        dist[k] = d;
+        float v = d + vol[k];
+        if (positivity && 0.0f > v) v = 0.0f;
+        vol[k] = v;
+        bad |= !isfinite(v);
+    }
+    if (bad) pflag[y] = 1;
+}
+
+// ---------------------------------------------------------------------------
 // Volume layout conversion between the host ImageStack layout [z][y][x] and the
 // device layout [y][z][x], for rows [y0, y0+nrows).
 // ---------------------------------------------------------------------------
@@ -1540,6 +1621,29 @@ hipError_t tv_denoise(hipStream_t st, float* vol, float* f, float* p, int W, int
         k_tv_u<<<nb, TJ_BLOCK, 0, st>>>(f, vol, p, p + n, p + 2 * n, W, H, D, (int)nreal, theta,
                                         it == iterations - 1 ? pflag : nullptr);
     }
+    return hipGetLastError();
+}
+
+hipError_t bayes_conv(hipStream_t st, const float* vol, float* dist, int W, int H, int D, double factor)
+{
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    const int nxt = (W + TJ_BLOCK - 1) / TJ_BLOCK;
+    const long nreal = (long)nxt * H * ((D + TJ_TV_ZR - 1) / TJ_TV_ZR);
+    if (nreal <= 0) return hipSuccess;
+    if (nreal > 0x7ffffff0L) return hipErrorInvalidValue;
+    k_bayes_conv<<<(int)((nreal + 7) & ~7L), TJ_BLOCK, 0, st>>>(vol, dist, W, H, D, (int)nreal, factor);
+    return hipGetLastError();
+}
+
+hipError_t bayes_end(hipStream_t st, float* vol, float* dist, size_t plane, int H, float alpha, int positivity,
+                     uint8_t* pflag)
+{
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    if (H <= 0 || plane == 0) return hipSuccess;
+    hipError_t e = hipMemsetAsync(pflag, 0, H, st);
+    if (e != hipSuccess) return e;
+    const unsigned bx = (unsigned)std::min<size_t>(64, (plane + 4 * TJ_BLOCK - 1) / (4 * TJ_BLOCK));
+    k_bayes_end<<<dim3(bx, H), TJ_BLOCK, 0, st>>>(vol, dist, plane, H, alpha, positivity, pflag);
     return hipGetLastError();
 }
 

@@ -19,13 +19,14 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from .projector import TVMProjector3DHIP, VoxelProjector3DHIP
+from .projector import BayesianVoxelProjector3DHIP, TVMProjector3DHIP, VoxelProjector3DHIP
 from .reconstruction import tilt_eulers
 
 BP = 0
 WBP = 1
 OSSART = 2
 TVM = 3        # AdvancedReconstructionParameters.java:10
+BAYESIAN = 4   # AdvancedReconstructionParameters.java:11
 
 
 @dataclass
@@ -189,12 +190,38 @@ class AdvancedReconstructionParameters(ReconstructionParameters):
             raise TypeError("createTVMParameters takes 5 or 9 arguments")
         return p
 
+    @staticmethod
+    def createBayesianParameters(width, height, depth, nbIterations, regularizationAlpha,
+                                 regularizationWeight) -> "AdvancedReconstructionParameters":
+        """:135-144: the alpha is stored as the relaxation coefficient; update 0."""
+        p = AdvancedReconstructionParameters(width, height, depth)
+        p.reconstructionType = BAYESIAN
+        p.nbIterations = int(nbIterations)
+        p.weightingRadius = float("nan")
+        p.relaxationCoefficient = float(regularizationAlpha)
+        p.regularizationWeight = float(regularizationWeight)
+        p.updateNb = 0
+        return p
+
     def getProjector(self, ts: TiltSeriesData, device: int = 0):
-        """The TVM branch of getProjector (:179-193): TVMProjector3D with the
-        relaxation as regularisation alpha, the positivity and long-object
-        settings, the TV settings, the scale when rescaleData, and the FISTA
-        wrapper; TomoJ leaves its regularisation lambda at 0.1 (the call that
-        would set it is commented out, :182).  Other types go to the base class."""
+        """The TVM and BAYESIAN branches of getProjector (:179-203).  TVM:
+        TVMProjector3D with the relaxation as regularisation alpha, the
+        positivity and long-object settings, the TV settings, the scale when
+        rescaleData, and the FISTA wrapper; TomoJ leaves its regularisation
+        lambda at 0.1 (the call that would set it is commented out, :182).
+        BAYESIAN: BayesianVoxelProjector3D with alpha = the relaxation
+        coefficient and lambda = the regularisation weight, positivity,
+        long-object setting and scale, never wrapped in FISTA.  Other types go
+        to the base class."""
+        if self.reconstructionType == BAYESIAN:
+            fista, self.fista = self.fista, False
+            try:
+                P = self._projector(BayesianVoxelProjector3DHIP, ts, device, self.positivityConstraint)
+            finally:
+                self.fista = fista
+            P.setRegularizationAlpha(self.relaxationCoefficient)
+            P.setRegularizationLambda(self.regularizationWeight)
+            return P
         if self.reconstructionType != TVM:
             return super().getProjector(ts, device)
         P = self._projector(TVMProjector3DHIP, ts, device, self.positivityConstraint)
@@ -235,9 +262,9 @@ class ReconstructionThreadManager:
         if params.reconstructionType in (BP, WBP):
             P.run_wbp()
             return None
-        if params.reconstructionType in (OSSART, TVM):
-            # TVM goes through TomoReconstruction2.regularization (:783-878), the
-            # OS-SART loop with the projector's own iteration hooks.
+        if params.reconstructionType in (OSSART, TVM, BAYESIAN):
+            # TVM and BAYESIAN go through TomoReconstruction2.regularization
+            # (:783-878), the OS-SART loop with the projector's own iteration hooks.
             return P.run_ossart(params.nbIterations, params.relaxationCoefficient, params.updateNb,
                                 first_index=first_index)
         raise ValueError(f"reconstruction type {params.reconstructionType} is not on this path")

@@ -214,6 +214,14 @@ class VoxelProjector3DHIP:
         iterations).  iterations = 0 turns it off."""
         call("tj_set_tvm", self._ctx, int(iterations), float(theta), float(g), float(lam))
 
+    def setBayesian(self, on: bool, alpha: float = 0.1, lam: float = 0.1) -> None:
+        """Turns the projector into BayesianVoxelProjector3D
+        (BayesianVoxelProjector3D.java:25-204): the iteration hooks keep a
+        distance volume (-lam x the [1,-2,1]^3 convolution of the volume plus
+        the backprojected differences) and add (float) alpha times it to the
+        volume at the end of each iteration."""
+        call("tj_set_bayesian", self._ctx, int(bool(on)), float(alpha), float(lam))
+
     def tv_denoise(self, theta: float, g: float, dt: float, iterations: int) -> None:
         """TotalVariationDenoising_3D.denoise3D (TotalVariationDenoising_3D.java:84-202)
         of the resident volume, once."""
@@ -325,6 +333,29 @@ class TVMProjector3DHIP(VoxelProjector3DHIP):
 
     def setTvmIteration(self, n: int) -> None:
         self.tvmIteration = int(n)
+        self._push()
+
+
+class BayesianVoxelProjector3DHIP(VoxelProjector3DHIP):
+    """BayesianVoxelProjector3D (TomoJ_AdvancedReconstruction/.../projectors/
+    BayesianVoxelProjector3D.java:25-204) on the GPU, with TomoJ's setter names
+    and defaults (:26-27)."""
+
+    def __init__(self, *args, **kw):
+        super().__init__(*args, **kw)
+        self.regularizationLambda = 0.1
+        self.regularizationAlpha = 0.1
+        self._push()
+
+    def _push(self) -> None:
+        self.setBayesian(True, self.regularizationAlpha, self.regularizationLambda)
+
+    def setRegularizationLambda(self, v: float) -> None:
+        self.regularizationLambda = float(v)
+        self._push()
+
+    def setRegularizationAlpha(self, v: float) -> None:
+        self.regularizationAlpha = float(v)
         self._push()
 
 
