@@ -200,6 +200,24 @@ int tj_set_fista(tj_ctx *ctx, int on);
  * (:64-74).  Holds 16 bytes per voxel of device memory while on. */
 int tj_set_tvm(tj_ctx *ctx, int iterations, double theta, double g, double lambda);
 
+/* ReconstructionParameters.getAvailableIndexes (ReconstructionParameters.java:212-256)
+ * for `nproj` images: projection_type 11 (EVEN_PROJECTIONS) or 12
+ * (ODD_PROJECTIONS) keeps the images of that parity, at most nproj/2 of them;
+ * any other type (ALL_PROJECTIONS = 10) keeps all.  Writes the list to `out`
+ * (room for nproj) and returns its length, or a negative TJ_E* code. */
+int tj_available_indexes(int nproj, int projection_type, int *out);
+
+/* The images tj_run_ossart and tj_run_wbp walk: the list getAvailableIndexes
+ * returns (or setAvailableIndexes sets, :258-261), as TomoReconstruction2.WBP
+ * (:500) and orderShuffleART (:736) use it; n = 0 restores all.  With a list,
+ * tj_run_ossart keeps TomoJ's arithmetic on the full count N: the order
+ * chain uses the Euler matrices of images 0..n-1 (:741) and emits the listed
+ * images, update 0 means N, a subset is flushed when it holds `update` images
+ * or at position N-1 (:662), so a last partial subset is never projected and
+ * with update 0 nothing is, and err[0] divides by N (:726).  first_index is a
+ * position in the list.  Checked against the loaded projections at run time. */
+int tj_set_available_indexes(tj_ctx *ctx, const int *indexes, int n);
+
 /* Bayesian regularisation: the projector getProjector builds for the BAYESIAN
  * type (AdvancedReconstructionParameters.java:194-203) is
  * BayesianVoxelProjector3D (TomoJ_AdvancedReconstruction/.../projectors/

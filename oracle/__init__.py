@@ -90,6 +90,9 @@ def _L():
             "or_order_shuffle": (None, [P, i, i, P]),
             "or_ossart": (i, [G, P, P, P, i, i, d, i, i, i, i, P]),
             "or_ossart_f64": (i, [G, P, P, P, i, i, d, i, i, i, i, P]),
+            "or_ossart_avail": (i, [G, P, P, P, i, P, i, i, d, i, i, i, i, P]),
+            "or_order_shuffle_avail": (None, [P, i, P, i, P]),
+            "or_available_indexes": (i, [i, i, P]),
             "or_weighting_mask": (None, [i, i, P, P, i, d, P]),
             "or_fft2": (None, [P, P, i, i, i]),
             "or_weighting": (None, [P, P, i, i]),
@@ -228,6 +231,22 @@ def backproject(g: Geom, vol, images, eulers, start_y=0, end_y=None, f64=False):
     return vol
 
 
+def available_indexes(n, projection_type) -> np.ndarray:
+    """ReconstructionParameters.getAvailableIndexes (:212-256): ALL 10, EVEN 11, ODD 12."""
+    out = np.zeros(max(n, 1), np.int32)
+    k = _L().or_available_indexes(int(n), int(projection_type), _p(out))
+    return out[:k].copy()
+
+
+def order_shuffle_available(eulers, available, first_index=0) -> np.ndarray:
+    """orderShuffleART (:735-781) over a list of available images."""
+    eulers = _c(eulers, np.float64).reshape(-1, 9)
+    av = _c(available, np.int32)
+    order = np.zeros(max(av.size, 1), np.int32)
+    _L().or_order_shuffle_avail(_p(eulers), av.size, _p(av), int(first_index), _p(order))
+    return order[:av.size].copy()
+
+
 def order_shuffle(eulers, first_index=0) -> np.ndarray:
     """TomoReconstruction2.orderShuffleART (:735-781) with a fixed first index."""
     eulers = _c(eulers, np.float64).reshape(-1, 9)
@@ -237,15 +256,25 @@ def order_shuffle(eulers, first_index=0) -> np.ndarray:
 
 
 def ossart(g: Geom, vol, projs, eulers, iterations, lam, update, first_index=0,
-           start_y=0, end_y=None, f64=False):
+           start_y=0, end_y=None, f64=False, available=None):
     """TomoReconstruction2.OSSART (:548-634).  Updates ``vol`` in place; returns the
-    per-iteration ``err1`` rows [avg, min, imin, max, imax] (iterations x 5)."""
+    per-iteration ``err1`` rows [avg, min, imin, max, imax] (iterations x 5).
+    ``available``: the list getAvailableIndexes gives (FP32 only)."""
     end_y = g.vh if end_y is None else end_y
     dt = np.float64 if f64 else np.float32
     assert vol.dtype == dt and vol.flags["C_CONTIGUOUS"]
     projs = _c(projs, dt)
     eulers = _c(eulers, np.float64).reshape(-1, 9)
     err = np.zeros((iterations, 5))
+    if available is not None:
+        assert not f64
+        av = _c(available, np.int32)
+        rc = _L().or_ossart_avail(C.byref(g), _p(vol), _p(projs), _p(eulers), eulers.shape[0], _p(av),
+                                  av.size, iterations, float(lam), int(update), int(first_index), start_y,
+                                  end_y, _p(err))
+        if rc != 0:
+            raise MemoryError("oracle OS-SART allocation failed")
+        return err
     fn = _L().or_ossart_f64 if f64 else _L().or_ossart
     rc = fn(C.byref(g), _p(vol), _p(projs), _p(eulers), eulers.shape[0], iterations,
             float(lam), int(update), int(first_index), start_y, end_y, _p(err))

@@ -556,16 +556,59 @@ void or_order_shuffle(const double *eul, int n, int first_index, int *order)
     free(todo);
 }
 
+/* orderShuffleART over a list of available images (TomoReconstruction2.java:735-781):
+ * the chain runs over positions 0..nav-1 with the Euler matrix of IMAGE im
+ * (getEulerMatrix(im), :741), not of the image avail[im] the position names,
+ * and emits avail[im].  avail NULL: every image. */
+void or_order_shuffle_avail(const double *eul, int nav, const int *avail, int first_index, int *order)
+{
+    if (nav <= 0) return;
+    or_order_shuffle(eul, nav, first_index, order);
+    if (avail)
+        for (int k = 0; k < nav; k++) order[k] = avail[order[k]];
+}
+
+/* ReconstructionParameters.getAvailableIndexes (ReconstructionParameters.java:212-256)
+ * for projectionType ALL (10, and any other value), EVEN (11) or ODD (12): the
+ * images of that parity, at most n/2 of them.  Returns the count. */
+int or_available_indexes(int n, int type, int *out)
+{
+    int cnt = 0, k = 0;
+    char *take = (char *)malloc(n > 0 ? n : 1);
+    for (int i = 0; i < n; i++) {
+        if (type == 11 || type == 12) {
+            const int par = (type == 11) ? (i % 2 == 0) : (i % 2 == 1);
+            take[i] = (char)par;
+            if (par) cnt++;
+            if (cnt > n / 2) {
+                take[i] = 0;
+                cnt--;
+            }
+        } else {
+            take[i] = 1;
+            cnt++;
+        }
+    }
+    for (int i = 0; i < n; i++)
+        if (take[i]) out[k++] = i;
+    free(take);
+    return cnt;
+}
+
 /* One osartIteration (TomoReconstruction2.java:636-733).  projs: n x ph x pw
  * (the already-supplied projections, ts.getPixels).  err5 receives err1[0..4].
- * Returns 0, or -1 on allocation failure. */
+ * With a list of available images (nav of them): the order runs over the
+ * list, but update (0 -> n), the flush rule "t == n - 1" and the error
+ * normalisation keep the full count n, as TomoJ's do (:639, :644, :662,
+ * :726), so a last partial subset is never projected, and with update 0
+ * nothing is.  Returns 0, or -1 on allocation failure. */
 static int osart_iteration_f32(const or_geom *g, float *vol, const float *projs, const double *eul,
-                               int n, double lambda, int update_nb, int first_index,
+                               int n, const int *avail, int nav, double lambda, int update_nb, int first_index,
                                int startY, int endY, double *err5)
 {
     const long npx = (long)g->pw * g->ph;
-    int *order = (int *)malloc(sizeof(int) * n);
-    or_order_shuffle(eul, n, first_index, order);
+    int *order = (int *)malloc(sizeof(int) * (nav > 0 ? nav : 1));
+    or_order_shuffle_avail(eul, nav, avail, first_index, order);
     const int update = (update_nb == 0) ? n : update_nb;
     const float factorimg = (float)(lambda / update);
     float *th = (float *)malloc(sizeof(float) * npx * update);
@@ -575,7 +618,7 @@ static int osart_iteration_f32(const or_geom *g, float *vol, const float *projs,
     if (!order || !th || !nm || !df || !errors) return -1;
     double som = 0, errmin = JAVA_FLOAT_MAX, errmax = 0;
     int imin = 0, imax = 0, diffIndex = 0;
-    for (int t = 0; t < n; t++) {
+    for (int t = 0; t < nav; t++) {
         diffIndex++;
         if (diffIndex == update || t == n - 1) {
             const int nadd = diffIndex;
@@ -614,12 +657,12 @@ static int osart_iteration_f32(const or_geom *g, float *vol, const float *projs,
 }
 
 static int osart_iteration_f64(const or_geom *g, double *vol, const double *projs, const double *eul,
-                               int n, double lambda, int update_nb, int first_index,
+                               int n, const int *avail, int nav, double lambda, int update_nb, int first_index,
                                int startY, int endY, double *err5)
 {
     const long npx = (long)g->pw * g->ph;
-    int *order = (int *)malloc(sizeof(int) * n);
-    or_order_shuffle(eul, n, first_index, order);
+    int *order = (int *)malloc(sizeof(int) * (nav > 0 ? nav : 1));
+    or_order_shuffle_avail(eul, nav, avail, first_index, order);
     const int update = (update_nb == 0) ? n : update_nb;
     const double factor = lambda / update;
     double *th = (double *)malloc(sizeof(double) * npx * update);
@@ -629,7 +672,7 @@ static int osart_iteration_f64(const or_geom *g, double *vol, const double *proj
     if (!order || !th || !nm || !df || !errors) return -1;
     double som = 0, errmin = JAVA_FLOAT_MAX, errmax = 0;
     int imin = 0, imax = 0, diffIndex = 0;
-    for (int t = 0; t < n; t++) {
+    for (int t = 0; t < nav; t++) {
         diffIndex++;
         if (diffIndex == update || t == n - 1) {
             const int nadd = diffIndex;
@@ -674,7 +717,19 @@ int or_ossart(const or_geom *g, float *vol, const float *projs, const double *eu
               int startY, int endY, double *err)
 {
     for (int it = 0; it < iterations; it++)
-        if (osart_iteration_f32(g, vol, projs, eul, n, lambda, update_nb, first_index, startY, endY,
+        if (osart_iteration_f32(g, vol, projs, eul, n, NULL, n, lambda, update_nb, first_index, startY, endY,
+                                err + 5 * it))
+            return -1;
+    return 0;
+}
+
+/* OSSART over the available images `avail` (nav of them). */
+int or_ossart_avail(const or_geom *g, float *vol, const float *projs, const double *eul, int n,
+                    const int *avail, int nav, int iterations, double lambda, int update_nb, int first_index,
+                    int startY, int endY, double *err)
+{
+    for (int it = 0; it < iterations; it++)
+        if (osart_iteration_f32(g, vol, projs, eul, n, avail, nav, lambda, update_nb, first_index, startY, endY,
                                 err + 5 * it))
             return -1;
     return 0;
@@ -685,7 +740,7 @@ int or_ossart_f64(const or_geom *g, double *vol, const double *projs, const doub
                   int startY, int endY, double *err)
 {
     for (int it = 0; it < iterations; it++)
-        if (osart_iteration_f64(g, vol, projs, eul, n, lambda, update_nb, first_index, startY, endY,
+        if (osart_iteration_f64(g, vol, projs, eul, n, NULL, n, lambda, update_nb, first_index, startY, endY,
                                 err + 5 * it))
             return -1;
     return 0;

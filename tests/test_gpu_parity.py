@@ -722,3 +722,76 @@ def test_bayesian_and_tvm_hooks_exclusive(tj):
         P.setBayesian(True)
         with pytest.raises(tj.TjError):
             P.setTvm(2)
+
+
+# --------------------------------------------------------------------------
+# projection subsets (getAvailableIndexes: FSC even / odd halves)
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("ptype,update,first", [(11, 4, 2), (12, 5, 0), (12, 0, 1), (11, 1, 3)])
+def test_available_subsets_bit_exact(tj, oracle, ptype, update, first):
+    """run_ossart and the SPI loop over the even / odd half against the
+    oracle, with TomoJ's full-count rules (dropped last subset, nothing at
+    update 0, err over N)."""
+    from tomoj_amd import reconstruction as R
+    W, H, D = 26, 6, 11
+    E = oracle.tilt_eulers(np.arange(-60, 61, 6.0))
+    exps = (oracle.phantom_project(oracle.ellipsoids(19, 4, W, H, D), E, W, H)
+            + 0.05 * oracle.gaussian_noise(6, len(E) * W * H).reshape(len(E), H, W)).astype(np.float32)
+    avail = oracle.available_indexes(len(E), ptype)
+    print(f"subset type={ptype} update={update} first={first}", flush=True)
+    ref = np.zeros((D, H, W), np.float32)
+    with make(tj, (W, H, D)) as P:
+        ref_err = oracle.ossart(ogeom(oracle, P), ref, exps, E, 2, 0.9, update, first_index=first,
+                                available=avail)
+        P.load_tilt_series(exps, E)
+        P.setAvailableIndexes(avail)
+        err = P.run_ossart(2, 0.9, update, first_index=first)
+        assert same(P.get_volume(), ref)
+        assert np.allclose(err[:, [0, 1, 3]], ref_err[:, [0, 1, 3]], rtol=1e-12)
+        assert np.array_equal(err[:, [2, 4]], ref_err[:, [2, 4]])
+        P.reset_volume()
+        spi_err = R.ossart(P, E, 2, 0.9, update, first_index=first, available=list(avail))
+        assert same(P.get_volume(), ref)
+        assert np.allclose(spi_err[:, [0, 1, 3]], ref_err[:, [0, 1, 3]], rtol=1e-12)
+        P.setAvailableIndexes(None)
+        P.reset_volume()
+        P.run_ossart(1, 0.9, update, first_index=first)
+        full = np.zeros((D, H, W), np.float32)
+        oracle.ossart(ogeom(oracle, P), full, exps, E, 1, 0.9, update, first_index=first)
+        assert same(P.get_volume(), full)
+
+
+def test_available_subsets_through_parameters(tj, oracle):
+    """setFscType(ODD/EVEN) on the parameters drives doRec over the half: BP
+    (no weighting) over the odd images equals the oracle's backprojection of
+    those images; OS-SART over the even half equals the oracle's subset loop."""
+    from tomoj_amd import parameters as RP
+    W, H, D = 24, 5, 9
+    tilts = list(np.arange(-60, 61, 8.0))
+    E = oracle.tilt_eulers(tilts)
+    exps = oracle.phantom_project(oracle.ellipsoids(21, 4, W, H, D), E, W, H).astype(np.float32)
+    ts = RP.TiltSeriesData(exps, tilts)
+    g = oracle.make_geom(W, H, D, nthreads=8)
+    odd = oracle.available_indexes(len(E), RP.ODD_PROJECTIONS)
+    even = oracle.available_indexes(len(E), RP.EVEN_PROJECTIONS)
+    bp = RP.ReconstructionParameters.createWBPParameters(W, H, D, float("nan"))
+    bp.setFscType(RP.ODD_PROJECTIONS)
+    m = RP.ReconstructionThreadManager(ts)
+    try:
+        m.doRec(bp)
+        ref = oracle.backproject(g, np.zeros((D, H, W), np.float32), exps[odd], E[odd])
+        assert same(m.getRec2(), ref)
+    finally:
+        m.close()
+    os_ = RP.ReconstructionParameters.createOSSARTParameters(W, H, D, 2, 1.0, 3)
+    os_.setFscType(RP.EVEN_PROJECTIONS)
+    m = RP.ReconstructionThreadManager(ts)
+    try:
+        err = m.doRec(os_)
+        ref = np.zeros((D, H, W), np.float32)
+        ref_err = oracle.ossart(g, ref, exps, E, 2, 1.0, 3, available=even)
+        assert same(m.getRec2(), ref)
+        assert np.allclose(err[:, 0], ref_err[:, 0], rtol=1e-12)
+    finally:
+        m.close()

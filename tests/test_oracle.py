@@ -673,3 +673,73 @@ def test_bayes_end_step(oracle):
     b = np.array([-3.0], np.float32)
     oracle._L().or_bayes_end(oracle._p(b), oracle._p(np.array([1.0], np.float32)), 1, 0.5, 0)
     assert b[0] == -2.5
+
+
+# --------------------------------------------------------------------------
+# projection subsets (ReconstructionParameters.getAvailableIndexes)
+# --------------------------------------------------------------------------
+
+def test_available_indexes_known_lists(oracle):
+    """EVEN / ODD keep that parity, at most N/2 images (so the last even image
+    of an odd N is dropped); ALL keeps every image.  The library's host
+    routine and the parameters object give the same lists."""
+    import ctypes as C
+    import tomoj_amd
+    from tomoj_amd import parameters as RP
+    cases = {
+        (121, 11): list(range(0, 119, 2)), (121, 12): list(range(1, 120, 2)),
+        (10, 11): [0, 2, 4, 6, 8], (10, 12): [1, 3, 5, 7, 9],
+        (7, 11): [0, 2, 4], (7, 12): [1, 3, 5], (1, 11): [], (1, 12): [],
+        (5, 10): [0, 1, 2, 3, 4], (5, 0): [0, 1, 2, 3, 4],
+    }
+    lib = tomoj_amd.load_library()
+    for (n, t), want in cases.items():
+        assert list(oracle.available_indexes(n, t)) == want
+        out = (C.c_int * max(n, 1))()
+        k = lib.tj_available_indexes(n, t, out)
+        assert [out[i] for i in range(k)] == want
+        p = RP.ReconstructionParameters.createOSSARTParameters(8, 8, 8, 1, 1.0, 0)
+        p.setFscType(t)
+        assert p.getAvailableIndexes(RP.TiltSeriesData(np.zeros((n, 2, 2), np.float32), [0.0] * n)) == want
+
+
+def test_order_shuffle_available_uses_leading_matrices(oracle):
+    """With a list of available images TomoJ chains positions 0..len-1 with
+    the Euler matrices of IMAGES 0..len-1 (getEulerMatrix(im)) and emits the
+    listed image at each position; the Python loop does the same."""
+    from tomoj_amd.reconstruction import order_shuffle
+    E = oracle.tilt_eulers(np.arange(-60, 61, 5.0))
+    odd = oracle.available_indexes(len(E), 12)
+    for first in (0, 3, len(odd) - 1):
+        chain = oracle.order_shuffle(E[:len(odd)], first)
+        want = [int(odd[k]) for k in chain]
+        assert list(oracle.order_shuffle_available(E, odd, first)) == want
+        assert order_shuffle(E, first, odd) == want
+        # not the chain over the listed images' own matrices
+        assert want != [int(odd[k]) for k in oracle.order_shuffle(E[odd], first)]
+
+
+def test_osart_available_subsets_follow_full_count(oracle):
+    """update 0 means N: with half the images listed no subset ever fills and
+    the volume is untouched (err[0] = 0); with update 4 over 10 listed images
+    the two full subsets run and the last two images are never projected."""
+    W, H, D = 20, 4, 8
+    E = oracle.tilt_eulers(np.arange(-60, 61, 6.0))          # 21 images
+    exps = oracle.phantom_project(oracle.ellipsoids(8, 3, W, H, D), E, W, H).astype(np.float32)
+    g = oracle.make_geom(W, H, D, nthreads=4)
+    even = oracle.available_indexes(len(E), 11)              # 10 images
+    vol = np.zeros((D, H, W), np.float32)
+    err = oracle.ossart(g, vol, exps, E, 2, 1.0, 0, available=even)
+    assert not vol.any() and (err[:, 0] == 0).all()
+    got = np.zeros((D, H, W), np.float32)
+    err = oracle.ossart(g, got, exps, E, 1, 1.0, 4, first_index=2, available=even)
+    order = oracle.order_shuffle_available(E, even, 2)
+    ref = np.zeros((D, H, W), np.float32)
+    factor = float(np.float32(1.0 / 4))
+    som = 0.0
+    for s in (order[0:4], order[4:8]):
+        diffs, errs, _, _ = oracle.project_and_difference(g, ref, exps[s], E[s], factor)
+        oracle.backproject(g, ref, diffs, E[s])
+        som += errs.sum()
+    assert np.array_equal(got, ref)
+    assert err[0, 0] == pytest.approx(np.sqrt(som / (len(E) * W * H)), rel=1e-12)

@@ -133,6 +133,10 @@ struct tj_ctx {
     void* wspace = nullptr;
     size_t wspace_size = 0;
 
+    // ReconstructionParameters.getAvailableIndexes: the images the fused loops
+    // walk (empty: all)
+    std::vector<int> avail;
+
     // FISTA wrapper (FistaProjector3D): volume at the start of the iteration, t
     int fista = 0;
     double fista_t = 0.0;
@@ -463,14 +467,31 @@ void tj_order_shuffle(const double* euler, int n, int first, int* order)
     }
 }
 
+// The images a fused loop walks: the available list, or all.
+static int num_available(const tj_ctx* c) { return c->avail.empty() ? c->nproj : (int)c->avail.size(); }
+
+static int check_available(tj_ctx* c, const char* who)
+{
+    if ((int)c->avail.size() > c->nproj)
+        return fail(TJ_EINVAL, "%s: %zu available images of %d", who, c->avail.size(), c->nproj);
+    for (int v : c->avail)
+        if (v < 0 || v >= c->nproj) return fail(TJ_EINVAL, "%s: available image %d of %d", who, v, c->nproj);
+    return TJ_OK;
+}
+
+// orderShuffleART (:735-781) over the available images: the chain runs over
+// positions 0..nav-1 with the Euler matrix of image im (getEulerMatrix(im),
+// :741; not of the image the position names), and emits availableImages[im].
 static void order_shuffle(const tj_ctx* c, int first, std::vector<int>& order)
 {
-    const int n = c->nproj;
+    const int n = num_available(c);
     std::vector<double> e((size_t)9 * n);
     for (int p = 0; p < n; ++p)
         for (int k = 0; k < 9; ++k) e[(size_t)9 * p + k] = c->hpp[p].e[k];
     order.assign(n, 0);
     if (n > 0) tj_order_shuffle(e.data(), n, first, order.data());
+    if (!c->avail.empty())
+        for (int& o : order) o = c->avail[o];
 }
 
 extern "C" {
@@ -1242,6 +1263,41 @@ static int end_hooks(tj_ctx* c)
     return fista_end(c);
 }
 
+int tj_available_indexes(int nproj, int projection_type, int* out)
+{
+    if (nproj < 0 || (nproj > 0 && !out)) return fail(TJ_EINVAL, "tj_available_indexes: bad arguments");
+    // ReconstructionParameters.getAvailableIndexes (:212-256): EVEN 11 / ODD 12
+    // keep that parity, at most nproj/2 images; any other type keeps all.
+    std::vector<char> take(nproj);
+    int cnt = 0;
+    for (int i = 0; i < nproj; ++i) {
+        if (projection_type == 11 || projection_type == 12) {
+            const bool par = (projection_type == 11) ? (i % 2 == 0) : (i % 2 == 1);
+            take[i] = par;
+            if (par) ++cnt;
+            if (cnt > nproj / 2) {
+                take[i] = 0;
+                --cnt;
+            }
+        } else {
+            take[i] = 1;
+            ++cnt;
+        }
+    }
+    int k = 0;
+    for (int i = 0; i < nproj; ++i)
+        if (take[i]) out[k++] = i;
+    return cnt;
+}
+
+int tj_set_available_indexes(tj_ctx* c, const int* indexes, int n)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_set_available_indexes: null context");
+    if (n < 0 || (n > 0 && !indexes)) return fail(TJ_EINVAL, "tj_set_available_indexes: bad list");
+    c->avail.assign(indexes, indexes + n);
+    return TJ_OK;
+}
+
 int tj_set_bayesian(tj_ctx* c, int on, double alpha, double lambda)
 {
     if (!c) return fail(TJ_EINVAL, "tj_set_bayesian: null context");
@@ -1349,7 +1405,10 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
     const int n = c->nproj;
     if (n <= 0) return fail(TJ_ESTATE, "tj_run_ossart: no resident projections");
     if (iterations < 0 || update_nb < 0) return fail(TJ_EINVAL, "tj_run_ossart: negative count");
-    if (first_index < 0 || first_index >= n) return fail(TJ_EINVAL, "tj_run_ossart: first index %d of %d", first_index, n);
+    if ((rc = check_available(c, "tj_run_ossart"))) return rc;
+    const int nav = num_available(c);
+    if (first_index < 0 || (nav > 0 && first_index >= nav))
+        return fail(TJ_EINVAL, "tj_run_ossart: first index %d of %d", first_index, nav);
     TJ_HIP(hipSetDevice(c->device));
     const int update = (update_nb == 0) ? n : update_nb;
     if ((rc = ensure_slots(c, std::min(update, n)))) return rc;
@@ -1360,7 +1419,7 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
 
     std::vector<int> order;
     order_shuffle(c, first_index, order);
-    TJ_HIP(hipMemcpy(c->order, order.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    if (nav > 0) TJ_HIP(hipMemcpy(c->order, order.data(), sizeof(int) * nav, hipMemcpyHostToDevice));
     const float factorimg = (float)(lambda / update);
     const double completion_inc = (end_y - start_y) / (double)c->g.vh;
 
@@ -1378,8 +1437,11 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
         // projector.startOfIteration() / endOfIteration() around each
         // osartIteration (TomoReconstruction2.java:593-595)
         if ((rc = start_hooks(c))) return rc;
-        int diffIndex = 0;
-        for (int t = 0; t < n; ++t) {
+        // With an available list the loop walks it, but update (0 -> n), the
+        // flush test t == n-1 and the error normalisation keep the full count
+        // n (:639-644, :662, :726): a last partial subset is never projected.
+        int diffIndex = 0, done = 0;
+        for (int t = 0; t < nav; ++t) {
             ++diffIndex;
             if (diffIndex == update || t == n - 1) {
                 const int off = t - diffIndex + 1;
@@ -1414,6 +1476,7 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
                     resev.emplace_back(b, r);
                     bpev.emplace_back(r, d);
                 }
+                done = t + 1;
                 diffIndex = 0;
                 if (c->interrupted.load()) {
                     (void)hipStreamSynchronize(c->stream);
@@ -1421,12 +1484,13 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
                 }
             }
         }
-        TJ_HIP(hipMemcpyAsync(c->h_err, c->err, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        if (done > 0)
+            TJ_HIP(hipMemcpyAsync(c->h_err, c->err, sizeof(double) * done, hipMemcpyDeviceToHost, c->stream));
         TJ_HIP(hipStreamSynchronize(c->stream));
         // error statistics, in the order osartIteration accumulates them (:692-701, :726-730)
         double som = 0, errmin = 3.4028234663852886e38, errmax = 0;
         int imin = 0, imax = 0;
-        for (int t = 0; t < n; ++t) {
+        for (int t = 0; t < done; ++t) {
             const double e = c->h_err[t];
             som += e;
             if (e > errmax) {
@@ -1491,12 +1555,15 @@ int tj_run_wbp(tj_ctx* c, int start_y, int end_y)
     // the Bayesian projector's backproject adds nothing without differences (:90)
     if (c->bayes) return TJ_OK;
     TJ_HIP(hipSetDevice(c->device));
+    if ((rc = check_available(c, "tj_run_wbp"))) return rc;
     if ((rc = ensure_slots(c, n))) return rc;
     if ((rc = ensure_iota(c, n))) return rc;
     c->added.clear();
     c->have_diff = c->have_proj = false;
-    std::vector<int> idx(n);
-    for (int k = 0; k < n; ++k) idx[k] = k;
+    // TomoReconstruction2.WBP walks getAvailableIndexes (:500)
+    std::vector<int> idx(num_available(c));
+    for (int k = 0; k < (int)idx.size(); ++k) idx[k] = c->avail.empty() ? k : c->avail[k];
+    if (idx.empty()) return TJ_OK;
     hipEvent_t t0 = nullptr, t1 = nullptr, tb = nullptr;
     if (c->timing_on) {
         t0 = ev(c, 0);
@@ -1504,11 +1571,16 @@ int tj_run_wbp(tj_ctx* c, int start_y, int end_y)
         t1 = ev(c, 2);
         TJ_HIP(hipEventRecord(t0, c->stream));
     }
+    const int* slots = c->iota;
+    if (!c->avail.empty()) {
+        TJ_HIP(hipMemcpy(c->slot_proj, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
+        slots = c->slot_proj;
+    }
     const float* images;
     const int* index;
-    if ((rc = prepare_originals(c, c->iota, idx, &images, &index))) return rc;
+    if ((rc = prepare_originals(c, slots, idx, &images, &index))) return rc;
     if (c->timing_on) TJ_HIP(hipEventRecord(tb, c->stream));
-    if ((rc = do_backproject(c, c->iota, idx, images, index, start_y, end_y))) return rc;
+    if ((rc = do_backproject(c, slots, idx, images, index, start_y, end_y))) return rc;
     if (c->timing_on) {
         TJ_HIP(hipEventRecord(t1, c->stream));
         TJ_HIP(hipEventSynchronize(t1));

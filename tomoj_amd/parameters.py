@@ -13,18 +13,25 @@ normalised on the device as ``ts.getPixels(i)`` does for unaligned images
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
 import numpy as np
 
+from ._lib import call
 from .projector import BayesianVoxelProjector3DHIP, TVMProjector3DHIP, VoxelProjector3DHIP
 from .reconstruction import tilt_eulers
 
 BP = 0
 WBP = 1
 OSSART = 2
+# projection subsets (ReconstructionParameters.java:17-20)
+ALL_PROJECTIONS = 10
+EVEN_PROJECTIONS = 11
+ODD_PROJECTIONS = 12
+DEFINED_PROJECTIONS = 13
 TVM = 3        # AdvancedReconstructionParameters.java:10
 BAYESIAN = 4   # AdvancedReconstructionParameters.java:11
 
@@ -78,6 +85,8 @@ class ReconstructionParameters:
         self.longObjectCompensation = False
         self.reconstructionCenterModifiers: Optional[Sequence[float]] = None
         self._tempNbCall = 0
+        self.projectionType = 0
+        self.availableIndexes: Optional[List[int]] = None
 
     # factories (:279-311)
     @staticmethod
@@ -118,6 +127,34 @@ class ReconstructionParameters:
 
     def isFista(self) -> bool:
         return self.fista
+
+    # projection subsets (:174-261)
+    def getFscType(self) -> int:
+        return self.projectionType
+
+    def setFscType(self, fscType: int) -> None:
+        self.projectionType = int(fscType)
+        self.availableIndexes = None
+
+    getProjectionType = getFscType
+    setProjectionType = setFscType
+
+    def getAvailableIndexes(self, ts: "TiltSeriesData") -> List[int]:
+        """:212-256, computed once and kept: EVEN / ODD keep that parity (at
+        most N/2 images), any other type all of them."""
+        if self.availableIndexes is None:
+            n = len(ts.tilts)
+            out = (C.c_int * max(n, 1))()
+            k = call("tj_available_indexes", n, int(self.projectionType), out)
+            self.availableIndexes = [int(out[i]) for i in range(k)]
+        return self.availableIndexes
+
+    def setAvailableIndexes(self, availableIndexes: Sequence[int]) -> None:
+        """:258-261.  TomoJ also sets reconstructionType (not projectionType) to
+        DEFINED_PROJECTIONS, which sends doCPUrec down its default branch (the
+        regularisation loop, OS-SART's with the projector's hooks)."""
+        self.availableIndexes = [int(v) for v in availableIndexes]
+        self.reconstructionType = DEFINED_PROJECTIONS
 
     def volume_centers(self) -> List[float]:
         """TomoReconstruction2 centres (n-1)/2 (:63-71) shifted by the centre
@@ -259,12 +296,16 @@ class ReconstructionThreadManager:
         if self.projector is None or params is not self.params:
             self.createRec2(params)
         P = self.projector
+        avail = params.getAvailableIndexes(self.ts)
+        P.setAvailableIndexes(None if len(avail) == len(self.ts.tilts) and avail == list(range(len(avail)))
+                              else avail)
         if params.reconstructionType in (BP, WBP):
             P.run_wbp()
             return None
-        if params.reconstructionType in (OSSART, TVM, BAYESIAN):
-            # TVM and BAYESIAN go through TomoReconstruction2.regularization
-            # (:783-878), the OS-SART loop with the projector's own iteration hooks.
+        if params.reconstructionType in (OSSART, TVM, BAYESIAN, DEFINED_PROJECTIONS):
+            # TVM, BAYESIAN and DEFINED_PROJECTIONS go through doCPUrec's default
+            # branch, TomoReconstruction2.regularization (:783-878): the OS-SART
+            # loop with the projector's own iteration hooks.
             return P.run_ossart(params.nbIterations, params.relaxationCoefficient, params.updateNb,
                                 first_index=first_index)
         raise ValueError(f"reconstruction type {params.reconstructionType} is not on this path")

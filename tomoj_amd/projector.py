@@ -260,6 +260,15 @@ class VoxelProjector3DHIP:
         call("tj_interrupt", self._ctx)
 
     # ------------------------------------------------------------------ fused loops
+    def setAvailableIndexes(self, indexes=None) -> None:
+        """The images run_ossart / run_wbp walk (ReconstructionParameters.getAvailableIndexes);
+        None restores all."""
+        if indexes is None:
+            call("tj_set_available_indexes", self._ctx, None, 0)
+            return
+        a = np.ascontiguousarray(indexes, dtype=np.int32)
+        call("tj_set_available_indexes", self._ctx, _ptr(a), int(a.size))
+
     def run_ossart(self, iterations: int, relaxation: float, update: int, first_index: int = 0,
                    startY: int = 0, endY: Optional[int] = None) -> np.ndarray:
         """TomoReconstruction2.OSSART over the resident projections (iterations x 5 err1 rows)."""

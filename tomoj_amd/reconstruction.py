@@ -11,7 +11,7 @@ must give the same volume.
 from __future__ import annotations
 
 import math
-from typing import List, Sequence
+from typing import List, Optional, Sequence
 
 import numpy as np
 
@@ -38,11 +38,13 @@ def tilt_eulers(tilts_deg: Sequence[float], tilt_axis_deg: float = 0.0, axis_ver
     return np.stack([tilt_euler(t, tilt_axis_deg, axis_vertical) for t in tilts_deg])
 
 
-def order_shuffle(eulers: np.ndarray, first_index: int) -> List[int]:
+def order_shuffle(eulers: np.ndarray, first_index: int, available: Optional[Sequence[int]] = None) -> List[int]:
     """orderShuffleART (:735-781) with the first pick fixed: each next image is the
-    remaining one with the smallest |row2(E_prev) . row2(E_o)|."""
+    remaining one with the smallest |row2(E_prev) . row2(E_o)|.  With a list of
+    available images TomoJ runs the chain over positions 0..len-1 using the
+    Euler matrices of images 0..len-1 (:741) and emits the listed images."""
     rows = np.asarray(eulers, np.float64).reshape(-1, 3, 3)[:, 2, :]
-    n = rows.shape[0]
+    n = rows.shape[0] if available is None else len(available)
     todo = [True] * n
     order: List[int] = []
     im = first_index
@@ -63,21 +65,24 @@ def order_shuffle(eulers: np.ndarray, first_index: int) -> List[int]:
         if todo[im]:
             order.append(im)
             todo[im] = False
-    return order
+    return order if available is None else [int(available[k]) for k in order]
 
 
 def osart_iteration(projector, eulers: np.ndarray, vol_w: int, vol_h: int, relaxation: float,
-                    update_nb: int, first_index: int, startY: int, endY: int) -> List[float]:
-    """TomoReconstruction2.osartIteration (:636-733); returns err1 (5 values)."""
+                    update_nb: int, first_index: int, startY: int, endY: int,
+                    available: Optional[Sequence[int]] = None) -> List[float]:
+    """TomoReconstruction2.osartIteration (:636-733); returns err1 (5 values).
+    ``available``: the images to walk (getAvailableIndexes); update, the flush
+    at position n-1 and the error normalisation keep the full count n."""
     projector.clearAllProjections()
     n = len(eulers)
-    indexes = order_shuffle(eulers, first_index)
+    indexes = order_shuffle(eulers, first_index, available)
     update = n if update_nb == 0 else update_nb
     factorimg = float(np.float32(relaxation / update))
     diff_index = 0
     som, errmin, errmax = 0.0, JAVA_FLOAT_MAX, 0.0
     imin = imax = 0
-    for t in range(n):
+    for t in range(len(indexes)):
         projector.addProjection(indexes[t])
         diff_index += 1
         if diff_index == update or t == n - 1:
@@ -99,26 +104,30 @@ def osart_iteration(projector, eulers: np.ndarray, vol_w: int, vol_h: int, relax
 
 
 def ossart(projector, eulers: np.ndarray, iterations: int, relaxation: float, update_nb: int,
-           first_index: int = 0, startY: int = 0, endY=None) -> np.ndarray:
+           first_index: int = 0, startY: int = 0, endY=None,
+           available: Optional[Sequence[int]] = None) -> np.ndarray:
     """TomoReconstruction2.OSSART (:548-634) through the projector interface."""
     endY = projector.H if endY is None else endY
     out = []
     for _ in range(iterations):
         projector.startOfIteration()
         out.append(osart_iteration(projector, eulers, projector.W, projector.H, relaxation, update_nb,
-                                   first_index, startY, endY))
+                                   first_index, startY, endY, available))
         projector.endOfIteration()
     return np.array(out, np.float64).reshape(iterations, 5)
 
 
-def wbp(projector, n: int, update_nb: int = 0, startY: int = 0, endY=None) -> None:
-    """TomoReconstruction2.WBP(ts, projector, params, startY, endY) (:499-538)."""
+def wbp(projector, n: int, update_nb: int = 0, startY: int = 0, endY=None,
+        available: Optional[Sequence[int]] = None) -> None:
+    """TomoReconstruction2.WBP(ts, projector, params, startY, endY) (:499-538)
+    over getAvailableIndexes (all n images by default)."""
     endY = projector.H if endY is None else endY
+    images = list(range(n)) if available is None else [int(v) for v in available]
     index = 0
-    for i in range(n):
-        projector.addProjection(i)
+    for i, im in enumerate(images):
+        projector.addProjection(im)
         index += 1
-        if index >= update_nb or i == n - 1:
+        if index >= update_nb or i == len(images) - 1:
             projector.backproject(startY, endY)
             projector.clearAllProjections()
             index = 0
