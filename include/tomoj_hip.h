@@ -218,6 +218,22 @@ int tj_available_indexes(int nproj, int projection_type, int *out);
  * position in the list.  Checked against the loaded projections at run time. */
 int tj_set_available_indexes(tj_ctx *ctx, const int *indexes, int n);
 
+/* Fourier shell correlation of two reconstructions:
+ * ResolutionEstimation.fsc(recFSCeven, recFSCodd)
+ * (tomography/ResolutionEstimation.java:456-557) of the resident volumes of
+ * `even` and `odd` (same size, same device).  out = fsc[4][W/2+1] row by row:
+ * frequency i/W, FSC, the 2/sqrt(n) threshold curve, mean |E - O| per shell.
+ * Both volumes are multiplied by TomoJ's soft spherical mask (radius = the
+ * slice count), transformed in 3D in FP32 and the shells summed in FP64 (the
+ * atomics make the last bits of the sums order-dependent).  Needs 24 bytes
+ * per voxel of scratch device memory during the call; W/2+1 <= 1638 shells,
+ * H and D <= 65535. */
+int tj_fsc(tj_ctx *even, tj_ctx *odd, double *out);
+
+/* The same for two host volumes in ImageStack layout [D][H][W] (the static
+ * fsc(ImagePlus, ImagePlus)); copies them to `device` for the call. */
+int tj_fsc_volumes(int device, const float *even_zyx, const float *odd_zyx, int W, int H, int D, double *out);
+
 /* Bayesian regularisation: the projector getProjector builds for the BAYESIAN
  * type (AdvancedReconstructionParameters.java:194-203) is
  * BayesianVoxelProjector3D (TomoJ_AdvancedReconstruction/.../projectors/

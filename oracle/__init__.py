@@ -479,6 +479,50 @@ def bayes_iteration(g: Geom, vol, projs, eulers, alpha, lam, first_index=0):
     return [math.sqrt(som / (n * wh)), math.sqrt(emin / wh), imin, math.sqrt(emax / wh), imax]
 
 
+def fsc_mask(D, H, W) -> np.ndarray:
+    """ResolutionEstimation.fsc's mask (ResolutionEstimation.java:470-489), (D, H, W)
+    float32: 1 within radius D of (W/2, H/2, D/2), a raised cosine over the next
+    20 voxels, 0 beyond; dist^2 = dx^2 + (dz^2 + dy^2)."""
+    z, y, x = np.meshgrid(np.arange(D), np.arange(H), np.arange(W), indexing="ij")
+    dz = (z - D / 2.0) ** 2
+    dy = (y - H / 2.0) ** 2
+    dist = np.sqrt((x - W / 2.0) ** 2 + (dz + dy))
+    taper = (1 + np.cos((dist - D) * math.pi / 20)).astype(np.float32) * np.float32(0.5)
+    return np.where(dist < D, np.float32(1), np.where(dist < D + 20, taper, np.float32(0))).astype(np.float32)
+
+
+def fsc(even, odd) -> np.ndarray:
+    """ResolutionEstimation.fsc(even, odd) (:456-557) in double precision: the masked
+    volumes (float products) transformed with an FP64 FFT, the shell index
+    round(|f| W) from convertToFrequency (:699-701), and per shell the sums of
+    Re(E conj O), |E|^2, |O|^2, |E - O| and the count; returns fsc[4][W/2+1]."""
+    even = _c(even, np.float32)
+    odd = _c(odd, np.float32)
+    D, H, W = even.shape
+    m = fsc_mask(D, H, W)
+    E = np.fft.fftn((even * m).astype(np.float64))
+    O = np.fft.fftn((odd * m).astype(np.float64))
+
+    def freq(n):
+        i = np.arange(n)
+        return np.where(i <= (n >> 1), i, i - n) / n
+
+    fz, fy, fx = np.meshgrid(freq(D) ** 2, freq(H) ** 2, freq(W) ** 2, indexing="ij")
+    wi = np.floor(np.sqrt(fx + fy + fz) * W + 0.5).astype(np.int64).ravel()
+    R = W // 2 + 1
+    keep = wi < R
+    wi = wi[keep]
+    e = E.ravel()[keep]
+    o = O.ravel()[keep]
+    num = np.bincount(wi, (e * np.conj(o)).real, R)
+    de = np.bincount(wi, np.abs(e) ** 2, R)
+    do = np.bincount(wi, np.abs(o) ** 2, R)
+    err = np.bincount(wi, np.abs(e - o), R)
+    cnt = np.bincount(wi, None, R).astype(np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.array([np.arange(R) / W, num / np.sqrt(de * do), 2 / np.sqrt(cnt), err / cnt])
+
+
 # --------------------------------------------------------------------------
 # synthetic data
 # --------------------------------------------------------------------------

@@ -743,3 +743,73 @@ def test_osart_available_subsets_follow_full_count(oracle):
         som += errs.sum()
     assert np.array_equal(got, ref)
     assert err[0, 0] == pytest.approx(np.sqrt(som / (len(E) * W * H)), rel=1e-12)
+
+
+# --------------------------------------------------------------------------
+# Fourier shell correlation (ResolutionEstimation.fsc)
+# --------------------------------------------------------------------------
+
+def test_fsc_mask_closed_form(oracle):
+    """1 inside radius D, (1 + cos((d - D) pi / 20)) / 2 over the next 20
+    voxels (0.5 at d = D + 10), 0 beyond."""
+    D, H, W = 4, 3, 64
+    m = oracle.fsc_mask(D, H, W)
+    zc, yc, xc = 2, 1, 32                      # (W/2, H/2, D/2) rounded to voxels
+    assert m[zc, yc, xc + 3] == 1.0             # dist sqrt(9 + 0.25) < 4
+    assert m[zc, 1, xc + 14] == np.float32((1 + np.cos((np.sqrt(14 ** 2 + 0.25) - 4) * np.pi / 20)) / 2)
+    assert m[zc, 1, xc + 30] == 0.0
+    mm = oracle.fsc_mask(4, 4, 64)              # H even: dy = 0 on row 2
+    assert mm[2, 2, 32 + 14] == np.float32(0.5)
+
+
+def test_fsc_known_answers(oracle):
+    """Identical halves correlate to 1 with no error, opposite ones to -1,
+    a scale changes nothing; the first column is i/W, the third 2/sqrt(n) with
+    n = 1 for the DC shell; the shell counts cover every frequency below R."""
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal((8, 12, 16)).astype(np.float32)
+    f = oracle.fsc(v, v)
+    R = 16 // 2 + 1
+    assert f.shape == (4, R)
+    assert np.allclose(f[1], 1.0, atol=1e-12) and np.all(f[3] == 0)
+    assert np.allclose(oracle.fsc(v, -v)[1], -1.0, atol=1e-12)
+    assert np.allclose(oracle.fsc(v, 3 * v)[1], 1.0, atol=1e-12)
+    assert np.array_equal(f[0], np.arange(R) / 16)
+    assert f[2][0] == 2.0
+    cnt = (2 / f[2]) ** 2
+    assert cnt.sum() == pytest.approx(np.sum(np.round(np.sqrt(
+        np.add.outer(np.add.outer(np.fft.fftfreq(8) ** 2, np.fft.fftfreq(12) ** 2), np.fft.fftfreq(16) ** 2))
+        * 16 + 1e-9) < R))
+
+
+def test_fsc_transform_convention(oracle):
+    """The shell sums use the forward, unscaled transform over all three axes:
+    for a tiny volume the DC shell's |E|^2 is the squared masked sum, and the
+    whole FSC matches a direct DFT evaluation."""
+    rng = np.random.default_rng(9)
+    D, H, W = 2, 3, 4
+    a = rng.standard_normal((D, H, W)).astype(np.float32)
+    b = rng.standard_normal((D, H, W)).astype(np.float32)
+    m = oracle.fsc_mask(D, H, W)
+    z, y, x = np.meshgrid(np.arange(D), np.arange(H), np.arange(W), indexing="ij")
+
+    def dft(v):
+        out = np.zeros((D, H, W), complex)
+        for kz in range(D):
+            for ky in range(H):
+                for kx in range(W):
+                    ph = np.exp(-2j * np.pi * (kz * z / D + ky * y / H + kx * x / W))
+                    out[kz, ky, kx] = np.sum((v * m).astype(np.float64) * ph)
+        return out
+
+    E, O = dft(a), dft(b)
+    f = oracle.fsc(a, b)
+    # shell of every (kz, ky, kx), then the FSC of each shell
+    fr = lambda n: np.where(np.arange(n) <= n // 2, np.arange(n), np.arange(n) - n) / n
+    wi = np.floor(np.sqrt(np.add.outer(np.add.outer(fr(D) ** 2, fr(H) ** 2), fr(W) ** 2)) * W + 0.5)
+    for s in range(W // 2 + 1):
+        sel = wi == s
+        if not sel.any():
+            continue
+        want = (E[sel] * np.conj(O[sel])).real.sum() / np.sqrt((abs(E[sel]) ** 2).sum() * (abs(O[sel]) ** 2).sum())
+        assert f[1][s] == pytest.approx(want, rel=1e-9)

@@ -111,6 +111,8 @@ _SIGS = {
     "tj_set_tvm": (C.c_int, [_P, C.c_int, C.c_double, C.c_double, C.c_double]),
     "tj_set_bayesian": (C.c_int, [_P, C.c_int, C.c_double, C.c_double]),
     "tj_available_indexes": (C.c_int, [C.c_int, C.c_int, _P]),
+    "tj_fsc": (C.c_int, [_P, _P, _P]),
+    "tj_fsc_volumes": (C.c_int, [C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, _P]),
     "tj_set_available_indexes": (C.c_int, [_P, _P, C.c_int]),
     "tj_tv_denoise": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.c_int]),
     "tj_get_completion": (C.c_double, [_P]),

@@ -46,6 +46,8 @@ hipError_t weighting_init(hipStream_t, int, int, double, const double*, const do
 hipError_t ramp_init(hipStream_t, int, int, int, double, double, float*);
 hipError_t weighting_apply(hipStream_t, int, int, int, const float*, const int*, const float*, float*, void*, size_t);
 size_t weighting_workspace(int, int, int);
+size_t fsc_workspace(int, int, int);
+hipError_t fsc(hipStream_t, const float*, const float*, int, int, int, void*, size_t, double*);
 }  // namespace tjk
 
 // ---------------------------------------------------------------------------
@@ -1261,6 +1263,86 @@ static int end_hooks(tj_ctx* c)
     if (rc) return rc;
     if ((rc = bayes_end(c))) return rc;
     return fista_end(c);
+}
+
+// ResolutionEstimation.fsc (:456-557) of two device volumes on `st`: shell
+// sums on the device, the four curves of fsc[4][W/2+1] on the host (:545-550).
+static int fsc_run(int device, hipStream_t st, const float* even, const float* odd, int W, int H, int D,
+                   double* out)
+{
+    const int R = W / 2 + 1;
+    const size_t ws_size = tjk::fsc_workspace(W, H, D);
+    void* ws = nullptr;
+    double* sums = nullptr;
+    if (hipMalloc(&ws, ws_size) != hipSuccess || hipMalloc((void**)&sums, sizeof(double) * 5 * R) != hipSuccess) {
+        (void)hipGetLastError();
+        if (ws) (void)hipFree(ws);
+        return fail(TJ_ENOMEM, "tj_fsc: cannot allocate %zu bytes of workspace", ws_size);
+    }
+    std::vector<double> h(5 * (size_t)R);
+    hipError_t e = tjk::fsc(st, even, odd, W, H, D, ws, ws_size, sums);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), sums, sizeof(double) * 5 * R, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(ws);
+    (void)hipFree(sums);
+    if (e != hipSuccess) return fail(TJ_EDEVICE, "tj_fsc: %s", hipGetErrorString(e));
+    (void)device;
+    for (int i = 0; i < R; ++i) {
+        const double num = h[i], de = h[R + i], dodd = h[2 * R + i], err = h[3 * R + i], cnt = h[4 * R + i];
+        out[i] = (double)i / W;
+        out[R + i] = num / std::sqrt(de * dodd);
+        out[2 * R + i] = 2 / std::sqrt(cnt);
+        out[3 * R + i] = err / cnt;
+    }
+    return TJ_OK;
+}
+
+static int fsc_check_dims(int W, int H, int D)
+{
+    if (W <= 0 || H <= 0 || D <= 0) return fail(TJ_EINVAL, "tj_fsc: empty volume");
+    if (H > 65535 || D > 65535 || sizeof(double) * 5 * (W / 2 + 1) > 64 * 1024)
+        return fail(TJ_EINVAL, "tj_fsc: volume %d x %d x %d beyond the supported size", W, H, D);
+    return TJ_OK;
+}
+
+int tj_fsc(tj_ctx* even, tj_ctx* odd, double* out)
+{
+    if (!even || !odd || !out) return fail(TJ_EINVAL, "tj_fsc: null argument");
+    const TjGeomDev &a = even->g, &b = odd->g;
+    if (a.vw != b.vw || a.vh != b.vh || a.vd != b.vd) return fail(TJ_EINVAL, "tj_fsc: volume sizes differ");
+    if (even->device != odd->device) return fail(TJ_EINVAL, "tj_fsc: the volumes are on different devices");
+    int rc = fsc_check_dims(a.vw, a.vh, a.vd);
+    if (rc) return rc;
+    TJ_HIP(hipSetDevice(even->device));
+    TJ_HIP(hipStreamSynchronize(odd->stream));
+    return fsc_run(even->device, even->stream, even->vol, odd->vol, a.vw, a.vh, a.vd, out);
+}
+
+int tj_fsc_volumes(int device, const float* even_zyx, const float* odd_zyx, int W, int H, int D, double* out)
+{
+    if (!even_zyx || !odd_zyx || !out) return fail(TJ_EINVAL, "tj_fsc_volumes: null argument");
+    int rc = fsc_check_dims(W, H, D);
+    if (rc) return rc;
+    TJ_HIP(hipSetDevice(device));
+    const size_t n = (size_t)W * H * D;
+    float *stage = nullptr, *ev = nullptr, *od = nullptr;
+    hipError_t e = hipMalloc((void**)&stage, n * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&ev, n * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&od, n * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(stage, even_zyx, n * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = tjk::zyx_to_yzx(nullptr, stage, ev, W, D, H, 0);
+    if (e == hipSuccess) e = hipMemcpy(stage, odd_zyx, n * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = tjk::zyx_to_yzx(nullptr, stage, od, W, D, H, 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) rc = fsc_run(device, nullptr, ev, od, W, H, D, out);
+    if (stage) (void)hipFree(stage);
+    if (ev) (void)hipFree(ev);
+    if (od) (void)hipFree(od);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(e == hipErrorOutOfMemory ? TJ_ENOMEM : TJ_EDEVICE, "tj_fsc_volumes: %s", hipGetErrorString(e));
+    }
+    return rc;
 }
 
 int tj_available_indexes(int nproj, int projection_type, int* out)

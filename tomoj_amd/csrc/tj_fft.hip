@@ -205,10 +205,156 @@ static hipError_t row_transform(hipStream_t st, const void* in, const int* idx, 
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Fourier shell correlation, ResolutionEstimation.fsc(even, odd)
+// (tomography/ResolutionEstimation.java:456-557).  Each volume is multiplied by
+// a soft spherical mask, transformed in 3D (FP32: x rows, a per-plane
+// transpose, z rows, a full transpose, y rows; the result is laid out
+// [x][z][y]), and the shell sums are accumulated in FP64.
+// ---------------------------------------------------------------------------
+
+// H = data * mask (:470-497): mask 1 within radius D (the slice count) of the
+// centre (W/2, H/2, D/2), a raised-cosine fall-off over the next 20 voxels, 0
+// beyond; dist^2 summed as dx^2 + (dz^2 + dy^2).  Volume in the device layout
+// [y][z][x].
+__global__ __launch_bounds__(256) void k_fsc_load(const float* __restrict__ vol, float2* __restrict__ out, int W,
+                                                  int H, int D)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int z = blockIdx.y, y = blockIdx.z;
+    if (x >= W) return;
+    double dz = z - D / 2.0;
+    dz *= dz;
+    double dy = y - H / 2.0;
+    dy *= dy;
+    double dist = x - W / 2.0;
+    dist *= dist;
+    dist += dz + dy;
+    dist = sqrt(dist);
+    float m = 0.0f;
+    if (dist < D) m = 1.0f;
+    else if (dist < D + 20) m = (float)(1 + cos((dist - D) * M_PI / 20)) * .5f;
+    const size_t k = ((size_t)y * D + z) * W + x;
+    out[k] = make_float2(vol[k] * m, 0.0f);
+}
+
+// convertToFrequency (:699-701)
+__device__ inline double fsc_freq(int i, int n) { return (double)((i <= (n >> 1)) ? i : i - n) / n; }
+
+// Shell sums over the transforms E and O ([x][z][y]): per shell, the real part
+// of E conj(O) (formed in float as multConjSecond does), |E|^2 and |O|^2 (the
+// float sums of squares, square-rooted and squared again in double), |E - O|
+// and the count.  Block partials in LDS, then FP64 atomics into sums[5][R].
+__global__ __launch_bounds__(256) void k_fsc_shells(const float2* __restrict__ E, const float2* __restrict__ O,
+                                                    int W, int H, int D, int R, double* __restrict__ sums)
+{
+    extern __shared__ double sh[];  // [5][R]
+    for (int k = threadIdx.x; k < 5 * R; k += blockDim.x) sh[k] = 0.0;
+    __syncthreads();
+    const size_t n = (size_t)W * H * D;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+        const int y = (int)(k % H);
+        const size_t r = k / H;
+        const int z = (int)(r % D), x = (int)(r / D);
+        double tz = fsc_freq(z, D);
+        tz *= tz;
+        double ty = fsc_freq(y, H);
+        ty *= ty;
+        double tx = fsc_freq(x, W);
+        tx *= tx;
+        const double w = sqrt(tx + ty + tz);
+        const long wi = (long)floor(w * W + 0.5);  // Math.round
+        if (wi >= R) continue;
+        const float2 e = E[k], o = O[k];
+        const double ae = sqrt((double)(e.x * e.x + e.y * e.y));
+        const double ao = sqrt((double)(o.x * o.x + o.y * o.y));
+        const float re = e.x * o.x + e.y * o.y;
+        const double d0 = (double)(e.x - o.x), d1 = (double)(e.y - o.y);
+        atomicAdd(&sh[wi], (double)re);
+        atomicAdd(&sh[R + wi], ae * ae);
+        atomicAdd(&sh[2 * R + wi], ao * ao);
+        atomicAdd(&sh[3 * R + wi], sqrt(d0 * d0 + d1 * d1));
+        atomicAdd(&sh[4 * R + wi], 1.0);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 5 * R; k += blockDim.x)
+        if (sh[k] != 0.0) atomicAdd(&sums[k], sh[k]);
+}
+
+// Row transforms of many rows: the DFT path's grid.y is bounded, so rows go in
+// chunks there.
+static hipError_t rows_fft(hipStream_t st, const float2* in, float2* out, const float2* tw, int n, size_t nrows)
+{
+    if (pow2(n) && n <= 4096) {
+        const size_t lds = sizeof(float2) * 2 * n;
+        for (size_t r0 = 0; r0 < nrows; r0 += 0x7fffffff) {
+            const size_t m = std::min<size_t>(nrows - r0, 0x7fffffff);
+            k_fft_row<kComplex><<<(unsigned)m, 256, lds, st>>>(in + r0 * n, nullptr, out + r0 * n, nullptr, tw, n,
+                                                               ilog2(n), 1, 0, 0, 1.0f);
+        }
+    } else {
+        for (size_t r0 = 0; r0 < nrows; r0 += 65535) {
+            const size_t m = std::min<size_t>(nrows - r0, 65535);
+            k_dft_row<kComplex><<<dim3((n + 255) / 256, (unsigned)m), 256, 0, st>>>(in + r0 * n, nullptr,
+                                                                                   out + r0 * n, nullptr, tw, n, 1,
+                                                                                   0, 0, 1.0f);
+        }
+    }
+    return hipGetLastError();
+}
+
 }  // namespace tjf
 
 namespace tjk {
 using namespace tjf;
+
+// Workspace of tjk::fsc: three complex volumes and the twiddle tables.
+size_t fsc_workspace(int W, int H, int D)
+{
+    const size_t n = (size_t)W * H * D;
+    return sizeof(float2) * (3 * n + (size_t)W + H + D) + 256;
+}
+
+// ResolutionEstimation.fsc (:456-557) of two device volumes ([y][z][x]) into
+// sums[5][R] (R = W/2 + 1; zeroed here): the caller forms the curves.
+hipError_t fsc(hipStream_t st, const float* even, const float* odd, int W, int H, int D, void* ws, size_t ws_size,
+               double* sums)
+{
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    if (ws_size < fsc_workspace(W, H, D)) return hipErrorInvalidValue;
+    if (H > 65535 || D > 65535 || sizeof(double) * 5 * (W / 2 + 1) > 64 * 1024) return hipErrorInvalidValue;
+    const size_t n = (size_t)W * H * D;
+    const int R = W / 2 + 1;
+    float2* A = (float2*)ws;
+    float2* B = A + n;
+    float2* X = B + n;
+    float2* tww = X + n;
+    float2* twh = tww + W;
+    float2* twd = twh + H;
+    k_twiddles<<<(W + 255) / 256, 256, 0, st>>>(W, tww);
+    k_twiddles<<<(H + 255) / 256, 256, 0, st>>>(H, twh);
+    k_twiddles<<<(D + 255) / 256, 256, 0, st>>>(D, twd);
+    hipError_t e = hipMemsetAsync(sums, 0, sizeof(double) * 5 * R, st);
+    if (e != hipSuccess) return e;
+    // per volume: mask -> A; x rows A -> B; planes [z][x] -> [x][z]: B -> A;
+    // z rows A -> B; [y][(x,z)] -> [(x,z)][y]: B -> A; y rows A -> result
+    // (even: X, odd: B, free again by then).
+    for (int v = 0; v < 2; ++v) {
+        float2* res = (v == 0) ? X : B;
+        k_fsc_load<<<dim3((W + 255) / 256, D, H), 256, 0, st>>>(v == 0 ? even : odd, A, W, H, D);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = rows_fft(st, A, B, tww, W, (size_t)H * D)) != hipSuccess) return e;
+        k_transpose<<<dim3((W + 31) / 32, (D + 31) / 32, H), dim3(32, 8), 0, st>>>(B, A, D, W);
+        if ((e = rows_fft(st, A, B, twd, D, (size_t)H * W)) != hipSuccess) return e;
+        k_transpose<<<dim3((unsigned)(((size_t)W * D + 31) / 32), (H + 31) / 32, 1), dim3(32, 8), 0, st>>>(
+            B, A, H, W * D);
+        if ((e = rows_fft(st, A, res, twh, H, (size_t)W * D)) != hipSuccess) return e;
+    }
+    const size_t lds = sizeof(double) * 5 * R;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 2048);
+    k_fsc_shells<<<blocks, 256, lds, st>>>(X, B, W, H, D, R, sums);
+    return hipGetLastError();
+}
 
 // workspace: two complex buffers of kMaxBatch images + twiddle tables (pw + ph entries)
 size_t weighting_workspace(int pw, int ph, int n)
