@@ -101,6 +101,47 @@ def cpu_baseline(w, P, E, oracle):
     }
 
 
+def regularisation_extras(w, P, tj, iters=3):
+    """SIRT, TVM (TomoJ's defaults: 5 TV steps, theta 25, g 1, lambda 0.1) and
+    Bayesian iterations on the workload's resident data, a TV step alone, and
+    the FSC of two resident volumes."""
+    import ctypes as C
+    res = {}
+    for key, setup in (("sirt", lambda: None), ("tvm", lambda: P.setTvm(5, 25.0, 1.0, 0.1)),
+                       ("bayesian", lambda: P.setBayesian(True, 1.0, 0.01))):
+        P.setTvm(0)
+        P.setBayesian(False)
+        setup()
+        P.reset_volume()
+        P.run_ossart(1, 1.0, 0)
+        t0 = time.perf_counter()
+        P.run_ossart(iters, 1.0, 0)
+        res[f"{key}_it_s"] = iters / (time.perf_counter() - t0)
+    P.setTvm(0)
+    P.setBayesian(False)
+    P.tv_denoise(25.0, 1.0, 0.1, 1)
+    t0 = time.perf_counter()
+    P.tv_denoise(25.0, 1.0, 0.1, 10)
+    t1 = time.perf_counter()
+    P.tv_denoise(25.0, 1.0, 0.1, 2)
+    t2 = time.perf_counter()
+    step = ((t1 - t0) - (t2 - t1)) / 8          # per step, set-up and flag read-back removed
+    nvox = w.W * w.H * w.D
+    res["tv_step_ms"] = step * 1e3
+    res["tv_step_GBs"] = 48.0 * nvox / step / 1e9   # 28 + 20 algorithmic bytes per voxel
+    Q = tj.VoxelProjector3DHIP((w.W, w.H, w.D))
+    try:
+        Q.set_volume(P.get_volume() * np.float32(0.9))
+        out = np.zeros((4, w.W // 2 + 1))
+        tj._lib.call("tj_fsc", P._ctx, Q._ctx, out.ctypes.data_as(C.c_void_p))
+        t0 = time.perf_counter()
+        tj._lib.call("tj_fsc", P._ctx, Q._ctx, out.ctypes.data_as(C.c_void_p))
+        res["fsc_ms"] = (time.perf_counter() - t0) * 1e3
+    finally:
+        Q.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -229,6 +270,11 @@ def main():
                                "roofline_frac": WL.algorithmic_bytes_per_iteration(c) * k / dtc / 1e9 / HBM_PEAK_GBS}
         log(f"[bench] SIRT 512^3: {k / dtc:.2f} it/s")
         Q.close()
+        out["regularisation"] = regularisation_extras(w, P, tj)
+        r = out["regularisation"]
+        log(f"[bench] {w.W}x{w.H}x{w.D}: SIRT {r['sirt_it_s']:.1f} it/s, TVM (5 TV steps) "
+            f"{r['tvm_it_s']:.1f} it/s, Bayesian {r['bayesian_it_s']:.1f} it/s, TV step {r['tv_step_ms']:.3f} ms "
+            f"({r['tv_step_GBs']:.0f} GB/s), FSC {r['fsc_ms']:.1f} ms")
 
     if not args.no_forms:
         from tomoj_amd import forms
