@@ -335,6 +335,17 @@ int tj_normalize_projections(tj_ctx *ctx, const double *tilt_deg, int min_size, 
 int tj_align_projections(tj_ctx *ctx, const double *transforms, const double *tilt_deg, int min_size,
                          int normalize, int fill, const double *stats_in, double *stats_out);
 
+/* The optional threshold of the projection supply: TiltSeries.getPixels applies
+ * applyThreshold (TiltSeries.java:682-684, :720-806) after the alignment when
+ * setThreshold(threshold, true) or setThresholdHisteresis(threshold, hysteresis,
+ * true) was called (:407-416).  On every resident projection, in place: the
+ * levels at the `threshold` and `hysteresis` fractions of ImageJ's 256-bin
+ * histogram of the image, values below the first take the fill value
+ * (TJ_FILL_NONE 0, TJ_FILL_AVG the image mean, TJ_FILL_NAN), values between the
+ * two take it when connected to such a value through index -+1 or -+width.
+ * setThreshold leaves the hysteresis at Double.MIN_VALUE (4.9e-324). */
+int tj_threshold_projections(tj_ctx *ctx, double threshold, double hysteresis, int fill);
+
 /* ---------------------------------------------------------------------------
  * MRC files (InputOutput/MRC_Reader.java, MRC_Writer.java): the container
  * TomoJ reads tilt series from and writes reconstructions to.  Host only.

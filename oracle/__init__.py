@@ -116,6 +116,7 @@ def _L():
             "or_tv_denoise2d": (None, [P, i, i, C.c_float, C.c_float, C.c_float, i]),
             "or_bayes_convolve": (None, [P, P, i, i, i, d]),
             "or_bayes_end": (None, [P, P, l, d, i]),
+            "or_threshold": (None, [P, i, i, d, d, i]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
@@ -477,6 +478,16 @@ def bayes_iteration(g: Geom, vol, projs, eulers, alpha, lam, first_index=0):
         elif e < emin:
             emin, imin = e, int(order[t])
     return [math.sqrt(som / (n * wh)), math.sqrt(emin / wh), imin, math.sqrt(emax / wh), imax]
+
+
+def threshold(img, threshold, hysteresis=5e-324, fill=1) -> np.ndarray:
+    """TiltSeries.applyThreshold (TiltSeries.java:720-806) of one (H, W) float32 image;
+    returns the thresholded copy.  fill: 0 none (0), 1 average, 2 NaN; the
+    hysteresis fraction defaults to Double.MIN_VALUE as setThreshold leaves it."""
+    out = np.array(img, np.float32, order="C", copy=True)
+    h, w = out.shape
+    _L().or_threshold(_p(out), w, h, float(threshold), float(hysteresis), int(fill))
+    return out
 
 
 def fsc_mask(D, H, W) -> np.ndarray:

@@ -21,6 +21,7 @@
  *        OSSART :548-634, osartIteration :636-733, orderShuffleART :735-781
  *   tomography/filters/FFTWeighting.java          createWeightingMask :31-65, weighting :87-112
  *   src/Utility/.../fr/curie/utils/MatrixUtils.java eulerAngles2Matrix :226-251
+ *   tomography/TiltSeries.java                    applyThreshold :720-806 (projection supply)
  *   src/Filters/.../fr/curie/filters/TotalVariationDenoising_3D.java denoise3D :84-202,
  *        denoise2D :206-260 (the step TVMProjector3D.endOfIteration runs)
  *   third party: ImageJ 1.53a ImageProcessor.getInterpolatedValue (not in the
@@ -1424,4 +1425,105 @@ void or_bayes_end(float *vol, float *dist, long n, double alpha, int positivity)
         if (positivity && 0.0f > v) v = 0.0f;
         vol[i] = v;
     }
+}
+
+/* ---------------------------------------------------------------------------
+ * Hysteresis threshold of the projection supply, TiltSeries.applyThreshold
+ * (src/TomoJ_Base/.../tomography/TiltSeries.java:720-806), applied by getPixels
+ * (:678-686) after the alignment.  One image in place.
+ *
+ * Statistics as ImageJ 1.53a's FloatStatistics (third party, restated from its
+ * published source): values v with -FLT_MAX <= v <= FLT_MAX (NaN and +-Inf
+ * left out), histMin/histMax = their min/max, 256 bins, bin (int)(scale *
+ * (v - histMin)) with scale = 256 / (histMax - histMin) (Java's (int) of NaN is
+ * 0: a constant image fills bin 0), clamped to 255; mean = sequential double
+ * sum / count.  Levels: th = (last bin i whose running count is still below
+ * (int)(W*H*fraction)) / 256 * (histMax - histMin) + histMin, for the
+ * threshold and the hysteresis fraction.  Pixels below th1 take the fill value
+ * (NaN, 0, or (float) mean); pixels in [th1, th2) are queued, and a queued
+ * pixel takes the fill value once a neighbour holds it (NaN in NaN mode; the
+ * neighbours are index-1 and index+1, wrapping across rows as TomoJ's test
+ * does, and index-W, index+W), sweeping the queue backwards until a sweep
+ * changes nothing.
+ * ------------------------------------------------------------------------- */
+static int java_d2i(double x)
+{
+    if (x != x) return 0;
+    if (x >= 2147483647.0) return 2147483647;
+    if (x <= -2147483648.0) return -2147483647 - 1;
+    return (int)x;
+}
+
+void or_threshold(float *res, int width, int height, double threshold, double hysteresis, int fill)
+{
+    const long len = (long)width * height;
+    double mn = DBL_MAX, mx = -DBL_MAX, sum = 0;
+    long cnt = 0;
+    int hist[256];
+    memset(hist, 0, sizeof hist);
+    for (long i = 0; i < len; i++) {
+        const double v = res[i];
+        if (v >= -FLT_MAX && v <= FLT_MAX) {
+            if (v < mn) mn = v;
+            if (v > mx) mx = v;
+        }
+    }
+    const double scale = 256 / (mx - mn);
+    for (long i = 0; i < len; i++) {
+        const double v = res[i];
+        if (v >= -FLT_MAX && v <= FLT_MAX && v >= mn && v <= mx) {
+            cnt++;
+            sum += v;
+            int index = java_d2i(scale * (v - mn));
+            if (index >= 256) index = 255;
+            hist[index]++;
+        }
+    }
+    const double mean = sum / cnt;
+    const int pixnb1 = java_d2i((double)width * height * threshold);
+    const int pixnb2 = java_d2i((double)width * height * hysteresis);
+    double th1 = 0, th2 = 0;
+    long run = 0;
+    for (int i = 0; i < 256; i++) {
+        run += hist[i];
+        if (run < pixnb1) th1 = i;
+        if (run < pixnb2) th2 = i;
+    }
+    th1 /= 256;
+    th1 = th1 * (mx - mn) + mn;
+    th2 /= 256;
+    th2 = th2 * (mx - mn) + mn;
+    float value = NAN;
+    if (fill == 1) value = (float)mean;
+    if (fill == 0) value = 0;
+    const int nan_mode = (value != value);
+    long *q = (long *)malloc(sizeof(long) * (len > 0 ? len : 1));
+    long nq = 0, count = 0;
+    for (long i = 0; i < len; i++) {
+        if (res[i] < th1) res[i] = value;
+        else if (res[i] < th2) {
+            count++;
+            q[nq++] = i;
+        }
+    }
+#define OR_MATCH(j) (nan_mode ? (res[j] != res[j]) : (res[j] == value))
+    while (count > 0) {
+        count = 0;
+        for (long i = nq - 1; i >= 0; i--) {
+            const long ind = q[i];
+            int hit = 0;
+            if (ind - 1 >= 0 && OR_MATCH(ind - 1)) hit = 1;
+            else if (ind + 1 < len && OR_MATCH(ind + 1)) hit = 1;
+            else if (ind - width >= 0 && OR_MATCH(ind - width)) hit = 1;
+            else if (ind + width < len && OR_MATCH(ind + width)) hit = 1;
+            if (hit) {
+                count++;
+                memmove(q + i, q + i + 1, sizeof(long) * (nq - i - 1));
+                nq--;
+                res[ind] = value;
+            }
+        }
+    }
+#undef OR_MATCH
+    free(q);
 }

@@ -859,3 +859,69 @@ def test_resolution_estimation_fsc(tj, oracle):
     ok = np.isfinite(ref[1])
     assert np.array_equal(np.isfinite(got[1]), ok)
     assert np.abs(got[1][ok] - ref[1][ok]).max() < 1e-5
+
+
+# --------------------------------------------------------------------------
+# hysteresis threshold of the projection supply (TiltSeries.applyThreshold)
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("fill", [0, 1, 2])
+def test_threshold_projections_bit_exact(tj, oracle, fill):
+    """Every resident image against the oracle: smooth images with noise so the
+    hysteresis band forms connected patches, a constant image, an image
+    holding NaN and Inf, and one with a long wrapped chain."""
+    W, H = 37, 23
+    rng = np.random.default_rng(fill)
+    y, x = np.mgrid[0:H, 0:W]
+    imgs = [(np.sin(x / 5.0 + k) * np.cos(y / 4.0) + 0.3 * rng.standard_normal((H, W))).astype(np.float32)
+            for k in range(4)]
+    imgs.append(np.full((H, W), 2.0, np.float32))
+    odd = imgs[0].copy()
+    odd[3, 4] = np.nan
+    odd[7, 9] = np.inf
+    imgs.append(odd)
+    chain = np.full((H, W), 100.0, np.float32)
+    chain[0, 0] = 0.0
+    chain.ravel()[1:400] = 50.0                   # a chain crossing many rows through the wrap
+    imgs.append(chain)
+    stack = np.stack(imgs)
+    E = oracle.tilt_eulers(np.linspace(-60, 60, len(imgs)))
+    for t, h in ((0.2, 0.45), (0.3, 5e-324), (0.05, 0.9)):
+        print(f"threshold fill={fill} t={t} h={h}", flush=True)
+        with make(tj, (W, H, 4)) as P:
+            P.load_tilt_series(stack, E)
+            P.threshold_projections(t, h, fill)
+            for k in range(len(imgs)):
+                assert same(P.get_loaded_projection(k), oracle.threshold(stack[k], t, h, fill)), k
+
+
+def test_threshold_in_the_supply_chain(tj, oracle):
+    """TiltSeriesData with normalisation and the threshold on: getPixels'
+    order (normalise, then threshold) and the SIRT result after it."""
+    from tomoj_amd import parameters as RP
+    W, H, D = 30, 6, 10
+    tilts = list(np.arange(-60, 61, 10.0))
+    E = oracle.tilt_eulers(tilts)
+    exps = (oracle.phantom_project(oracle.ellipsoids(13, 5, W, H, D), E, W, H)
+            + 0.05 * oracle.gaussian_noise(2, len(E) * W * H).reshape(len(E), H, W)).astype(np.float32)
+    ts = RP.TiltSeriesData(exps, tilts, normalize=True, apply_threshold=True, threshold=0.1,
+                           threshold_hysteresis=0.3, fill=0)
+    # the normalisation's own parity (ROI sums in another order, images within
+    # 1 ulp) is tested elsewhere: threshold the device's normalised images
+    with make(tj, (W, H, D)) as Q:
+        Q.load_tilt_series(exps, E)
+        Q.normalize_projections(tilts)
+        normed = np.stack([Q.get_loaded_projection(k) for k in range(len(E))])
+    assert np.allclose(normed, oracle.normalized_projections(exps, tilts)[0], rtol=1e-6, atol=1e-6)
+    supplied = np.stack([oracle.threshold(im, 0.1, 0.3, 0) for im in normed])
+    p = RP.ReconstructionParameters.createOSSARTParameters(W, H, D, 2, 1.0, 0)
+    m = RP.ReconstructionThreadManager(ts)
+    try:
+        m.doRec(p)
+        for k in range(len(E)):
+            assert same(m.projector.get_loaded_projection(k), supplied[k])
+        ref = np.zeros((D, H, W), np.float32)
+        oracle.ossart(oracle.make_geom(W, H, D, nthreads=8), ref, supplied, E, 2, 1.0, 0)
+        assert same(m.getRec2(), ref)
+    finally:
+        m.close()

@@ -813,3 +813,47 @@ def test_fsc_transform_convention(oracle):
             continue
         want = (E[sel] * np.conj(O[sel])).real.sum() / np.sqrt((abs(E[sel]) ** 2).sum() * (abs(O[sel]) ** 2).sum())
         assert f[1][s] == pytest.approx(want, rel=1e-9)
+
+
+# --------------------------------------------------------------------------
+# hysteresis threshold of the projection supply (TiltSeries.applyThreshold)
+# --------------------------------------------------------------------------
+
+def test_threshold_ramp_levels(oracle):
+    """A 0..99 ramp: 256 bins over [0, 99]; with the 25% fraction the last bin
+    whose running count stays below 25 is bin 61, level 61/256*99 = 23.6, so
+    exactly the values 0..23 take the fill value."""
+    img = np.arange(100, dtype=np.float32).reshape(10, 10)
+    for fill, val in ((2, None), (0, 0.0), (1, np.float32(49.5))):
+        out = oracle.threshold(img, 0.25, fill=fill)
+        low = out.ravel()[:24]
+        assert (np.isnan(low).all() if val is None else (low == val).all())
+        assert np.array_equal(out.ravel()[24:], img.ravel()[24:])
+
+
+def test_threshold_hysteresis_connectivity(oracle):
+    """Low (0), middle (50) and high (100) values: the middle ones take the
+    fill value only when connected to a filled pixel through index -+1 (which
+    wraps from the end of one row to the start of the next) or -+width; a NaN
+    already in the image seeds the NaN fill."""
+    W, H = 8, 6
+    img = np.full((H, W), 100.0, np.float32)
+    img[0, 7] = 0.0                               # low
+    for y, x in ((1, 0), (1, 1), (2, 1), (4, 5)):
+        img[y, x] = 50.0                          # middle
+    n_low, n_mid, n = 1, 4, W * H
+    t, h = (n_low + 1.5) / n, (n_low + n_mid + 1.5) / n
+    out = oracle.threshold(img, t, h, fill=0)
+    want = img.copy()
+    for y, x in ((0, 7), (1, 0), (1, 1), (2, 1)):
+        want[y, x] = 0.0
+    assert np.array_equal(out, want)              # (4, 5) is not connected
+    nan_img = img.copy()
+    nan_img[5, 5] = np.nan                        # index 45 = (4, 5) + width
+    out = oracle.threshold(nan_img, t, h, fill=2)
+    assert np.isnan(out[4, 5]) and np.isnan(out[1, 0]) and out[3, 3] == 100.0
+
+
+def test_threshold_constant_image_unchanged(oracle):
+    img = np.full((5, 7), 3.0, np.float32)
+    assert np.array_equal(oracle.threshold(img, 0.5, 0.9, fill=2), img)

@@ -111,6 +111,7 @@ _SIGS = {
     "tj_set_tvm": (C.c_int, [_P, C.c_int, C.c_double, C.c_double, C.c_double]),
     "tj_set_bayesian": (C.c_int, [_P, C.c_int, C.c_double, C.c_double]),
     "tj_available_indexes": (C.c_int, [C.c_int, C.c_int, _P]),
+    "tj_threshold_projections": (C.c_int, [_P, C.c_double, C.c_double, C.c_int]),
     "tj_fsc": (C.c_int, [_P, _P, _P]),
     "tj_fsc_volumes": (C.c_int, [C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, _P]),
     "tj_set_available_indexes": (C.c_int, [_P, _P, C.c_int]),

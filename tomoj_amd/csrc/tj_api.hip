@@ -39,6 +39,7 @@ hipError_t tv_denoise(hipStream_t, float*, float*, float*, int, int, int, float,
 hipError_t bayes_conv(hipStream_t, const float*, float*, int, int, int, double);
 hipError_t bayes_end(hipStream_t, float*, float*, size_t, int, float, int, uint8_t*);
 hipError_t normalize(hipStream_t, float*, int, int, int, const int*, double*);
+hipError_t threshold(hipStream_t, float*, int, int, int, double, double, int, double*, uint8_t*, int*);
 hipError_t roi_stats(hipStream_t, const float*, int, int, int, const int*, double*);
 hipError_t affine(hipStream_t, const float*, float*, int, int, int, const double*, const int*, const double*, int,
                   int);
@@ -713,6 +714,33 @@ int tj_normalize_projections(tj_ctx* c, const double* tilt_deg, int min_size, do
     dfree(droi);
     dfree(dstats);
     if (e != hipSuccess) return fail(TJ_EDEVICE, "tj_normalize_projections: %s", hipGetErrorString(e));
+    return TJ_OK;
+}
+
+int tj_threshold_projections(tj_ctx* c, double threshold, double hysteresis, int fill)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_threshold_projections: null context");
+    if (c->nproj <= 0) return fail(TJ_ESTATE, "tj_threshold_projections: no resident projections");
+    if (fill < TJ_FILL_NONE || fill > TJ_FILL_NAN) return fail(TJ_EINVAL, "tj_threshold_projections: fill %d", fill);
+    TJ_HIP(hipSetDevice(c->device));
+    double* lv = nullptr;
+    uint8_t* q = nullptr;
+    int* changed = nullptr;
+    hipError_t e = dalloc(&lv, 4 * (size_t)c->nproj);
+    if (e == hipSuccess) e = dalloc(&q, c->npx * c->nproj);
+    if (e == hipSuccess) e = dalloc(&changed, 1);
+    if (e == hipSuccess)
+        e = tjk::threshold(c->stream, c->exps, c->g.pw, c->g.ph, c->nproj, threshold, hysteresis, fill, lv, q,
+                           changed);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(lv);
+    dfree(q);
+    dfree(changed);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(e == hipErrorOutOfMemory ? TJ_ENOMEM : TJ_EDEVICE, "tj_threshold_projections: %s",
+                    hipGetErrorString(e));
+    }
     return TJ_OK;
 }
 

@@ -33,6 +33,7 @@ struct AddD { __device__ double operator()(double a, double b) const { return a 
 struct MaxF { __device__ float operator()(float a, float b) const { return a > b ? a : b; } };
 struct MaxD { __device__ double operator()(double a, double b) const { return a > b ? a : b; } };
 struct OrU { __device__ unsigned operator()(unsigned a, unsigned b) const { return a | b; } };
+struct MinD { __device__ double operator()(double a, double b) const { return a < b ? a : b; } };
 
 // jcentered of detector row j: VoxelProjector3D.project splits the rows from
 // `start` into tasks of two (:350-352) and projectJosephPartial sets
@@ -1078,6 +1079,137 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_fista(float* __restrict__ vol, con
 }
 
 // ---------------------------------------------------------------------------
+// Hysteresis threshold of the projection supply, TiltSeries.applyThreshold
+// (TiltSeries.java:720-806).  k_thr_levels (one block per image): ImageJ
+// FloatStatistics' min, max and 256-bin histogram over the values within
+// +-FLT_MAX, the mean as Java's sequential double sum (one lane, only for the
+// average fill), and the two levels.  k_thr_apply: below th1 -> fill value,
+// [th1, th2) -> queued.  k_thr_grow: a queued pixel whose neighbour (index
+// -+1, wrapping across rows as TomoJ's test does, and -+W) holds the fill value
+// (NaN in NaN mode) takes it; repeated until nothing changes, which reaches the
+// same closure as TomoJ's backward sweeps.
+// ---------------------------------------------------------------------------
+__device__ inline int java_d2i(double x)
+{
+    if (x != x) return 0;
+    if (x >= 2147483647.0) return 2147483647;
+    if (x <= -2147483648.0) return -2147483647 - 1;
+    return (int)x;
+}
+
+__device__ inline bool fsv_in(double v) { return v >= -3.4028234663852886e38 && v <= 3.4028234663852886e38; }
+
+// lv[img] = {th1, th2, value bits, mean}
+__global__ __launch_bounds__(TJ_BLOCK) void k_thr_levels(const float* __restrict__ imgs, int w, int h,
+                                                         double thr, double hys, int fill, double* __restrict__ lv)
+{
+    __shared__ double shd[TJ_BLOCK];
+    __shared__ int hist[256];
+    const size_t len = (size_t)w * h;
+    const float* im = imgs + (size_t)blockIdx.x * len;
+    double mn = 1.7976931348623157e308, mx = -1.7976931348623157e308;
+    for (size_t i = threadIdx.x; i < len; i += blockDim.x) {
+        const double v = im[i];
+        if (fsv_in(v)) {
+            mn = v < mn ? v : mn;
+            mx = v > mx ? v : mx;
+        }
+    }
+    mn = block_reduce(mn, shd, MinD());
+    mx = block_reduce(mx, shd, MaxD());
+    for (int b = threadIdx.x; b < 256; b += blockDim.x) hist[b] = 0;
+    __syncthreads();
+    const double scale = 256 / (mx - mn);
+    for (size_t i = threadIdx.x; i < len; i += blockDim.x) {
+        const double v = im[i];
+        if (fsv_in(v) && v >= mn && v <= mx) {
+            int index = java_d2i(scale * (v - mn));
+            if (index >= 256) index = 255;
+            atomicAdd(&hist[index], 1);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mean = 0.0;
+        if (fill == 1) {
+            double sum = 0.0;
+            long cnt = 0;
+            for (size_t i = 0; i < len; ++i) {
+                const double v = im[i];
+                if (fsv_in(v) && v >= mn && v <= mx) {
+                    sum += v;
+                    ++cnt;
+                }
+            }
+            mean = sum / cnt;
+        }
+        const int pixnb1 = java_d2i((double)w * h * thr);
+        const int pixnb2 = java_d2i((double)w * h * hys);
+        double th1 = 0, th2 = 0;
+        long run = 0;
+        for (int i = 0; i < 256; ++i) {
+            run += hist[i];
+            if (run < pixnb1) th1 = i;
+            if (run < pixnb2) th2 = i;
+        }
+        th1 /= 256;
+        th1 = th1 * (mx - mn) + mn;
+        th2 /= 256;
+        th2 = th2 * (mx - mn) + mn;
+        float value = __int_as_float(0x7fc00000);
+        if (fill == 1) value = (float)mean;
+        if (fill == 0) value = 0.0f;
+        double* o = lv + 4 * (size_t)blockIdx.x;
+        o[0] = th1;
+        o[1] = th2;
+        o[2] = (double)__float_as_uint(value);
+        o[3] = mean;
+    }
+}
+
+__global__ __launch_bounds__(TJ_BLOCK) void k_thr_apply(float* __restrict__ imgs, size_t len,
+                                                        const double* __restrict__ lv, uint8_t* __restrict__ q)
+{
+    const size_t base = (size_t)blockIdx.y * len;
+    const double* o = lv + 4 * (size_t)blockIdx.y;
+    const double th1 = o[0], th2 = o[1];
+    const float value = __uint_as_float((unsigned)o[2]);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (size_t)gridDim.x * blockDim.x) {
+        const double v = imgs[base + i];
+        uint8_t qq = 0;
+        if (v < th1) imgs[base + i] = value;
+        else if (v < th2) qq = 1;
+        q[base + i] = qq;
+    }
+}
+
+__global__ __launch_bounds__(TJ_BLOCK) void k_thr_grow(float* __restrict__ imgs, size_t len, int w,
+                                                       const double* __restrict__ lv, uint8_t* __restrict__ q,
+                                                       int* __restrict__ changed)
+{
+    const size_t base = (size_t)blockIdx.y * len;
+    const float value = __uint_as_float((unsigned)lv[4 * (size_t)blockIdx.y + 2]);
+    const bool nan_mode = value != value;
+    float* im = imgs + base;
+    int hit_any = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (size_t)gridDim.x * blockDim.x) {
+        if (!q[base + i]) continue;
+        auto match = [&](size_t j) {
+            const float v = im[j];
+            return nan_mode ? (v != v) : (v == value);
+        };
+        const bool hit = (i >= 1 && match(i - 1)) || (i + 1 < len && match(i + 1)) ||
+                         (i >= (size_t)w && match(i - w)) || (i + w < len && match(i + w));
+        if (hit) {
+            im[i] = value;
+            q[base + i] = 0;
+            hit_any = 1;
+        }
+    }
+    if (hit_any) *changed = 1;
+}
+
+// ---------------------------------------------------------------------------
 // Total-variation denoising, TotalVariationDenoising_3D.denoise3D
 // (Filters/.../TotalVariationDenoising_3D.java:84-202), on the device layout
 // [y][z][x] with p held as three planes px, py, pz of the volume's size.  A
@@ -1598,7 +1730,32 @@ hipError_t fista(hipStream_t st, float* vol, const float* prev, size_t n, double
     return hipGetLastError();
 }
 
-// denoise3D (:84-113): f keeps the input, u (the volume) starts equal to it,
+// TiltSeries.applyThreshold on n resident images (in place); lv: 4 doubles
+// per image, q: a byte per pixel, changed: one int (device scratch).
+hipError_t threshold(hipStream_t st, float* imgs, int w, int h, int n, double thr, double hys, int fill, double* lv,
+                     uint8_t* q, int* changed)
+{
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    if (n <= 0 || w <= 0 || h <= 0) return hipSuccess;
+    if (n > 65535) return hipErrorInvalidValue;
+    const size_t len = (size_t)w * h;
+    k_thr_levels<<<n, TJ_BLOCK, 0, st>>>(imgs, w, h, thr, hys, fill, lv);
+    const unsigned bx = (unsigned)std::min<size_t>((len + TJ_BLOCK - 1) / TJ_BLOCK, 1024);
+    k_thr_apply<<<dim3(bx, n), TJ_BLOCK, 0, st>>>(imgs, len, lv, q);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // grow until a pass changes nothing; four passes per check of the flag
+    int hc = 1;
+    for (size_t pass = 0; hc && pass < len + 4; pass += 4) {
+        if ((e = hipMemsetAsync(changed, 0, sizeof(int), st)) != hipSuccess) return e;
+        for (int k = 0; k < 4; ++k) k_thr_grow<<<dim3(bx, n), TJ_BLOCK, 0, st>>>(imgs, len, w, lv, q, changed);
+        if ((e = hipMemcpyAsync(&hc, changed, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    }
+    return hipGetLastError();
+}
+
+// denoise3D (:84-113): f keeps the input, u (the volume) starts equal to it,// denoise3D (:84-113): f keeps the input, u (the volume) starts equal to it,
 // p starts at zero.
 hipError_t tv_denoise(hipStream_t st, float* vol, float* f, float* p, int W, int H, int D, float theta, float g,
                       float dt, int iterations, uint8_t* pflag)

@@ -51,6 +51,11 @@ class TiltSeriesData:
     transforms: Optional[np.ndarray] = None
     fill: int = 1                      # TiltSeries.FILL_AVG (TiltSeries.java:130)
     stats: Optional[np.ndarray] = None  # n x (mean, stdDev) of the ImageJ ROI, when rotated
+    # setThreshold / setThresholdHisteresis (TiltSeries.java:407-416): fractions,
+    # applied by getPixels after the alignment when apply_threshold is set
+    threshold: float = 5e-324          # Double.MIN_VALUE (:101-103)
+    threshold_hysteresis: float = 5e-324
+    apply_threshold: bool = False
 
     @property
     def width(self) -> int:
@@ -184,6 +189,9 @@ class ReconstructionParameters:
         elif ts.normalize:
             # getPixels(i) with normalisation, for unaligned images (identity alignment)
             P.normalize_projections(ts.tilts, ts.normalization_min_size)
+        if ts.apply_threshold:
+            # then applyThreshold (TiltSeries.java:682-684)
+            P.threshold_projections(ts.threshold, ts.threshold_hysteresis, ts.fill)
         if self.reconstructionType == WBP:
             P.setWeightingFilter(self.weightingRadius * ts.width, ts.tilts, ts.tilt_axes())
         if self.fista:

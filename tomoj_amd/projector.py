@@ -130,6 +130,12 @@ class VoxelProjector3DHIP:
              1 if normalize else 0, int(fill), _ptr(sin) if sin is not None else None, _ptr(out))
         return out
 
+    def threshold_projections(self, threshold: float, hysteresis: float = 5e-324,
+                              fill: int = _lib.TJ_FILL_AVG) -> None:
+        """TiltSeries.applyThreshold (TiltSeries.java:720-806) on every resident
+        projection, as getPixels applies it after the alignment."""
+        call("tj_threshold_projections", self._ctx, float(threshold), float(hysteresis), int(fill))
+
     def get_loaded_projection(self, index: int) -> np.ndarray:
         out = np.empty((self.ph, self.pw), np.float32)
         call("tj_get_loaded_projection", self._ctx, int(index), _ptr(out))
