@@ -925,3 +925,31 @@ def test_threshold_in_the_supply_chain(tj, oracle):
         assert same(m.getRec2(), ref)
     finally:
         m.close()
+
+
+# --------------------------------------------------------------------------
+# the benchmark configurations at full size
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("wname,iters", [("sirt_256x256x128_61", 10), ("ossart_512x512x256_121", 1)])
+def test_configurations_at_full_size_bit_exact(tj, oracle, wname, iters):
+    """256x256x128 with 61 tilts, SIRT, all 10 iterations (the size TomoJ's CPU
+    path runs), and 512x512x256 with 121 tilts, OS-SART in 10 subsets, one
+    iteration: the fused loop on the device against the oracle on the same
+    device-generated projections, volume bit for bit and error rows."""
+    from tomoj_amd import workloads as WL
+    w = WL.ALL[wname]
+    E = WL.eulers_for(w)
+    print(f"full-size {wname} x{iters}", flush=True)
+    with make(tj, (w.W, w.H, w.D)) as P:
+        P.synth_tilt_series(tj.phantom_ellipsoids(w.seed, w.ellipsoids, w.W, w.H, w.D), E, w.noise_rel,
+                            w.noise_seed)
+        exps = np.stack([P.get_loaded_projection(k) for k in range(w.n)])
+        err = P.run_ossart(iters, w.relaxation, w.update)
+        got = P.get_volume()
+    ref = np.zeros((w.D, w.H, w.W), np.float32)
+    ref_err = oracle.ossart(oracle.make_geom(w.W, w.H, w.D, nthreads=16), ref, exps, E, iters, w.relaxation,
+                            w.update)
+    assert same(got, ref)
+    assert np.allclose(err[:, [0, 1, 3]], ref_err[:, [0, 1, 3]], rtol=1e-12)
+    assert np.array_equal(err[:, [2, 4]], ref_err[:, [2, 4]])
