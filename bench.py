@@ -33,6 +33,15 @@ METRIC = "SIRT iters/s + voxel-ray GUPS at 512³ (HBM %roofline); serve p50/p99 
 HBM_PEAK_GBS = 8000.0
 
 
+# MI355X vector issue rate: 256 CUs x 4 SIMDs x 16 lanes per clock x 2.4 GHz
+# (FP32 and FP64 adds and multiplies both issue at this rate; packed FP32 is
+# not used).
+VALU_PEAK = 256 * 64 * 2.4e9
+# VALU lane operations per voxel-ray update of the plain kernels on the
+# OS-SART 512x512x256 workload (SQ_INSTS_VALU x 64 / updates per launch).
+VALU_LANE_OPS_PER_UPDATE = {"fp": 12.9, "bp": 11.8}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -214,6 +223,14 @@ def main():
         "limiter": "VALU and LDS: FP64 positions and ImageJ's FP64 bilinear weights per voxel update "
                    "(SQ_INSTS_VALU in profiles/); measured HBM traffic is within 8% of the algorithmic bytes",
     }
+    # The same launch against the VALU issue rate: lane operations per voxel-ray
+    # update from SQ_INSTS_VALU x 64 / updates (rocprofv3 counters of this
+    # workload), times the updates per launch, over the live launch time.
+    per_update = VALU_LANE_OPS_PER_UPDATE[dom]
+    upd = w.voxel_ray_updates_per_iteration() / 2 * args.steps / max(tim[f"{dom}_launches"], 1)
+    valu = per_update * upd / (kernels[dom]["avg_ms"] * 1e-3)
+    roofline["valu"] = {"lane_ops_per_update": per_update, "achieved": valu / 1e12, "peak": VALU_PEAK / 1e12,
+                        "unit": "T lane-ops/s", "frac": valu / VALU_PEAK}
     alg = WL.algorithmic_bytes_per_iteration(w)
     out = {
         "metric": METRIC,

@@ -953,3 +953,39 @@ def test_configurations_at_full_size_bit_exact(tj, oracle, wname, iters):
     assert same(got, ref)
     assert np.allclose(err[:, [0, 1, 3]], ref_err[:, [0, 1, 3]], rtol=1e-12)
     assert np.array_equal(err[:, [2, 4]], ref_err[:, [2, 4]])
+
+
+def test_new_entry_points_report_errors(tj):
+    """Argument and state errors of the regularisation, subset, threshold and
+    FSC entry points come back as TjError, and the context stays usable."""
+    import ctypes as C
+    from tomoj_amd import TjError
+    from tomoj_amd._lib import call
+    from tomoj_amd.resolution import fsc_volumes
+    with make(tj, (16, 8, 8)) as P, make(tj, (16, 8, 6)) as Q:
+        with pytest.raises(TjError):
+            P.setTvm(-1)
+        with pytest.raises(TjError):
+            P.tv_denoise(25.0, 1.0, 0.1, -2)
+        with pytest.raises(TjError):
+            P.threshold_projections(0.1)            # nothing loaded
+        P.load_tilt_series(np.ones((3, 8, 16), np.float32), np.stack([np.eye(3)] * 3))
+        with pytest.raises(TjError):
+            P.threshold_projections(0.1, fill=7)
+        P.setAvailableIndexes([0, 5])
+        with pytest.raises(TjError):
+            P.run_ossart(1, 1.0, 0)                 # image 5 of 3
+        with pytest.raises(TjError):
+            P.run_wbp()
+        P.setAvailableIndexes([0, 2])
+        with pytest.raises(TjError):
+            P.run_ossart(1, 1.0, 0, first_index=2)  # a position in the 2-image list
+        P.setAvailableIndexes(None)
+        P.run_ossart(1, 1.0, 0)
+        out = np.zeros((4, 9))
+        with pytest.raises(TjError):
+            call("tj_fsc", P._ctx, Q._ctx, out.ctypes.data_as(C.c_void_p))   # sizes differ
+        with pytest.raises(ValueError):
+            fsc_volumes(np.zeros((4, 4, 4), np.float32), np.zeros((4, 4, 5), np.float32))
+        with pytest.raises(TjError):
+            fsc_volumes(np.zeros((4, 70000, 1), np.float32), np.zeros((4, 70000, 1), np.float32))
