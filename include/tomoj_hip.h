@@ -327,9 +327,11 @@ int tj_normalize_projections(tj_ctx *ctx, const double *tilt_deg, int min_size, 
  * and row nearest), is normalised with the image's statistics when `normalize`,
  * and pixels mapping outside take `fill`.  The statistics are those of
  * TiltSeries.getImageStatistics(i) (:809-835): pass them in stats_in (n x
- * {mean, stdDev}, e.g. from ImageJ), or NULL to compute them on the device from
- * the ROI rectangle mapped by T_i (only for transforms without rotation or
- * shear, m01 = m10 = 0; TJ_EUNSUP otherwise).  tilt_deg is needed only then.
+ * {mean, stdDev}, e.g. from ImageJ), or NULL to compute them on the device over
+ * the ROI rectangle's corners mapped by T_i and cast to int: the rectangle
+ * they span for transforms without rotation or shear, else the pixels whose
+ * centres lie in their polygon (even-odd test; ImageJ's PolygonRoi mask rule,
+ * restated).  tilt_deg is needed only then.
  * stats_out (may be NULL) receives the statistics used.  TJ_EINVAL for a
  * non-invertible transform. */
 int tj_align_projections(tj_ctx *ctx, const double *transforms, const double *tilt_deg, int min_size,

@@ -117,6 +117,8 @@ def _L():
             "or_bayes_convolve": (None, [P, P, i, i, i, d]),
             "or_bayes_end": (None, [P, P, l, d, i]),
             "or_threshold": (None, [P, i, i, d, d, i]),
+            "or_poly_stats": (None, [P, i, i, P, P, P]),
+            "or_roi_poly_affine": (None, [i, i, d, i, P, P]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(lib, name)
@@ -347,7 +349,8 @@ def rect_stats(img, rect):
 def aligned_projections(images, transforms, tilts_deg=None, min_size=0, normalize=True, fill=1, stats=None):
     """TiltSeries.getPixels(i) through AffineAlignment.applyTransformOnImage
     (AffineAlignment.java:136-254) for every image; stats n x (mean, stdDev) or
-    None (the rectangle ROI of rotation-free transforms).  Returns (images, stats)."""
+    None (the mapped ROI: its rectangle for rotation-free transforms, else the
+    pixels whose centres lie in its polygon).  Returns (images, stats)."""
     imgs = np.array(images, np.float32, copy=True)
     n, h, w = imgs.shape
     T = _c(transforms, np.float64).reshape(n, 6)
@@ -358,8 +361,10 @@ def aligned_projections(images, transforms, tilts_deg=None, min_size=0, normaliz
         if need:
             if stats is not None:
                 st[k] = stats[k]
-            else:
+            elif T[k][1] == 0.0 and T[k][2] == 0.0:
                 st[k] = rect_stats(src, roi_rect_affine(w, h, tilts_deg[k], T[k], min_size))
+            else:
+                st[k] = poly_stats(src, roi_poly_affine(w, h, tilts_deg[k], T[k], min_size))
         if np.array_equal(T[k], [1.0, 0.0, 0.0, 1.0, 0.0, 0.0]):
             if normalize:
                 _L().or_normalize(_p(src), src.size, st[k][0], st[k][1])
@@ -478,6 +483,26 @@ def bayes_iteration(g: Geom, vol, projs, eulers, alpha, lam, first_index=0):
         elif e < emin:
             emin, imin = e, int(order[t])
     return [math.sqrt(som / (n * wh)), math.sqrt(emin / wh), imin, math.sqrt(emax / wh), imax]
+
+
+def roi_poly_affine(width, height, tilt_deg, transform, min_size=0) -> np.ndarray:
+    """The int corners (xs[4], ys[4]) of getImageStatistics' ROI polygon under
+    the forward transform (m00, m10, m01, m11, m02, m12)."""
+    m = _c(transform, np.float64)
+    out = np.zeros(8, np.int32)
+    _L().or_roi_poly_affine(int(width), int(height), float(tilt_deg), int(min_size), _p(m), _p(out))
+    return out
+
+
+def poly_stats(img, poly):
+    """Mean and sample standard deviation over the pixels whose centres lie in
+    the polygon poly = (xs[4], ys[4])."""
+    img = _c(img, np.float32)
+    h, w = img.shape
+    pl = _c(poly, np.int32)
+    m, sd = C.c_double(), C.c_double()
+    _L().or_poly_stats(_p(img), w, h, _p(pl), C.byref(m), C.byref(sd))
+    return m.value, sd.value
 
 
 def threshold(img, threshold, hysteresis=5e-324, fill=1) -> np.ndarray:

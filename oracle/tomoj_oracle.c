@@ -1527,3 +1527,79 @@ void or_threshold(float *res, int width, int height, double threshold, double hy
 #undef OR_MATCH
     free(q);
 }
+
+/* ---------------------------------------------------------------------------
+ * Statistics of TiltSeries.getImageStatistics' polygon ROI under a rotating or
+ * shearing alignment (TiltSeries.java:809-835): the four int corners
+ * (xs[k], ys[k]) of the mapped rectangle define an ImageJ PolygonRoi, and
+ * FloatStatistics sums the pixels of its mask.  Mask restated from ImageJ's
+ * documented rule (third party, not in the tree): a pixel belongs to the
+ * polygon when its centre (x + 0.5, y + 0.5) lies inside by the even-odd
+ * crossing test; the test's half-open convention decides centres exactly on
+ * an edge.  Pixels outside the image are left out.  Sums row by row over the
+ * bounding box, mean and sample deviation as or_rect_stats.
+ * ------------------------------------------------------------------------- */
+int or_poly_inside(const int *xs, const int *ys, double px, double py)
+{
+    int in = 0;
+    for (int i = 0, j = 3; i < 4; j = i++) {
+        const double xi = xs[i], yi = ys[i], xj = xs[j], yj = ys[j];
+        if ((yi > py) != (yj > py) && px < (xj - xi) * (py - yi) / (yj - yi) + xi) in = !in;
+    }
+    return in;
+}
+
+void or_poly_stats(const float *img, int width, int height, const int *poly, double *mean, double *sd)
+{
+    const int *xs = poly, *ys = poly + 4;
+    int x0 = xs[0], x1 = xs[0], y0 = ys[0], y1 = ys[0];
+    for (int k = 1; k < 4; k++) {
+        if (xs[k] < x0) x0 = xs[k];
+        if (xs[k] > x1) x1 = xs[k];
+        if (ys[k] < y0) y0 = ys[k];
+        if (ys[k] > y1) y1 = ys[k];
+    }
+    if (x0 < 0) x0 = 0;
+    if (y0 < 0) y0 = 0;
+    if (x1 > width) x1 = width;
+    if (y1 > height) y1 = height;
+    double n = 0, sum = 0, sum2 = 0, s = 0.0;
+    for (int y = y0; y < y1; ++y)
+        for (int x = x0; x < x1; ++x)
+            if (or_poly_inside(xs, ys, x + 0.5, y + 0.5)) {
+                const double v = img[(long)y * width + x];
+                sum += v;
+                sum2 += v * v;
+                n += 1;
+            }
+    *mean = n > 0 ? sum / n : 0.0;
+    if (n > 0) {
+        s = (n * sum2 - sum * sum) / n;
+        s = (s > 0.0) ? sqrt(s / (n - 1.0)) : 0.0;
+    }
+    *sd = s;
+}
+
+/* The ROI polygon's int corners (xs[4], ys[4]) for an image of width x height
+ * at tilt `tilt_deg` under the forward transform m (m00, m10, m01, m11, m02, m12),
+ * TiltSeries.getImageStatistics (:809-828). */
+void or_roi_poly_affine(int width, int height, double tilt_deg, int min_size, const double *m, int *poly)
+{
+    double rad = tilt_deg / 180.0 * M_PI;
+    long r = (long)floor(width * cos(rad) + 0.5);
+    int roisx = (int)(r > min_size ? r : min_size);
+    double cx = (width - 1.0) / 2, cy = (height - 1.0) / 2;
+    double pts[8];
+    pts[0] = (double)((width - roisx) / 2) - cx;
+    pts[1] = -cy;
+    pts[2] = pts[0] + roisx;
+    pts[3] = pts[1];
+    pts[4] = pts[2];
+    pts[5] = height - cy;
+    pts[6] = pts[0];
+    pts[7] = pts[5];
+    for (int k = 0; k < 4; ++k) {
+        poly[k] = (int)((m[0] * pts[2 * k] + m[2] * pts[2 * k + 1] + m[4]) + cx);
+        poly[4 + k] = (int)((m[1] * pts[2 * k] + m[3] * pts[2 * k + 1] + m[5]) + cy);
+    }
+}

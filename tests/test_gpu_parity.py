@@ -538,15 +538,32 @@ def test_aligned_projection_supply(tj, oracle, case):
     assert np.array_equal(np.nan_to_num(got), np.nan_to_num(ref))
 
 
-def test_aligned_supply_rejects_rotation_without_statistics(tj, oracle):
-    W, H = 16, 8
-    imgs = np.ones((2, H, W), np.float32)
-    with make(tj, (W, H, 4)) as P:
-        P.load_tilt_series(imgs, oracle.tilt_eulers([0.0, 10.0]))
+@pytest.mark.parametrize("normalize,fill", [(True, 1), (False, 1), (True, 2)])
+def test_aligned_supply_rotation_polygon_statistics(tj, oracle, normalize, fill):
+    """Rotated alignments without caller statistics: the polygon ROI's
+    statistics on the device against the oracle (1e-12, reduction order), then
+    the images against the oracle run with the device's statistics; a
+    singular transform is refused."""
+    rng = np.random.default_rng(33)
+    W, H, n = 64, 40, 4
+    tilts = [-55.0, -20.0, 5.0, 48.0]
+    imgs = (rng.standard_normal((n, H, W)) * 2 + 6).astype(np.float32)
+    T = np.zeros((n, 6))
+    for k in range(n):
+        a, s = rng.uniform(-0.5, 0.5), rng.uniform(0.9, 1.1)
+        T[k] = [s * np.cos(a), s * np.sin(a), -s * np.sin(a), s * np.cos(a), rng.uniform(-4, 4), rng.uniform(-3, 3)]
+    T[1] = [1, 0, 0, 1, 2, -1]                        # rotation-free in the same call
+    with make(tj, (W, H, 8)) as P:
+        P.load_tilt_series(imgs, oracle.tilt_eulers(tilts))
+        st = P.align_projections(T, tilts, 0, normalize, fill, None)
+        got = np.stack([P.get_loaded_projection(k) for k in range(n)])
         with pytest.raises(tj.TjError):
-            P.align_projections([[1, 0, 0, 1, 0, 0], [0.9, 0.1, -0.1, 0.9, 0, 0]], [0.0, 10.0], 0, True, 1)
-        with pytest.raises(tj.TjError):
-            P.align_projections([[1, 0, 0, 1, 0, 0], [0, 0, 0, 0, 1, 1]], [0.0, 10.0], 0, False, 0)
+            P.align_projections([[1, 0, 0, 1, 0, 0], [0, 0, 0, 0, 1, 1]] * 2, tilts, 0, False, 0)
+    _, ref_st = oracle.aligned_projections(imgs, T, tilts, 0, normalize, fill)
+    assert np.allclose(st, ref_st, rtol=1e-12)
+    ref, _ = oracle.aligned_projections(imgs, T, tilts, 0, normalize, fill, stats=st)
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    assert np.array_equal(np.nan_to_num(got), np.nan_to_num(ref))
 
 
 # --------------------------------------------------------------------------

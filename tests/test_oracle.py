@@ -857,3 +857,30 @@ def test_threshold_hysteresis_connectivity(oracle):
 def test_threshold_constant_image_unchanged(oracle):
     img = np.full((5, 7), 3.0, np.float32)
     assert np.array_equal(oracle.threshold(img, 0.5, 0.9, fill=2), img)
+
+
+def test_polygon_roi_statistics(oracle):
+    """The polygon ROI of a rotating alignment: an axis-aligned polygon gives
+    the rectangle's statistics exactly; a rotated quadrilateral takes the
+    pixels whose centres pass a convex-polygon sign test (no centre lies on an
+    edge here), checked against numpy's mean and sample deviation; a polygon
+    larger than the image takes every pixel."""
+    rng = np.random.default_rng(12)
+    h, w = 30, 40
+    img = rng.standard_normal((h, w)).astype(np.float32)
+    rect = oracle.poly_stats(img, [3, 20, 20, 3, 4, 4, 17, 17])
+    assert rect == oracle.rect_stats(img, [3, 20, 4, 17])
+    xs, ys = np.array([2, 33, 27, -4]), np.array([1, 9, 36, 24])   # odd dx - dy: no centre on an edge line
+    yy, xx = np.mgrid[0:h, 0:w] + 0.5
+    inside = np.ones((h, w), bool)
+    for i in range(4):
+        j = (i + 1) % 4
+        cross = (xs[j] - xs[i]) * (yy - ys[i]) - (ys[j] - ys[i]) * (xx - xs[i])
+        assert not (cross == 0).any()
+        inside &= cross > 0
+    m, sd = oracle.poly_stats(img, list(xs) + list(ys))
+    sel = img[inside].astype(np.float64)
+    assert m == pytest.approx(sel.mean(), rel=1e-12)
+    assert sd == pytest.approx(sel.std(ddof=1), rel=1e-9)
+    m, sd = oracle.poly_stats(img, [-5, 50, 50, -5, -5, -5, 40, 40])
+    assert m == pytest.approx(img.astype(np.float64).mean(), rel=1e-12)

@@ -41,6 +41,7 @@ hipError_t bayes_end(hipStream_t, float*, float*, size_t, int, float, int, uint8
 hipError_t normalize(hipStream_t, float*, int, int, int, const int*, double*);
 hipError_t threshold(hipStream_t, float*, int, int, int, double, double, int, double*, uint8_t*, int*);
 hipError_t roi_stats(hipStream_t, const float*, int, int, int, const int*, double*);
+hipError_t poly_stats(hipStream_t, const float*, int, int, int, const int*, double*);
 hipError_t affine(hipStream_t, const float*, float*, int, int, int, const double*, const int*, const double*, int,
                   int);
 hipError_t weighting_init(hipStream_t, int, int, double, const double*, const double*, int, float*);
@@ -787,7 +788,22 @@ static bool affine_inverse(const double* m, double* r)
 // rectangle of the identity case, its corners mapped by T and cast to int.  For
 // a transform without rotation or shear the polygon is the axis-aligned
 // rectangle [min x, max x) x [min y, max y).
+static void roi_poly_affine(int width, int height, double tilt_deg, int min_size, const double* m, int* xs,
+                            int* ys);
+
 static void roi_rect_affine(int width, int height, double tilt_deg, int min_size, const double* m, int* r4)
+{
+    int xs[4], ys[4];
+    roi_poly_affine(width, height, tilt_deg, min_size, m, xs, ys);
+    r4[0] = std::min(std::min(xs[0], xs[1]), std::min(xs[2], xs[3]));
+    r4[1] = std::max(std::max(xs[0], xs[1]), std::max(xs[2], xs[3]));
+    r4[2] = std::min(std::min(ys[0], ys[1]), std::min(ys[2], ys[3]));
+    r4[3] = std::max(std::max(ys[0], ys[1]), std::max(ys[2], ys[3]));
+}
+
+// The polygon's int corners.
+static void roi_poly_affine(int width, int height, double tilt_deg, int min_size, const double* m, int* xs,
+                            int* ys)
 {
     const double rad = tilt_deg / 180.0 * M_PI;
     const long rr = (long)std::floor(width * std::cos(rad) + 0.5);
@@ -802,16 +818,11 @@ static void roi_rect_affine(int width, int height, double tilt_deg, int min_size
     pts[5] = height - cy;
     pts[6] = pts[0];
     pts[7] = pts[5];
-    int xs[4], ys[4];
     for (int k = 0; k < 4; ++k) {
         const double x = pts[2 * k], y = pts[2 * k + 1];
         xs[k] = (int)((m[0] * x + m[2] * y + m[4]) + cx);
         ys[k] = (int)((m[1] * x + m[3] * y + m[5]) + cy);
     }
-    r4[0] = std::min(std::min(xs[0], xs[1]), std::min(xs[2], xs[3]));
-    r4[1] = std::max(std::max(xs[0], xs[1]), std::max(xs[2], xs[3]));
-    r4[2] = std::min(std::min(ys[0], ys[1]), std::min(ys[2], ys[3]));
-    r4[3] = std::max(std::max(ys[0], ys[1]), std::max(ys[2], ys[3]));
 }
 
 int tj_align_projections(tj_ctx* c, const double* transforms, const double* tilt_deg, int min_size, int normalize,
@@ -823,7 +834,11 @@ int tj_align_projections(tj_ctx* c, const double* transforms, const double* tilt
     const int n = c->nproj, w = c->g.pw, h = c->g.ph;
     const bool need_stats = normalize || fill == TJ_FILL_AVG;
     std::vector<double> inv(6 * (size_t)n);
-    std::vector<int> ident(n), roi(4 * (size_t)n);
+    // ROI of each image's statistics: a rectangle [x0,x1) x [y0,y1) when the
+    // transform neither rotates nor shears (roi), else the polygon's corners
+    // (poly); `rotated` says which.
+    std::vector<int> ident(n), roi(4 * (size_t)n), poly(8 * (size_t)n);
+    bool rotated = false;
     for (int p = 0; p < n; ++p) {
         const double* m = transforms + 6 * (size_t)p;
         ident[p] = (m[0] == 1.0 && m[1] == 0.0 && m[2] == 0.0 && m[3] == 1.0 && m[4] == 0.0 && m[5] == 0.0);
@@ -831,12 +846,9 @@ int tj_align_projections(tj_ctx* c, const double* transforms, const double* tilt
             return fail(TJ_EINVAL, "tj_align_projections: transform %d is not invertible", p);
         if (need_stats && !stats_in) {
             if (!tilt_deg) return fail(TJ_EINVAL, "tj_align_projections: tilt angles needed for the statistics");
-            if (m[1] != 0.0 || m[2] != 0.0)
-                return fail(TJ_EUNSUP,
-                            "tj_align_projections: transform %d rotates or shears the ROI polygon; pass its "
-                            "statistics (ImageJ polygon ROI) in stats_in",
-                            p);
             roi_rect_affine(w, h, tilt_deg[p], min_size, m, &roi[4 * (size_t)p]);
+            roi_poly_affine(w, h, tilt_deg[p], min_size, m, &poly[8 * (size_t)p], &poly[8 * (size_t)p + 4]);
+            rotated |= (m[1] != 0.0 || m[2] != 0.0);
         }
     }
     TJ_HIP(hipSetDevice(c->device));
@@ -856,10 +868,15 @@ int tj_align_projections(tj_ctx* c, const double* transforms, const double* tilt
         if (stats_in) {
             e = hipMemcpyAsync(dstats, stats_in, sizeof(double) * 2 * n, hipMemcpyHostToDevice, c->stream);
         } else {
-            e = dalloc(&droi, roi.size());
+            // the polygon test gives the rectangle's pixels for rotation-free
+            // transforms; those keep the rectangle kernel
+            const std::vector<int>& r = rotated ? poly : roi;
+            e = dalloc(&droi, r.size());
             if (e == hipSuccess)
-                e = hipMemcpyAsync(droi, roi.data(), sizeof(int) * roi.size(), hipMemcpyHostToDevice, c->stream);
-            if (e == hipSuccess) e = tjk::roi_stats(c->stream, orig, w, h, n, droi, dstats);
+                e = hipMemcpyAsync(droi, r.data(), sizeof(int) * r.size(), hipMemcpyHostToDevice, c->stream);
+            if (e == hipSuccess)
+                e = rotated ? tjk::poly_stats(c->stream, orig, w, h, n, droi, dstats)
+                            : tjk::roi_stats(c->stream, orig, w, h, n, droi, dstats);
         }
     }
     if (e == hipSuccess)

@@ -990,6 +990,61 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_roi_stats(const float* __restrict_
     }
 }
 
+// Statistics over getImageStatistics' polygon ROI when the alignment rotates
+// or shears it (TiltSeries.java:809-835): poly = 8 ints per image (xs[4],
+// ys[4]).  A pixel counts when its centre lies inside by the even-odd crossing
+// test (ImageJ's documented polygon-mask rule); bounding box clipped to the
+// image; mean and sample deviation as k_roi_stats.
+__device__ inline bool poly_inside(const int* xs, const int* ys, double px, double py)
+{
+    bool in = false;
+    for (int i = 0, j = 3; i < 4; j = i++) {
+        const double xi = xs[i], yi = ys[i], xj = xs[j], yj = ys[j];
+        if ((yi > py) != (yj > py) && px < (xj - xi) * (py - yi) / (yj - yi) + xi) in = !in;
+    }
+    return in;
+}
+
+__global__ __launch_bounds__(TJ_BLOCK) void k_poly_stats(const float* __restrict__ imgs, int pw, int ph,
+                                                         const int* __restrict__ poly, double* __restrict__ stats)
+{
+    __shared__ double sh[TJ_BLOCK];
+    const int p = blockIdx.x;
+    int xs[4], ys[4];
+    for (int k = 0; k < 4; ++k) {
+        xs[k] = poly[8 * p + k];
+        ys[k] = poly[8 * p + 4 + k];
+    }
+    const int x0 = max(min(min(xs[0], xs[1]), min(xs[2], xs[3])), 0);
+    const int x1 = min(max(max(xs[0], xs[1]), max(xs[2], xs[3])), pw);
+    const int y0 = max(min(min(ys[0], ys[1]), min(ys[2], ys[3])), 0);
+    const int y1 = min(max(max(ys[0], ys[1]), max(ys[2], ys[3])), ph);
+    const int rw = max(x1 - x0, 0);
+    const size_t cnt = (size_t)rw * max(y1 - y0, 0);
+    const float* img = imgs + (size_t)p * pw * ph;
+    double sum = 0.0, sum2 = 0.0, n = 0.0;
+    for (size_t k = threadIdx.x; k < cnt; k += TJ_BLOCK) {
+        const int y = y0 + (int)(k / rw), x = x0 + (int)(k - (size_t)(y - y0) * rw);
+        if (!poly_inside(xs, ys, x + 0.5, y + 0.5)) continue;
+        const double v = img[(size_t)y * pw + x];
+        sum += v;
+        sum2 += v * v;
+        n += 1.0;
+    }
+    sum = block_reduce(sum, sh, AddD());
+    sum2 = block_reduce(sum2, sh, AddD());
+    n = block_reduce(n, sh, AddD());
+    if (threadIdx.x == 0) {
+        double sd = 0.0;
+        if (n > 0) {
+            sd = (n * sum2 - sum * sum) / n;
+            sd = (sd > 0.0) ? sqrt(sd / (n - 1.0)) : 0.0;
+        }
+        stats[2 * p] = n > 0 ? sum / n : 0.0;
+        stats[2 * p + 1] = sd;
+    }
+}
+
 __global__ __launch_bounds__(TJ_BLOCK) void k_normalize(float* __restrict__ imgs, size_t npx,
                                                         const double* __restrict__ stats)
 {
@@ -1698,6 +1753,14 @@ hipError_t roi_stats(hipStream_t st, const float* imgs, int pw, int ph, int n, c
     (void)hipGetLastError();
     if (n <= 0) return hipSuccess;
     k_roi_stats<<<n, TJ_BLOCK, 0, st>>>(imgs, pw, ph, roi, stats);
+    return hipGetLastError();
+}
+
+hipError_t poly_stats(hipStream_t st, const float* imgs, int pw, int ph, int n, const int* poly, double* stats)
+{
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    if (n <= 0) return hipSuccess;
+    k_poly_stats<<<n, TJ_BLOCK, 0, st>>>(imgs, pw, ph, poly, stats);
     return hipGetLastError();
 }
 
