@@ -230,6 +230,21 @@ int tj_set_available_indexes(tj_ctx *ctx, const int *indexes, int n);
  * H and D <= 65535. */
 int tj_fsc(tj_ctx *even, tj_ctx *odd, double *out);
 
+/* Spectral SNR of a reconstruction: ResolutionEstimation.SSNR(false)
+ * (tomography/ResolutionEstimation.java:167-260 with addToVectors1D :581-697).
+ * `signal` holds the resident (supplied) tilt series and its reconstruction,
+ * `noise` a tilt series of Gaussian noise (generateSSNRNoiseData, :93-112)
+ * reconstructed with the same parameters; same sizes.  For every tilt both
+ * reconstructions are re-projected (ReconstructionThreadManager.getProjection,
+ * :257-270), NaN set to 0, and the power spectra of measured - projected and
+ * of projected summed over rings ring_width wide (TomoJ: 4).  out (9 x
+ * (W/2+1) doubles, row by row) receives TomoJ's nine rows: index, frequency /
+ * pixel_size, corrected SSNR in dB (-1000 where SSNR <= 1), signal SNR, signal
+ * and noise powers in dB, noise SNR and its powers in dB; *nfreq = the number
+ * of columns filled (W/2+1).  FP32 transforms (TomoJ's are FP64), FP64 sums.
+ * The VSSNR map (computeVSSNR) is not built. */
+int tj_ssnr(tj_ctx *signal, tj_ctx *noise, int ring_width, double pixel_size, double *out, int *nfreq);
+
 /* The same for two host volumes in ImageStack layout [D][H][W] (the static
  * fsc(ImagePlus, ImagePlus)); copies them to `device` for the call. */
 int tj_fsc_volumes(int device, const float *even_zyx, const float *odd_zyx, int W, int H, int D, double *out);

@@ -515,6 +515,56 @@ def threshold(img, threshold, hysteresis=5e-324, fill=1) -> np.ndarray:
     return out
 
 
+def ssnr_sums(sig_exps, sig_th, noi_exps, noi_th, ring_width=4) -> np.ndarray:
+    """ResolutionEstimation.addToVectors1D (:581-697) over every tilt, FP64
+    transforms: NaN -> 0, the float differences measured - projected, the power
+    spectra |fft2|^2 of difference and projection for signal and noise, and
+    ring sums (a frequency at radius w*W adds to rings
+    [max(0, ceil(w*W - ringWidth)), floor(w*W))).  Returns sums[5][W]:
+    thSignal, signal - thSignal, thNoise, noise - thNoise, count."""
+    se, st, ne, nt = (np.nan_to_num(_c(a, np.float32), nan=0.0, posinf=np.inf, neginf=-np.inf)
+                      for a in (sig_exps, sig_th, noi_exps, noi_th))
+    n, h, w = se.shape
+
+    def freq(m):
+        i = np.arange(m)
+        return np.where(i <= (m >> 1), i, i - m) / m
+
+    fy, fx = np.meshgrid(freq(h) ** 2, freq(w) ** 2, indexing="ij")
+    wi = np.sqrt(fx + fy) * w
+    l0 = np.maximum(0, np.ceil(wi - ring_width)).astype(np.int64)
+    lf = np.floor(wi).astype(np.int64)
+    sums = np.zeros((5, w))
+    for t in range(n):
+        pw = [np.abs(np.fft.fft2(a.astype(np.float64))) ** 2
+              for a in (st[t], (se[t] - st[t]).astype(np.float32), nt[t], (ne[t] - nt[t]).astype(np.float32))]
+        for k in range(ring_width + 1):
+            l = l0 + k
+            ok = l < lf
+            for row in range(4):
+                np.add.at(sums[row], l[ok], pw[row][ok])
+            np.add.at(sums[4], l[ok], 1.0)
+    return sums
+
+
+def ssnr_rows(sums, pixel_size=1.0) -> np.ndarray:
+    """SSNR (:212-260) from the ring sums: nine rows over the non-negative
+    frequencies; the dB rows divide by TomoJ's counter imgno = W + 1."""
+    SS, SN, NS, NN, K = (np.array(r, np.float64) for r in sums)
+    w = SS.size
+    with np.errstate(divide="ignore", invalid="ignore"):
+        s_snr, n_snr = SS / SN, NS / NN
+        SS, SN, NS, NN = SS / K, SN / K, NS / K, NN / K
+        imgno = w + 1
+        r = w // 2 + 1
+        i = np.arange(r)
+        ssnr = s_snr[:r] / n_snr[:r]
+        corr = np.where(ssnr > 1, 10 * np.log10(np.where(ssnr > 1, ssnr - 1, 1.0)), -1000.0)
+        return np.array([i, (i / w) / pixel_size, corr, s_snr[:r], 10 * np.log10(SS[:r] / imgno),
+                         10 * np.log10(SN[:r] / imgno), n_snr[:r], 10 * np.log10(NS[:r] / imgno),
+                         10 * np.log10(NN[:r] / imgno)])
+
+
 def fsc_mask(D, H, W) -> np.ndarray:
     """ResolutionEstimation.fsc's mask (ResolutionEstimation.java:470-489), (D, H, W)
     float32: 1 within radius D of (W/2, H/2, D/2), a raised cosine over the next

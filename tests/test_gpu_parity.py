@@ -1006,3 +1006,40 @@ def test_new_entry_points_report_errors(tj):
             fsc_volumes(np.zeros((4, 4, 4), np.float32), np.zeros((4, 4, 5), np.float32))
         with pytest.raises(TjError):
             fsc_volumes(np.zeros((4, 70000, 1), np.float32), np.zeros((4, 70000, 1), np.float32))
+
+
+def test_resolution_estimation_ssnr(tj, oracle):
+    """ResolutionEstimation.SSNR(): the signal and noise reconstructions on the
+    device, then the nine rows against the oracle's FP64 restatement on the
+    same supplied images and the oracle's projections of the same volumes
+    (FP32 transforms on the device: ratios to 1e-4, dB rows to 1e-3 dB)."""
+    from tomoj_amd import parameters as RP
+    from tomoj_amd.resolution import ResolutionEstimation
+    W, H, D = 48, 10, 16
+    tilts = list(np.arange(-60, 61, 6.0))
+    E = oracle.tilt_eulers(tilts)
+    exps = (oracle.phantom_project(oracle.ellipsoids(37, 6, W, H, D), E, W, H)
+            + 0.2 * oracle.gaussian_noise(8, len(E) * W * H).reshape(len(E), H, W)).astype(np.float32)
+    ts = RP.TiltSeriesData(exps, tilts)
+    p = RP.ReconstructionParameters.createOSSARTParameters(W, H, D, 3, 1.0, 0)
+    g = oracle.make_geom(W, H, D, nthreads=8)
+    r = ResolutionEstimation(ts, p)
+    try:
+        got = r.SSNR(noise_seed=5)
+        sets = []
+        for m in (r.recThManSignal, r.recThManNoise):
+            sup = np.stack([m.projector.get_loaded_projection(k) for k in range(len(E))])
+            th, _ = oracle.project(g, m.getRec2(), E)
+            sets += [sup, th]
+    finally:
+        r.close()
+    ref = oracle.ssnr_rows(oracle.ssnr_sums(*sets, ring_width=4))
+    assert got.shape == ref.shape == (9, W // 2 + 1)
+    assert np.array_equal(got[0], ref[0]) and np.allclose(got[1], ref[1], rtol=0, atol=1e-15)
+    ok = np.isfinite(ref[3])
+    for row in (3, 6):
+        assert np.allclose(got[row][ok], ref[row][ok], rtol=1e-4)
+    for row in (4, 5, 7, 8):
+        assert np.allclose(got[row][ok], ref[row][ok], atol=1e-3)
+    both = (got[2] > -1000) & (ref[2] > -1000)
+    assert np.allclose(got[2][both], ref[2][both], atol=1e-2)

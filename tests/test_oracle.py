@@ -884,3 +884,52 @@ def test_polygon_roi_statistics(oracle):
     assert sd == pytest.approx(sel.std(ddof=1), rel=1e-9)
     m, sd = oracle.poly_stats(img, [-5, 50, 50, -5, -5, -5, 40, 40])
     assert m == pytest.approx(img.astype(np.float64).mean(), rel=1e-12)
+
+
+# --------------------------------------------------------------------------
+# spectral SNR (ResolutionEstimation.SSNR)
+# --------------------------------------------------------------------------
+
+def test_ssnr_ring_sums_brute_force(oracle):
+    """The ring sums against a direct evaluation: explicit DFT sums, each
+    frequency's radius from convertToFrequency, and its rings walked one by one
+    with ceil/floor (NaN in an input counts as 0)."""
+    rng = np.random.default_rng(14)
+    n, h, w = 2, 5, 6
+    a = [rng.standard_normal((n, h, w)).astype(np.float32) for _ in range(4)]
+    a[0][1, 2, 3] = np.nan
+    sums = oracle.ssnr_sums(*a, ring_width=2)
+    want = np.zeros((5, w))
+    y, x = np.mgrid[0:h, 0:w]
+    for t in range(n):
+        se, st, ne, nt = (np.nan_to_num(b[t]) for b in a)
+        imgs = (st, (se - st).astype(np.float32), nt, (ne - nt).astype(np.float32))
+        for ky in range(h):
+            for kx in range(w):
+                fxx = (kx if kx <= w // 2 else kx - w) / w
+                fyy = (ky if ky <= h // 2 else ky - h) / h
+                wi = np.sqrt(fxx * fxx + fyy * fyy) * w
+                ph = np.exp(-2j * np.pi * (ky * y / h + kx * x / w))
+                pw = [abs(np.sum(im.astype(np.float64) * ph)) ** 2 for im in imgs]
+                for l in range(max(0, int(np.ceil(wi - 2))), int(np.floor(wi))):
+                    for r in range(4):
+                        want[r][l] += pw[r]
+                    want[4][l] += 1
+    assert np.allclose(sums, want, rtol=1e-10, atol=1e-9)
+
+
+def test_ssnr_rows_known_answers(oracle):
+    """Equal signal and noise sums give SSNR 1 (corrected -1000) and equal
+    signal/noise rows; a signal ten times the noise power ratio gives
+    10 log10(10 - 1) dB; the frequency row is i/W over pixel size."""
+    w = 8
+    K = np.arange(1, w + 1, dtype=np.float64)
+    base = np.array([np.full(w, 4.0), np.full(w, 2.0), np.full(w, 4.0), np.full(w, 2.0), K])
+    r = oracle.ssnr_rows(base, pixel_size=2.0)
+    assert r.shape == (9, w // 2 + 1)
+    assert np.all(r[2] == -1000) and np.allclose(r[3], r[6])
+    assert np.allclose(r[1], np.arange(w // 2 + 1) / w / 2.0)
+    assert np.allclose(r[4], 10 * np.log10(4.0 / K[:5] / (w + 1)))
+    strong = base.copy()
+    strong[0] *= 10
+    assert np.allclose(oracle.ssnr_rows(strong)[2], 10 * np.log10(9.0))

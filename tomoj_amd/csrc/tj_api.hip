@@ -49,6 +49,9 @@ hipError_t ramp_init(hipStream_t, int, int, int, double, double, float*);
 hipError_t weighting_apply(hipStream_t, int, int, int, const float*, const int*, const float*, float*, void*, size_t);
 size_t weighting_workspace(int, int, int);
 size_t fsc_workspace(int, int, int);
+size_t ssnr_workspace(int, int);
+hipError_t ssnr(hipStream_t, const float*, const float*, const float*, const float*, int, int, int, int, void*,
+                size_t, double*);
 hipError_t fsc(hipStream_t, const float*, const float*, int, int, int, void*, size_t, double*);
 }  // namespace tjk
 
@@ -1388,6 +1391,93 @@ int tj_fsc_volumes(int device, const float* even_zyx, const float* odd_zyx, int 
         return fail(e == hipErrorOutOfMemory ? TJ_ENOMEM : TJ_EDEVICE, "tj_fsc_volumes: %s", hipGetErrorString(e));
     }
     return rc;
+}
+
+// ReconstructionThreadManager.getProjection(i) for every image
+// (ReconstructionThreadManager.java:257-270): addProjection(i), project(),
+// getProjection(0); all n at once into the context's subset buffers.
+static int project_all(tj_ctx* c)
+{
+    int rc = ensure_slots(c, c->nproj);
+    if (rc) return rc;
+    c->added.resize(c->nproj);
+    for (int k = 0; k < c->nproj; ++k) c->added[k] = k;
+    if ((rc = upload_added(c))) return rc;
+    if ((rc = do_project(c, c->slot_proj, c->added, 0))) return rc;
+    c->added.clear();
+    c->have_diff = c->have_proj = false;
+    return TJ_OK;
+}
+
+int tj_ssnr(tj_ctx* sig, tj_ctx* noi, int ring_width, double pixel_size, double* out, int* nfreq)
+{
+    if (!sig || !noi || !out || !nfreq) return fail(TJ_EINVAL, "tj_ssnr: null argument");
+    const TjGeomDev &a = sig->g, &b = noi->g;
+    if (a.pw != b.pw || a.ph != b.ph || sig->nproj != noi->nproj || a.vw != b.vw || a.vh != b.vh || a.vd != b.vd)
+        return fail(TJ_EINVAL, "tj_ssnr: the signal and noise sets differ in size");
+    if (sig->device != noi->device) return fail(TJ_EINVAL, "tj_ssnr: the sets are on different devices");
+    if (sig->nproj <= 0) return fail(TJ_ESTATE, "tj_ssnr: no resident projections");
+    if (ring_width < 1) return fail(TJ_EINVAL, "tj_ssnr: ring width %d", ring_width);
+    const int W = a.pw, H = a.ph, n = sig->nproj;
+    if (sizeof(double) * 5 * W > 64 * 1024) return fail(TJ_EINVAL, "tj_ssnr: detector width %d too large", W);
+    TJ_HIP(hipSetDevice(sig->device));
+    int rc = project_all(sig);
+    if (!rc) rc = project_all(noi);
+    if (rc) return rc;
+    TJ_HIP(hipStreamSynchronize(noi->stream));
+    const size_t ws_size = tjk::ssnr_workspace(W, H);
+    void* ws = nullptr;
+    double* sums = nullptr;
+    if (hipMalloc(&ws, ws_size) != hipSuccess || hipMalloc((void**)&sums, sizeof(double) * 5 * W) != hipSuccess) {
+        (void)hipGetLastError();
+        if (ws) (void)hipFree(ws);
+        return fail(TJ_ENOMEM, "tj_ssnr: cannot allocate %zu bytes of workspace", ws_size);
+    }
+    std::vector<double> h(5 * (size_t)W);
+    hipError_t e = tjk::ssnr(sig->stream, sig->exps, sig->sproj, noi->exps, noi->sproj, W, H, n, ring_width, ws,
+                             ws_size, sums);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), sums, sizeof(double) * 5 * W, hipMemcpyDeviceToHost, sig->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(sig->stream);
+    (void)hipFree(ws);
+    (void)hipFree(sums);
+    if (e != hipSuccess) return fail(TJ_EDEVICE, "tj_ssnr: %s", hipGetErrorString(e));
+    // SSNR (:212-260): ratios, per-ring means, then rows for the non-negative
+    // frequencies; TomoJ divides the dB rows by its counter imgno, which ends
+    // at W + 1.
+    double *SS = h.data(), *SN = SS + W, *NS = SN + W, *NN = NS + W, *K = NN + W;
+    std::vector<double> ssnr1(W), nsnr1(W);
+    int imgno = 1;
+    for (int i = 0; i < W; ++i) {
+        ssnr1[i] = SS[i] / SN[i];
+        SS[i] /= K[i];
+        SN[i] /= K[i];
+        nsnr1[i] = NS[i] / NN[i];
+        NS[i] /= K[i];
+        NN[i] /= K[i];
+        imgno++;
+    }
+    const int R = W / 2 + 1;
+    int imax = 0;
+    for (int i = 0; i < W; ++i) {
+        const double w = (double)((i <= (W >> 1)) ? i : i - W) / W;  // convertToFrequency
+        if (w < 0) {
+            imax = i;
+            break;
+        }
+        const double ssnr = ssnr1[i] / nsnr1[i];
+        out[0 * R + i] = i;
+        out[1 * R + i] = w / pixel_size;
+        out[2 * R + i] = ssnr > 1 ? 10 * std::log10(ssnr - 1) : -1000;
+        out[3 * R + i] = ssnr1[i];
+        out[4 * R + i] = 10 * std::log10(SS[i] / imgno);
+        out[5 * R + i] = 10 * std::log10(SN[i] / imgno);
+        out[6 * R + i] = nsnr1[i];
+        out[7 * R + i] = 10 * std::log10(NS[i] / imgno);
+        out[8 * R + i] = 10 * std::log10(NN[i] / imgno);
+        imax++;
+    }
+    *nfreq = imax;
+    return TJ_OK;
 }
 
 int tj_available_indexes(int nproj, int projection_type, int* out)

@@ -303,10 +303,116 @@ static hipError_t rows_fft(hipStream_t st, const float2* in, float2* out, const 
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// SSNR per-tilt vectors, ResolutionEstimation.addToVectors1D
+// (ResolutionEstimation.java:581-697, the computeVSSNR = false path): for
+// each tilt, NaN -> 0 in the measured and re-projected images of the signal
+// and noise reconstructions, the differences measured - projected (float),
+// the 2-D power spectra of difference and projection for both (FP32
+// transforms, powers in FP64), and ring sums: a frequency at radius w*W adds
+// to rings [max(0, ceil(w*W - ringWidth)), floor(w*W)).
+// ---------------------------------------------------------------------------
+
+// out[4b + {0,1,2,3}] = signal - thSignal, thSignal, noise - thNoise, thNoise
+__global__ __launch_bounds__(256) void k_ssnr_prep(const float* __restrict__ sexp, const float* __restrict__ sth,
+                                                   const float* __restrict__ nexp, const float* __restrict__ nth,
+                                                   size_t npx, int t0, float2* __restrict__ out)
+{
+    const int b = blockIdx.y;
+    const size_t t = (size_t)(t0 + b);
+    auto nz = [](float v) { return v != v ? 0.0f : v; };  // convertNaNToZeros
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < npx; k += (size_t)gridDim.x * blockDim.x) {
+        const float s = nz(sexp[t * npx + k]), ts = nz(sth[t * npx + k]);
+        const float n = nz(nexp[t * npx + k]), tn = nz(nth[t * npx + k]);
+        out[(4 * (size_t)b + 0) * npx + k] = make_float2(s - ts, 0.0f);
+        out[(4 * (size_t)b + 1) * npx + k] = make_float2(ts, 0.0f);
+        out[(4 * (size_t)b + 2) * npx + k] = make_float2(n - tn, 0.0f);
+        out[(4 * (size_t)b + 3) * npx + k] = make_float2(tn, 0.0f);
+    }
+}
+
+// spectra: nb quads of images laid out [x][y] (ph fastest); sums[5][W]:
+// S_S (thSignal), S_N (signal - thSignal), N_S (thNoise), N_N (noise -
+// thNoise), K (count).
+__global__ __launch_bounds__(256) void k_ssnr_rings(const float2* __restrict__ F, int pw, int ph, int nb,
+                                                    int ring_width, double* __restrict__ sums)
+{
+    extern __shared__ double sh[];  // [5][pw]
+    for (int k = threadIdx.x; k < 5 * pw; k += blockDim.x) sh[k] = 0.0;
+    __syncthreads();
+    const size_t npx = (size_t)pw * ph;
+    const size_t total = npx * nb;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t b = e / npx, r = e - b * npx;
+        const int x = (int)(r / ph), y = (int)(r - (size_t)x * ph);
+        double ty = fsc_freq(y, ph);
+        ty *= ty;
+        double tx = fsc_freq(x, pw);
+        tx *= tx;
+        const double wi = sqrt(tx + ty) * pw;
+        const int l0 = max(0, (int)ceil(wi - ring_width));
+        const int lf = (int)floor(wi);
+        if (l0 >= lf) continue;
+        const float2* q = F + 4 * b * npx + r;
+        auto pw2 = [](float2 v) { return (double)v.x * v.x + (double)v.y * v.y; };
+        const double sn = pw2(q[0]), ss = pw2(q[npx]), nn = pw2(q[2 * npx]), ns = pw2(q[3 * npx]);
+        for (int l = l0; l < lf; ++l) {
+            atomicAdd(&sh[l], ss);
+            atomicAdd(&sh[pw + l], sn);
+            atomicAdd(&sh[2 * pw + l], ns);
+            atomicAdd(&sh[3 * pw + l], nn);
+            atomicAdd(&sh[4 * pw + l], 1.0);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 5 * pw; k += blockDim.x)
+        if (sh[k] != 0.0) atomicAdd(&sums[k], sh[k]);
+}
+
 }  // namespace tjf
 
 namespace tjk {
 using namespace tjf;
+
+constexpr int kSsnrBatch = 8;  // tilts per pass
+
+size_t ssnr_workspace(int pw, int ph)
+{
+    return sizeof(float2) * (2 * 4 * (size_t)kSsnrBatch * pw * ph + (size_t)pw + ph) + 256;
+}
+
+// Ring sums of all n tilts into sums[5][pw] (zeroed here).  sexp/nexp: the
+// measured images, sth/nth: the projections of the reconstructions, n x ph x pw.
+hipError_t ssnr(hipStream_t st, const float* sexp, const float* sth, const float* nexp, const float* nth, int pw,
+                int ph, int n, int ring_width, void* ws, size_t ws_size, double* sums)
+{
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    if (ws_size < ssnr_workspace(pw, ph)) return hipErrorInvalidValue;
+    if (sizeof(double) * 5 * pw > 64 * 1024) return hipErrorInvalidValue;
+    const size_t npx = (size_t)pw * ph;
+    float2* A = (float2*)ws;
+    float2* B = A + 4 * (size_t)kSsnrBatch * npx;
+    float2* tww = B + 4 * (size_t)kSsnrBatch * npx;
+    float2* twh = tww + pw;
+    k_twiddles<<<(pw + 255) / 256, 256, 0, st>>>(pw, tww);
+    k_twiddles<<<(ph + 255) / 256, 256, 0, st>>>(ph, twh);
+    hipError_t e = hipMemsetAsync(sums, 0, sizeof(double) * 5 * pw, st);
+    if (e != hipSuccess) return e;
+    for (int t0 = 0; t0 < n; t0 += kSsnrBatch) {
+        const int m = std::min(kSsnrBatch, n - t0);
+        const unsigned bx = (unsigned)std::min<size_t>((npx + 255) / 256, 1024);
+        k_ssnr_prep<<<dim3(bx, m), 256, 0, st>>>(sexp, sth, nexp, nth, npx, t0, A);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        // rows along x, transpose each image to [x][y], rows along y
+        if ((e = rows_fft(st, A, B, tww, pw, (size_t)4 * m * ph)) != hipSuccess) return e;
+        k_transpose<<<dim3((pw + 31) / 32, (ph + 31) / 32, 4 * m), dim3(32, 8), 0, st>>>(B, A, ph, pw);
+        if ((e = rows_fft(st, A, B, twh, ph, (size_t)4 * m * pw)) != hipSuccess) return e;
+        const unsigned blocks = (unsigned)std::min<size_t>((npx * m + 255) / 256, 1024);
+        k_ssnr_rings<<<blocks, 256, sizeof(double) * 5 * pw, st>>>(B, pw, ph, m, ring_width, sums);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipGetLastError();
+}
 
 // Workspace of tjk::fsc: three complex volumes and the twiddle tables.
 size_t fsc_workspace(int W, int H, int D)

@@ -5,11 +5,13 @@
 two half reconstructions of generateFSCData (:115-165: the odd images, then the
 even ones, each through the CPU-manager driver with the user's parameters) and
 correlates them on the device (``tj_fsc``); ``fsc_volumes`` is the static
-fsc(ImagePlus, ImagePlus) (:456-557) for host volumes.
+fsc(ImagePlus, ImagePlus) (:456-557) for host volumes; ``SSNR()`` the
+spectral SNR of a reconstruction against one of pure noise (:167-260).
 """
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 
 import numpy as np
 
@@ -36,7 +38,7 @@ def fsc_volumes(even: np.ndarray, odd: np.ndarray, device: int = 0) -> np.ndarra
 
 
 class ResolutionEstimation:
-    """ResolutionEstimation(tsSignal, parameters) (:70-74), FSC part."""
+    """ResolutionEstimation(tsSignal, parameters) (:70-74): FSC and SSNR."""
 
     def __init__(self, ts: TiltSeriesData, params: ReconstructionParameters, device: int = 0):
         self.ts = ts
@@ -44,6 +46,10 @@ class ResolutionEstimation:
         self.device = device
         self.recManFSCodd: ReconstructionThreadManager | None = None
         self.recManFSCeven: ReconstructionThreadManager | None = None
+        self.recThManSignal: ReconstructionThreadManager | None = None
+        self.recThManNoise: ReconstructionThreadManager | None = None
+        self.ringWidth = 4          # :37
+        self.pixelSize = 1.0        # :38
 
     def generateFSCData(self, first_index: int = 0) -> None:
         """:115-165: the odd half, then the even half, then back to all images."""
@@ -66,7 +72,37 @@ class ResolutionEstimation:
         call("tj_fsc", self.recManFSCeven.projector._ctx, self.recManFSCodd.projector._ctx, _ptr(out))
         return out
 
+    def SSNR(self, noise_seed: int = 1, first_index: int = 0) -> np.ndarray:
+        """SSNR(false) (:167-260): the signal reconstruction, a tilt series of
+        N(0, 1) noise images (generateSSNRNoiseData, :93-112; TomoJ draws them
+        from ImageJ's unseeded generator, here from `noise_seed`) supplied and
+        reconstructed the same way, and the nine rows of the ring statistics
+        (``tj_ssnr``).  Returns a 9 x (W/2+1) array."""
+        p = self.recParameters
+        p.setFscType(ALL_PROJECTIONS)
+        self.recThManSignal = self.recThManSignal or ReconstructionThreadManager(self.ts, self.device)
+        self.recThManSignal.createRec2(p)
+        self.recThManSignal.doRec(p, first_index)
+        rng = np.random.default_rng(noise_seed)
+        noise = rng.standard_normal(np.shape(self.ts.images)).astype(np.float32)
+        tsn = dataclasses.replace(self.ts, images=noise)
+        self.recThManNoise = self.recThManNoise or ReconstructionThreadManager(tsn, self.device)
+        self.recThManNoise.ts = tsn
+        self.recThManNoise.createRec2(p)
+        self.recThManNoise.doRec(p, first_index)
+        W = self.ts.width
+        out = np.zeros((9, W // 2 + 1), np.float64)
+        nf = C.c_int()
+        call("tj_ssnr", self.recThManSignal.projector._ctx, self.recThManNoise.projector._ctx, int(self.ringWidth),
+             float(self.pixelSize), _ptr(out), C.byref(nf))
+        self.ssnr = out[:, :nf.value]
+        return self.ssnr
+
     def close(self) -> None:
+        for m in (self.recThManSignal, self.recThManNoise):
+            if m is not None:
+                m.close()
+        self.recThManSignal = self.recThManNoise = None
         for m in (self.recManFSCodd, self.recManFSCeven):
             if m is not None:
                 m.close()
