@@ -242,8 +242,12 @@ int tj_fsc(tj_ctx *even, tj_ctx *odd, double *out);
  * pixel_size, corrected SSNR in dB (-1000 where SSNR <= 1), signal SNR, signal
  * and noise powers in dB, noise SNR and its powers in dB; *nfreq = the number
  * of columns filled (W/2+1).  FP32 transforms (TomoJ's are FP64), FP64 sums.
- * The VSSNR map (computeVSSNR) is not built. */
-int tj_ssnr(tj_ctx *signal, tj_ctx *noise, int ring_width, double pixel_size, double *out, int *nfreq);
+ * vssnr (may be NULL): n x proj_h x proj_w floats, the per-pixel SSNR maps of
+ * the computeVSSNR path (:648-663), 10 log10(max(ISSNR/alpha - 1, 0) + 1) in
+ * swapQuadrant2D's order (:720-733; an odd width's last column stays 0): the
+ * tilt series SSNR(true) reconstructs into the VSSNR volume. */
+int tj_ssnr(tj_ctx *signal, tj_ctx *noise, int ring_width, double pixel_size, double *out, int *nfreq,
+            float *vssnr);
 
 /* The same for two host volumes in ImageStack layout [D][H][W] (the static
  * fsc(ImagePlus, ImagePlus)); copies them to `device` for the call. */

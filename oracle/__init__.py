@@ -547,6 +547,30 @@ def ssnr_sums(sig_exps, sig_th, noi_exps, noi_th, ring_width=4) -> np.ndarray:
     return sums
 
 
+def vssnr_maps(sig_exps, sig_th, noi_exps, noi_th) -> np.ndarray:
+    """The per-pixel SSNR maps of SSNR(true) (:648-663) with FP64 transforms:
+    10 log10(max(ISSNR/alpha - 1, 0) + 1) (NaN kept, as Java's Math.max keeps
+    it), laid out by swapQuadrant2D (:720-733): row y to (y + H/2) mod H, its
+    first W/2 entries to the right half, the next W/2 to the left, an odd
+    width's last column left 0."""
+    se, st, ne, nt = (np.nan_to_num(_c(a, np.float32), nan=0.0, posinf=np.inf, neginf=-np.inf)
+                      for a in (sig_exps, sig_th, noi_exps, noi_th))
+    n, h, w = se.shape
+    cx, cy = w // 2, h // 2
+    out = np.zeros((n, h, w), np.float32)
+    for t in range(n):
+        p = [np.abs(np.fft.fft2(a.astype(np.float64))) ** 2
+             for a in ((se[t] - st[t]).astype(np.float32), st[t], (ne[t] - nt[t]).astype(np.float32), nt[t])]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            aux = (p[1] / p[0]) / (p[3] / p[2]) - 1.0
+            m = (10.0 * np.log10(np.where(np.isnan(aux), aux, np.maximum(aux, 0.0)) + 1.0)).astype(np.float32)
+        for y in range(h):
+            yc = (y + cy) % h
+            out[t, yc, cx:2 * cx] = m[y, :cx]
+            out[t, yc, :cx] = m[y, cx:2 * cx]
+    return out
+
+
 def ssnr_rows(sums, pixel_size=1.0) -> np.ndarray:
     """SSNR (:212-260) from the ring sums: nine rows over the non-negative
     frequencies; the dB rows divide by TomoJ's counter imgno = W + 1."""

@@ -1025,7 +1025,8 @@ def test_resolution_estimation_ssnr(tj, oracle):
     g = oracle.make_geom(W, H, D, nthreads=8)
     r = ResolutionEstimation(ts, p)
     try:
-        got = r.SSNR(noise_seed=5)
+        got = r.SSNR(computeVSSNR=True, noise_seed=5)
+        maps = r.tsVSSNR.images
         sets = []
         for m in (r.recThManSignal, r.recThManNoise):
             sup = np.stack([m.projector.get_loaded_projection(k) for k in range(len(E))])
@@ -1043,3 +1044,11 @@ def test_resolution_estimation_ssnr(tj, oracle):
         assert np.allclose(got[row][ok], ref[row][ok], atol=1e-3)
     both = (got[2] > -1000) & (ref[2] > -1000)
     assert np.allclose(got[2][both], ref[2][both], atol=1e-2)
+    # the VSSNR maps: 10 log10(SSNR + 1) per frequency; compared where the
+    # oracle's value is finite and not at the max(., 0) clamp's edge
+    ref_maps = oracle.vssnr_maps(*sets)
+    assert maps.shape == ref_maps.shape
+    ok = np.isfinite(ref_maps) & np.isfinite(maps) & (np.abs(ref_maps) > 1e-3)
+    assert ok.mean() > 0.5
+    assert np.allclose(maps[ok], ref_maps[ok], rtol=1e-3, atol=1e-3)
+    assert r.recVSSNR.shape == (D, H, W) and np.isfinite(r.recVSSNR).all()

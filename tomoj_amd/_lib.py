@@ -113,7 +113,7 @@ _SIGS = {
     "tj_available_indexes": (C.c_int, [C.c_int, C.c_int, _P]),
     "tj_threshold_projections": (C.c_int, [_P, C.c_double, C.c_double, C.c_int]),
     "tj_fsc": (C.c_int, [_P, _P, _P]),
-    "tj_ssnr": (C.c_int, [_P, _P, C.c_int, C.c_double, _P, _P]),
+    "tj_ssnr": (C.c_int, [_P, _P, C.c_int, C.c_double, _P, _P, _P]),
     "tj_fsc_volumes": (C.c_int, [C.c_int, _P, _P, C.c_int, C.c_int, C.c_int, _P]),
     "tj_set_available_indexes": (C.c_int, [_P, _P, C.c_int]),
     "tj_tv_denoise": (C.c_int, [_P, C.c_float, C.c_float, C.c_float, C.c_int]),

@@ -51,7 +51,7 @@ size_t weighting_workspace(int, int, int);
 size_t fsc_workspace(int, int, int);
 size_t ssnr_workspace(int, int);
 hipError_t ssnr(hipStream_t, const float*, const float*, const float*, const float*, int, int, int, int, void*,
-                size_t, double*);
+                size_t, double*, float*);
 hipError_t fsc(hipStream_t, const float*, const float*, int, int, int, void*, size_t, double*);
 }  // namespace tjk
 
@@ -1409,7 +1409,7 @@ static int project_all(tj_ctx* c)
     return TJ_OK;
 }
 
-int tj_ssnr(tj_ctx* sig, tj_ctx* noi, int ring_width, double pixel_size, double* out, int* nfreq)
+int tj_ssnr(tj_ctx* sig, tj_ctx* noi, int ring_width, double pixel_size, double* out, int* nfreq, float* vssnr)
 {
     if (!sig || !noi || !out || !nfreq) return fail(TJ_EINVAL, "tj_ssnr: null argument");
     const TjGeomDev &a = sig->g, &b = noi->g;
@@ -1428,18 +1428,24 @@ int tj_ssnr(tj_ctx* sig, tj_ctx* noi, int ring_width, double pixel_size, double*
     const size_t ws_size = tjk::ssnr_workspace(W, H);
     void* ws = nullptr;
     double* sums = nullptr;
-    if (hipMalloc(&ws, ws_size) != hipSuccess || hipMalloc((void**)&sums, sizeof(double) * 5 * W) != hipSuccess) {
+    float* maps = nullptr;
+    if (hipMalloc(&ws, ws_size) != hipSuccess || hipMalloc((void**)&sums, sizeof(double) * 5 * W) != hipSuccess ||
+        (vssnr && hipMalloc((void**)&maps, sizeof(float) * sig->npx * n) != hipSuccess)) {
         (void)hipGetLastError();
         if (ws) (void)hipFree(ws);
+        if (sums) (void)hipFree(sums);
         return fail(TJ_ENOMEM, "tj_ssnr: cannot allocate %zu bytes of workspace", ws_size);
     }
     std::vector<double> h(5 * (size_t)W);
     hipError_t e = tjk::ssnr(sig->stream, sig->exps, sig->sproj, noi->exps, noi->sproj, W, H, n, ring_width, ws,
-                             ws_size, sums);
+                             ws_size, sums, maps);
     if (e == hipSuccess) e = hipMemcpyAsync(h.data(), sums, sizeof(double) * 5 * W, hipMemcpyDeviceToHost, sig->stream);
+    if (e == hipSuccess && maps)
+        e = hipMemcpyAsync(vssnr, maps, sizeof(float) * sig->npx * n, hipMemcpyDeviceToHost, sig->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(sig->stream);
     (void)hipFree(ws);
     (void)hipFree(sums);
+    if (maps) (void)hipFree(maps);
     if (e != hipSuccess) return fail(TJ_EDEVICE, "tj_ssnr: %s", hipGetErrorString(e));
     // SSNR (:212-260): ratios, per-ring means, then rows for the non-negative
     // frequencies; TomoJ divides the dB rows by its counter imgno, which ends

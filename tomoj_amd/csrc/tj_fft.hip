@@ -369,6 +369,35 @@ __global__ __launch_bounds__(256) void k_ssnr_rings(const float2* __restrict__ F
         if (sh[k] != 0.0) atomicAdd(&sums[k], sh[k]);
 }
 
+// The per-pixel SSNR map of the VSSNR path (:648-663): ISSNR = P(thSignal) /
+// P(signal - thSignal), alpha the same for the noise, SSNR = max(ISSNR/alpha -
+// 1, 0) (Java's max keeps NaN), value (float)(10 log10(SSNR + 1)), placed by
+// swapQuadrant2D (:720-733) with centres (W/2, H/2): the first W/2 entries of a
+// row move right by W/2, the next W/2 to the start, the row to (y + H/2) mod H;
+// an odd width's last column is not copied (the caller zeroes the maps).
+__global__ __launch_bounds__(256) void k_vssnr_map(const float2* __restrict__ F, int pw, int ph, int nb, int t0,
+                                                   float* __restrict__ maps)
+{
+    const size_t npx = (size_t)pw * ph;
+    const size_t total = npx * nb;
+    const int cx = pw / 2, cy = ph / 2;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t b = e / npx, r = e - b * npx;
+        const int x = (int)(r / ph), y = (int)(r - (size_t)x * ph);
+        if (x >= 2 * cx) continue;
+        const float2* q = F + 4 * b * npx + r;
+        auto pw2 = [](float2 v) { return (double)v.x * v.x + (double)v.y * v.y; };
+        const double issnr = pw2(q[npx]) / pw2(q[0]);
+        const double alpha = pw2(q[3 * npx]) / pw2(q[2 * npx]);
+        const double aux = issnr / alpha - 1.0;
+        const double ssnr = (aux != aux) ? aux : (aux > 0.0 ? aux : 0.0);
+        int yy = y + cy;
+        if (yy >= ph) yy -= ph;
+        const int xx = (x < cx) ? x + cx : x - cx;
+        maps[((size_t)(t0 + b) * ph + yy) * pw + xx] = (float)(10.0 * log10(ssnr + 1.0));
+    }
+}
+
 }  // namespace tjf
 
 namespace tjk {
@@ -384,7 +413,7 @@ size_t ssnr_workspace(int pw, int ph)
 // Ring sums of all n tilts into sums[5][pw] (zeroed here).  sexp/nexp: the
 // measured images, sth/nth: the projections of the reconstructions, n x ph x pw.
 hipError_t ssnr(hipStream_t st, const float* sexp, const float* sth, const float* nexp, const float* nth, int pw,
-                int ph, int n, int ring_width, void* ws, size_t ws_size, double* sums)
+                int ph, int n, int ring_width, void* ws, size_t ws_size, double* sums, float* maps)
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (ws_size < ssnr_workspace(pw, ph)) return hipErrorInvalidValue;
@@ -397,6 +426,7 @@ hipError_t ssnr(hipStream_t st, const float* sexp, const float* sth, const float
     k_twiddles<<<(pw + 255) / 256, 256, 0, st>>>(pw, tww);
     k_twiddles<<<(ph + 255) / 256, 256, 0, st>>>(ph, twh);
     hipError_t e = hipMemsetAsync(sums, 0, sizeof(double) * 5 * pw, st);
+    if (e == hipSuccess && maps) e = hipMemsetAsync(maps, 0, sizeof(float) * npx * n, st);
     if (e != hipSuccess) return e;
     for (int t0 = 0; t0 < n; t0 += kSsnrBatch) {
         const int m = std::min(kSsnrBatch, n - t0);
@@ -409,6 +439,7 @@ hipError_t ssnr(hipStream_t st, const float* sexp, const float* sth, const float
         if ((e = rows_fft(st, A, B, twh, ph, (size_t)4 * m * pw)) != hipSuccess) return e;
         const unsigned blocks = (unsigned)std::min<size_t>((npx * m + 255) / 256, 1024);
         k_ssnr_rings<<<blocks, 256, sizeof(double) * 5 * pw, st>>>(B, pw, ph, m, ring_width, sums);
+        if (maps) k_vssnr_map<<<blocks, 256, 0, st>>>(B, pw, ph, m, t0, maps);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipGetLastError();

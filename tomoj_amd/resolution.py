@@ -72,12 +72,15 @@ class ResolutionEstimation:
         call("tj_fsc", self.recManFSCeven.projector._ctx, self.recManFSCodd.projector._ctx, _ptr(out))
         return out
 
-    def SSNR(self, noise_seed: int = 1, first_index: int = 0) -> np.ndarray:
-        """SSNR(false) (:167-260): the signal reconstruction, a tilt series of
-        N(0, 1) noise images (generateSSNRNoiseData, :93-112; TomoJ draws them
-        from ImageJ's unseeded generator, here from `noise_seed`) supplied and
-        reconstructed the same way, and the nine rows of the ring statistics
-        (``tj_ssnr``).  Returns a 9 x (W/2+1) array."""
+    def SSNR(self, computeVSSNR: bool = False, noise_seed: int = 1, first_index: int = 0) -> np.ndarray:
+        """SSNR(computeVSSNR) (:167-260): the signal reconstruction, a tilt
+        series of N(0, 1) noise images (generateSSNRNoiseData, :93-112; TomoJ
+        draws them from ImageJ's unseeded generator, here from `noise_seed`)
+        supplied and reconstructed the same way, and the nine rows of the ring
+        statistics (``tj_ssnr``); returns a 9 x (W/2+1) array.  With
+        computeVSSNR the per-pixel SSNR maps form a new tilt series (TomoJ's
+        defaults: identity alignment, normalised) reconstructed with the same
+        parameters into ``recVSSNR`` (:250-258)."""
         p = self.recParameters
         p.setFscType(ALL_PROJECTIONS)
         self.recThManSignal = self.recThManSignal or ReconstructionThreadManager(self.ts, self.device)
@@ -93,9 +96,19 @@ class ResolutionEstimation:
         W = self.ts.width
         out = np.zeros((9, W // 2 + 1), np.float64)
         nf = C.c_int()
+        maps = np.zeros(np.shape(self.ts.images), np.float32) if computeVSSNR else None
         call("tj_ssnr", self.recThManSignal.projector._ctx, self.recThManNoise.projector._ctx, int(self.ringWidth),
-             float(self.pixelSize), _ptr(out), C.byref(nf))
+             float(self.pixelSize), _ptr(out), C.byref(nf), _ptr(maps) if computeVSSNR else None)
         self.ssnr = out[:, :nf.value]
+        if computeVSSNR:
+            self.tsVSSNR = TiltSeriesData(maps, list(self.ts.tilts), normalize=True)
+            m = ReconstructionThreadManager(self.tsVSSNR, self.device)
+            try:
+                m.createRec2(p)
+                m.doRec(p, first_index)
+                self.recVSSNR = m.getRec2()
+            finally:
+                m.close()
         return self.ssnr
 
     def close(self) -> None:
