@@ -936,16 +936,17 @@ def test_ssnr_rows_known_answers(oracle):
 
 
 def test_vssnr_map_closed_form(oracle):
-    """Signal images twice the noise ones (projections and differences alike):
-    ISSNR/alpha = 4 at every frequency, so every map value is 10 log10(4);
-    with an odd width swapQuadrant2D leaves the last column at 0."""
+    """The signal's projections twice the noise's, the differences measured -
+    projected equal (small integers, so every float sum is exact): ISSNR/alpha
+    = 4 at every frequency, each map value 10 log10(4); with an odd width
+    swapQuadrant2D leaves the last column at 0."""
     rng = np.random.default_rng(19)
     n, h, w = 2, 6, 7
-    th_n = rng.standard_normal((n, h, w)).astype(np.float32)
-    d = rng.standard_normal((n, h, w)).astype(np.float32)
+    th_n = rng.integers(-8, 9, (n, h, w)).astype(np.float32)
+    d = rng.integers(-8, 9, (n, h, w)).astype(np.float32)
     noi_e = (th_n + d).astype(np.float32)
     sig_th = 2 * th_n
-    sig_e = (sig_th + 2 * d).astype(np.float32)
+    sig_e = (sig_th + d).astype(np.float32)
     maps = oracle.vssnr_maps(sig_e, sig_th, noi_e, th_n)
     assert np.all(maps[:, :, w - 1] == 0)
     assert np.allclose(maps[:, :, :w - 1], 10 * np.log10(4.0), atol=1e-4)
