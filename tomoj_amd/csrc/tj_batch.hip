@@ -55,6 +55,10 @@ struct Bf16 {
 // flat instructions (flat loads also count against the LDS wait counter).
 #define TJ_GLOBAL __attribute__((address_space(1)))
 #define TJ_LDS __attribute__((address_space(3)))
+// LDS addresses are 32-bit byte offsets computed in registers and turned into
+// address-space-3 pointers where they are read; the host pass, which sees
+// 64-bit pointers, would warn about each of those casts.
+#pragma clang diagnostic ignored "-Wint-to-pointer-cast"
 template <typename T>
 __device__ __forceinline__ const TJ_GLOBAL T* gp(const T* p) { return (const TJ_GLOBAL T*)p; }
 template <typename T>
