@@ -81,6 +81,11 @@ const char *tj_version(void);
  * role for OpenCL devices). */
 int tj_device_info(int device, char *buf, int len);
 
+/* ProjectorGPU.computeMaximumSizeForVolume (ProjectorGPU.java:162-166): the
+ * bytes of volume a slab driver may place on `device`, min(0.4 x its global
+ * memory, its largest allocation). */
+int tj_max_volume_bytes(int device, size_t *bytes);
+
 /* Copies `bytes` between two device buffers `repeats` times with a plain
  * streaming kernel and reports the average milliseconds per copy.  No TomoJ
  * counterpart: it is the known-size reference used to calibrate HBM byte

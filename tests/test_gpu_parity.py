@@ -1052,3 +1052,16 @@ def test_resolution_estimation_ssnr(tj, oracle):
     assert ok.mean() > 0.5
     assert np.allclose(maps[ok], ref_maps[ok], rtol=1e-3, atol=1e-3)
     assert r.recVSSNR.shape == (D, H, W) and np.isfinite(r.recVSSNR).all()
+
+
+def test_max_volume_bytes(tj):
+    """computeMaximumSizeForVolume: 40% of the device's global memory."""
+    import ctypes as C
+    import json
+    from tomoj_amd._lib import call
+    v = C.c_size_t()
+    call("tj_max_volume_bytes", 0, C.byref(v))
+    buf = C.create_string_buffer(1024)
+    call("tj_device_info", 0, buf, 1024)
+    hbm = json.loads(buf.value.decode())["hbm_bytes"]
+    assert v.value == int(hbm * 0.4)

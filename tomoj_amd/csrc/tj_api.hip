@@ -552,6 +552,17 @@ int tj_device_info(int device, char* buf, int len)
     return TJ_OK;
 }
 
+int tj_max_volume_bytes(int device, size_t* bytes)
+{
+    if (!bytes) return fail(TJ_EINVAL, "tj_max_volume_bytes: null argument");
+    hipDeviceProp_t prop;
+    TJ_HIP(hipGetDeviceProperties(&prop, device));
+    // min(0.4 x global memory, largest allocation); HIP allocates up to the
+    // whole memory, so the first term decides
+    *bytes = (size_t)(prop.totalGlobalMem * 0.4);
+    return TJ_OK;
+}
+
 void tj_geometry_default(tj_geometry* g, int vw, int vh, int vd, int pw, int ph)
 {
     std::memset(g, 0, sizeof *g);
