@@ -1065,3 +1065,43 @@ def test_max_volume_bytes(tj):
     call("tj_device_info", 0, buf, 1024)
     hbm = json.loads(buf.value.decode())["hbm_bytes"]
     assert v.value == int(hbm * 0.4)
+
+
+# --------------------------------------------------------------------------
+# Y-slab reconstruction (doGPURec's work sharing on one device)
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("kind", ["ossart", "sirt_positivity_fista", "bp", "wbp", "even_half"])
+def test_slab_reconstruction_equals_whole_volume(tj, oracle, kind):
+    """Slabs of 3 rows (the last one shorter), each its own context with
+    shifted centres, stitched: bit-identical to the whole volume."""
+    from tomoj_amd import parameters as RP
+    from tomoj_amd.slabs import reconstruct_in_slabs
+    W, H, D = 30, 11, 12
+    tilts = list(np.arange(-60, 61, 10.0))
+    E = oracle.tilt_eulers(tilts)
+    exps = (oracle.phantom_project(oracle.ellipsoids(51, 5, W, H, D), E, W, H)
+            + 0.05 * oracle.gaussian_noise(12, len(E) * W * H).reshape(len(E), H, W)).astype(np.float32)
+    ts = RP.TiltSeriesData(exps, tilts, normalize=True)
+    if kind == "ossart":
+        p = RP.ReconstructionParameters.createOSSARTParameters(W, H, D, 2, 0.8, 4)
+    elif kind == "sirt_positivity_fista":
+        p = RP.ReconstructionParameters.createOSSARTParameters(W, H, D, 3, 1.0, 0)
+        p.positivityConstraint = True
+        p.setFista(True)
+    elif kind == "bp":
+        p = RP.ReconstructionParameters.createWBPParameters(W, H, D, float("nan"))
+    elif kind == "wbp":
+        p = RP.ReconstructionParameters.createWBPParameters(W, H, D, 0.5)
+    else:
+        p = RP.ReconstructionParameters.createOSSARTParameters(W, H, D, 2, 1.0, 3)
+        p.setFscType(RP.EVEN_PROJECTIONS)
+    print(f"slabs {kind}", flush=True)
+    m = RP.ReconstructionThreadManager(ts)
+    try:
+        m.doRec(p)
+        whole = m.getRec2()
+    finally:
+        m.close()
+    vol, _ = reconstruct_in_slabs(ts, p, slab_rows=3)
+    assert same(vol, whole)

@@ -72,6 +72,23 @@ class TiltSeriesData:
         return np.full(len(self.tilts), 0.0 if self.axis_vertical else float(self.tilt_axis_deg))
 
 
+def supply(P: VoxelProjector3DHIP, ts: TiltSeriesData) -> None:
+    """Makes TiltSeries.getPixels(i) resident in P for every image
+    (TiltSeries.java:678-687): the raw images, then the affine alignment or,
+    for unaligned images, the normalisation, then the optional threshold."""
+    P.load_tilt_series(ts.images, ts.eulers())
+    if ts.transforms is not None:
+        # getPixels(i) through the affine alignment (AffineAlignment.applyTransformOnImage)
+        P.align_projections(ts.transforms, ts.tilts, ts.normalization_min_size, ts.normalize, ts.fill,
+                            ts.stats)
+    elif ts.normalize:
+        # getPixels(i) with normalisation, for unaligned images (identity alignment)
+        P.normalize_projections(ts.tilts, ts.normalization_min_size)
+    if ts.apply_threshold:
+        # then applyThreshold (TiltSeries.java:682-684)
+        P.threshold_projections(ts.threshold, ts.threshold_hysteresis, ts.fill)
+
+
 class ReconstructionParameters:
     """ReconstructionParameters.java:12-411 (reconstruction type, iterations,
     relaxation, update size, weighting radius, constraints)."""
@@ -181,17 +198,7 @@ class ReconstructionParameters:
         P = cls((self.width, self.height, self.depth), (ts.width, ts.height), device=device,
                 scale=scale, positivity=positivity, long_object_compensation=self.longObjectCompensation,
                 vol_center=self.volume_centers())
-        P.load_tilt_series(ts.images, ts.eulers())
-        if ts.transforms is not None:
-            # getPixels(i) through the affine alignment (AffineAlignment.applyTransformOnImage)
-            P.align_projections(ts.transforms, ts.tilts, ts.normalization_min_size, ts.normalize, ts.fill,
-                                ts.stats)
-        elif ts.normalize:
-            # getPixels(i) with normalisation, for unaligned images (identity alignment)
-            P.normalize_projections(ts.tilts, ts.normalization_min_size)
-        if ts.apply_threshold:
-            # then applyThreshold (TiltSeries.java:682-684)
-            P.threshold_projections(ts.threshold, ts.threshold_hysteresis, ts.fill)
+        supply(P, ts)
         if self.reconstructionType == WBP:
             P.setWeightingFilter(self.weightingRadius * ts.width, ts.tilts, ts.tilt_axes())
         if self.fista:
