@@ -111,15 +111,18 @@ def cpu_baseline(w, P, E, oracle):
 
 
 def regularisation_extras(w, P, tj, iters=3):
-    """SIRT, TVM (TomoJ's defaults: 5 TV steps, theta 25, g 1, lambda 0.1) and
-    Bayesian iterations on the workload's resident data, a TV step alone, and
-    the FSC of two resident volumes."""
+    """SIRT, TVM (TomoJ's defaults: 5 TV steps, theta 25, g 1, lambda 0.1),
+    Bayesian and BgART (k 1, dark background, cross median) iterations on the
+    workload's resident data, a TV step alone, and the FSC of two resident
+    volumes."""
     import ctypes as C
     res = {}
     for key, setup in (("sirt", lambda: None), ("tvm", lambda: P.setTvm(5, 25.0, 1.0, 0.1)),
-                       ("bayesian", lambda: P.setBayesian(True, 1.0, 0.01))):
+                       ("bayesian", lambda: P.setBayesian(True, 1.0, 0.01)),
+                       ("bgart", lambda: P.setBgart(True, 1.0, True, True))):
         P.setTvm(0)
         P.setBayesian(False)
+        P.setBgart(False)
         setup()
         P.reset_volume()
         P.run_ossart(1, 1.0, 0)
@@ -128,6 +131,7 @@ def regularisation_extras(w, P, tj, iters=3):
         res[f"{key}_it_s"] = iters / (time.perf_counter() - t0)
     P.setTvm(0)
     P.setBayesian(False)
+    P.setBgart(False)
     P.tv_denoise(25.0, 1.0, 0.1, 1)
     t0 = time.perf_counter()
     P.tv_denoise(25.0, 1.0, 0.1, 10)
@@ -290,7 +294,7 @@ def main():
         out["regularisation"] = regularisation_extras(w, P, tj)
         r = out["regularisation"]
         log(f"[bench] {w.W}x{w.H}x{w.D}: SIRT {r['sirt_it_s']:.1f} it/s, TVM (5 TV steps) "
-            f"{r['tvm_it_s']:.1f} it/s, Bayesian {r['bayesian_it_s']:.1f} it/s, TV step {r['tv_step_ms']:.3f} ms "
+            f"{r['tvm_it_s']:.1f} it/s, Bayesian {r['bayesian_it_s']:.1f} it/s, BgART {r['bgart_it_s']:.1f} it/s, TV step {r['tv_step_ms']:.3f} ms "
             f"({r['tv_step_GBs']:.0f} GB/s), FSC {r['fsc_ms']:.1f} ms")
 
     if not args.no_forms:

@@ -276,6 +276,24 @@ int tj_fsc_volumes(int device, const float *even_zyx, const float *odd_zyx, int 
  * bytes per voxel of device memory while on. */
 int tj_set_bayesian(tj_ctx *ctx, int on, double alpha, double lambda);
 
+/* BgART: the projector getProjector builds for the BGART type
+ * (AdvancedReconstructionParameters.java:145-155, 214-218) is
+ * BgARTVoxelProjector3D (TomoJ_AdvancedReconstruction/.../projectors/
+ * BgARTVoxelProjector3D.java:13-152).  With `on`, the collapse filter runs
+ * once at once (the constructor, :20-27) and at every tj_end_iteration
+ * (:119-121): the stack's min, max and 256-bin histogram (ImageJ
+ * StackStatistics), the background mv = min + (first fullest bin) x binSize,
+ * or bg_value when it is not NaN, sigma from the bins below it, and every
+ * voxel below mv + k sigma (dark_bg) or above mv - k sigma (otherwise) set to
+ * (float) mv (collapseFilter, :30-78).  With `median`, tj_start_iteration (and
+ * each iteration of tj_run_ossart) first replaces every voxel with the median
+ * of the cross it forms with its neighbours -+1 in z (0 beyond the volume) and
+ * -+1, -+W in its slice image's row-major index, those wrapping across rows
+ * and read as 0 where TomoJ's tests idx+1 < W*H, idx-1 > 0, idx-W > 0,
+ * idx+W < W*H fail (startOfIteration, :123-151).  Refused while TVM or
+ * Bayesian is on.  Holds 4 bytes per voxel of device memory while on. */
+int tj_set_bgart(tj_ctx *ctx, int on, double k, int dark_bg, int median, double bg_value);
+
 /* TotalVariationDenoising_3D.denoise3D(stack, theta, g, dt, iterations)
  * (TotalVariationDenoising_3D.java:84-202) of the resident volume, once: the
  * plugin's run (:66-73) with the volume as the stack. */

@@ -118,6 +118,8 @@ def _L():
             "or_bayes_end": (None, [P, P, l, d, i]),
             "or_threshold": (None, [P, i, i, d, d, i]),
             "or_poly_stats": (None, [P, i, i, P, P, P]),
+            "or_median_cross": (None, [P, i, i, i]),
+            "or_collapse": (d, [P, l, d, i, d]),
             "or_roi_poly_affine": (None, [i, i, d, i, P, P]),
         }
         for name, (res, args) in sigs.items():
@@ -483,6 +485,32 @@ def bayes_iteration(g: Geom, vol, projs, eulers, alpha, lam, first_index=0):
         elif e < emin:
             emin, imin = e, int(order[t])
     return [math.sqrt(som / (n * wh)), math.sqrt(emin / wh), imin, math.sqrt(emax / wh), imax]
+
+
+def median_cross(vol) -> np.ndarray:
+    """BgARTVoxelProjector3D.medianFilter3Dcross (:123-151) of a (D, H, W) volume (a copy)."""
+    out = np.array(vol, np.float32, order="C", copy=True)
+    D, H, W = out.shape
+    _L().or_median_cross(_p(out), W, H, D)
+    return out
+
+
+def collapse(vol, k, dark_bg=True, bg_value=float("nan")):
+    """BgARTVoxelProjector3D.collapseFilter (:34-78); returns (volume copy, background)."""
+    out = np.array(vol, np.float32, order="C", copy=True)
+    mv = _L().or_collapse(_p(out), out.size, float(k), int(bool(dark_bg)), float(bg_value))
+    return out, mv
+
+
+def bgart_iteration(g: Geom, vol, projs, eulers, lam, update, k, dark_bg=True, median=True, first_index=0):
+    """One iteration of TomoReconstruction2.regularization with BgARTVoxelProjector3D:
+    the cross median (startOfIteration), osartIteration, the collapse filter
+    (endOfIteration).  ``vol`` in place; returns the err row."""
+    if median:
+        vol[...] = median_cross(vol)
+    e = ossart(g, vol, projs, eulers, 1, lam, update, first_index)[0]
+    vol[...] = collapse(vol, k, dark_bg)[0]
+    return e
 
 
 def roi_poly_affine(width, height, tilt_deg, transform, min_size=0) -> np.ndarray:

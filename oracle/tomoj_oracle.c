@@ -22,6 +22,8 @@
  *   tomography/filters/FFTWeighting.java          createWeightingMask :31-65, weighting :87-112
  *   src/Utility/.../fr/curie/utils/MatrixUtils.java eulerAngles2Matrix :226-251
  *   tomography/TiltSeries.java                    applyThreshold :720-806 (projection supply)
+ *   TomoJ_AdvancedReconstruction/.../projectors/BgARTVoxelProjector3D.java collapseFilter :34-78,
+ *        medianFilter3Dcross :123-151
  *   src/Filters/.../fr/curie/filters/TotalVariationDenoising_3D.java denoise3D :84-202,
  *        denoise2D :206-260 (the step TVMProjector3D.endOfIteration runs)
  *   third party: ImageJ 1.53a ImageProcessor.getInterpolatedValue (not in the
@@ -1602,4 +1604,110 @@ void or_roi_poly_affine(int width, int height, double tilt_deg, int min_size, co
         poly[k] = (int)((m[0] * pts[2 * k] + m[2] * pts[2 * k + 1] + m[4]) + cx);
         poly[4 + k] = (int)((m[1] * pts[2 * k] + m[3] * pts[2 * k + 1] + m[5]) + cy);
     }
+}
+
+/* ---------------------------------------------------------------------------
+ * BgART projector, BgARTVoxelProjector3D (src/TomoJ_AdvancedReconstruction/src/
+ * main/java/fr/curie/tomoj/tomography/projectors/BgARTVoxelProjector3D.java).
+ * Volumes in ImageStack layout [z][y][x].
+ * ------------------------------------------------------------------------- */
+
+/* Java's Float.compare order (Arrays.sort(float[])): -0.0 before 0.0, NaN last. */
+static int java_float_key(float v)
+{
+    int32_t b;
+    if (v != v) return 0x7fc00000;
+    memcpy(&b, &v, 4);
+    return b >= 0 ? b : (int32_t)(b ^ 0x7fffffff);
+}
+
+/* medianFilter3Dcross (:123-151): each voxel takes the median (sorted[3]) of
+ * itself, the voxels of the slices before and after (0 beyond the stack) and
+ * the pixels index -+1 and -+W of its slice image (index +-1 wraps across rows;
+ * the tests are idx+1 < W*H, idx-1 > 0, idx-W > 0, idx+W < W*H, 0 otherwise),
+ * all read from a copy of the volume. */
+void or_median_cross(float *vol, int W, int H, int D)
+{
+    const long plane = (long)W * H, n = plane * D;
+    float *tmp = (float *)malloc(sizeof(float) * (n > 0 ? n : 1));
+    memcpy(tmp, vol, sizeof(float) * n);
+    for (int z = 0; z < D; z++) {
+        const float *pixs = tmp + z * plane;
+        const float *pmin = z > 0 ? tmp + (z - 1) * plane : NULL;
+        const float *pmax = z < D - 1 ? tmp + (z + 1) * plane : NULL;
+        for (long i = 0; i < plane; i++) {
+            float v[7];
+            v[0] = pmin ? pmin[i] : 0.0f;
+            v[1] = pmax ? pmax[i] : 0.0f;
+            v[2] = pixs[i];
+            v[3] = (i + 1 < plane) ? pixs[i + 1] : 0.0f;
+            v[4] = (i - 1 > 0) ? pixs[i - 1] : 0.0f;
+            v[5] = (i - W > 0) ? pixs[i - W] : 0.0f;
+            v[6] = (i + W < plane) ? pixs[i + W] : 0.0f;
+            for (int a = 1; a < 7; a++) {   /* insertion sort by Float.compare */
+                float x = v[a];
+                int b = a - 1;
+                while (b >= 0 && java_float_key(v[b]) > java_float_key(x)) {
+                    v[b + 1] = v[b];
+                    b--;
+                }
+                v[b + 1] = x;
+            }
+            vol[z * plane + i] = v[3];
+        }
+    }
+    free(tmp);
+}
+
+/* collapseFilter (:34-78): ImageJ StackStatistics of the volume (third party,
+ * restated as FloatStatistics: values within +-FLT_MAX, min/max, 256 bins of
+ * (int)(256/(max-min) * (v - min)) clamped to 255), the background value =
+ * min + (first fullest bin) * binSize (or bg_value when it is not NaN), sigma
+ * from the bins below it, and every voxel below mv + k sigma (dark
+ * background; above mv - k sigma otherwise) set to the background.  Returns
+ * the background value. */
+double or_collapse(float *vol, long n, double k, int dark_bg, double bg_value)
+{
+    double mn = DBL_MAX, mx = -DBL_MAX;
+    int hist[256];
+    memset(hist, 0, sizeof hist);
+    for (long i = 0; i < n; i++) {
+        const double v = vol[i];
+        if (v >= -FLT_MAX && v <= FLT_MAX) {
+            if (v < mn) mn = v;
+            if (v > mx) mx = v;
+        }
+    }
+    const double scale = 256 / (mx - mn);
+    for (long i = 0; i < n; i++) {
+        const double v = vol[i];
+        if (v >= -FLT_MAX && v <= FLT_MAX && v >= mn && v <= mx) {
+            int index = java_d2i(scale * (v - mn));
+            if (index >= 256) index = 255;
+            hist[index]++;
+        }
+    }
+    const double bin = (mx - mn) / 256;
+    double best = 0;
+    int bi = 0;
+    for (int i = 0; i < 256; i++)
+        if (hist[i] > best) {
+            best = hist[i];
+            bi = i;
+        }
+    const double mv = (bg_value != bg_value) ? mn + bi * bin : bg_value;
+    double sigma = 0;
+    int count = 0;
+    for (int i = 0; i < bi; i++) {
+        sigma += (mn + i * bin - mv) * (mn + i * bin - mv) * hist[i];
+        count += hist[i];
+    }
+    sigma = sqrt(sigma / count);
+    const double th1 = mv - k * sigma, th2 = mv + k * sigma;
+    for (long i = 0; i < n; i++) {
+        const double v = vol[i];
+        if (dark_bg && v < th2) vol[i] = (float)mv;
+        if (!dark_bg && v > th1) vol[i] = (float)mv;
+    }
+    return mv;
 }

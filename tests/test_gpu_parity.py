@@ -742,6 +742,87 @@ def test_bayesian_and_tvm_hooks_exclusive(tj):
 
 
 # --------------------------------------------------------------------------
+# BgART: cross median at the start of an iteration, collapse at the end
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("shape,dark", [((64, 64, 80), True), ((37, 11, 9), False)])
+def test_bgart_collapse_bit_exact(tj, oracle, shape, dark):
+    """The constructor's collapse filter on a resident volume: statistics over
+    more voxels than one reduction pass holds (64 x 64 x 80) and a ragged one;
+    NaN and +-inf voxels included."""
+    W, H, D = shape
+    rng = np.random.default_rng(5)
+    vol = rng.normal(1.0, 0.2, (D, H, W)).astype(np.float32)
+    vol[rng.random(vol.shape) < 0.1] = 3.0
+    vol.flat[[7, 100, 222]] = [np.nan, np.inf, -np.inf]
+    print(f"bgart collapse {shape} dark={dark}", flush=True)
+    ref, _ = oracle.collapse(vol, 1.5, dark)
+    with make(tj, (W, H, D)) as P:
+        P.set_volume(vol)
+        P.setBgart(True, 1.5, dark, True)
+        assert same(P.get_volume(), ref)
+        # the cross median runs at the start of an iteration, on the device layout
+        P.startOfIteration()
+        assert same(P.get_volume(), oracle.median_cross(ref))
+
+
+@pytest.mark.parametrize("median,dark,fista", [(True, True, False), (False, True, True), (True, False, False)])
+def test_bgart_reconstruction_bit_exact(tj, oracle, median, dark, fista):
+    """The BGART type end to end (createBgARTParameters -> getProjector -> doRec,
+    i.e. TomoReconstruction2.regularization) against the oracle's iteration
+    (median, OS-SART pass, collapse), with and without the FISTA wrapper, and
+    the same through the SPI hooks."""
+    from tomoj_amd import parameters as RP
+    from tomoj_amd import reconstruction as R
+    W, H, D = 30, 6, 14
+    tilts = list(np.arange(-60, 61, 12.0))
+    E = oracle.tilt_eulers(tilts)
+    ell = oracle.ellipsoids(47, 5, W, H, D)
+    exps = (oracle.phantom_project(ell, E, W, H) + 0.05 * oracle.gaussian_noise(8, len(tilts) * W * H)
+            .reshape(len(tilts), H, W)).astype(np.float32)
+    ts = RP.TiltSeriesData(exps, tilts)
+    g = oracle.make_geom(W, H, D, nthreads=8)
+    iters, relax, update, k = 3, 0.7, 4, 0.8
+    print(f"bgart median={median} dark={dark} fista={fista}", flush=True)
+    ref = np.zeros((D, H, W), np.float32)
+    ref_err, t = [], 0.0
+    for _ in range(iters):
+        prev = ref.copy()
+        ref_err.append(oracle.bgart_iteration(g, ref, exps, E, relax, update, k, dark, median))
+        if fista:
+            t = oracle.fista_step(ref, prev, t)
+    ref_err = np.array(ref_err)
+    p = RP.AdvancedReconstructionParameters.createBgARTParameters(W, H, D, iters, relax, update, k, median, dark)
+    p.setFista(fista)
+    m = RP.ReconstructionThreadManager(ts)
+    try:
+        err = m.doRec(p)
+        assert same(m.getRec2(), ref)
+        assert np.allclose(err[:, [0, 1, 3]], ref_err[:, [0, 1, 3]], rtol=1e-12)
+        assert np.array_equal(err[:, [2, 4]], ref_err[:, [2, 4]])
+        if not fista:
+            P = m.projector
+            P.reset_volume()
+            R.ossart(P, E, iters, relax, update, first_index=0)
+            assert same(P.get_volume(), ref)
+    finally:
+        m.close()
+
+
+def test_bgart_excludes_other_regularisers(tj):
+    with make(tj, (8, 2, 4)) as P:
+        P.setBgart(True)
+        with pytest.raises(tj.TjError):
+            P.setTvm(2)
+        with pytest.raises(tj.TjError):
+            P.setBayesian(True)
+        P.setBgart(False)
+        P.setBayesian(True)
+        with pytest.raises(tj.TjError):
+            P.setBgart(True)
+
+
+# --------------------------------------------------------------------------
 # projection subsets (getAvailableIndexes: FSC even / odd halves)
 # --------------------------------------------------------------------------
 

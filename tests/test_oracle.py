@@ -950,3 +950,80 @@ def test_vssnr_map_closed_form(oracle):
     maps = oracle.vssnr_maps(sig_e, sig_th, noi_e, th_n)
     assert np.all(maps[:, :, w - 1] == 0)
     assert np.allclose(maps[:, :, :w - 1], 10 * np.log10(4.0), atol=1e-4)
+
+
+# --------------------------------------------------------------------------
+# BgART: the 3-D cross median and the collapse filter
+# --------------------------------------------------------------------------
+
+def test_median_cross_boundary_zeros(oracle):
+    """A constant volume: sorted[3] of the 7 cross values stays c while at most
+    3 of them are TomoJ's zeros (beyond the stack, or a failed index test
+    idx+1 < W*H, idx-1 > 0, idx-W > 0, idx+W < W*H) and is 0 from 4 on."""
+    W, H, c = 5, 4, 2.5
+    one = oracle.median_cross(np.full((1, H, W), c, np.float32))[0]
+    # one slice: both z values are 0.  A second in-plane failure makes 4 zeros:
+    # idx 0 and 1 fail idx-1 > 0 and idx-W > 0; idx W*H-1 fails idx+1 < W*H and
+    # idx+W < W*H; every other voxel fails at most one test
+    exp = np.full((H, W), c, np.float32)
+    exp[0, 0] = exp[0, 1] = exp[H - 1, W - 1] = 0.0
+    assert np.array_equal(one, exp)
+    three = oracle.median_cross(np.full((3, H, W), c, np.float32))
+    # the middle slice has both z neighbours, so never more than 2 zeros
+    assert np.array_equal(three[1], np.full((H, W), c, np.float32))
+    # the first slice has one z zero: idx 0, 1 and W*H-1 reach 3 zeros and keep c
+    assert np.array_equal(three[0], np.full((H, W), c, np.float32))
+
+
+def test_median_cross_linear_ramp_and_spike(oracle):
+    """Away from the index tests the cross is symmetric about a linear
+    function, so an integer ramp is a fixed point; a lone spike goes."""
+    D, H, W = 5, 6, 7
+    z, y, x = np.meshgrid(np.arange(D), np.arange(H), np.arange(W), indexing="ij")
+    ramp = (3 * x + 5 * y + 11 * z + 100).astype(np.float32)
+    out = oracle.median_cross(ramp)
+    inner = (slice(1, D - 1), slice(1, H - 1), slice(1, W - 1))
+    assert np.array_equal(out[inner], ramp[inner])
+    vol = np.full((D, H, W), 1.0, np.float32)
+    vol[2, 3, 3] = 50.0
+    assert np.array_equal(oracle.median_cross(vol)[1:-1, 1:-1, 1:-1], vol[1:-1, 1:-1, 1:-1] * 0 + 1.0)
+
+
+def test_median_cross_signed_zero_order(oracle):
+    """Arrays.sort(float[]) puts -0.0 before 0.0: on one slice of -0.0 the first
+    row's voxels see four +0.0 and three -0.0 (sorted[3] = +0.0); inner voxels
+    see two +0.0 (sorted[3] = -0.0)."""
+    W, H = 4, 4
+    out = oracle.median_cross(np.full((1, H, W), -0.0, np.float32))[0]
+    assert not np.signbit(out[0, 1]) and np.signbit(out[2, 2])
+
+
+def test_collapse_closed_form(oracle):
+    """40 voxels at 0.5, 200 at 1.0, 16 at 5.0: min 0.5, max 5, bin 4.5/256;
+    1.0 falls in bin (int)(256/4.5*0.5) = 28, so mv = 0.5 + 28*4.5/256 =
+    0.9921875 and sigma (bins below 28: only bin 0) = 0.4921875."""
+    vals = np.array([0.5] * 40 + [1.0] * 200 + [5.0] * 16, np.float32)
+    vol = vals.reshape(4, 8, 8)
+    mv, sig = 0.5 + 28 * 4.5 / 256, 28 * 4.5 / 256
+    dark, got = oracle.collapse(vol, 1.0, True)
+    assert got == mv
+    exp = np.where(vol < mv + sig, np.float32(mv), vol)
+    assert np.array_equal(dark, exp)
+    light, _ = oracle.collapse(vol, 1.0, False)
+    assert np.array_equal(light, np.where(vol > mv - sig, np.float32(mv), vol))   # 0.5 == th1 stays
+    assert np.array_equal(light.ravel()[:40], vals[:40])
+    # a given background replaces the histogram's; sigma follows it
+    fixed, got = oracle.collapse(vol, 2.0, True, bg_value=1.0)
+    assert got == 1.0
+    assert np.array_equal(fixed, np.where(vol < 1.0 + 2.0 * 0.5, np.float32(1.0), vol))
+
+
+def test_collapse_skips_non_finite(oracle):
+    """NaN and +-inf are outside the statistics; NaN and +inf survive the dark
+    collapse, -inf is below the threshold and collapses."""
+    vals = np.array([0.5] * 40 + [1.0] * 200 + [5.0] * 13 + [np.nan, np.inf, -np.inf], np.float32)
+    vol = vals.reshape(4, 8, 8)
+    out, mv = oracle.collapse(vol, 1.0, True)
+    assert mv == 0.5 + 28 * 4.5 / 256
+    flat = out.ravel()
+    assert np.isnan(flat[-3]) and flat[-2] == np.inf and flat[-1] == np.float32(mv)

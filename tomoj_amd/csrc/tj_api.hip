@@ -37,6 +37,9 @@ hipError_t copy(hipStream_t, const void*, void*, size_t);
 hipError_t fista(hipStream_t, float*, const float*, size_t, double);
 hipError_t tv_denoise(hipStream_t, float*, float*, float*, int, int, int, float, float, float, int, uint8_t*);
 hipError_t bayes_conv(hipStream_t, const float*, float*, int, int, int, double);
+hipError_t median_cross(hipStream_t, const float*, float*, int, int, int);
+hipError_t stack_stats(hipStream_t, const float*, size_t, double*, int*);
+hipError_t collapse(hipStream_t, float*, size_t, double, double, float, int);
 hipError_t bayes_end(hipStream_t, float*, float*, size_t, int, float, int, uint8_t*);
 hipError_t normalize(hipStream_t, float*, int, int, int, const int*, double*);
 hipError_t threshold(hipStream_t, float*, int, int, int, double, double, int, double*, uint8_t*, int*);
@@ -162,6 +165,13 @@ struct tj_ctx {
     double bayes_alpha = 0.1, bayes_lambda = 0.1;
     float* dist = nullptr;      // [vh][vd][vw]
 
+    // BgARTVoxelProjector3D (BgARTVoxelProjector3D.java:13-31)
+    int bgart = 0, bg_dark = 1, bg_median = 1;
+    double bg_k = 1.0, bg_value = NAN;
+    float* bg_tmp = nullptr;    // [vh][vd][vw] copy read by the cross median
+    double* bg_mm = nullptr;    // [2 x 4096] min/max partials
+    int* bg_hist = nullptr;     // [256]
+
     // FP64 pair images of the backprojected images (LDS backprojection)
     double2* bpairs = nullptr;  // [bpairs_cap][ph][pw]
     int bpairs_cap = 0;
@@ -186,6 +196,9 @@ static void ctx_free(tj_ctx* c)
     dfree(c->tv_f);
     dfree(c->tv_p);
     dfree(c->dist);
+    dfree(c->bg_tmp);
+    dfree(c->bg_mm);
+    dfree(c->bg_hist);
     dfree(c->vol);
     dfree(c->pflag);
     dfree(c->norm_tab);
@@ -1306,14 +1319,64 @@ static int bayes_end(tj_ctx* c)
     return refresh_plane_flags(c, true);
 }
 
+// BgARTVoxelProjector3D.collapseFilter (:30-78): the stack's 256-bin
+// histogram on the device, the background level and thresholds on the host
+// in Java's order, the collapse on the device.
+static int bgart_collapse(tj_ctx* c)
+{
+    TJ_HIP(tjk::stack_stats(c->stream, c->vol, c->nvox, c->bg_mm, c->bg_hist));
+    double mm[2];
+    int hist[256];
+    TJ_HIP(hipMemcpyAsync(mm, c->bg_mm, sizeof mm, hipMemcpyDeviceToHost, c->stream));
+    TJ_HIP(hipMemcpyAsync(hist, c->bg_hist, sizeof hist, hipMemcpyDeviceToHost, c->stream));
+    TJ_HIP(hipStreamSynchronize(c->stream));
+    const double histmin = mm[0], histmax = mm[1];
+    const double bin = (histmax - histmin) / 256;
+    double best = 0;
+    int bi = 0;
+    for (int i = 0; i < 256; ++i)
+        if (hist[i] > best) {
+            best = hist[i];
+            bi = i;
+        }
+    const double mv = std::isnan(c->bg_value) ? histmin + bi * bin : c->bg_value;
+    double sigma = 0;
+    int count = 0;
+    for (int i = 0; i < bi; ++i) {
+        sigma += (histmin + i * bin - mv) * (histmin + i * bin - mv) * hist[i];
+        count += hist[i];
+    }
+    sigma = std::sqrt(sigma / count);
+    TJ_HIP(tjk::collapse(c->stream, c->vol, c->nvox, mv - c->bg_k * sigma, mv + c->bg_k * sigma, (float)mv,
+                         c->bg_dark));
+    return refresh_plane_flags(c);
+}
+
+// BgARTVoxelProjector3D.startOfIteration (:123-151): the cross median.
+static int bgart_start(tj_ctx* c)
+{
+    if (!c->bgart || !c->bg_median) return TJ_OK;
+    TJ_HIP(hipMemcpyAsync(c->bg_tmp, c->vol, c->nvox * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    TJ_HIP(tjk::median_cross(c->stream, c->bg_tmp, c->vol, c->g.vw, c->g.vh, c->g.vd));
+    return refresh_plane_flags(c);
+}
+
+// BgARTVoxelProjector3D.endOfIteration (:119-121): the collapse filter.
+static int bgart_end(tj_ctx* c)
+{
+    if (!c->bgart) return TJ_OK;
+    return bgart_collapse(c);
+}
+
 // FistaProjector3D wraps the working projector: its startOfIteration keeps the
 // volume, then runs the wrapped one's (:62-67); its endOfIteration runs the
-// wrapped one's first (:44).  TVM and Bayesian are never both on.
+// wrapped one's first (:44).  TVM, Bayesian and BgART are never on together.
 static int start_hooks(tj_ctx* c)
 {
     int rc = fista_start(c);
     if (rc) return rc;
-    return bayes_start(c);
+    if ((rc = bayes_start(c))) return rc;
+    return bgart_start(c);
 }
 
 static int end_hooks(tj_ctx* c)
@@ -1321,7 +1384,30 @@ static int end_hooks(tj_ctx* c)
     int rc = tvm_end(c);
     if (rc) return rc;
     if ((rc = bayes_end(c))) return rc;
+    if ((rc = bgart_end(c))) return rc;
     return fista_end(c);
+}
+
+int tj_set_bgart(tj_ctx* c, int on, double k, int dark_bg, int median, double bg_value)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_set_bgart: null context");
+    if (on && (c->tvm_iter > 0 || c->bayes)) return fail(TJ_ESTATE, "tj_set_bgart: another regulariser is on");
+    TJ_HIP(hipSetDevice(c->device));
+    c->bgart = 0;
+    if (!on) return TJ_OK;
+    if ((!c->bg_tmp && dalloc(&c->bg_tmp, c->nvox) != hipSuccess) ||
+        (!c->bg_mm && dalloc(&c->bg_mm, 2 * 4096) != hipSuccess) ||
+        (!c->bg_hist && dalloc(&c->bg_hist, 256) != hipSuccess)) {
+        (void)hipGetLastError();
+        return fail(TJ_ENOMEM, "tj_set_bgart: cannot allocate its workspace");
+    }
+    c->bg_k = k;
+    c->bg_dark = dark_bg ? 1 : 0;
+    c->bg_median = median ? 1 : 0;
+    c->bg_value = bg_value;
+    c->bgart = 1;
+    // the constructor runs the collapse filter once (:20-27)
+    return bgart_collapse(c);
 }
 
 // ResolutionEstimation.fsc (:456-557) of two device volumes on `st`: shell
@@ -1535,7 +1621,7 @@ int tj_set_available_indexes(tj_ctx* c, const int* indexes, int n)
 int tj_set_bayesian(tj_ctx* c, int on, double alpha, double lambda)
 {
     if (!c) return fail(TJ_EINVAL, "tj_set_bayesian: null context");
-    if (on && c->tvm_iter > 0) return fail(TJ_ESTATE, "tj_set_bayesian: the TVM hook is on");
+    if (on && (c->tvm_iter > 0 || c->bgart)) return fail(TJ_ESTATE, "tj_set_bayesian: another regulariser is on");
     TJ_HIP(hipSetDevice(c->device));
     if (on) {
         int rc = ensure_dist(c);
@@ -1551,7 +1637,7 @@ int tj_set_tvm(tj_ctx* c, int iterations, double theta, double g, double lambda)
 {
     if (!c) return fail(TJ_EINVAL, "tj_set_tvm: null context");
     if (iterations < 0) return fail(TJ_EINVAL, "tj_set_tvm: negative iteration count %d", iterations);
-    if (iterations > 0 && c->bayes) return fail(TJ_ESTATE, "tj_set_tvm: the Bayesian hook is on");
+    if (iterations > 0 && (c->bayes || c->bgart)) return fail(TJ_ESTATE, "tj_set_tvm: another regulariser is on");
     TJ_HIP(hipSetDevice(c->device));
     if (iterations > 0) {
         int rc = ensure_tv(c);

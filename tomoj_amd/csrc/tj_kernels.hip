@@ -1360,6 +1360,126 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_tv_u(const float* __restrict__ f, 
 }
 
 // ---------------------------------------------------------------------------
+// BgART projector (BgARTVoxelProjector3D): the cross median of
+// startOfIteration (medianFilter3Dcross, :123-151) and the stack statistics
+// and collapse of endOfIteration (collapseFilter, :34-78).  Device layout
+// [y][z][x]; TomoJ indexes a slice image as i = x + y*W.
+// ---------------------------------------------------------------------------
+__device__ inline int java_float_key(float v)
+{
+    if (v != v) return 0x7fc00000;  // Float.compare: every NaN after +inf
+    const int b = __float_as_int(v);
+    return b >= 0 ? b : (b ^ 0x7fffffff);
+}
+
+__global__ __launch_bounds__(TJ_BLOCK) void k_median_cross(const float* __restrict__ src, float* __restrict__ dst,
+                                                           int W, int H, int D)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int z = blockIdx.y, y = blockIdx.z;
+    if (x >= W) return;
+    const long plane = (long)W * H;
+    const long i = x + (long)y * W;
+    auto at = [&](int xx, int yy, int zz) { return src[((size_t)yy * D + zz) * W + xx]; };
+    float v[7];
+    v[0] = z > 0 ? at(x, y, z - 1) : 0.0f;
+    v[1] = z < D - 1 ? at(x, y, z + 1) : 0.0f;
+    v[2] = at(x, y, z);
+    // index -+1 runs into the neighbouring row at the row ends
+    v[3] = (i + 1 < plane) ? (x + 1 < W ? at(x + 1, y, z) : at(0, y + 1, z)) : 0.0f;
+    v[4] = (i - 1 > 0) ? (x > 0 ? at(x - 1, y, z) : at(W - 1, y - 1, z)) : 0.0f;
+    v[5] = (i - W > 0) ? at(x, y - 1, z) : 0.0f;
+    v[6] = (i + W < plane) ? at(x, y + 1, z) : 0.0f;
+    int kk[7];
+#pragma unroll
+    for (int a = 0; a < 7; ++a) kk[a] = java_float_key(v[a]);
+    // insertion sort by Float.compare (stable: equal keys keep their order)
+#pragma unroll
+    for (int a = 1; a < 7; ++a) {
+#pragma unroll
+        for (int b = a; b > 0; --b) {
+            const bool sw = kk[b - 1] > kk[b];
+            const int tk = kk[b];
+            const float tv = v[b];
+            kk[b] = sw ? kk[b - 1] : tk;
+            v[b] = sw ? v[b - 1] : tv;
+            kk[b - 1] = sw ? tk : kk[b - 1];
+            v[b - 1] = sw ? tv : v[b - 1];
+        }
+    }
+    dst[((size_t)y * D + z) * W + x] = v[3];
+}
+
+// per-block min and max over the values within +-FLT_MAX
+__global__ __launch_bounds__(TJ_BLOCK) void k_stack_minmax(const float* __restrict__ vol, size_t n,
+                                                           double* __restrict__ part)
+{
+    __shared__ double sh[TJ_BLOCK];
+    double mn = 1.7976931348623157e308, mx = -1.7976931348623157e308;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+        const double v = vol[k];
+        if (fsv_in(v)) {
+            mn = v < mn ? v : mn;
+            mx = v > mx ? v : mx;
+        }
+    }
+    mn = block_reduce(mn, sh, MinD());
+    mx = block_reduce(mx, sh, MaxD());
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = mn;
+        part[2 * blockIdx.x + 1] = mx;
+    }
+}
+
+// reduces the per-block partials into part[0] (min) and part[1] (max); one block
+__global__ __launch_bounds__(TJ_BLOCK) void k_stack_reduce(double* __restrict__ part, int np)
+{
+    __shared__ double sh[TJ_BLOCK];
+    double mn = 1.7976931348623157e308, mx = -1.7976931348623157e308;
+    for (int k = threadIdx.x; k < np; k += blockDim.x) {
+        mn = part[2 * k] < mn ? part[2 * k] : mn;
+        mx = part[2 * k + 1] > mx ? part[2 * k + 1] : mx;
+    }
+    mn = block_reduce(mn, sh, MinD());
+    mx = block_reduce(mx, sh, MaxD());
+    if (threadIdx.x == 0) {
+        part[0] = mn;
+        part[1] = mx;
+    }
+}
+
+__global__ __launch_bounds__(TJ_BLOCK) void k_stack_hist(const float* __restrict__ vol, size_t n,
+                                                         const double* __restrict__ mm, int* __restrict__ hist)
+{
+    __shared__ int h[256];
+    for (int b = threadIdx.x; b < 256; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    const double mn = mm[0], mx = mm[1];
+    const double scale = 256 / (mx - mn);
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+        const double v = vol[k];
+        if (fsv_in(v) && v >= mn && v <= mx) {
+            int index = java_d2i(scale * (v - mn));
+            if (index >= 256) index = 255;
+            atomicAdd(&h[index], 1);
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < 256; b += blockDim.x)
+        if (h[b]) atomicAdd(&hist[b], h[b]);
+}
+
+__global__ __launch_bounds__(TJ_BLOCK) void k_collapse(float* __restrict__ vol, size_t n, double th1, double th2,
+                                                       float mv, int dark_bg)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) {
+        const double v = vol[k];
+        if (dark_bg && v < th2) vol[k] = mv;
+        if (!dark_bg && v > th1) vol[k] = mv;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Bayesian regularisation (BayesianVoxelProjector3D): the distance volume of
 // startOfIteration, convolve(rec, kernel, dist, -lambda) (:175-198), and the
 // update of endOfIteration (:121-132).  Same tiles and block order as the TV
@@ -1818,7 +1938,41 @@ hipError_t threshold(hipStream_t st, float* imgs, int w, int h, int n, double th
     return hipGetLastError();
 }
 
-// denoise3D (:84-113): f keeps the input, u (the volume) starts equal to it,// denoise3D (:84-113): f keeps the input, u (the volume) starts equal to it,
+hipError_t median_cross(hipStream_t st, const float* src, float* dst, int W, int H, int D)
+{
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    if ((size_t)W * H * D == 0) return hipSuccess;
+    if (H > 65535 || D > 65535) return hipErrorInvalidValue;
+    k_median_cross<<<dim3((W + TJ_BLOCK - 1) / TJ_BLOCK, D, H), TJ_BLOCK, 0, st>>>(src, dst, W, H, D);
+    return hipGetLastError();
+}
+
+// StackStatistics' min, max (mm[0..1]) and 256-bin histogram of the volume;
+// mm needs 2 x 4096 doubles, hist 256 ints.  The histogram pass keeps to 1024
+// blocks: each block merges up to 256 bins into the global histogram.
+hipError_t stack_stats(hipStream_t st, const float* vol, size_t n, double* mm, int* hist)
+{
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    if (n == 0) return hipErrorInvalidValue;
+    const int blocks = (int)std::min<size_t>((n + TJ_BLOCK - 1) / TJ_BLOCK, 4096);
+    hipError_t e = hipMemsetAsync(hist, 0, 256 * sizeof(int), st);
+    if (e != hipSuccess) return e;
+    k_stack_minmax<<<blocks, TJ_BLOCK, 0, st>>>(vol, n, mm);
+    k_stack_reduce<<<1, TJ_BLOCK, 0, st>>>(mm, blocks);
+    k_stack_hist<<<std::min(blocks, 1024), TJ_BLOCK, 0, st>>>(vol, n, mm, hist);
+    return hipGetLastError();
+}
+
+hipError_t collapse(hipStream_t st, float* vol, size_t n, double th1, double th2, float mv, int dark_bg)
+{
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    if (n == 0) return hipSuccess;
+    const int blocks = (int)std::min<size_t>((n + TJ_BLOCK - 1) / TJ_BLOCK, 8192);
+    k_collapse<<<blocks, TJ_BLOCK, 0, st>>>(vol, n, th1, th2, mv, dark_bg);
+    return hipGetLastError();
+}
+
+// denoise3D (:84-113): f keeps the input, u (the volume) starts equal to it,
 // p starts at zero.
 hipError_t tv_denoise(hipStream_t st, float* vol, float* f, float* p, int W, int H, int D, float theta, float g,
                       float dt, int iterations, uint8_t* pflag)

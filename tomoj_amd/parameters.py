@@ -21,7 +21,8 @@ from typing import List, Optional, Sequence
 import numpy as np
 
 from ._lib import call
-from .projector import BayesianVoxelProjector3DHIP, TVMProjector3DHIP, VoxelProjector3DHIP
+from .projector import (BayesianVoxelProjector3DHIP, BgARTVoxelProjector3DHIP, TVMProjector3DHIP,
+                        VoxelProjector3DHIP)
 from .reconstruction import tilt_eulers
 
 BP = 0
@@ -34,6 +35,7 @@ ODD_PROJECTIONS = 12
 DEFINED_PROJECTIONS = 13
 TVM = 3        # AdvancedReconstructionParameters.java:10
 BAYESIAN = 4   # AdvancedReconstructionParameters.java:11
+BGART = 6      # AdvancedReconstructionParameters.java:13
 
 
 @dataclass
@@ -206,11 +208,22 @@ class ReconstructionParameters:
         return P
 
 
+class _BgartFactory:
+    """Builds BgARTVoxelProjector3DHIP with the parameters' (k, darkBG, median)."""
+
+    def __init__(self, k: float, dark: bool, median: bool):
+        self.k, self.dark, self.median = k, dark, median
+
+    def __call__(self, *args, **kw):
+        return BgARTVoxelProjector3DHIP(*args, k=self.k, darkBG=self.dark, doMedian=self.median, **kw)
+
+
 class AdvancedReconstructionParameters(ReconstructionParameters):
     """AdvancedReconstructionParameters (TomoJ_AdvancedReconstruction/.../tomography/
-    AdvancedReconstructionParameters.java:8-283) for the TVM type: the TV settings
-    and their defaults (:24-27), the two createTVMParameters factories (:112-133)
-    and getProjector's TVM branch (:179-193)."""
+    AdvancedReconstructionParameters.java:8-283) for the TVM, BAYESIAN and BGART
+    types: their settings and defaults (:19-27), the createTVMParameters,
+    createBayesianParameters and createBgARTParameters factories (:112-155) and
+    getProjector's branches for them (:179-203, 214-218)."""
 
     def __init__(self, width: int, height: int, depth: int):
         super().__init__(width, height, depth)
@@ -219,6 +232,9 @@ class AdvancedReconstructionParameters(ReconstructionParameters):
         self.tvmG = 1.0
         self.tvmDt = 0.1
         self.tvmIterations = 5
+        self.bgartK = 1.0
+        self.bgartMedian = True
+        self.bgartDarkBG = True
 
     @staticmethod
     def createTVMParameters(width, height, depth, nbIterations, *args) -> "AdvancedReconstructionParameters":
@@ -255,16 +271,40 @@ class AdvancedReconstructionParameters(ReconstructionParameters):
         p.updateNb = 0
         return p
 
+    @staticmethod
+    def createBgARTParameters(width, height, depth, nbIterations, relaxationCoefficient, updateNb,
+                              bgartK, bgartMedian, bgartDarkBG) -> "AdvancedReconstructionParameters":
+        """:145-155: OS-SART settings plus the BgART k, median and dark-background flags."""
+        p = AdvancedReconstructionParameters(width, height, depth)
+        p.reconstructionType = BGART
+        p.nbIterations = int(nbIterations)
+        p.weightingRadius = float("nan")
+        p.relaxationCoefficient = float(relaxationCoefficient)
+        p.updateNb = int(updateNb)
+        p.bgartK = float(bgartK)
+        p.bgartMedian = bool(bgartMedian)
+        p.bgartDarkBG = bool(bgartDarkBG)
+        return p
+
     def getProjector(self, ts: TiltSeriesData, device: int = 0):
-        """The TVM and BAYESIAN branches of getProjector (:179-203).  TVM:
+        """The TVM, BAYESIAN and BGART branches of getProjector (:179-203).  TVM:
         TVMProjector3D with the relaxation as regularisation alpha, the
         positivity and long-object settings, the TV settings, the scale when
         rescaleData, and the FISTA wrapper; TomoJ leaves its regularisation
         lambda at 0.1 (the call that would set it is commented out, :182).
         BAYESIAN: BayesianVoxelProjector3D with alpha = the relaxation
         coefficient and lambda = the regularisation weight, positivity,
-        long-object setting and scale, never wrapped in FISTA.  Other types go
-        to the base class."""
+        long-object setting and scale, never wrapped in FISTA.  BGART (:214-218):
+        BgARTVoxelProjector3D(k, darkBG, median) with the scale when rescaleData
+        and the FISTA wrapper, TomoJ's defaults otherwise (no positivity, no
+        long-object compensation).  Other types go to the base class."""
+        if self.reconstructionType == BGART:
+            loc, self.longObjectCompensation = self.longObjectCompensation, False
+            try:
+                return self._projector(_BgartFactory(self.bgartK, self.bgartDarkBG, self.bgartMedian), ts,
+                                       device, False)
+            finally:
+                self.longObjectCompensation = loc
         if self.reconstructionType == BAYESIAN:
             fista, self.fista = self.fista, False
             try:
@@ -317,8 +357,8 @@ class ReconstructionThreadManager:
         if params.reconstructionType in (BP, WBP):
             P.run_wbp()
             return None
-        if params.reconstructionType in (OSSART, TVM, BAYESIAN, DEFINED_PROJECTIONS):
-            # TVM, BAYESIAN and DEFINED_PROJECTIONS go through doCPUrec's default
+        if params.reconstructionType in (OSSART, TVM, BAYESIAN, BGART, DEFINED_PROJECTIONS):
+            # TVM, BAYESIAN, BGART and DEFINED_PROJECTIONS go through doCPUrec's default
             # branch, TomoReconstruction2.regularization (:783-878): the OS-SART
             # loop with the projector's own iteration hooks.
             return P.run_ossart(params.nbIterations, params.relaxationCoefficient, params.updateNb,

@@ -228,6 +228,16 @@ class VoxelProjector3DHIP:
         volume at the end of each iteration."""
         call("tj_set_bayesian", self._ctx, int(bool(on)), float(alpha), float(lam))
 
+    def setBgart(self, on: bool, k: float = 1.0, darkBG: bool = True, doMedian: bool = True,
+                 bgValue: float = float("nan")) -> None:
+        """Turns the projector into BgARTVoxelProjector3D
+        (BgARTVoxelProjector3D.java:13-152): the collapse filter runs once now
+        (the constructor) and at the end of every iteration; with doMedian each
+        iteration starts with the 3-D cross median.  bgValue NaN takes the
+        background from the histogram."""
+        call("tj_set_bgart", self._ctx, int(bool(on)), float(k), int(bool(darkBG)), int(bool(doMedian)),
+             float(bgValue))
+
     def tv_denoise(self, theta: float, g: float, dt: float, iterations: int) -> None:
         """TotalVariationDenoising_3D.denoise3D (TotalVariationDenoising_3D.java:84-202)
         of the resident volume, once."""
@@ -371,6 +381,28 @@ class BayesianVoxelProjector3DHIP(VoxelProjector3DHIP):
 
     def setRegularizationAlpha(self, v: float) -> None:
         self.regularizationAlpha = float(v)
+        self._push()
+
+
+class BgARTVoxelProjector3DHIP(VoxelProjector3DHIP):
+    """BgARTVoxelProjector3D (TomoJ_AdvancedReconstruction/.../projectors/
+    BgARTVoxelProjector3D.java:13-152) on the GPU.  The constructor takes
+    TomoJ's (k, darkBG, doMedian) after the projector arguments and, like
+    TomoJ's (:20-27), runs the collapse filter once on the volume."""
+
+    def __init__(self, *args, k: float = 1.0, darkBG: bool = True, doMedian: bool = True, **kw):
+        super().__init__(*args, **kw)
+        self.k = float(k)
+        self.darkBG = bool(darkBG)
+        self.doMedian = bool(doMedian)
+        self.bgValue = float("nan")
+        self._push()
+
+    def _push(self) -> None:
+        self.setBgart(True, self.k, self.darkBG, self.doMedian, self.bgValue)
+
+    def collapseFilter(self) -> None:
+        """collapseFilter (:30-78) of the resident volume, once."""
         self._push()
 
 
