@@ -1393,21 +1393,21 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_median_cross(const float* __restri
     int kk[7];
 #pragma unroll
     for (int a = 0; a < 7; ++a) kk[a] = java_float_key(v[a]);
-    // insertion sort by Float.compare (stable: equal keys keep their order)
-#pragma unroll
-    for (int a = 1; a < 7; ++a) {
-#pragma unroll
-        for (int b = a; b > 0; --b) {
-            const bool sw = kk[b - 1] > kk[b];
-            const int tk = kk[b];
-            const float tv = v[b];
-            kk[b] = sw ? kk[b - 1] : tk;
-            v[b] = sw ? v[b - 1] : tv;
-            kk[b - 1] = sw ? tk : kk[b - 1];
-            v[b - 1] = sw ? tv : v[b - 1];
-        }
-    }
-    dst[((size_t)y * D + z) * W + x] = v[3];
+    // 16-comparator sorting network on the keys; a key maps back to its float
+    // (every NaN to the canonical one)
+    auto cx = [&](int a, int b) {
+        const int lo = min(kk[a], kk[b]), hi = max(kk[a], kk[b]);
+        kk[a] = lo;
+        kk[b] = hi;
+    };
+    cx(0, 6); cx(2, 3); cx(4, 5);
+    cx(0, 2); cx(1, 4); cx(3, 6);
+    cx(0, 1); cx(2, 5); cx(3, 4);
+    cx(1, 2); cx(4, 6);
+    cx(2, 3); cx(4, 5);
+    cx(1, 2); cx(3, 4); cx(5, 6);
+    const int m = kk[3];
+    dst[((size_t)y * D + z) * W + x] = __int_as_float(m >= 0 ? m : (m ^ 0x7fffffff));
 }
 
 // per-block min and max over the values within +-FLT_MAX
@@ -1942,6 +1942,10 @@ hipError_t median_cross(hipStream_t st, const float* src, float* dst, int W, int
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if ((size_t)W * H * D == 0) return hipSuccess;
+    // blocks in (x tile, z, y) order: the rows and slices a block reads around
+    // its own are read by blocks running at the same time.  Handing each XCD a
+    // contiguous eighth of this order measured slower (0.34 against 0.27 ms at
+    // 512 x 512 x 256).
     if (H > 65535 || D > 65535) return hipErrorInvalidValue;
     k_median_cross<<<dim3((W + TJ_BLOCK - 1) / TJ_BLOCK, D, H), TJ_BLOCK, 0, st>>>(src, dst, W, H, D);
     return hipGetLastError();
