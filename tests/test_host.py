@@ -148,3 +148,23 @@ def test_reconstruction_parameters_factories():
     assert [s.getRelaxationCoefficient() for _ in range(3)] == [1.0, 0.5, 1.0 / 3]
     s.resetTmpNbCall()
     assert s.getRelaxationCoefficient() == 1.0
+
+
+def test_bgart_parameters_and_slab_refusal():
+    """createBgARTParameters (AdvancedReconstructionParameters.java:145-155)
+    keeps the OS-SART settings and the BgART flags; the slab driver refuses
+    the regularised types before touching a device."""
+    from tomoj_amd import parameters as RP
+    from tomoj_amd import slabs
+    p = RP.AdvancedReconstructionParameters.createBgARTParameters(64, 32, 16, 7, 0.6, 5, 1.25, False, True)
+    assert p.reconstructionType == RP.BGART == 6
+    assert (p.nbIterations, p.relaxationCoefficient, p.updateNb) == (7, 0.6, 5)
+    assert (p.bgartK, p.bgartMedian, p.bgartDarkBG) == (1.25, False, True)
+    assert np.isnan(p.weightingRadius)
+    q = RP.AdvancedReconstructionParameters(8, 8, 8)
+    assert (q.bgartK, q.bgartMedian, q.bgartDarkBG) == (1.0, True, True)   # :29-31
+    ts = RP.TiltSeriesData(np.zeros((3, 32, 64), np.float32), [-10.0, 0.0, 10.0])
+    for t in (RP.TVM, RP.BAYESIAN, RP.BGART):
+        p.reconstructionType = t
+        with pytest.raises(ValueError):
+            slabs.reconstruct_in_slabs(ts, p, slab_rows=8)

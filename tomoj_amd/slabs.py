@@ -19,7 +19,7 @@ from typing import List, Optional, Tuple
 import numpy as np
 
 from ._lib import call
-from .parameters import (BAYESIAN, BP, OSSART, TVM, WBP, DEFINED_PROJECTIONS, ReconstructionParameters,
+from .parameters import (BAYESIAN, BGART, BP, OSSART, TVM, WBP, DEFINED_PROJECTIONS, ReconstructionParameters,
                          TiltSeriesData, supply)
 from .projector import VoxelProjector3DHIP
 
@@ -36,12 +36,13 @@ def reconstruct_in_slabs(ts: TiltSeriesData, params: ReconstructionParameters, s
     """The volume (D, H, W) and each OS-SART slab's err rows.  Supported:
     BP, WBP and the OS-SART family with the vertical tilt axis, scale 1,
     detector height = volume height and no long-object compensation (its norm
-    is a whole-image maximum); TVM and Bayesian couple neighbouring rows and
-    are refused."""
+    is a whole-image maximum); TVM, Bayesian and BgART couple neighbouring
+    rows (BgART also takes its background from the whole volume's histogram)
+    and are refused."""
     W, H, D = params.width, params.height, params.depth
     t = params.reconstructionType
-    if t in (TVM, BAYESIAN):
-        raise ValueError("TVM and Bayesian regularisation couple neighbouring rows: no slabs")
+    if t in (TVM, BAYESIAN, BGART):
+        raise ValueError("TVM, Bayesian and BgART regularisation couple neighbouring rows: no slabs")
     if t not in (BP, WBP, OSSART, DEFINED_PROJECTIONS):
         raise ValueError(f"reconstruction type {t} has no slab form")
     if not ts.axis_vertical or ts.height != H or ts.width != W or params.longObjectCompensation:
