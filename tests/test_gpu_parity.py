@@ -1053,6 +1053,25 @@ def test_configurations_at_full_size_bit_exact(tj, oracle, wname, iters):
     assert np.array_equal(err[:, [2, 4]], ref_err[:, [2, 4]])
 
 
+def test_bgart_hooks_at_full_size_bit_exact(tj, oracle):
+    """BgART's collapse (constructor) and cross median (start of an iteration)
+    on a 512x512x256 volume: the statistics reduce 134 M values over the
+    widest partial grid, the median runs over the full block grid."""
+    W, H, D = 512, 512, 256
+    rng = np.random.default_rng(11)
+    vol = rng.normal(0.0, 1.0, (D, H, W)).astype(np.float32)
+    vol[:, 100:180, 200:330] += 4.0
+    print(f"bgart full size {W}x{H}x{D}", flush=True)
+    ref, _ = oracle.collapse(vol, 1.0, True)
+    with make(tj, (W, H, D)) as P:
+        P.set_volume(vol)
+        P.setBgart(True, 1.0, True, True)
+        assert same(P.get_volume(), ref)
+        P.startOfIteration()
+        got = P.get_volume()
+    assert same(got, oracle.median_cross(ref))
+
+
 def test_new_entry_points_report_errors(tj):
     """Argument and state errors of the regularisation, subset, threshold and
     FSC entry points come back as TjError, and the context stays usable."""
