@@ -47,14 +47,17 @@ def log(*a):
 
 
 def kernel_bytes(w):
-    """Algorithmic HBM bytes per iteration of each kernel (FP32 storage):
-    forward projection streams the volume once per projection and writes proj+norm;
-    the residual reads exp/proj/norm and writes diff; the backprojection reads and
-    writes the volume once per subset and reads every diff image."""
+    """Algorithmic HBM bytes per iteration of each kernel (FP32 storage): the
+    forward projection, fused with the residual, streams the volume once per
+    projection, reads each measured image and writes each difference image;
+    the residual step left after it folds the per-block error partials and row
+    flags; the backprojection reads and writes the volume once per subset and
+    reads every diff image."""
     nvox, npx, n, s = w.nvox, w.npx, w.n, w.subsets()
+    ntile = (w.W + 255) // 256
     return {
         "fp": n * nvox * 4 + n * npx * 8,
-        "res": n * npx * 16,
+        "res": n * (((w.H + 3) // 4) * ntile * 8 + w.H * ntile + w.H),
         "bp": s * 2 * nvox * 4 + n * npx * 4,
     }
 

@@ -26,6 +26,8 @@ hipError_t norm_max_fill(hipStream_t, int, int, float*);
 hipError_t residual(hipStream_t, int, int, int, const float*, const int*, const float*, const float*, double,
                     float*, double*, uint8_t*, double*);
 hipError_t row_flags(hipStream_t, int, int, int, const float*, const int*, uint8_t*);
+hipError_t fp_residual(hipStream_t, const TjGeomDev&, const float*, const TjProj*, const int*, int, int, const float*,
+                       int, int, const float*, double, float*, double*, uint8_t*, uint8_t*, double*);
 hipError_t bp(hipStream_t, const TjGeomDev&, float*, const TjProj*, const int*, int, const float*, const int*,
               const uint8_t*, TjBpRow*, int, int, int, int, int, int, int, double2*);
 hipError_t zyx_to_yzx(hipStream_t, const float*, float*, int, int, int, int);
@@ -125,6 +127,7 @@ struct tj_ctx {
     int swgt_cap = 0;
     double* row_err = nullptr;  // [cap][ph]
     uint8_t* rowflag = nullptr; // [cap][ph]
+    uint8_t* rbad = nullptr;    // [cap][ph][column tiles] (fused forward + residual)
     TjBpRow* bprows = nullptr;  // [cap][vh]
     double* err = nullptr;      // [max(cap,nproj)]
     int* slot_proj = nullptr;   // [cap]
@@ -356,12 +359,15 @@ static int ensure_slots(tj_ctx* c, int n)
     dfree(c->swgt);
     dfree(c->row_err);
     dfree(c->rowflag);
+    dfree(c->rbad);
     dfree(c->bprows);
     dfree(c->slot_proj);
     c->cap = 0;
     const size_t npx = c->npx;
+    const size_t ntile = (size_t)(c->g.pw + 255) / 256;  // column tiles of the fused residual
     if (dalloc(&c->sproj, npx * n) || dalloc(&c->snorm, npx * n) || dalloc(&c->sdiff, npx * n) ||
-        dalloc(&c->row_err, (size_t)c->g.ph * n) || dalloc(&c->rowflag, (size_t)c->g.ph * n) ||
+        dalloc(&c->row_err, (size_t)c->g.ph * n * ntile) || dalloc(&c->rowflag, (size_t)c->g.ph * n) ||
+        dalloc(&c->rbad, (size_t)c->g.ph * n * ntile) ||
         dalloc(&c->bprows, (size_t)c->g.vh * n) || dalloc(&c->slot_proj, n)) {
         (void)hipGetLastError();
         return fail(TJ_ENOMEM, "cannot allocate %d projection slots of %dx%d", n, c->g.pw, c->g.ph);
@@ -1778,13 +1784,26 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
                 const int start = fp_start_row(c, start_y);
                 const int fk = pick_fp_kernel(c, hs, start);
                 const int bk = pick_bp_kernel(c, hs);
-                TJ_HIP(tjk::fp(c->stream, c->g, c->vol, c->pp, slots, diffIndex, start, fk, c->pflag, c->norm_tab,
-                               c->sproj, c->snorm, c->tune.fp_rows, c->tune.fp_rays, c->tune.fp_zchunk));
-                if (c->geo.long_object_comp)
-                    TJ_HIP(tjk::norm_max_fill(c->stream, (int)c->npx, diffIndex, c->snorm));
-                if (c->timing_on) TJ_HIP(hipEventRecord(b, c->stream));
-                if ((rc = do_residual(c, slots, diffIndex, (double)factorimg, c->err + off))) return rc;
-                if (c->timing_on) TJ_HIP(hipEventRecord(r, c->stream));
+                if (fk == 3 && !c->geo.long_object_comp) {
+                    // projectAndDifference in one pass: the LDS forward kernel
+                    // writes the differences and the row errors itself
+                    TJ_HIP(tjk::fp_residual(c->stream, c->g, c->vol, c->pp, slots, diffIndex, start, c->norm_tab,
+                                            c->tune.fp_rays, c->tune.fp_zchunk, c->exps, (double)factorimg,
+                                            c->sdiff, c->row_err, c->rbad, c->rowflag, c->err + off));
+                    if (c->timing_on) {
+                        TJ_HIP(hipEventRecord(b, c->stream));
+                        TJ_HIP(hipEventRecord(r, c->stream));
+                    }
+                } else {
+                    TJ_HIP(tjk::fp(c->stream, c->g, c->vol, c->pp, slots, diffIndex, start, fk, c->pflag,
+                                   c->norm_tab, c->sproj, c->snorm, c->tune.fp_rows, c->tune.fp_rays,
+                                   c->tune.fp_zchunk));
+                    if (c->geo.long_object_comp)
+                        TJ_HIP(tjk::norm_max_fill(c->stream, (int)c->npx, diffIndex, c->snorm));
+                    if (c->timing_on) TJ_HIP(hipEventRecord(b, c->stream));
+                    if ((rc = do_residual(c, slots, diffIndex, (double)factorimg, c->err + off))) return rc;
+                    if (c->timing_on) TJ_HIP(hipEventRecord(r, c->stream));
+                }
                 if (bk == 3)
                     if ((rc = ensure_bpairs(c, diffIndex))) return rc;
                 TJ_HIP(tjk::bp(c->stream, c->g, bp_target(c), c->pp, slots, diffIndex, c->sdiff, c->iota,

@@ -844,11 +844,23 @@ __device__ __forceinline__ void fp_lds_chunks(const TjGeomDev& g, const float* _
     }
 }
 
-template <int G>
+// With FUSE the kernel also does k_residual's work for its pixels: instead of
+// proj and norm it writes diff = (float)(((exp - th) / norm) * factor), the
+// FP64 sum of the squared differences per (slot, 4-row block, column tile) and
+// a non-finite flag per (slot, row, column tile); k_err_fold folds both.
+struct TjFpFuse {
+    const float* exps;  // [n][ph][pw], image slot_proj[s] for slot s
+    double factor;
+    float* diff;        // [nslot][ph][pw]
+    double* rerr;       // [nslot][ceil(ph / 4)][ntile]
+    uint8_t* rbad;      // [nslot][ph][ntile]
+};
+
+template <int G, bool FUSE>
 __global__ __launch_bounds__(TJ_BLOCK) void k_fp_lds(
     TjGeomDev g, const float* __restrict__ vol, const TjProj* __restrict__ pp,
     const int* __restrict__ slot_proj, int nslot, int start_row, const float* __restrict__ norm_tab, int zc,
-    int nrb, int nsub, int ntile, float* __restrict__ proj, float* __restrict__ norm)
+    int nrb, int nsub, int ntile, float* __restrict__ proj, float* __restrict__ norm, TjFpFuse fu)
 {
     extern __shared__ float4 lds4[];
     const int tid = threadIdx.x;
@@ -922,6 +934,71 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_fp_lds(
         else if (ns == 3) run(std::integral_constant<int, (G > 3 ? 3 : G)>{});
         else if (ns == 4) run(std::integral_constant<int, (G > 4 ? 4 : G)>{});
         else run(std::integral_constant<int, G>{});
+    }
+    if constexpr (FUSE) {
+        __shared__ double red[TJ_BLOCK / 64][G];
+        __shared__ unsigned redb[TJ_BLOCK / 64][G * 4];
+        const int lane = tid & 63, wv = tid >> 6;
+        const size_t npx = (size_t)g.pw * g.ph;
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            if (k >= ns) break;  // uniform over the block
+            const int p = slot_proj[s0 + k];
+            const double ninc = pp[p].norminc;
+            const float nt = col ? norm_tab[(size_t)p * g.pw + i] : 0.0f;
+            const float rv[4] = {rs[k].x, rs[k].y, rs[k].z, rs[k].w};
+            double v2 = 0.0;  // this lane's squared differences over the block's rows
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int j = j0 + r;
+                if (j >= g.ph) break;  // uniform over the block
+                bool bad = false;
+                if (col) {
+                    float th, n;
+                    if (j < start_row) {
+                        th = 0.0f;
+                        n = 0.0f;
+                    } else if (ok[r] && nt > 0) {
+                        th = (float)((double)rv[r] * ninc);
+                        n = nt;
+                    } else {
+                        th = __builtin_nanf("");
+                        n = 0.0f;
+                    }
+                    const float ex = fu.exps[(size_t)p * npx + (size_t)j * g.pw + i];
+                    float d = 0.0f;
+                    if (!isnan(ex) && !isnan(th)) {
+                        const double value = (double)((ex - th) / n) * fu.factor;
+                        d = (float)value;
+                        v2 += value * value;
+                    }
+                    fu.diff[((size_t)(s0 + k) * g.ph + j) * g.pw + i] = d;
+                    bad = !isfinite(d);
+                }
+                const unsigned wbad = __ballot(bad) != 0 ? 1u : 0u;
+                if (lane == 0) redb[wv][k * 4 + r] = wbad;
+            }
+#pragma unroll
+            for (int m = 32; m >= 1; m >>= 1) v2 += __shfl_xor(v2, m);
+            if (lane == 0) red[wv][k] = v2;
+        }
+        __syncthreads();
+        if (tid < ns * 4) {
+            const int k = tid >> 2, r = tid & 3, j = j0 + r;
+            if (j < g.ph) {
+                unsigned b = 0;
+#pragma unroll
+                for (int w = 0; w < TJ_BLOCK / 64; ++w) b |= redb[w][tid];
+                fu.rbad[((size_t)(s0 + k) * g.ph + j) * ntile + tile] = (uint8_t)b;
+            }
+            if (r == 0) {
+                double sum = 0.0;
+#pragma unroll
+                for (int w = 0; w < TJ_BLOCK / 64; ++w) sum += red[w][k];
+                fu.rerr[((size_t)(s0 + k) * nrb + rb) * ntile + tile] = sum;
+            }
+        }
+        return;
     }
     if (!col) return;
 #pragma unroll
@@ -1698,14 +1775,16 @@ static int g_max_lds = 65536;
 
 void set_max_lds(int bytes) { g_max_lds = bytes; }
 
-template <int G>
+template <int G, bool FUSE = false>
 static hipError_t launch_fp_lds(hipStream_t st, const TjGeomDev& g, const float* vol, const TjProj* pp,
                                 const int* slot_proj, int nslot, int start_row, const float* norm_tab, int zc,
-                                float* proj, float* norm)
+                                float* proj, float* norm, TjFpFuse fu = TjFpFuse{})
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     const int per = (g.vw + 1) * (int)sizeof(float4);
-    int zmax = (g_max_lds - 2 * (int)sizeof(float4)) / per;
+    // the fused form's reduction arrays are static LDS beside the staged slices
+    const int fixed = FUSE ? (TJ_BLOCK / 64) * G * (int)(sizeof(double) + 4 * sizeof(unsigned)) : 0;
+    int zmax = (g_max_lds - 2 * (int)sizeof(float4) - fixed) / per;
     if (zmax < 1) return hipErrorInvalidValue;
     if (zc <= 0 || zc > zmax) zc = zmax;
     if (zc > g.vd) zc = g.vd;
@@ -1715,13 +1794,13 @@ static hipError_t launch_fp_lds(hipStream_t st, const TjGeomDev& g, const float*
     const int ngrp = (nslot + G - 1) / G;
     const int nsub = ntile * ngrp;
     const int nblk = ((nrb + 7) / 8) * 8 * nsub;
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_fp_lds<G>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds);
+    if (lds + fixed > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)k_fp_lds<G, FUSE>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    k_fp_lds<G><<<nblk, TJ_BLOCK, lds, st>>>(g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nrb, nsub,
-                                              ntile, proj, norm);
+    k_fp_lds<G, FUSE><<<nblk, TJ_BLOCK, lds, st>>>(g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nrb,
+                                                    nsub, ntile, proj, norm, fu);
     return hipGetLastError();
 }
 
@@ -1783,6 +1862,48 @@ hipError_t residual(hipStream_t st, int pw, int ph, int nslot, const float* exps
     if (nslot <= 0) return hipSuccess;
     k_residual<<<dim3(ph, nslot), TJ_BLOCK, 0, st>>>(pw, ph, exps, exp_index, proj, norm, factor, diff, row_err, rowflag);
     k_err_sum<<<nslot, TJ_BLOCK, 0, st>>>(ph, row_err, err);
+    return hipGetLastError();
+}
+
+// Per slot: the sum of the error partials of its row blocks and column tiles
+// (in that order), and each row's non-finite flag over its column tiles.
+__global__ __launch_bounds__(TJ_BLOCK) void k_err_fold(int ph, int ntile, const double* __restrict__ rerr,
+                                                       const uint8_t* __restrict__ rbad,
+                                                       uint8_t* __restrict__ rowflag, double* __restrict__ err)
+{
+    __shared__ double sh[TJ_BLOCK];
+    const int s = blockIdx.x;
+    const int nq = ((ph + 3) / 4) * ntile;
+    double acc = 0.0;
+    for (int q = threadIdx.x; q < nq; q += blockDim.x) acc += rerr[(size_t)s * nq + q];
+    for (int j = threadIdx.x; j < ph; j += blockDim.x) {
+        const size_t q = ((size_t)s * ph + j) * ntile;
+        uint8_t b = 0;
+        for (int t = 0; t < ntile; ++t) b |= rbad[q + t];
+        rowflag[(size_t)s * ph + j] = b;
+    }
+    acc = block_reduce(acc, sh, AddD());
+    if (threadIdx.x == 0) err[s] = acc;
+}
+
+// The forward projection of the LDS kernel fused with the residual (the
+// projectAndDifference of one subset): diff, rowflag and the per-slot errors as
+// fp + residual give them (the errors summed in another order), without
+// writing proj and norm.  rerr holds nslot x ceil(ph / 4) x ceil(pw / 256)
+// entries, rbad nslot x ph x ceil(pw / 256).
+hipError_t fp_residual(hipStream_t st, const TjGeomDev& g, const float* vol, const TjProj* pp, const int* slot_proj,
+                       int nslot, int start_row, const float* norm_tab, int rpt, int zc, const float* exps,
+                       double factor, float* diff, double* rerr, uint8_t* rbad, uint8_t* rowflag, double* err)
+{
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    if (nslot <= 0) return hipSuccess;
+    const TjFpFuse fu{exps, factor, diff, rerr, rbad};
+    hipError_t e;
+    if (rpt <= 2) e = launch_fp_lds<2, true>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nullptr, nullptr, fu);
+    else if (rpt <= 4) e = launch_fp_lds<4, true>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nullptr, nullptr, fu);
+    else e = launch_fp_lds<8, true>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nullptr, nullptr, fu);
+    if (e != hipSuccess) return e;
+    k_err_fold<<<nslot, TJ_BLOCK, 0, st>>>(g.ph, (g.pw + TJ_BLOCK - 1) / TJ_BLOCK, rerr, rbad, rowflag, err);
     return hipGetLastError();
 }
 
