@@ -2073,8 +2073,8 @@ hipError_t median_cross(hipStream_t st, const float* src, float* dst, int W, int
 }
 
 // StackStatistics' min, max (mm[0..1]) and 256-bin histogram of the volume;
-// mm needs 2 x 4096 doubles, hist 256 ints.  The histogram pass keeps to 1024
-// blocks: each block merges up to 256 bins into the global histogram.
+// mm needs 2 x 4096 doubles, hist 256 ints.  Both passes run 4096 blocks; each
+// histogram block merges its non-empty bins into the global histogram.
 hipError_t stack_stats(hipStream_t st, const float* vol, size_t n, double* mm, int* hist)
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
@@ -2084,7 +2084,7 @@ hipError_t stack_stats(hipStream_t st, const float* vol, size_t n, double* mm, i
     if (e != hipSuccess) return e;
     k_stack_minmax<<<blocks, TJ_BLOCK, 0, st>>>(vol, n, mm);
     k_stack_reduce<<<1, TJ_BLOCK, 0, st>>>(mm, blocks);
-    k_stack_hist<<<std::min(blocks, 1024), TJ_BLOCK, 0, st>>>(vol, n, mm, hist);
+    k_stack_hist<<<blocks, TJ_BLOCK, 0, st>>>(vol, n, mm, hist);
     return hipGetLastError();
 }
 
