@@ -114,6 +114,30 @@ def test_backprojection_bit_exact(tj, oracle, name, positivity):
         assert same(got, ref), (name, int((got != ref).sum()))
 
 
+@pytest.mark.parametrize("update", [0, 3])
+def test_fused_loop_nan_measured_pixels(tj, oracle, update):
+    """The OS-SART loop, whose forward kernel writes the differences itself:
+    NaN measured pixels and a NaN row give zero differences (difference skips
+    them), so volume and error rows follow the oracle; ragged 300-column
+    detector (two column tiles, the second partial)."""
+    W, H, D = 300, 9, 10
+    E = oracle.tilt_eulers(np.arange(-50, 51, 10.0))
+    exps = (oracle.phantom_project(oracle.ellipsoids(29, 4, W, H, D), E, W, H)
+            + 0.05 * oracle.gaussian_noise(3, len(E) * W * H).reshape(len(E), H, W)).astype(np.float32)
+    exps[2, 3, 17:40] = np.nan
+    exps[5, 6, :] = np.nan
+    exps[7, 0, 299] = np.nan
+    print(f"fused loop NaN pixels update={update}", flush=True)
+    ref = np.zeros((D, H, W), np.float32)
+    ref_err = oracle.ossart(oracle.make_geom(W, H, D, nthreads=8), ref, exps, E, 2, 0.8, update)
+    with make(tj, (W, H, D)) as P:
+        P.load_tilt_series(exps, E)
+        err = P.run_ossart(2, 0.8, update)
+        assert same(P.get_volume(), ref)
+    assert np.allclose(err[:, [0, 1, 3]], ref_err[:, [0, 1, 3]], rtol=1e-12)
+    assert np.array_equal(err[:, [2, 4]], ref_err[:, [2, 4]])
+
+
 def test_backprojection_nan_inf_rows(tj, oracle):
     """NaN pixels (and the zero-weight next-row NaN rule) and Inf sums follow the oracle."""
     W, H, D = 18, 7, 6
