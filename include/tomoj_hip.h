@@ -154,6 +154,30 @@ int tj_get_projection(tj_ctx *ctx, int slot, float *proj, float *norm);
  * tj_project_and_difference (VoxelProjector3D.getDifferenceImage, :139-141). */
 int tj_get_difference(tj_ctx *ctx, int slot, float *diff);
 
+/* Allocation-free projectAndDifference.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same results as
+ * tj_project_and_difference on a context holding this volume and these images
+ * added in order 0..n-1: the n difference images bit for bit, the FP64 errors
+ * summed in another order (a relative difference of order 1e-15).  Inputs and
+ * outputs are the caller's device buffers: `vol_yzx` the volume in the
+ * library's resident layout (row y, slice z, x fastest: vol_h x vol_d x vol_w),
+ * `measured` and `diff` n x proj_h x proj_w; `euler` (n x 9) and `errors` (n)
+ * are host arrays.  Every workspace lives in one device array of
+ * TJ_NOALLOC_SCRATCH_BYTES per device, held until the process ends; the call
+ * allocates nothing.  Covers the geometry of the LDS forward kernel: one vertical
+ * tilt axis, scale 1, whole volume rows, a finite volume and no long-object
+ * compensation (otherwise TJ_EUNSUP); a size whose workspaces exceed the array
+ * (tj_noalloc_scratch_bytes) gives TJ_ENOMEM.  Runs on `stream` (a hipStream_t;
+ * NULL: the null stream) and returns when the work is done.  Calls on one device
+ * share the array and are serialised by the library; the array keeps the last
+ * call's contents and nothing carries over from one call to the next. */
+#define TJ_NOALLOC_SCRATCH_BYTES (16u << 20)
+size_t tj_noalloc_scratch_bytes(const tj_geometry *g, int n);
+int tj_project_and_difference_noalloc(int device, const tj_geometry *g, const float *vol_yzx, const float *measured,
+                                      const double *euler, int n, double factor, int start_y, int end_y, float *diff,
+                                      double *errors, void *stream);
+
 /* Projector.backproject(startY,endY) (Projector.java:138; VoxelProjector3D.java:155-308):
  * adds the difference images of every added projection (or, when no difference
  * was computed since the last clear, the projections themselves, weighted if a

@@ -31,6 +31,7 @@ TJ_STORAGE_F64 = 3
 STORAGE_NAMES = {"f32": TJ_STORAGE_F32, "bf16": TJ_STORAGE_BF16, "f16": TJ_STORAGE_F16, "f64": TJ_STORAGE_F64}
 
 TJ_RUN_GRAPH = 1
+TJ_NOALLOC_SCRATCH_BYTES = 16 << 20   # include/tomoj_hip.h
 
 
 class TjError(RuntimeError):
@@ -100,6 +101,9 @@ _SIGS = {
     "tj_project_and_difference": (C.c_int, [_P, C.c_double, C.c_int, C.c_int, _P]),
     "tj_get_projection": (C.c_int, [_P, C.c_int, _P, _P]),
     "tj_get_difference": (C.c_int, [_P, C.c_int, _P]),
+    "tj_noalloc_scratch_bytes": (C.c_size_t, [C.POINTER(tj_geometry), C.c_int]),
+    "tj_project_and_difference_noalloc": (C.c_int, [C.c_int, C.POINTER(tj_geometry), _P, _P, _P, C.c_int,
+                                                    C.c_double, C.c_int, C.c_int, _P, _P, _P]),
     "tj_backproject": (C.c_int, [_P, C.c_int, C.c_int]),
     "tj_set_weighting": (C.c_int, [_P, C.c_double, _P, _P, C.c_int]),
     "tj_set_ramp_filter": (C.c_int, [_P, C.c_int, C.c_double]),

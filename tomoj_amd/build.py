@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtomoj_hip.so")
-SOURCES = ["tj_kernels.hip", "tj_fft.hip", "tj_api.hip", "tj_batch.hip", "tj_mrc.cpp"]
+SOURCES = ["tj_kernels.hip", "tj_fft.hip", "tj_api.hip", "tj_batch.hip", "tj_noalloc.hip", "tj_mrc.cpp"]
 ARCH = os.environ.get("TOMOJ_OFFLOAD_ARCH", "gfx950")
 
 

@@ -232,7 +232,7 @@ static void ctx_free(tj_ctx* c)
 // Per-projection constants of projectJosephPartial (VoxelProjector3D.java:682-694),
 // same operation order as the Java source (host code is built without FMA
 // contraction).
-static TjProj make_proj(const double* r)
+TjProj tj_make_proj(const double* r)
 {
     TjProj p{};
     for (int k = 0; k < 9; ++k) p.e[k] = r[k];
@@ -261,13 +261,12 @@ static bool all_axis(const tj_ctx* c, const std::vector<int>& idx)
 // launch.  Checked here with the kernel's own formulas: every projection must
 // give the same y for each detector row, and the inside rows must map to
 // consecutive volume rows.
-static bool fp_lds_ok(const tj_ctx* c, const std::vector<int>& idx, int start_row)
+bool tj_fp_lds_ok(const TjGeomDev& g, const TjProj* hpp, const int* idx, int n, int start_row)
 {
-    const TjGeomDev& g = c->g;
-    if (g.scale != 1.0 || idx.empty() || c->any_nonfinite || !c->norm_tab) return false;
-    const TjProj& a = c->hpp[idx[0]];
-    for (int i : idx) {
-        const TjProj& b = c->hpp[i];
+    if (g.scale != 1.0 || n <= 0) return false;
+    const TjProj& a = hpp[idx[0]];
+    for (int k = 0; k < n; ++k) {
+        const TjProj& b = hpp[idx[k]];
         if (!b.axis || b.e[4] != 1.0 || b.icf != a.icf || b.zcf != a.zcf || b.divy != a.divy || b.diry != a.diry)
             return false;
     }
@@ -291,6 +290,12 @@ static bool fp_lds_ok(const tj_ctx* c, const std::vector<int>& idx, int start_ro
         }
     }
     return true;
+}
+
+static bool fp_lds_ok(const tj_ctx* c, const std::vector<int>& idx, int start_row)
+{
+    if (idx.empty() || c->any_nonfinite || !c->norm_tab) return false;
+    return tj_fp_lds_ok(c->g, c->hpp.data(), idx.data(), (int)idx.size(), start_row);
 }
 
 // The LDS backprojection bounds the detector columns a block reaches by
@@ -400,11 +405,36 @@ static int check_range(const tj_ctx* c, int start_y, int end_y)
 }
 
 // First detector row VoxelProjector3D.project computes (:338, :347).
-static int fp_start_row(const tj_ctx* c, int start_y)
+int tj_fp_start_row(const tj_geometry& g, int start_y)
 {
-    const tj_geometry& g = c->geo;
     const int offset = (g.proj_h > g.vol_h && g.scale == 1) ? (g.proj_h - g.vol_h) / 2 : 0;
     return (int)std::floor((start_y + offset) * g.scale + 0.5);
+}
+
+static int fp_start_row(const tj_ctx* c, int start_y) { return tj_fp_start_row(c->geo, start_y); }
+
+// TjGeomDev from the ABI geometry: borders W + ACCURACY and H + ACCURACY are
+// evaluated in float as in VoxelProjector3D.java:700-701.
+void tj_geom_to_dev(const tj_geometry* geo, TjGeomDev* out)
+{
+    TjGeomDev& g = *out;
+    g = TjGeomDev{};
+    g.vw = geo->vol_w;
+    g.vh = geo->vol_h;
+    g.vd = geo->vol_d;
+    g.pw = geo->proj_w;
+    g.ph = geo->proj_h;
+    g.positivity = geo->positivity ? 1 : 0;
+    g.pc0 = geo->proj_center[0];
+    g.pc1 = geo->proj_center[1];
+    g.pc2 = geo->proj_center[2];
+    g.pc3 = geo->proj_center[3];
+    g.vcx = geo->vol_center[0];
+    g.vcy = geo->vol_center[1];
+    g.vcz = geo->vol_center[2];
+    g.scale = geo->scale;
+    g.bxmax = (double)((float)g.vw + 1e-10f);
+    g.bymax = (double)((float)g.vh + 1e-10f);
 }
 
 static hipEvent_t ev(tj_ctx* c, size_t k)
@@ -619,23 +649,8 @@ int tj_open(int device, const tj_geometry* geo, tj_ctx** out)
     tj_ctx* c = new tj_ctx();
     c->device = device;
     c->geo = *geo;
-    TjGeomDev& g = c->g;
-    g.vw = geo->vol_w;
-    g.vh = geo->vol_h;
-    g.vd = geo->vol_d;
-    g.pw = geo->proj_w;
-    g.ph = geo->proj_h;
-    g.positivity = geo->positivity ? 1 : 0;
-    g.pc0 = geo->proj_center[0];
-    g.pc1 = geo->proj_center[1];
-    g.pc2 = geo->proj_center[2];
-    g.pc3 = geo->proj_center[3];
-    g.vcx = geo->vol_center[0];
-    g.vcy = geo->vol_center[1];
-    g.vcz = geo->vol_center[2];
-    g.scale = geo->scale;
-    g.bxmax = (double)((float)g.vw + 1e-10f);
-    g.bymax = (double)((float)g.vh + 1e-10f);
+    tj_geom_to_dev(geo, &c->g);
+    const TjGeomDev& g = c->g;
     c->npx = (size_t)g.pw * g.ph;
     c->plane = (size_t)g.vd * g.vw;
     c->nvox = c->plane * g.vh;
@@ -673,7 +688,7 @@ int tj_set_eulers(tj_ctx* c, const double* euler, int n)
     if (n != c->nproj) return fail(TJ_EINVAL, "tj_set_eulers: %d matrices for %d projections", n, c->nproj);
     TJ_HIP(hipSetDevice(c->device));
     c->hpp.resize(n);
-    for (int k = 0; k < n; ++k) c->hpp[k] = make_proj(euler + 9 * k);
+    for (int k = 0; k < n; ++k) c->hpp[k] = tj_make_proj(euler + 9 * k);
     TJ_HIP(hipMemcpy(c->pp, c->hpp.data(), sizeof(TjProj) * n, hipMemcpyHostToDevice));
     TJ_HIP(tjk::norm_table(c->stream, c->g, c->pp, n, c->norm_tab));
     TJ_HIP(hipStreamSynchronize(c->stream));

@@ -56,3 +56,9 @@ void tj_record_error(const char* msg);
 
 // TomoReconstruction2.orderShuffleART with the first image fixed (host code).
 void tj_order_shuffle(const double* euler, int n, int first, int* order);
+
+// Host helpers shared by the context API (tj_api.hip) and the allocation-free
+// entry point (tj_noalloc.hip); tj_geom_to_dev and tj_fp_start_row, which take
+// the ABI's tj_geometry, are declared where tomoj_hip.h is included.
+TjProj tj_make_proj(const double* r);
+bool tj_fp_lds_ok(const TjGeomDev& g, const TjProj* hpp, const int* idx, int n, int start_row);
