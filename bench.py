@@ -15,7 +15,15 @@ the serving form (a Poisson stream, p50/p99 latency), and TomoJ's CPU
 algorithm (the C restatement in oracle/) timed on the host cores for one full
 iteration of the same workload.
 
+Roofline figures: `roofline` is the dominant kernel (algorithmic bytes per
+launch over its HIP-event time, with its counter-measured traffic and its VALU
+issue fraction); `hbm_iteration` is the counter-measured HBM bytes of a whole
+iteration (profiles/traffic.json) over the measured iteration time; the
+`iteration_model` entry is SURVEY's per-projection streaming model, a model
+figure that counts more volume reads than the kernels make.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-forms]
+       python bench.py --workload sirt_512x512x512_121 ...   (another timed workload)
        python bench.py --precision-study W H D N   (one precision study only)
 """
 import argparse
@@ -37,41 +45,148 @@ HBM_PEAK_GBS = 8000.0
 # (FP32 and FP64 adds and multiplies both issue at this rate; packed FP32 is
 # not used).
 VALU_PEAK = 256 * 64 * 2.4e9
-# VALU lane operations per voxel-ray update of the plain kernels on the
-# OS-SART 512x512x256 workload (SQ_INSTS_VALU x 64 / updates per launch).
-VALU_LANE_OPS_PER_UPDATE = {"fp": 12.9, "bp": 11.8}
-
-
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
 def kernel_bytes(w):
-    """Algorithmic HBM bytes per iteration of each kernel (FP32 storage): the
-    forward projection, fused with the residual, streams the volume once per
-    projection, reads each measured image and writes each difference image;
-    the residual step left after it folds the per-block error partials and row
-    flags; the backprojection reads and writes the volume once per subset and
-    reads every diff image."""
+    """Algorithmic HBM bytes per iteration of each timed kernel group (FP32
+    storage).  fp: the forward projection fused with the residual, in the
+    per-projection slab-streaming model (the volume streamed once per
+    projection; the kernel itself reads it about once per subset, so this term
+    exceeds the bytes that cross HBM), plus the measured images read and the
+    differences written.  res: the fold of the per-block error partials and
+    row flags.  bpprep: the backprojection's row table and FP64 pair images
+    (difference read, 16-byte pair written per pixel).  bp: the backprojection
+    kernel, the volume read and written once per subset plus the differences
+    (the model's 4 bytes per pixel; the kernel stages the pair images)."""
     nvox, npx, n, s = w.nvox, w.npx, w.n, w.subsets()
     ntile = (w.W + 255) // 256
+    nrb = (w.H + 3) // 4
     return {
         "fp": n * nvox * 4 + n * npx * 8,
-        "res": n * (((w.H + 3) // 4) * ntile * 8 + w.H * ntile + w.H),
+        "res": n * (nrb * ntile * 8 + w.H * ntile + w.H + 8),
+        "bpprep": n * npx * (4 + 16) + s * w.H * 32,
         "bp": s * 2 * nvox * 4 + n * npx * 4,
     }
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE/WRITE_SIZE runs
-    summarised in profiles/traffic.json (scaled by the copy-kernel calibration), or None."""
+# Kernels of one plain OS-SART/SIRT iteration on the LDS path, as rocprofv3
+# names them, grouped as tj_timing groups them.
+ITERATION_KERNELS = {"fp": "k_fp_lds", "res": "k_err_fold", "bpprep": ("k_bp_rows", "k_bp_pairs"), "bp": "k_bp_lds"}
+
+
+def load_traffic():
+    """profiles/traffic.json: HBM bytes per launch of each kernel of a workload
+    from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs, scaled by the copy-kernel
+    calibration.  None when the file is absent."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as fh:
-            data = json.load(fh)
-        return data["kernels"][kernel]["hbm_bytes_per_launch"]
-    except (OSError, KeyError, ValueError):
+            return json.load(fh)
+    except (OSError, ValueError):
         return None
+
+
+def _counter_record(traffic, workload, kernel):
+    try:
+        ks = traffic["workloads"][workload]["kernels"]
+    except (TypeError, KeyError):
+        return None
+    for name, rec in ks.items():
+        if name == kernel or name.startswith(kernel + "<"):
+            return rec
+    return None
+
+
+def measured_traffic(traffic, workload, kernel):
+    """Calibrated HBM bytes per launch of `kernel` (a rocprofv3 name prefix) on
+    `workload`, or None."""
+    rec = _counter_record(traffic, workload, kernel)
+    return rec["hbm_bytes_per_launch"] if rec else None
+
+
+def valu_lane_ops_per_update(traffic, w, kernel):
+    """VALU lane operations per voxel-ray update of `kernel` on workload `w`:
+    SQ_INSTS_VALU (wave instructions per launch, rocprofv3) x 64 lanes over the
+    updates of an average launch (N x Nvox / subsets)."""
+    rec = _counter_record(traffic, w.name, kernel)
+    if not rec or not rec.get("valu_wave_instr"):
+        return None
+    return rec["valu_wave_instr"] * 64 / (w.n * w.nvox / w.subsets())
+
+
+def iteration_traffic(traffic, w, launches_per_iteration):
+    """Counter-measured HBM bytes of one iteration: the calibrated bytes per
+    launch of every kernel of the iteration times its launches per iteration."""
+    total = 0.0
+    parts = {}
+    for group, names in ITERATION_KERNELS.items():
+        for kname in (names if isinstance(names, tuple) else (names,)):
+            b = measured_traffic(traffic, w.name, kname)
+            if b is None:
+                return None, None
+            parts[kname] = b * launches_per_iteration[group]
+            total += parts[kname]
+    return total, parts
+
+
+def kernel_table(w, tim, steps):
+    """Per timed group: launches, average launch time (HIP events on the
+    library's stream) and algorithmic bytes per launch."""
+    kb = kernel_bytes(w)
+    out = {}
+    for k in ("fp", "res", "bpprep", "bp"):
+        launches = tim[f"{k}_launches"]
+        ms_k = tim[f"{k}_ms"]
+        total_bytes = kb[k] * steps
+        out[k] = {
+            "launches": launches,
+            "avg_ms": ms_k / max(launches, 1),
+            "bytes_per_launch": total_bytes / max(launches, 1),
+            "achieved_GBs": total_bytes / (ms_k * 1e-3) / 1e9 if ms_k > 0 else None,
+        }
+    return out
+
+
+def roofline_of(w, kernels, tim, steps, traffic):
+    """The roofline object of the dominant kernel (the longest of the forward
+    and backprojection launches): algorithmic bytes per launch over its live
+    launch time against the 8 TB/s HBM peak, its counter-measured traffic, and
+    the same launch against the vector issue rate."""
+    dom = max(("fp", "bp"), key=lambda k: tim[f"{k}_ms"])
+    kname = ITERATION_KERNELS[dom]
+    achieved = kernels[dom]["achieved_GBs"]
+    r = {
+        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(traffic, w.name, kname),
+        "kernel": kname,
+        "bytes_per_launch": kernels[dom]["bytes_per_launch"],
+        "avg_launch_ms": kernels[dom]["avg_ms"],
+        "limiter": "VALU issue and LDS: FP64 positions and ImageJ's FP64 bilinear weights per voxel update",
+    }
+    if r["traffic"]:
+        r["traffic_over_algorithmic"] = r["traffic"] / r["bytes_per_launch"]
+    per_update = valu_lane_ops_per_update(traffic, w, kname)
+    if per_update:
+        upd = w.voxel_ray_updates_per_iteration() / 2 * steps / max(tim[f"{dom}_launches"], 1)
+        valu = per_update * upd / (kernels[dom]["avg_ms"] * 1e-3)
+        r["valu"] = {"lane_ops_per_update": per_update, "achieved": valu / 1e12, "peak": VALU_PEAK / 1e12,
+                     "unit": "T lane-ops/s", "frac": valu / VALU_PEAK}
+    return r
+
+
+def hbm_iteration(w, tim, steps, ms_per_step, traffic):
+    """Counter-measured HBM bytes of one whole iteration over the measured
+    iteration time, against the 8 TB/s peak."""
+    lpi = {k: tim[f"{k}_launches"] / steps for k in ITERATION_KERNELS}
+    total, parts = iteration_traffic(traffic, w, lpi)
+    if total is None:
+        return None
+    gbs = total / (ms_per_step * 1e-3) / 1e9
+    return {"bytes_per_iteration": total, "achieved_GBs": gbs, "peak": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS,
+            "by_kernel": parts,
+            "source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, copy-calibrated (profiles/traffic.json)"}
 
 
 def run_gpu(w, steps, warmup, tj):
@@ -91,10 +206,34 @@ def run_gpu(w, steps, warmup, tj):
     return P, E, err, (t1 - t0), tim
 
 
+def host_cpu():
+    """Host CPU description: the model name, the logical CPUs the OS reports,
+    the CPUs this process may run on, and OMP_NUM_THREADS."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count()
+    return {"model": model, "nproc": os.cpu_count(), "affinity": affinity,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(w, P, E, oracle):
     """TomoJ's CPU algorithm (C restatement, TomoJ's 2-row task decomposition with a
-    join per projection) for one full OS-SART iteration of the same workload."""
-    cores = max(1, min(16, os.cpu_count() or 1))
+    join per projection) for one full OS-SART iteration of the same workload, on
+    the host CPUs this process is given: OMP_NUM_THREADS when it is set (the
+    share of the host a GPU job gets), else every CPU in the affinity mask."""
+    cpu = host_cpu()
+    cores = int(cpu["omp_num_threads"] or 0) or cpu["affinity"] or 1
+    cores = max(1, min(cores, cpu["affinity"] or cores))
     exps = np.stack([P.get_loaded_projection(k) for k in range(w.n)])
     g = oracle.make_geom(w.W, w.H, w.D, nthreads=cores)
     vol = np.zeros((w.D, w.H, w.W), np.float32)
@@ -107,9 +246,10 @@ def cpu_baseline(w, P, E, oracle):
         "cores": cores,
         "kind": "port",
         "sample": f"one full OS-SART iteration ({w.subsets()} subsets, {w.n} projections) of "
-                  f"{w.W}x{w.H}x{w.D}, C restatement of VoxelProjector3D/TomoReconstruction2, "
-                  f"{cores} OpenMP threads, {dt:.1f} s",
+                  f"{w.W}x{w.H}x{w.D}, C restatement of VoxelProjector3D/TomoReconstruction2 "
+                  f"(TomoJ CPU algorithm, not the JVM), {cores} OpenMP threads, {dt:.1f} s",
         "gups": w.voxel_ray_updates_per_iteration() / dt / 1e9,
+        "host": cpu,
     }
 
 
@@ -161,6 +301,9 @@ def regularisation_extras(w, P, tj, iters=3):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="ossart_512x512x256_121",
+                    choices=["ossart_512x512x256_121", "sirt_512x512x512_121", "sirt_256x256x128_61"],
+                    help="the timed workload (default: the production OS-SART configuration)")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -196,53 +339,25 @@ def main():
         print(json.dumps({"precision_study": res}), flush=True)
         return
 
-    w = WL.OSSART_512
+    w = WL.ALL[args.workload]
+    traffic = load_traffic()
     log(f"[bench] workload {w.name}: {w.W}x{w.H}x{w.D}, {w.n} projections, update {w.update}")
     P, E, err, dt, tim = run_gpu(w, args.steps, args.warmup, tj)
     ms = dt / args.steps * 1e3
     iters = args.steps / dt
     gups = w.voxel_ray_updates_per_iteration() * iters / 1e9
     log(f"[bench] {args.steps} iterations in {dt:.3f} s: {iters:.2f} it/s, {gups:.1f} GUPS; "
-        f"fp {tim['fp_ms']:.1f} ms, res {tim['res_ms']:.1f} ms, bp {tim['bp_ms']:.1f} ms, "
-        f"other {tim['other_ms']:.1f} ms")
+        f"fp {tim['fp_ms']:.1f} ms, res {tim['res_ms']:.1f} ms, bp prep {tim['bpprep_ms']:.1f} ms, "
+        f"bp {tim['bp_ms']:.1f} ms, other {tim['other_ms']:.1f} ms; fused subsets {tim['fused_subsets']}")
 
-    kb = kernel_bytes(w)
-    kernels = {}
-    for k in ("fp", "res", "bp"):
-        launches = tim[f"{k}_launches"]
-        ms_k = tim[f"{k}_ms"]
-        total_bytes = kb[k] * args.steps
-        kernels[k] = {
-            "launches": launches,
-            "avg_ms": ms_k / max(launches, 1),
-            "bytes_per_launch": total_bytes / max(launches, 1),
-            "achieved_GBs": total_bytes / (ms_k * 1e-3) / 1e9 if ms_k > 0 else None,
-        }
-    dom = max(("fp", "bp"), key=lambda k: tim[f"{k}_ms"])
-    achieved = kernels[dom]["achieved_GBs"]
-    kname = {"fp": "k_fp_lds", "bp": "k_bp_lds"}[dom]
-    roofline = {
-        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kname),
-        "kernel": kname,
-        "bytes_per_launch": kernels[dom]["bytes_per_launch"],
-        "avg_launch_ms": kernels[dom]["avg_ms"],
-        "limiter": "VALU and LDS: FP64 positions and ImageJ's FP64 bilinear weights per voxel update "
-                   "(SQ_INSTS_VALU in profiles/); measured HBM traffic is within 8% of the algorithmic bytes",
-    }
-    # The same launch against the VALU issue rate: lane operations per voxel-ray
-    # update from SQ_INSTS_VALU x 64 / updates (rocprofv3 counters of this
-    # workload), times the updates per launch, over the live launch time.
-    per_update = VALU_LANE_OPS_PER_UPDATE[dom]
-    upd = w.voxel_ray_updates_per_iteration() / 2 * args.steps / max(tim[f"{dom}_launches"], 1)
-    valu = per_update * upd / (kernels[dom]["avg_ms"] * 1e-3)
-    roofline["valu"] = {"lane_ops_per_update": per_update, "achieved": valu / 1e12, "peak": VALU_PEAK / 1e12,
-                        "unit": "T lane-ops/s", "frac": valu / VALU_PEAK}
+    kernels = kernel_table(w, tim, args.steps)
+    roofline = roofline_of(w, kernels, tim, args.steps, traffic)
     alg = WL.algorithmic_bytes_per_iteration(w)
+    algo_name = "OS-SART" if w.update else "SIRT"
     out = {
         "metric": METRIC,
         "value": iters,
-        "unit": "OS-SART iterations/s",
+        "unit": f"{algo_name} iterations/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -251,19 +366,34 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded 40-ellipsoid phantom, analytic FP64 projections + 1% noise, generated on device)",
+        "data": "synthetic (seeded ellipsoid phantom, analytic FP64 projections + 1% noise, generated on device)",
         "config": {"workload": w.name, "volume": [w.W, w.H, w.D], "projections": w.n,
-                   "detector": [w.W, w.H], "tilts_deg": [w.tilts[0], w.tilts[-1], 1.0],
-                   "algorithm": "OS-SART", "update": w.update, "subsets": w.subsets(),
-                   "relaxation": w.relaxation, "first_index": 0},
+                   "detector": [w.W, w.H], "tilts_deg": [w.tilts[0], w.tilts[-1], w.tilts[1] - w.tilts[0]],
+                   "algorithm": algo_name, "update": w.update, "subsets": w.subsets(),
+                   "relaxation": w.relaxation, "first_index": 0, "ellipsoids": w.ellipsoids},
         "gups": gups,
         "roofline": roofline,
-        "iteration_roofline": {"model_bytes": alg, "achieved_GBs": alg * iters / 1e9,
-                               "frac": alg * iters / 1e9 / HBM_PEAK_GBS,
-                               "compulsory_bytes": WL.compulsory_bytes_per_iteration(w)},
+        # counter-measured HBM bytes of the whole iteration over its time: the
+        # honest bandwidth figure of the path (it is bound by VALU issue and
+        # LDS, not by HBM)
+        "hbm_iteration": hbm_iteration(w, tim, args.steps, ms, traffic),
+        # SURVEY's per-projection slab-streaming model: it counts one volume
+        # read per projection, which the LDS kernels never make (they read the
+        # volume about once per subset), so it is a model figure, above the
+        # bytes that cross HBM, not a measurement of HBM use
+        "iteration_model": {"model_bytes": alg, "model_GBs": alg * iters / 1e9,
+                            "model_frac": alg * iters / 1e9 / HBM_PEAK_GBS,
+                            "compulsory_bytes": WL.compulsory_bytes_per_iteration(w),
+                            "note": "model, not measured traffic: counts one volume read per projection"},
         "kernels": kernels,
+        "fused_subsets": tim["fused_subsets"],
         "final_residual": float(err[-1, 0]),
     }
+    log(f"[bench] dominant kernel {roofline['kernel']}: {roofline['avg_launch_ms']:.4f} ms/launch, "
+        f"{roofline['achieved']:.0f} GB/s algorithmic ({roofline['frac']:.3f} of HBM peak)" +
+        (f", VALU {roofline['valu']['frac']:.3f}" if "valu" in roofline else "") +
+        (f"; iteration HBM (counters) {out['hbm_iteration']['achieved_GBs']:.0f} GB/s = "
+         f"{out['hbm_iteration']['frac']:.3f}" if out["hbm_iteration"] else ""))
 
     if not args.no_extras:
         # weighted backprojection pass (Radermacher weighting, diameter 0.5*W) on the same data
@@ -279,21 +409,21 @@ def main():
         out["wbp_ramp_hamming_ms"] = (time.perf_counter() - t0) * 1e3
         log(f"[bench] WBP: radermacher {out['wbp_radermacher_ms']:.1f} ms, "
             f"ramp+hamming {out['wbp_ramp_hamming_ms']:.1f} ms")
-        # literal 512^3 SIRT
+        # literal 512^3 SIRT, with its own kernel times and roofline
         c = WL.SIRT_512_CUBE
-        Q = tj.VoxelProjector3DHIP((c.W, c.H, c.D))
-        Q.synth_tilt_series(tj.phantom_ellipsoids(c.seed, c.ellipsoids, c.W, c.H, c.D),
-                            WL.eulers_for(c), c.noise_rel, c.noise_seed)
-        Q.run_ossart(1, 1.0, 0)
-        k = 3
-        t0 = time.perf_counter()
-        Q.run_ossart(k, 1.0, 0)
-        dtc = time.perf_counter() - t0
-        out["sirt_512cube"] = {"iters_per_s": k / dtc,
-                               "gups": c.voxel_ray_updates_per_iteration() * k / dtc / 1e9,
-                               "roofline_frac": WL.algorithmic_bytes_per_iteration(c) * k / dtc / 1e9 / HBM_PEAK_GBS}
-        log(f"[bench] SIRT 512^3: {k / dtc:.2f} it/s")
-        Q.close()
+        if c.name != w.name:
+            Q, _, _, dtc, timc = run_gpu(c, 3, 1, tj)
+            kc = kernel_table(c, timc, 3)
+            out["sirt_512cube"] = {"iters_per_s": 3 / dtc,
+                                   "ms_per_iteration": dtc / 3 * 1e3,
+                                   "gups": c.voxel_ray_updates_per_iteration() * 3 / dtc / 1e9,
+                                   "roofline": roofline_of(c, kc, timc, 3, traffic),
+                                   "hbm_iteration": hbm_iteration(c, timc, 3, dtc / 3 * 1e3, traffic),
+                                   "iteration_model_frac": WL.algorithmic_bytes_per_iteration(c) * 3 / dtc / 1e9
+                                   / HBM_PEAK_GBS,
+                                   "kernels": kc}
+            log(f"[bench] SIRT 512^3: {3 / dtc:.2f} it/s")
+            Q.close()
         out["regularisation"] = regularisation_extras(w, P, tj)
         r = out["regularisation"]
         log(f"[bench] {w.W}x{w.H}x{w.D}: SIRT {r['sirt_it_s']:.1f} it/s, TVM (5 TV steps) "

@@ -446,12 +446,19 @@ int tj_get_loaded_projection(tj_ctx *ctx, int index, float *out);
 
 /* Timing of the most recent tj_run_ossart / tj_run_wbp, from HIP events on the
  * context's stream: total milliseconds, and the summed milliseconds and launch
- * counts of the forward-projection kernel, the residual kernels (difference and
- * error sums) and the backprojection kernel (with its row table). */
+ * counts of the forward-projection kernel (in the fused loop: the forward kernel
+ * that writes the differences), the residual kernels (difference images and
+ * error sums; in the fused loop the fold of the error partials), the
+ * backprojection's preparation kernels (row table and, for the LDS kernel, the
+ * FP64 pair images) and the backprojection kernel itself. */
 typedef struct {
     double total_ms;
     double fp_ms, res_ms, bp_ms, other_ms;
     int fp_launches, res_launches, bp_launches;
+    double bpprep_ms;
+    int bpprep_launches;
+    int fused_subsets;  /* subsets whose projectAndDifference ran as the fused
+                           forward kernel (counted with timing on or off) */
 } tj_timing;
 int tj_get_timing(tj_ctx *ctx, tj_timing *t);
 

@@ -57,7 +57,8 @@ class tj_geometry(C.Structure):
 class tj_timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("fp_ms", C.c_double), ("res_ms", C.c_double),
                 ("bp_ms", C.c_double), ("other_ms", C.c_double), ("fp_launches", C.c_int),
-                ("res_launches", C.c_int), ("bp_launches", C.c_int)]
+                ("res_launches", C.c_int), ("bp_launches", C.c_int),
+                ("bpprep_ms", C.c_double), ("bpprep_launches", C.c_int), ("fused_subsets", C.c_int)]
 
 
 class tj_tuning(C.Structure):

@@ -1890,10 +1890,12 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_err_fold(int ph, int ntile, const 
 // projectAndDifference of one subset): diff, rowflag and the per-slot errors as
 // fp + residual give them (the errors summed in another order), without
 // writing proj and norm.  rerr holds nslot x ceil(ph / 4) x ceil(pw / 256)
-// entries, rbad nslot x ph x ceil(pw / 256).
+// entries, rbad nslot x ph x ceil(pw / 256).  `mid` (or null) is recorded
+// between the forward kernel and the fold.
 hipError_t fp_residual(hipStream_t st, const TjGeomDev& g, const float* vol, const TjProj* pp, const int* slot_proj,
                        int nslot, int start_row, const float* norm_tab, int rpt, int zc, const float* exps,
-                       double factor, float* diff, double* rerr, uint8_t* rbad, uint8_t* rowflag, double* err)
+                       double factor, float* diff, double* rerr, uint8_t* rbad, uint8_t* rowflag, double* err,
+                       hipEvent_t mid)
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nslot <= 0) return hipSuccess;
@@ -1903,6 +1905,7 @@ hipError_t fp_residual(hipStream_t st, const TjGeomDev& g, const float* vol, con
     else if (rpt <= 4) e = launch_fp_lds<4, true>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nullptr, nullptr, fu);
     else e = launch_fp_lds<8, true>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nullptr, nullptr, fu);
     if (e != hipSuccess) return e;
+    if (mid && (e = hipEventRecord(mid, st)) != hipSuccess) return e;
     k_err_fold<<<nslot, TJ_BLOCK, 0, st>>>(g.ph, (g.pw + TJ_BLOCK - 1) / TJ_BLOCK, rerr, rbad, rowflag, err);
     return hipGetLastError();
 }
@@ -1919,12 +1922,17 @@ hipError_t row_flags(hipStream_t st, int pw, int ph, int nslot, const float* ima
 template <int R, int KC, int G>
 static hipError_t launch_bp_lds(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp,
                                 const int* slot_proj, int nslot, const float* images, const int* image_index,
-                                const TjBpRow* rows, int start_y, int end_y, int finite_volume, double2* pairs)
+                                const TjBpRow* rows, int start_y, int end_y, int finite_volume, double2* pairs,
+                                hipEvent_t mid)
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     const size_t npx = (size_t)g.pw * g.ph;
     k_bp_pairs<<<dim3((unsigned)std::min<size_t>((npx + TJ_BLOCK - 1) / TJ_BLOCK, 1024), nslot), TJ_BLOCK, 0, st>>>(
         images, image_index, g.pw, g.ph, pairs);
+    if (mid) {
+        const hipError_t e = hipEventRecord(mid, st);
+        if (e != hipSuccess) return e;
+    }
     const int ny = end_y - start_y;
     const unsigned grid = (unsigned)(((g.vw + TJ_BLOCK - 1) / TJ_BLOCK) * ((ny + R - 1) / R) * ((g.vd + KC - 1) / KC));
     if (g.positivity)
@@ -1938,13 +1946,19 @@ static hipError_t launch_bp_lds(hipStream_t st, const TjGeomDev& g, float* vol, 
 hipError_t bp(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp, const int* slot_proj, int nslot,
               const float* images, const int* image_index, const uint8_t* rowflag, TjBpRow* rows, int start_y,
               int end_y, int kernel, int rows_per_thread, int kchunk, int group, int finite_volume,
-              double2* pairs)
+              double2* pairs, hipEvent_t mid)
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nslot <= 0 || end_y <= start_y) return hipSuccess;
     const int bx = (g.vw + TJ_BLOCK - 1) / TJ_BLOCK;
     const int ny = end_y - start_y;
+    // `mid` (or null) is recorded after the preparation kernels (row table,
+    // pair images), before the backprojection kernel itself
     if (kernel == 1) {
+        if (mid) {
+            const hipError_t e = hipEventRecord(mid, st);
+            if (e != hipSuccess) return e;
+        }
         if (g.positivity)
             k_bp_generic<true><<<dim3(bx, ny, g.vd), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, start_y);
         else
@@ -1953,7 +1967,7 @@ hipError_t bp(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp, 
     }
     k_bp_rows<<<dim3((g.vh + 255) / 256, nslot), 256, 0, st>>>(g, pp, slot_proj, nslot, rowflag, rows);
     if (kernel == 3) {
-#define TJ_BPL(RR, KK, GG) return launch_bp_lds<RR, KK, GG>(st, g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume, pairs)
+#define TJ_BPL(RR, KK, GG) return launch_bp_lds<RR, KK, GG>(st, g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume, pairs, mid)
         if (rows_per_thread == 2) {
             if (kchunk >= 16) { if (group >= 4) TJ_BPL(2, 16, 4); TJ_BPL(2, 16, 2); }
             if (group >= 8) TJ_BPL(2, 8, 8);
@@ -1966,6 +1980,10 @@ hipError_t bp(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp, 
         if (group >= 3) TJ_BPL(4, 8, 3);
         TJ_BPL(4, 8, 2);
 #undef TJ_BPL
+    }
+    if (mid) {
+        const hipError_t e = hipEventRecord(mid, st);
+        if (e != hipSuccess) return e;
     }
     if (kchunk <= 0) kchunk = 8;
     const int bz = (g.vd + kchunk - 1) / kchunk;
