@@ -168,3 +168,87 @@ def test_bgart_parameters_and_slab_refusal():
         p.reconstructionType = t
         with pytest.raises(ValueError):
             slabs.reconstruct_in_slabs(ts, p, slab_rows=8)
+
+
+class _FakeEngine:
+    """Stands in for JobEngine on the CPU: records runs, counts iterations."""
+
+    def __init__(self, budget):
+        self.budget = budget
+        self.used = 0
+        self.specs = {}
+        self.iters = {}
+        self.calls = []
+        self._next = 1
+
+    def fits(self, spec):
+        return self.used + spec.bytes() <= self.budget
+
+    def add(self, spec):
+        j = self._next
+        self._next += 1
+        self.specs[j] = spec
+        self.iters[j] = 0
+        self.used += spec.bytes()
+        return j
+
+    def synth(self, *a):
+        pass
+
+    def run(self, ids, k, graph=False):
+        self.calls.append((tuple(ids), k))
+        for j in ids:
+            self.iters[j] += k
+        return 0.0
+
+    def get_volume(self, j):
+        return np.full(1, self.iters[j])
+
+    def remove(self, j):
+        self.used -= self.specs.pop(j).bytes()
+
+
+def _spec(W, n=41):
+    from tomoj_amd.batch import JobSpec
+
+    class S(JobSpec):
+        def bytes(self):
+            return self.W * self.H * self.D * 4
+    return S(W, W, W // 2, list(np.linspace(-60, 60, n)))
+
+
+def test_serving_tick_gives_small_jobs_several_iterations():
+    """A tick gives each job about one quantum of voxel-ray updates (the
+    largest job's iteration), never more iterations than it has left; the
+    group with the least remaining work runs first."""
+    big, mid, small = _spec(256), _spec(64), _spec(32)
+    reqs = [S.Request(0, 0.0, big, 5, 4, 1), S.Request(1, 0.0, mid, 30, 4, 2), S.Request(2, 0.0, small, 3, 4, 3)]
+    loop = S.ServeLoop(_FakeEngine(1 << 40))
+    active = {1: reqs[0], 2: reqs[1], 3: reqs[2]}
+    left = {1: 5, 2: 30, 3: 3}
+    plan = loop.plan_tick(active, left)
+    ks = {j: k for k, js in plan for j in js}
+    assert ks[1] == 1                       # the largest job: one iteration
+    assert ks[2] == 30                      # 64 = 256/4: 64 iterations fit the quantum, 30 left
+    assert ks[3] == 3                       # capped by what it has left
+    assert plan[0][1] == [3]                # least remaining work first
+
+
+def test_serving_loop_runs_each_request_exactly_its_iterations():
+    reqs = [S.Request(i, 0.0, _spec(w), it, 4, i) for i, (w, it) in
+            enumerate([(256, 4), (32, 17), (64, 9), (128, 2), (32, 1)])]
+    eng = _FakeEngine(1 << 40)
+    out = S.ServeLoop(eng, keep_volumes=True).run(reqs, realtime=False)
+    assert out["requests"] == len(reqs) and eng.used == 0
+    for d in out["done"]:
+        assert int(d.volume[0]) == reqs[d.rid].iterations
+    assert set(out["by_width"]) == {"32", "64", "128", "256"}
+    # the small jobs finish before the large one
+    fin = {d.rid: d.finished for d in out["done"]}
+    assert fin[1] <= fin[0] and fin[4] <= fin[0]
+
+
+def test_serving_rejects_requests_without_iterations():
+    reqs = [S.Request(0, 0.0, _spec(32), 0, 4, 1)]
+    with pytest.raises(ValueError):
+        S.ServeLoop(_FakeEngine(1 << 40)).run(reqs, realtime=False)

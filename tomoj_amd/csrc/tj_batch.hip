@@ -28,6 +28,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <list>
 #include <map>
 #include <string>
 #include <vector>
@@ -1181,7 +1182,6 @@ struct RunPlan {
     std::vector<int> ids;
     int iterations = 0;
     tj_batch_tuning tune{};
-    long epoch = -1;
     TjJob* djobs = nullptr;
     std::vector<int4*> dtiles;  // owned tile tables
     hipGraphExec_t exec = nullptr;
@@ -1201,10 +1201,12 @@ struct tj_batch {
     hipStream_t stream = nullptr;
     size_t budget = 0, used = 0;
     int next_id = 1;
-    long epoch = 0;
     std::map<int, BJob*> jobs;
     tj_batch_tuning tune{4, 128, 4, 16, 2, 8, 1, 2};  // the autotuner's pick at 512^3 BF16
-    RunPlan plan;
+    // run plans of the recent job sets, most recently used first (a serving
+    // loop alternates between a few sets, each with its tile tables and graph)
+    std::list<RunPlan> plans;
+    RunPlan* cur = nullptr;     // the plan of the current run (front of plans)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int ncu = 256;
@@ -1222,7 +1224,22 @@ static void free_plan(RunPlan& p)
     p.dtiles.clear();
     p.launches.clear();
     p.ids.clear();
-    p.epoch = -1;
+}
+
+static constexpr size_t kMaxPlans = 8;
+
+// Frees every cached plan that runs job `id` (all plans when id < 0).
+static void drop_plans(tj_batch* b, int id)
+{
+    for (auto it = b->plans.begin(); it != b->plans.end();) {
+        if (id < 0 || std::find(it->ids.begin(), it->ids.end(), id) != it->ids.end()) {
+            if (b->cur == &*it) b->cur = nullptr;
+            free_plan(*it);
+            it = b->plans.erase(it);
+        } else {
+            ++it;
+        }
+    }
 }
 
 static void free_job(BJob* j)
@@ -1592,13 +1609,34 @@ static TjJob dev_desc(const BJob* j)
     return t;
 }
 
+// Frees the cached plan of exactly this job set (stale tuning).
+static void drop_plans_of_set(tj_batch* b, const std::vector<int>& ids)
+{
+    for (auto it = b->plans.begin(); it != b->plans.end();) {
+        if (it->ids == ids) {
+            if (b->cur == &*it) b->cur = nullptr;
+            free_plan(*it);
+            it = b->plans.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
 // Builds the launches of one iteration over `ids`: per storage type a forward
 // launch, one error launch over every job, per (storage, positivity) a back
 // launch.  Jobs are ordered by storage in the descriptor table.
 static int build_plan(tj_batch* b, const std::vector<int>& ids)
 {
-    RunPlan& p = b->plan;
-    free_plan(p);
+    drop_plans_of_set(b, ids);
+    while (b->plans.size() >= kMaxPlans) {
+        if (b->cur == &b->plans.back()) b->cur = nullptr;
+        free_plan(b->plans.back());
+        b->plans.pop_back();
+    }
+    b->plans.emplace_front();
+    b->cur = &b->plans.front();
+    RunPlan& p = *b->cur;
     std::vector<BJob*> js;
     for (int id : ids) {
         auto it = b->jobs.find(id);
@@ -1675,13 +1713,12 @@ static int build_plan(tj_batch* b, const std::vector<int>& ids)
     p.launches.insert(p.launches.end(), back.begin(), back.end());
     p.ids = ids;
     p.tune = b->tune;
-    p.epoch = b->epoch;
     return TJ_OK;
 }
 
 static hipError_t enqueue_launch(tj_batch* b, const RunPlan::L& l)
 {
-    const RunPlan& p = b->plan;
+    const RunPlan& p = *b->cur;
     if (l.kind == 0)
         return launch_fp(b->stream, l.storage, l.pstorage, p.tune, p.djobs, l.tiles, l.ntiles, l.grid, l.pos);
     if (l.kind == 2) return launch_bp(b->stream, l.storage, l.pos, p.tune, p.djobs, l.tiles, l.ntiles, l.grid);
@@ -1697,17 +1734,25 @@ static hipError_t enqueue_launch(tj_batch* b, const RunPlan::L& l)
 static hipError_t enqueue_iterations(tj_batch* b, int iterations)
 {
     for (int it = 0; it < iterations; ++it)
-        for (const RunPlan::L& l : b->plan.launches) {
+        for (const RunPlan::L& l : b->cur->launches) {
             hipError_t e = enqueue_launch(b, l);
             if (e != hipSuccess) return e;
         }
     return hipSuccess;
 }
 
-static bool plan_matches(const tj_batch* b, const std::vector<int>& ids)
+// Makes the cached plan of this job set (built with the current tuning) the
+// current one; false when there is none.
+static bool plan_matches(tj_batch* b, const std::vector<int>& ids)
 {
-    const RunPlan& p = b->plan;
-    return p.epoch == b->epoch && p.ids == ids && memcmp(&p.tune, &b->tune, sizeof(tj_batch_tuning)) == 0;
+    for (auto it = b->plans.begin(); it != b->plans.end(); ++it) {
+        if (it->ids == ids && memcmp(&it->tune, &b->tune, sizeof(tj_batch_tuning)) == 0) {
+            b->plans.splice(b->plans.begin(), b->plans, it);
+            b->cur = &b->plans.front();
+            return true;
+        }
+    }
+    return false;
 }
 
 template <typename TO>
@@ -1788,7 +1833,7 @@ int tj_batch_close(tj_batch* b)
     if (!b) return TJ_OK;
     (void)hipSetDevice(b->device);
     (void)hipStreamSynchronize(b->stream);
-    free_plan(b->plan);
+    drop_plans(b, -1);
     for (auto& kv : b->jobs) free_job(kv.second);
     (void)hipEventDestroy(b->ev0);
     (void)hipEventDestroy(b->ev1);
@@ -1854,7 +1899,6 @@ int tj_batch_add_job(tj_batch* b, const tj_job_desc* d, const double* euler, int
     j->bytes = need;
     b->used += need;
     b->jobs[j->id] = j;
-    b->epoch++;
     *job = j->id;
     return TJ_OK;
 }
@@ -1866,11 +1910,10 @@ int tj_batch_remove_job(tj_batch* b, int id)
     if (!j) return bfail(TJ_EINVAL, "tj_batch_remove_job: unknown job %d", id);
     TB_HIP(hipSetDevice(b->device));
     TB_HIP(hipStreamSynchronize(b->stream));
-    if (std::find(b->plan.ids.begin(), b->plan.ids.end(), id) != b->plan.ids.end()) free_plan(b->plan);
+    drop_plans(b, id);
     b->used -= j->bytes;
     b->jobs.erase(id);
     free_job(j);
-    b->epoch++;
     return TJ_OK;
 }
 
@@ -2093,8 +2136,7 @@ int tj_batch_convert(tj_batch* b, int id, int storage)
     b->used = b->used - j->bytes + need;
     j->bytes = need;
     j->d.storage = storage;
-    free_plan(b->plan);
-    b->epoch++;
+    drop_plans(b, id);
     return TJ_OK;
 }
 
@@ -2128,7 +2170,7 @@ int tj_batch_run(tj_batch* b, const int* ids, int nids, int iterations, int flag
     if (!reuse) {
         if (int rc = build_plan(b, v)) return rc;
     }
-    RunPlan& p = b->plan;
+    RunPlan& p = *b->cur;
     TB_HIP(hipEventRecord(b->ev0, b->stream));
     if (graph) {
         if (!(reuse && p.exec && p.iterations == iterations)) {
@@ -2205,7 +2247,7 @@ int tj_batch_get_tuning(const tj_batch* b, tj_batch_tuning* t)
 // backprojection is timed.
 static int time_kind(tj_batch* b, int kind, int reps, double* ms_out)
 {
-    RunPlan& p = b->plan;
+    RunPlan& p = *b->cur;
     // (device buffer, saved copy, bytes) of every volume the backprojection writes
     struct Saved {
         void* buf;
@@ -2332,7 +2374,7 @@ int tj_batch_autotune(tj_batch* b, const int* ids, int nids, tj_batch_tuning* be
         }
     }
     b->tune = pick;
-    free_plan(b->plan);
+    drop_plans_of_set(b, v);
     if (best) *best = pick;
     if (best_ms) *best_ms = tot_best;
     return TJ_OK;

@@ -126,13 +126,14 @@ def batched_run(n_jobs: int = 64, iterations: int = 15, storage: str = "f32", ma
                 "device_bytes": eng.used, "tuning": tun}
 
 
-def serving_run(n_requests: int = 24, rate: float = 20.0, budget: int = 96 * GIB, seed: int = 0x5E21_0020,
-                widths: Sequence[int] = (128, 256, 512, 1024), graph: bool = False) -> Dict:
-    """The serving form: a seeded Poisson stream served in real time."""
+def serving_run(n_requests: int = 200, rate: float = 20.0, budget: int = 96 * GIB, seed: int = 0x5E21_0020,
+                widths: Sequence[int] = (128, 256, 512, 1024), graph: bool = True) -> Dict:
+    """The serving form: a seeded Poisson stream served in real time (graph
+    ticks by default)."""
     reqs = S.poisson_requests(n_requests, rate=rate, seed=seed, widths=widths)
     with B.JobEngine(budget) as eng:
         res = S.ServeLoop(eng, graph=graph).run(reqs, realtime=True)
     res.pop("done")
     res.update({"rate_per_s": rate, "budget_bytes": budget, "widths": list(widths),
-                "storages": ["f32", "bf16"]})
+                "storages": ["f32", "bf16"], "graph": graph})
     return res

@@ -3,12 +3,18 @@ together on one GPU (continuous batching).
 
 Requests arrive over time (a seeded Poisson stream in the benchmark).  The
 loop keeps every admitted job resident in the job engine (tomoj_amd.batch)
-and advances all of them by one SIRT iteration per tick, in the same
-persistent launches; between ticks it retires finished jobs (freeing their
-memory) and admits waiting requests in arrival order while they fit the
-budget.  A request's latency is its completion time minus its arrival time,
-including queueing, setup (phantom projections generated on the device) and
-the read-back of its volume.
+and advances them in ticks.  A tick gives every active job about the same
+amount of work: a job whose one SIRT iteration is small gets several
+iterations (about one "quantum" of voxel-ray updates, by default the largest
+active job's one iteration, and never more than it has left), so a small job
+is not held behind the iterations of a large one.  Jobs with the same
+iteration count share persistent launches (optionally one HIP graph per
+group, cached by the engine); the groups run smallest remaining work first,
+and a job that finishes is retired (memory freed, volume read back) as soon
+as its group's launch ends.  Between ticks waiting requests are admitted in
+arrival order while they fit the budget.  A request's latency is its
+completion time minus its arrival time, including queueing, setup (phantom
+projections generated on the device) and the read-back of its volume.
 
 No TomoJ counterpart: TomoJ reconstructs one tilt series per run.  Each job's
 arithmetic is independent of the other jobs in its launches, so a result is
@@ -20,7 +26,7 @@ import math
 import time
 from collections import deque
 from dataclasses import dataclass, field
-from typing import Callable, Dict, List, Optional, Sequence
+from typing import Callable, Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -47,6 +53,8 @@ class Done:
     iterations: int
     updates: int            # voxel-ray updates of the job
     volume: Optional[np.ndarray] = None
+    width: int = 0
+    storage: str = ""
 
     @property
     def latency(self) -> float:
@@ -89,11 +97,12 @@ class ServeLoop:
     """Continuous batching over a JobEngine with a fixed HBM budget."""
 
     def __init__(self, engine: JobEngine, keep_volumes: bool = False, graph: bool = False,
-                 clock: Callable[[], float] = time.perf_counter):
+                 clock: Callable[[], float] = time.perf_counter, quantum: Optional[int] = None):
         self.engine = engine
         self.keep_volumes = keep_volumes
         self.graph = graph
         self.clock = clock
+        self.quantum = quantum      # voxel-ray updates per job per tick (None: largest active iteration)
 
     def _setup(self, req: Request) -> int:
         from .projector import phantom_ellipsoids
@@ -103,6 +112,20 @@ class ServeLoop:
         self.engine.synth(jid, ell, req.noise_rel, req.seed ^ 0xA5A5A5A5)
         return jid
 
+    def plan_tick(self, active: Dict[int, Request], left: Dict[int, int]) -> List[Tuple[int, List[int]]]:
+        """The launches of one tick: (iterations, jobs) groups.  Each job gets
+        max(1, quantum // its per-iteration updates) iterations, capped by what
+        it has left; jobs with equal counts share a group; groups are ordered
+        by the largest remaining work of their jobs, smallest first."""
+        upd = {j: active[j].spec.voxel_ray_updates_per_iteration() for j in active}
+        quantum = self.quantum if self.quantum else max(upd.values())
+        groups: Dict[int, List[int]] = {}
+        for j in active:
+            k = max(1, min(left[j], quantum // max(upd[j], 1)))
+            groups.setdefault(k, []).append(j)
+        return sorted(((k, sorted(js)) for k, js in groups.items()),
+                      key=lambda kv: (max(upd[j] * left[j] for j in kv[1]), kv[0]))
+
     def run(self, requests: Sequence[Request], realtime: bool = True) -> Dict:
         """Serves `requests` (sorted by arrival).  With realtime, a request is
         visible only once the wall clock has passed its arrival time; without,
@@ -110,6 +133,8 @@ class ServeLoop:
         Returns per-request records and aggregate statistics."""
         pending = deque(sorted(requests, key=lambda q: q.arrival))
         for q in pending:
+            if q.iterations < 1:
+                raise ValueError(f"request {q.rid} asks for {q.iterations} iterations; at least 1 is needed")
             if q.spec.bytes() > self.engine.budget:
                 raise ValueError(f"request {q.rid} needs {q.spec.bytes()} bytes, more than the whole budget")
         active: Dict[int, Request] = {}
@@ -118,6 +143,7 @@ class ServeLoop:
         done: List[Done] = []
         t0 = self.clock()
         ticks = 0
+        launches = 0
         max_active = 0
         busy = 0.0
         while pending or active:
@@ -134,35 +160,46 @@ class ServeLoop:
                     time.sleep(max(0.0, pending[0].arrival - (self.clock() - t0)))
                 continue
             max_active = max(max_active, len(active))
-            ids = sorted(active)
-            tb = self.clock()
-            self.engine.run(ids, 1, graph=self.graph)
-            busy += self.clock() - tb
+            for k, ids in self.plan_tick(active, left):
+                tb = self.clock()
+                self.engine.run(ids, k, graph=self.graph)
+                busy += self.clock() - tb
+                launches += 1
+                for jid in ids:
+                    left[jid] -= k
+                    if left[jid] <= 0:
+                        q = active.pop(jid)
+                        vol = self.engine.get_volume(jid) if self.keep_volumes else None
+                        self.engine.remove(jid)
+                        done.append(Done(q.rid, q.arrival if realtime else 0.0, admitted_at.pop(jid),
+                                         self.clock() - t0, q.iterations,
+                                         q.spec.voxel_ray_updates_per_iteration() * q.iterations, vol,
+                                         q.spec.W, q.spec.storage))
+                        del left[jid]
             ticks += 1
-            for jid in ids:
-                left[jid] -= 1
-                if left[jid] == 0:
-                    q = active.pop(jid)
-                    vol = self.engine.get_volume(jid) if self.keep_volumes else None
-                    self.engine.remove(jid)
-                    done.append(Done(q.rid, q.arrival if realtime else 0.0, admitted_at.pop(jid),
-                                     self.clock() - t0, q.iterations,
-                                     q.spec.voxel_ray_updates_per_iteration() * q.iterations, vol))
-                    del left[jid]
         wall = self.clock() - t0
         lat = [d.latency for d in done]
         updates = sum(d.updates for d in done)
+        by_width = {}
+        for w in sorted({d.width for d in done}):
+            lw = [d.latency for d in done if d.width == w]
+            by_width[str(w)] = {"n": len(lw), "p50_s": percentile(lw, 50), "p99_s": percentile(lw, 99)}
         return {
             "done": sorted(done, key=lambda d: d.rid),
             "requests": len(done),
             "wall_s": wall,
             "ticks": ticks,
+            "launches": launches,
             "max_active": max_active,
             "busy_s": busy,
             "p50_s": percentile(lat, 50),
             "p99_s": percentile(lat, 99),
             "mean_s": float(np.mean(lat)) if lat else float("nan"),
+            "by_width": by_width,
             "jobs_per_s": len(done) / wall if wall > 0 else float("nan"),
+            # completed jobs per second of device-busy time: the highest
+            # arrival rate of this request mix the loop can sustain
+            "capacity_jobs_per_s": len(done) / busy if busy > 0 else float("nan"),
             "gups": updates / wall / 1e9 if wall > 0 else float("nan"),
         }
 
