@@ -122,6 +122,22 @@ int tj_get_volume(tj_ctx *ctx, float *zyx, int start_y, int end_y);
 /* Sets every voxel to zero (TomoReconstruction2.resetPixels, :414-428). */
 int tj_reset_volume(tj_ctx *ctx);
 
+/* VoxelProjector3D.createErrorVolume (VoxelProjector3D.java:858-864) when `on`
+ * (a zero volume of the reconstruction's size on the device), or drops it.
+ * While it exists, every backprojection of differences (tj_backproject after
+ * tj_project_and_difference, and every subset of tj_run_ossart) also squares
+ * the differences in place (convertDiffToErrorImage, :315-324; NaN pixels stay
+ * NaN) and backprojects the squares into it (backproject, :181-195), after the
+ * reconstruction's own backprojection of the same differences. */
+int tj_set_error_volume(tj_ctx *ctx, int on);
+
+/* Copies the error volume to the host in ImageStack layout (the data
+ * VoxelProjector3D.endOfIteration saves as a TIFF stack, :828-836). */
+int tj_get_error_volume(tj_ctx *ctx, float *zyx);
+
+/* VoxelProjector3D.clearErrorVolume (:842-849); no-op without an error volume. */
+int tj_clear_error_volume(tj_ctx *ctx);
+
 /* Projector.addProjection(int index) (Projector.java:130; VoxelProjector3D.java:111-118):
  * appends resident projection `index` to the ordered list of added projections. */
 int tj_add_projection(tj_ctx *ctx, int index);

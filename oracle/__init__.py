@@ -89,6 +89,8 @@ def _L():
             "or_backproject_one_f64": (None, [G, P, P, P, i, i]),
             "or_order_shuffle": (None, [P, i, i, P]),
             "or_ossart": (i, [G, P, P, P, i, i, d, i, i, i, i, P]),
+            "or_ossart_errvol": (i, [G, P, P, P, i, i, d, i, i, i, i, P, P]),
+            "or_convert_diff_to_error": (None, [P, l]),
             "or_ossart_f64": (i, [G, P, P, P, i, i, d, i, i, i, i, P]),
             "or_ossart_avail": (i, [G, P, P, P, i, P, i, i, d, i, i, i, i, P]),
             "or_order_shuffle_avail": (None, [P, i, P, i, P]),
@@ -260,17 +262,35 @@ def order_shuffle(eulers, first_index=0) -> np.ndarray:
     return order
 
 
+def convert_diff_to_error(diff):
+    """VoxelProjector3D.convertDiffToErrorImage (:315-324) on a copy: squared in
+    float, NaN kept."""
+    out = _c(diff, np.float32).copy()
+    _L().or_convert_diff_to_error(_p(out), out.size)
+    return out
+
+
 def ossart(g: Geom, vol, projs, eulers, iterations, lam, update, first_index=0,
-           start_y=0, end_y=None, f64=False, available=None):
+           start_y=0, end_y=None, f64=False, available=None, errvol=None):
     """TomoReconstruction2.OSSART (:548-634).  Updates ``vol`` in place; returns the
     per-iteration ``err1`` rows [avg, min, imin, max, imax] (iterations x 5).
-    ``available``: the list getAvailableIndexes gives (FP32 only)."""
+    ``available``: the list getAvailableIndexes gives (FP32 only).  ``errvol``
+    (FP32, ImageStack layout): VoxelProjector3D's error volume, accumulating the
+    backprojected squared differences of every subset (:181-195)."""
     end_y = g.vh if end_y is None else end_y
     dt = np.float64 if f64 else np.float32
     assert vol.dtype == dt and vol.flags["C_CONTIGUOUS"]
     projs = _c(projs, dt)
     eulers = _c(eulers, np.float64).reshape(-1, 9)
     err = np.zeros((iterations, 5))
+    if errvol is not None:
+        assert not f64 and available is None
+        assert errvol.dtype == np.float32 and errvol.flags["C_CONTIGUOUS"] and errvol.shape == vol.shape
+        rc = _L().or_ossart_errvol(C.byref(g), _p(vol), _p(projs), _p(eulers), eulers.shape[0], iterations,
+                                   float(lam), int(update), int(first_index), start_y, end_y, _p(err), _p(errvol))
+        if rc != 0:
+            raise MemoryError("oracle OS-SART allocation failed")
+        return err
     if available is not None:
         assert not f64
         av = _c(available, np.int32)

@@ -598,6 +598,16 @@ int or_available_indexes(int n, int type, int *out)
     return cnt;
 }
 
+/* VoxelProjector3D.convertDiffToErrorImage (VoxelProjector3D.java:315-324):
+ * each pixel squared in float, NaN left as it is. */
+void or_convert_diff_to_error(float *diff, long n)
+{
+    for (long i = 0; i < n; i++) {
+        const float f = diff[i];
+        if (!isnan(f)) diff[i] = f * f;
+    }
+}
+
 /* One osartIteration (TomoReconstruction2.java:636-733).  projs: n x ph x pw
  * (the already-supplied projections, ts.getPixels).  err5 receives err1[0..4].
  * With a list of available images (nav of them): the order runs over the
@@ -607,7 +617,7 @@ int or_available_indexes(int n, int type, int *out)
  * nothing is.  Returns 0, or -1 on allocation failure. */
 static int osart_iteration_f32(const or_geom *g, float *vol, const float *projs, const double *eul,
                                int n, const int *avail, int nav, double lambda, int update_nb, int first_index,
-                               int startY, int endY, double *err5)
+                               int startY, int endY, double *err5, float *errvol)
 {
     const long npx = (long)g->pw * g->ph;
     int *order = (int *)malloc(sizeof(int) * (nav > 0 ? nav : 1));
@@ -642,9 +652,17 @@ static int osart_iteration_f32(const or_geom *g, float *vol, const float *projs,
                 if (errors[e] > errmax) { errmax = errors[e]; imax = order[t - nadd + e + 1]; }
                 else if (errors[e] < errmin) { errmin = errors[e]; imin = order[t - nadd + e + 1]; }
             }
+            /* VoxelProjector3D.backproject (:155-208): per image the
+             * reconstruction's backprojection of the difference, then, with an
+             * error volume, the difference squared in place and backprojected
+             * into it (the reconstruction reads the unsquared difference) */
             for (int p = 0; p < nadd; p++) {
                 const int idx = order[t - nadd + 1 + p];
                 or_backproject_one(g, vol, df + p * npx, eul + (size_t)idx * 9, startY, endY);
+                if (errvol) {
+                    or_convert_diff_to_error(df + p * npx, npx);
+                    or_backproject_one(g, errvol, df + p * npx, eul + (size_t)idx * 9, startY, endY);
+                }
             }
             diffIndex = 0;
         }
@@ -721,7 +739,23 @@ int or_ossart(const or_geom *g, float *vol, const float *projs, const double *eu
 {
     for (int it = 0; it < iterations; it++)
         if (osart_iteration_f32(g, vol, projs, eul, n, NULL, n, lambda, update_nb, first_index, startY, endY,
-                                err + 5 * it))
+                                err + 5 * it, NULL))
+            return -1;
+    return 0;
+}
+
+/* OSSART with VoxelProjector3D's error volume (createErrorVolume, :858-864):
+ * errvol (ImageStack layout like vol) accumulates the backprojected squared
+ * differences of every subset of the `iterations` iterations (TomoJ saves and
+ * clears it at each endOfIteration, :828-852; a caller runs one iteration per
+ * call to get each iteration's volume). */
+int or_ossart_errvol(const or_geom *g, float *vol, const float *projs, const double *eul, int n,
+                     int iterations, double lambda, int update_nb, int first_index,
+                     int startY, int endY, double *err, float *errvol)
+{
+    for (int it = 0; it < iterations; it++)
+        if (osart_iteration_f32(g, vol, projs, eul, n, NULL, n, lambda, update_nb, first_index, startY, endY,
+                                err + 5 * it, errvol))
             return -1;
     return 0;
 }
@@ -733,7 +767,7 @@ int or_ossart_avail(const or_geom *g, float *vol, const float *projs, const doub
 {
     for (int it = 0; it < iterations; it++)
         if (osart_iteration_f32(g, vol, projs, eul, n, avail, nav, lambda, update_nb, first_index, startY, endY,
-                                err + 5 * it))
+                                err + 5 * it, NULL))
             return -1;
     return 0;
 }

@@ -506,6 +506,141 @@ def test_parameters_driver_matches_oracle(tj, oracle):
     m.close()
 
 
+def _errvol_case(oracle, W=40, H=9, D=14, seed=31):
+    tilts = list(np.arange(-60, 61, 10.0))
+    E = oracle.tilt_eulers(tilts)
+    exps = oracle.phantom_project(oracle.ellipsoids(seed, 5, W, H, D), E, W, H).astype(np.float32)
+    exps[3, 4, 7:12] = np.nan
+    return tilts, E, exps
+
+
+@pytest.mark.parametrize("update", [0, 4])
+def test_error_volume_fused_loop(tj, oracle, update):
+    """VoxelProjector3D's error volume in the OS-SART loop: after each subset's
+    backprojection the squared differences are backprojected into it; one
+    iteration per call with a clear in between (endOfIteration's save and
+    clear).  Volume and error volume bit-exact with the oracle."""
+    W, H, D = 40, 9, 14
+    tilts, E, exps = _errvol_case(oracle, W, H, D)
+    g = oracle.make_geom(W, H, D, nthreads=8)
+    ref = np.zeros((D, H, W), np.float32)
+    with make(tj, (W, H, D)) as P:
+        P.load_tilt_series(exps, E)
+        P.createErrorVolume()
+        for it in range(2):
+            print(f"error volume, fused loop, update {update}, iteration {it}", flush=True)
+            err = P.run_ossart(1, 0.9, update, first_index=1)
+            ev_ref = np.zeros((D, H, W), np.float32)
+            ref_err = oracle.ossart(g, ref, exps, E, 1, 0.9, update, first_index=1, errvol=ev_ref)
+            assert same(P.get_volume(), ref), it
+            assert same(P.getErrorVolume(), ev_ref), it
+            assert np.allclose(err[:, 0], ref_err[:, 0], rtol=1e-12)
+            P.clearErrorVolume()
+            assert not P.getErrorVolume().any()
+
+
+def test_error_volume_projector_interface(tj, oracle):
+    """projectAndDifference then backproject with an error volume: the
+    reconstruction adds the differences, the error volume their squares, and
+    the difference images are left squared (convertDiffToErrorImage works in
+    place)."""
+    W, H, D = 40, 9, 14
+    tilts, E, exps = _errvol_case(oracle, W, H, D, seed=7)
+    g = oracle.make_geom(W, H, D, nthreads=8)
+    rng = np.random.default_rng(4)
+    vol = (rng.random((D, H, W), dtype=np.float32) * np.float32(0.02)).astype(np.float32)
+    idx = [5, 0, 9]
+    factor = float(np.float32(1.0 / 3))
+    with make(tj, (W, H, D)) as P:
+        P.load_tilt_series(exps, E)
+        P.set_volume(vol)
+        P.createErrorVolume()
+        for k in idx:
+            P.addProjection(k)
+        P.projectAndDifference(factor)
+        P.backproject()
+        got, got_ev = P.get_volume(), P.getErrorVolume()
+        got_d = np.stack([P.getDifferenceImage(s) for s in range(len(idx))])
+    diffs, _, _, _ = oracle.project_and_difference(g, vol, exps[idx], E[idx], factor)
+    ref = oracle.backproject(g, vol.copy(), diffs, E[idx])
+    sq = np.stack([oracle.convert_diff_to_error(d) for d in diffs])
+    ref_ev = oracle.backproject(g, np.zeros((D, H, W), np.float32), sq, E[idx])
+    assert same(got, ref)
+    assert same(got_ev, ref_ev)
+    assert same(got_d, sq)
+
+
+@pytest.mark.parametrize("for_all", [False, True])
+def test_driver_save_error_volume(tj, oracle, for_all):
+    """doCPUrec's error-volume branches (ReconstructionThreadManager.java:
+    229-246): saveErrorVol alone runs nbIterations - 1 iterations, creates the
+    error volume and runs the last one (its volume saved as iteration n);
+    with saveForAll every iteration's error volume is saved and cleared."""
+    from tomoj_amd import parameters as RP
+    W, H, D = 40, 9, 14
+    tilts, E, exps = _errvol_case(oracle, W, H, D, seed=13)
+    ts = RP.TiltSeriesData(exps, tilts)
+    g = oracle.make_geom(W, H, D, nthreads=8)
+    n = 3
+    p = RP.ReconstructionParameters.createOSSARTParameters(W, H, D, n, 1.0, 5)
+    p.setSaveErrorVolume(True, for_all)
+    m = RP.ReconstructionThreadManager(ts)
+    try:
+        m.createRec2(p)
+        err = m.doRec(p, first_index=2)
+        got = m.getRec2()
+        saved = dict(m.errorVolumes)
+    finally:
+        m.close()
+    ref = np.zeros((D, H, W), np.float32)
+    ref_err = []
+    ref_saved = {}
+    for it in range(n):
+        if for_all or it == n - 1:
+            ev = np.zeros((D, H, W), np.float32)
+            ref_err.append(oracle.ossart(g, ref, exps, E, 1, 1.0, 5, first_index=2, errvol=ev))
+            ref_saved[it + 1] = ev
+        else:
+            ref_err.append(oracle.ossart(g, ref, exps, E, 1, 1.0, 5, first_index=2))
+    assert same(got, ref)
+    assert np.allclose(err[:, 0], np.concatenate(ref_err)[:, 0], rtol=1e-12)
+    assert sorted(saved) == sorted(ref_saved)
+    for k in saved:
+        assert same(saved[k], ref_saved[k]), k
+
+
+def test_driver_warm_start(tj, oracle):
+    """createRec2 keeps an existing reconstruction of the same size ("add some
+    iterations", ReconstructionThreadManager.java:176-185): two performs of 2
+    and 3 iterations equal 5 iterations from zero; a new size starts from
+    zero; resetRec2 forgets the old one."""
+    from tomoj_amd import parameters as RP
+    W, H, D = 36, 8, 12
+    tilts = list(np.arange(-60, 61, 15.0))
+    E = oracle.tilt_eulers(tilts)
+    exps = oracle.phantom_project(oracle.ellipsoids(3, 4, W, H, D), E, W, H).astype(np.float32)
+    ts = RP.TiltSeriesData(exps, tilts)
+    g = oracle.make_geom(W, H, D, nthreads=8)
+    m = RP.ReconstructionThreadManager(ts)
+    try:
+        m.performReconstruction(RP.ReconstructionParameters.createOSSARTParameters(W, H, D, 2, 1.0, 3))
+        m.performReconstruction(RP.ReconstructionParameters.createOSSARTParameters(W, H, D, 3, 1.0, 3))
+        ref = np.zeros((D, H, W), np.float32)
+        oracle.ossart(g, ref, exps, E, 5, 1.0, 3)
+        assert same(m.getRec2(), ref)
+        # another depth: a new reconstruction from zero
+        D2 = 10
+        m.performReconstruction(RP.ReconstructionParameters.createOSSARTParameters(W, H, D2, 2, 1.0, 3))
+        ref2 = np.zeros((D2, H, W), np.float32)
+        oracle.ossart(oracle.make_geom(W, H, D2, nthreads=8), ref2, exps, E, 2, 1.0, 3)
+        assert same(m.getRec2(), ref2)
+        m.resetRec2()
+        m.performReconstruction(RP.ReconstructionParameters.createOSSARTParameters(W, H, D2, 2, 1.0, 3))
+        assert same(m.getRec2(), ref2)
+    finally:
+        m.close()
+
+
 @pytest.mark.parametrize("min_size", [0, 30])
 def test_normalised_projection_supply(tj, oracle, min_size):
     """TiltSeries.getPixels with normalisation (identity alignment): ROI

@@ -303,6 +303,67 @@ def test_sirt_fixed_point(oracle):
     assert err[0, 0] == 0.0
 
 
+def test_convert_diff_to_error_known_answers(oracle):
+    """convertDiffToErrorImage (VoxelProjector3D.java:315-324): float squares,
+    NaN kept, overflow to +inf as Java's float multiply gives."""
+    d = np.array([1.5, -2.0, np.nan, -0.0, 3e19, -1e-30], np.float32)
+    got = oracle.convert_diff_to_error(d)
+    assert got[0] == 2.25 and got[1] == 4.0 and np.isnan(got[2])
+    assert got[3] == 0.0 and not np.signbit(got[3])
+    assert got[4] == np.inf and got[5] == 0.0
+
+
+@pytest.mark.parametrize("n", [1, 4])
+def test_error_volume_constant_difference(oracle, n):
+    """The error volume of one OS-SART iteration in closed form.  A zero
+    volume, n copies of the theta = 0 projection, measured images all equal to
+    c: the forward projection is 0 with ray norm D, every difference is
+    (c / D) * (1/n) (the factor lambda / update is inside it), so the volume
+    becomes n * (c / D) / n = c / D and the error volume, the backprojection of
+    the squared differences image by image, n * (c / (D n))^2 = c^2 / (D^2 n)
+    at every voxel."""
+    W, H, D = 12, 3, 8
+    c = 0.75
+    g = geom(oracle, W, H, D)
+    E = oracle.tilt_eulers([0.0] * n)
+    exps = np.full((n, H, W), c, np.float32)
+    vol = np.zeros((D, H, W), np.float32)
+    ev = np.zeros((D, H, W), np.float32)
+    oracle.ossart(g, vol, exps, E, 1, 1.0, n, errvol=ev)
+    d = np.float32(np.float32(c / D) * np.float32(1.0 / n))
+    assert np.allclose(vol, c / D, rtol=1e-6)
+    assert np.allclose(ev, np.float32(d * d) * n, rtol=1e-6)
+    # without an error volume the reconstruction is the same
+    vol2 = np.zeros((D, H, W), np.float32)
+    oracle.ossart(g, vol2, exps, E, 1, 1.0, n)
+    assert np.array_equal(vol, vol2)
+
+
+def test_error_volume_is_backprojection_of_squares(oracle):
+    """One subset: the error volume equals the backprojection of the subset's
+    squared differences, and the reconstruction does not see the squares."""
+    W, H, D = 20, 4, 10
+    g = geom(oracle, W, H, D)
+    tilts = [-30.0, 10.0, 45.0]
+    E = oracle.tilt_eulers(tilts)
+    ell = oracle.ellipsoids(5, 3, W, H, D)
+    exps = oracle.phantom_project(ell, E, W, H).astype(np.float32)
+    exps[1, 2, 3] = np.nan
+    vol = np.zeros((D, H, W), np.float32)
+    ev = np.zeros((D, H, W), np.float32)
+    oracle.ossart(g, vol, exps, E, 1, 1.0, 3, errvol=ev)
+    order = oracle.order_shuffle(E, 0)
+    factor = float(np.float32(1.0 / 3))
+    diffs, _, _, _ = oracle.project_and_difference(g, np.zeros((D, H, W), np.float32), exps[order], E[order],
+                                                   factor)
+    ref_vol = oracle.backproject(g, np.zeros((D, H, W), np.float32), diffs, E[order])
+    ref_ev = oracle.backproject(g, np.zeros((D, H, W), np.float32),
+                                np.stack([oracle.convert_diff_to_error(d) for d in diffs]), E[order])
+    assert np.array_equal(vol, ref_vol)
+    assert np.array_equal(ev, ref_ev)
+    assert (ev >= 0).all()
+
+
 def test_sirt_residual_decreases(oracle):
     W, H, D = 32, 4, 16
     g = geom(oracle, W, H, D)

@@ -37,6 +37,7 @@ hipError_t plane_flags(hipStream_t, const float*, size_t, int, int, uint8_t*);
 hipError_t phantom_project(hipStream_t, const double*, int, const TjProj*, int, int, int, double, double, double*);
 hipError_t add_noise(hipStream_t, const double*, size_t, uint64_t, double, double*, int, float*);
 hipError_t copy(hipStream_t, const void*, void*, size_t);
+hipError_t square_diffs(hipStream_t, float*, size_t);
 hipError_t fista(hipStream_t, float*, const float*, size_t, double);
 hipError_t tv_denoise(hipStream_t, float*, float*, float*, int, int, int, float, float, float, int, uint8_t*);
 hipError_t bayes_conv(hipStream_t, const float*, float*, int, int, int, double);
@@ -176,6 +177,11 @@ struct tj_ctx {
     double* bg_mm = nullptr;    // [2 x 4096] min/max partials
     int* bg_hist = nullptr;     // [256]
 
+    // VoxelProjector3D.errorVolume (createErrorVolume, :858-864): while it
+    // exists, every backprojection of differences also backprojects their
+    // squares into it
+    float* errvol = nullptr;    // [vh][vd][vw]
+
     // FP64 pair images of the backprojected images (LDS backprojection)
     double2* bpairs = nullptr;  // [bpairs_cap][ph][pw]
     int bpairs_cap = 0;
@@ -195,6 +201,7 @@ struct tj_ctx {
 
 static void ctx_free(tj_ctx* c)
 {
+    dfree(c->errvol);
     dfree(c->bpairs);
     dfree(c->prev);
     dfree(c->tv_f);
@@ -483,6 +490,33 @@ static int do_backproject(tj_ctx* c, const int* slots, const std::vector<int>& h
     TJ_HIP(tjk::bp(c->stream, c->g, bp_target(c), c->pp, slots, (int)hslots.size(), images, image_index,
                    c->rowflag, c->bprows, start_y, end_y, kernel, c->tune.bp_rows, c->tune.bp_zchunk,
                    c->tune.bp_group, bp_finite(c), c->bpairs));
+    return TJ_OK;
+}
+
+// The error-volume half of VoxelProjector3D.backproject (:181-195): the
+// differences of the `nslot` slots squared in place (convertDiffToErrorImage,
+// :315-324) and backprojected into the error volume, image by image in slot
+// order, over rows [start_y, end_y).  TomoJ squares each difference image on
+// the calling thread while the reconstruction's backprojection tasks of the
+// same image may still be reading it; here the reconstruction's
+// backprojection of the subset has been issued first on the same stream, so
+// it always reads the differences themselves.  TomoJ's error tasks step
+// `yy + increment` without the endY clamp the reconstruction tasks use; rows
+// past endY are not touched here.
+static int do_error_backproject(tj_ctx* c, const int* slots, const std::vector<int>& hslots, int start_y,
+                                int end_y)
+{
+    const int n = (int)hslots.size();
+    if (!c->errvol || n == 0) return TJ_OK;
+    TJ_HIP(tjk::square_diffs(c->stream, c->sdiff, c->npx * (size_t)n));
+    TJ_HIP(tjk::row_flags(c->stream, c->g.pw, c->g.ph, n, c->sdiff, c->iota, c->rowflag));
+    const int kernel = pick_bp_kernel(c, hslots);
+    if (kernel == 3)
+        if (int rc = ensure_bpairs(c, n)) return rc;
+    // the error volume holds sums of squares with the NaN/Inf scrub after
+    // every image, so it is always finite
+    TJ_HIP(tjk::bp(c->stream, c->g, c->errvol, c->pp, slots, n, c->sdiff, c->iota, c->rowflag, c->bprows, start_y,
+                   end_y, kernel, c->tune.bp_rows, c->tune.bp_zchunk, c->tune.bp_group, 1, c->bpairs));
     return TJ_OK;
 }
 
@@ -1011,6 +1045,52 @@ int tj_reset_volume(tj_ctx* c)
     return TJ_OK;
 }
 
+int tj_set_error_volume(tj_ctx* c, int on)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_set_error_volume: null context");
+    TJ_HIP(hipSetDevice(c->device));
+    if (!on) {
+        TJ_HIP(hipStreamSynchronize(c->stream));
+        dfree(c->errvol);
+        return TJ_OK;
+    }
+    if (!c->errvol) {
+        if (dalloc(&c->errvol, c->nvox) != hipSuccess) {
+            (void)hipGetLastError();
+            c->errvol = nullptr;
+            return fail(TJ_ENOMEM, "tj_set_error_volume: cannot allocate a %dx%dx%d error volume", c->g.vw, c->g.vh,
+                        c->g.vd);
+        }
+    }
+    return tj_clear_error_volume(c);
+}
+
+int tj_clear_error_volume(tj_ctx* c)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_clear_error_volume: null context");
+    if (!c->errvol) return TJ_OK;
+    TJ_HIP(hipSetDevice(c->device));
+    TJ_HIP(hipMemsetAsync(c->errvol, 0, c->nvox * sizeof(float), c->stream));
+    TJ_HIP(hipStreamSynchronize(c->stream));
+    return TJ_OK;
+}
+
+int tj_get_error_volume(tj_ctx* c, float* zyx)
+{
+    if (!c || !zyx) return fail(TJ_EINVAL, "tj_get_error_volume: null argument");
+    if (!c->errvol) return fail(TJ_ESTATE, "tj_get_error_volume: no error volume (tj_set_error_volume)");
+    TJ_HIP(hipSetDevice(c->device));
+    const TjGeomDev& g = c->g;
+    float* stage = nullptr;
+    TJ_HIP(dalloc(&stage, c->nvox));
+    hipError_t e = tjk::yzx_to_zyx(c->stream, c->errvol, stage, g.vw, g.vd, g.vh, 0);
+    if (e == hipSuccess) e = hipMemcpyAsync(zyx, stage, c->nvox * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(stage);
+    if (e != hipSuccess) return fail(TJ_EDEVICE, "tj_get_error_volume: %s", hipGetErrorString(e));
+    return TJ_OK;
+}
+
 int tj_add_projection(tj_ctx* c, int index)
 {
     if (!c) return fail(TJ_EINVAL, "tj_add_projection: null context");
@@ -1162,6 +1242,7 @@ int tj_backproject(tj_ctx* c, int start_y, int end_y)
         if ((rc = prepare_originals(c, c->slot_proj, c->added, &images, &index))) return rc;
     }
     if ((rc = do_backproject(c, c->slot_proj, c->added, images, index, start_y, end_y))) return rc;
+    if (c->have_diff && (rc = do_error_backproject(c, c->slot_proj, c->added, start_y, end_y))) return rc;
     TJ_HIP(hipStreamSynchronize(c->stream));
     return TJ_OK;
 }
@@ -1824,6 +1905,7 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
                 TJ_HIP(tjk::bp(c->stream, c->g, bp_target(c), c->pp, slots, diffIndex, c->sdiff, c->iota,
                                c->rowflag, c->bprows, start_y, end_y, bk, c->tune.bp_rows, c->tune.bp_zchunk,
                                c->tune.bp_group, bp_finite(c), c->bpairs, q));
+                if ((rc = do_error_backproject(c, slots, hs, start_y, end_y))) return rc;
                 if (c->timing_on) {
                     TJ_HIP(hipEventRecord(d, c->stream));
                     fpev.emplace_back(a, b);

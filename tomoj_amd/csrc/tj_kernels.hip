@@ -1756,6 +1756,16 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_add_noise(const double* __restrict
 }
 
 // Plain streaming copy (16-byte vectors), used to calibrate HBM byte counters.
+// VoxelProjector3D.convertDiffToErrorImage (VoxelProjector3D.java:315-324):
+// every difference pixel squared in FP32, NaN left as it is.
+__global__ __launch_bounds__(TJ_BLOCK) void k_square_diffs(float* __restrict__ d, size_t n)
+{
+    for (size_t k = (size_t)blockIdx.x * TJ_BLOCK + threadIdx.x; k < n; k += (size_t)gridDim.x * TJ_BLOCK) {
+        const float f = d[k];
+        if (!isnan(f)) d[k] = f * f;
+    }
+}
+
 __global__ __launch_bounds__(TJ_BLOCK) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst,
                                                    size_t n4)
 {
@@ -2161,6 +2171,15 @@ hipError_t bayes_end(hipStream_t st, float* vol, float* dist, size_t plane, int 
     if (e != hipSuccess) return e;
     const unsigned bx = (unsigned)std::min<size_t>(64, (plane + 4 * TJ_BLOCK - 1) / (4 * TJ_BLOCK));
     k_bayes_end<<<dim3(bx, H), TJ_BLOCK, 0, st>>>(vol, dist, plane, H, alpha, positivity, pflag);
+    return hipGetLastError();
+}
+
+hipError_t square_diffs(hipStream_t st, float* d, size_t n)
+{
+    (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + TJ_BLOCK - 1) / TJ_BLOCK, 4096);
+    k_square_diffs<<<blocks, TJ_BLOCK, 0, st>>>(d, n);
     return hipGetLastError();
 }
 

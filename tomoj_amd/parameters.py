@@ -16,7 +16,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 from dataclasses import dataclass
-from typing import List, Optional, Sequence
+from typing import Callable, Dict, List, Optional, Sequence
 
 import numpy as np
 
@@ -111,6 +111,8 @@ class ReconstructionParameters:
         self._tempNbCall = 0
         self.projectionType = 0
         self.availableIndexes: Optional[List[int]] = None
+        self.saveErrorVolume = False       # (:39-40)
+        self.saveErrorVolumeAll = False
 
     # factories (:279-311)
     @staticmethod
@@ -145,6 +147,18 @@ class ReconstructionParameters:
 
     def resetTmpNbCall(self) -> None:
         self._tempNbCall = 0
+
+    def setSaveErrorVolume(self, saveErrorVolume: bool, saveErrorVolumeAll: Optional[bool] = None) -> None:
+        """:196-202 (both overloads)."""
+        self.saveErrorVolume = bool(saveErrorVolume)
+        if saveErrorVolumeAll is not None:
+            self.saveErrorVolumeAll = bool(saveErrorVolumeAll)
+
+    def isSaveErrorVolume(self) -> bool:
+        return self.saveErrorVolume
+
+    def isSaveErrorVolumeAll(self) -> bool:
+        return self.saveErrorVolumeAll
 
     def setFista(self, fista: bool) -> None:
         self.fista = bool(fista)
@@ -188,18 +202,22 @@ class ReconstructionParameters:
             c = [c[k] + float(self.reconstructionCenterModifiers[k]) for k in range(3)]
         return c
 
-    def getProjector(self, ts: TiltSeriesData, device: int = 0) -> VoxelProjector3DHIP:
+    def getProjector(self, ts: TiltSeriesData, device: int = 0,
+                     vol_center: Optional[Sequence[float]] = None) -> VoxelProjector3DHIP:
         """:313-333: a voxel projector with the weighting filter for WBP, the
         positivity constraint for OS-SART, the scale ts.W / rec.W when
-        rescaleData, long-object compensation, and the FISTA wrapper."""
+        rescaleData, long-object compensation, and the FISTA wrapper.
+        `vol_center`: the centres of the reconstruction the projector works on
+        (TomoJ's projector takes them from rec2; default: this object's)."""
         return self._projector(VoxelProjector3DHIP, ts, device,
-                               self.reconstructionType == OSSART and self.positivityConstraint)
+                               self.reconstructionType == OSSART and self.positivityConstraint, vol_center)
 
-    def _projector(self, cls, ts: TiltSeriesData, device: int, positivity: bool):
+    def _projector(self, cls, ts: TiltSeriesData, device: int, positivity: bool,
+                   vol_center: Optional[Sequence[float]] = None):
         scale = ts.width / float(self.width) if self.rescaleData else 1.0
         P = cls((self.width, self.height, self.depth), (ts.width, ts.height), device=device,
                 scale=scale, positivity=positivity, long_object_compensation=self.longObjectCompensation,
-                vol_center=self.volume_centers())
+                vol_center=self.volume_centers() if vol_center is None else list(vol_center))
         supply(P, ts)
         if self.reconstructionType == WBP:
             P.setWeightingFilter(self.weightingRadius * ts.width, ts.tilts, ts.tilt_axes())
@@ -286,7 +304,7 @@ class AdvancedReconstructionParameters(ReconstructionParameters):
         p.bgartDarkBG = bool(bgartDarkBG)
         return p
 
-    def getProjector(self, ts: TiltSeriesData, device: int = 0):
+    def getProjector(self, ts: TiltSeriesData, device: int = 0, vol_center: Optional[Sequence[float]] = None):
         """The TVM, BAYESIAN and BGART branches of getProjector (:179-203).  TVM:
         TVMProjector3D with the relaxation as regularisation alpha, the
         positivity and long-object settings, the TV settings, the scale when
@@ -302,21 +320,21 @@ class AdvancedReconstructionParameters(ReconstructionParameters):
             loc, self.longObjectCompensation = self.longObjectCompensation, False
             try:
                 return self._projector(_BgartFactory(self.bgartK, self.bgartDarkBG, self.bgartMedian), ts,
-                                       device, False)
+                                       device, False, vol_center)
             finally:
                 self.longObjectCompensation = loc
         if self.reconstructionType == BAYESIAN:
             fista, self.fista = self.fista, False
             try:
-                P = self._projector(BayesianVoxelProjector3DHIP, ts, device, self.positivityConstraint)
+                P = self._projector(BayesianVoxelProjector3DHIP, ts, device, self.positivityConstraint, vol_center)
             finally:
                 self.fista = fista
             P.setRegularizationAlpha(self.relaxationCoefficient)
             P.setRegularizationLambda(self.regularizationWeight)
             return P
         if self.reconstructionType != TVM:
-            return super().getProjector(ts, device)
-        P = self._projector(TVMProjector3DHIP, ts, device, self.positivityConstraint)
+            return super().getProjector(ts, device, vol_center)
+        P = self._projector(TVMProjector3DHIP, ts, device, self.positivityConstraint, vol_center)
         P.setRegularizationAlpha(self.relaxationCoefficient)
         P.setTvmTheta(self.tvmTheta)
         P.setTvmG(self.tvmG)
@@ -327,36 +345,127 @@ class AdvancedReconstructionParameters(ReconstructionParameters):
 
 class ReconstructionThreadManager:
     """The CPU-manager driver (ReconstructionThreadManager.java:79-255) with the
-    projector on the GPU: one zero-initialised reconstruction, WBP/BP or
-    OS-SART over all of its rows."""
+    projector on the GPU.  `rec2` is the reconstruction TomoJ keeps across
+    calls; its device copy lives in the projector and is read back when the
+    host asks for it."""
 
     def __init__(self, ts: TiltSeriesData, device: int = 0):
         self.ts = ts
         self.device = device
         self.projector: Optional[VoxelProjector3DHIP] = None
         self.params: Optional[ReconstructionParameters] = None
+        self.rec2: Optional[np.ndarray] = None          # (D, H, W), ImageStack layout
+        self.rec2_centers: Optional[List[float]] = None
+        self._device_newer = False                      # the projector's volume is newer than rec2
+        self._rec2_zero = True                          # rec2 holds only zeros (a fresh reconstruction)
+        self.errorVolumes: Dict[int, np.ndarray] = {}   # iteration -> error volume (endOfIteration's saves)
+        self.error_volume_sink: Optional[Callable[[int, np.ndarray], None]] = None
+
+    def _sync_host(self) -> None:
+        if self._device_newer and self.projector is not None:
+            self.rec2 = self.projector.get_volume()
+            self._device_newer = False
+            self._rec2_zero = False
 
     def createRec2(self, params: ReconstructionParameters) -> None:
+        """createRec2 (:169-196): a new zero reconstruction of the parameters'
+        size, with centres (n-1)/2 shifted by reconstructionCenterModifiers;
+        when one of that size already exists TomoJ keeps it ("reconstruction
+        exists, add some iterations", :184) together with the centres it was
+        created with, and the next reconstruction continues from it."""
+        self._sync_host()
+        shape = (params.depth, params.height, params.width)
+        if self.rec2 is None or self.rec2.shape != shape:
+            self.rec2 = np.zeros(shape, np.float32)
+            self.rec2_centers = params.volume_centers()
+            self._rec2_zero = True
+        if self.projector is not None:
+            self.projector.close()
+            self.projector = None
+        self.params = params
+
+    def resetRec2(self) -> None:
+        """resetRec2 / setRec2(null) (:199-207): the next createRec2 starts from zero."""
+        if self.projector is not None:
+            self.projector.close()
+            self.projector = None
+        self.rec2 = None
+        self.rec2_centers = None
+        self._device_newer = False
+        self._rec2_zero = True
+
+    def _projector_for(self, params: ReconstructionParameters) -> VoxelProjector3DHIP:
+        """doCPUrec's params.getProjector(ts, rec2) (:223): a projector over rec2
+        (created for these parameters if rec2 does not exist or has another
+        size); a projector already made for the same parameters object keeps
+        its device volume."""
+        if self.rec2 is None or self.rec2.shape != (params.depth, params.height, params.width):
+            self.createRec2(params)
+        if self.projector is not None and params is self.params:
+            return self.projector
+        self._sync_host()
         if self.projector is not None:
             self.projector.close()
         self.params = params
-        self.projector = params.getProjector(self.ts, self.device)
+        self.projector = params.getProjector(self.ts, self.device, vol_center=self.rec2_centers)
+        if not self._rec2_zero:
+            self.projector.set_volume(self.rec2)
+        return self.projector
 
-    def doRec(self, params: ReconstructionParameters, first_index: int = 0) -> Optional[np.ndarray]:
+    def _save_error_volume(self, iteration: int, P: VoxelProjector3DHIP) -> None:
+        """VoxelProjector3D.endOfIteration with an error volume (:828-838):
+        save it (here: keep it, and hand it to error_volume_sink if set) and
+        clear it."""
+        ev = P.getErrorVolume()
+        self.errorVolumes[iteration] = ev
+        if self.error_volume_sink is not None:
+            self.error_volume_sink(iteration, ev)
+        P.clearErrorVolume()
+
+    def _ossart_saving_errors(self, P, params, iterations: int, done: int, first_index: int) -> np.ndarray:
+        rows = []
+        for k in range(iterations):
+            rows.append(P.run_ossart(1, params.relaxationCoefficient, params.updateNb, first_index=first_index))
+            self._save_error_volume(done + k + 1, P)
+        return np.concatenate(rows, axis=0) if rows else np.zeros((0, 5))
+
+    def doRec(self, params: ReconstructionParameters, saveErrorVol: Optional[bool] = None,
+              saveForAll: Optional[bool] = None, first_index: int = 0) -> Optional[np.ndarray]:
         """doCPUrec (:218-255): WBP/BP return None; OS-SART returns the
         per-iteration err1 rows.  OSSART reads the relaxation field directly
         (TomoReconstruction2.java:645), so the 1/k schedule never applies.
         TomoJ draws the first image of each iteration at random
-        (TomoReconstruction2.java:755); here it is `first_index`."""
-        if self.projector is None or params is not self.params:
-            self.createRec2(params)
-        P = self.projector
+        (TomoReconstruction2.java:755); here it is `first_index`.
+
+        Error volume (OSSART with a plain VoxelProjector3D; TomoJ prints that a
+        FISTA-wrapped projector is not a VoxelProjector3D and makes none):
+        saveErrorVol and saveForAll create it before the loop and save it at
+        every iteration's end; saveErrorVol alone runs nbIterations - 1
+        iterations, then creates it and runs the last iteration, whose error
+        volume is saved (:229-246).  Saved volumes go to `errorVolumes`
+        keyed by the projector's iteration count (TomoJ's errorVolume_<n>.tif).
+        The flags default to the parameters' isSaveErrorVolume /
+        isSaveErrorVolumeAll, as performReconstruction passes them (:83-84)."""
+        saveErrorVol = params.isSaveErrorVolume() if saveErrorVol is None else bool(saveErrorVol)
+        saveForAll = params.isSaveErrorVolumeAll() if saveForAll is None else bool(saveForAll)
+        P = self._projector_for(params)
         avail = params.getAvailableIndexes(self.ts)
         P.setAvailableIndexes(None if len(avail) == len(self.ts.tilts) and avail == list(range(len(avail)))
                               else avail)
+        self._device_newer = True
         if params.reconstructionType in (BP, WBP):
             P.run_wbp()
             return None
+        if params.reconstructionType == OSSART and saveErrorVol and not params.fista:
+            n = params.nbIterations
+            if saveForAll:
+                P.createErrorVolume()
+                return self._ossart_saving_errors(P, params, n, 0, first_index)
+            first = P.run_ossart(max(n - 1, 0), params.relaxationCoefficient, params.updateNb,
+                                 first_index=first_index)
+            P.createErrorVolume()
+            last = self._ossart_saving_errors(P, params, 1, max(n - 1, 0), first_index)
+            return np.concatenate([first.reshape(-1, 5), last], axis=0)
         if params.reconstructionType in (OSSART, TVM, BAYESIAN, BGART, DEFINED_PROJECTIONS):
             # TVM, BAYESIAN, BGART and DEFINED_PROJECTIONS go through doCPUrec's default
             # branch, TomoReconstruction2.regularization (:783-878): the OS-SART
@@ -365,9 +474,15 @@ class ReconstructionThreadManager:
                                 first_index=first_index)
         raise ValueError(f"reconstruction type {params.reconstructionType} is not on this path")
 
+    def performReconstruction(self, params: ReconstructionParameters, first_index: int = 0):
+        """createRec2 then doRec with the parameters' error-volume flags (:83-84, :70-71)."""
+        self.createRec2(params)
+        return self.doRec(params, first_index=first_index)
+
     def getRec2(self) -> np.ndarray:
         """The reconstruction in ImageStack layout (D, H, W)."""
-        return self.projector.get_volume()
+        self._sync_host()
+        return self.rec2
 
     def close(self) -> None:
         if self.projector is not None:

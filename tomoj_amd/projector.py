@@ -61,6 +61,7 @@ class VoxelProjector3DHIP:
         self._ctx = C.c_void_p()
         call("tj_open", int(device), C.byref(g), C.byref(self._ctx))
         self.n_loaded = 0
+        self.has_error_volume = False
 
     # ------------------------------------------------------------------ lifecycle
     def close(self) -> None:
@@ -166,6 +167,27 @@ class VoxelProjector3DHIP:
         call("tj_reset_volume", self._ctx)
 
     # ------------------------------------------------------------------ Projector SPI
+    def createErrorVolume(self) -> None:
+        """VoxelProjector3D.createErrorVolume (:858-864): from now on every
+        backprojection of differences also backprojects their squares into a
+        zero-initialised error volume on the device."""
+        call("tj_set_error_volume", self._ctx, 1)
+        self.has_error_volume = True
+
+    def dropErrorVolume(self) -> None:
+        call("tj_set_error_volume", self._ctx, 0)
+        self.has_error_volume = False
+
+    def getErrorVolume(self) -> np.ndarray:
+        """The error volume in ImageStack layout (D, H, W)."""
+        out = np.empty((self.D, self.H, self.W), np.float32)
+        call("tj_get_error_volume", self._ctx, _ptr(out))
+        return out
+
+    def clearErrorVolume(self) -> None:
+        """clearErrorVolume (:842-849)."""
+        call("tj_clear_error_volume", self._ctx)
+
     def addProjection(self, index: int) -> None:
         call("tj_add_projection", self._ctx, int(index))
 
