@@ -70,6 +70,22 @@ class TiltSeriesData:
     def eulers(self) -> np.ndarray:
         return tilt_eulers(self.tilts, self.tilt_axis_deg, self.axis_vertical)
 
+    @classmethod
+    def from_mrc(cls, path: str, tilts: Optional[Sequence[float]] = None, **kw) -> "TiltSeriesData":
+        """A tilt series opened from an MRC stack as TomoJ opens one
+        (TomoJ.java:42-49 -> MRC_Reader, InputOutput/MRC_Reader.java:176-256):
+        the images as floats, the tilt angles from the FEI extended header when
+        the file has one, else `tilts`."""
+        from . import mrc
+        data, info, fei = mrc.read(path)
+        if fei is not None and tilts is None:
+            tilts = [float(a) for a in fei]
+        if tilts is None:
+            raise ValueError(f"{path}: no FEI tilt angles in the file; pass the tilts")
+        if len(tilts) != data.shape[0]:
+            raise ValueError(f"{path}: {data.shape[0]} images but {len(tilts)} tilt angles")
+        return cls(data, list(tilts), **kw)
+
     def tilt_axes(self) -> np.ndarray:
         return np.full(len(self.tilts), 0.0 if self.axis_vertical else float(self.tilt_axis_deg))
 
@@ -358,6 +374,7 @@ class ReconstructionThreadManager:
         self.rec2_centers: Optional[List[float]] = None
         self._device_newer = False                      # the projector's volume is newer than rec2
         self._rec2_zero = True                          # rec2 holds only zeros (a fresh reconstruction)
+        self._thprojs: Optional[Dict[int, np.ndarray]] = None   # getProjection's cache (thprojs)
         self.errorVolumes: Dict[int, np.ndarray] = {}   # iteration -> error volume (endOfIteration's saves)
         self.error_volume_sink: Optional[Callable[[int, np.ndarray], None]] = None
 
@@ -374,6 +391,7 @@ class ReconstructionThreadManager:
         exists, add some iterations", :184) together with the centres it was
         created with, and the next reconstruction continues from it."""
         self._sync_host()
+        self._thprojs = None
         shape = (params.depth, params.height, params.width)
         if self.rec2 is None or self.rec2.shape != shape:
             self.rec2 = np.zeros(shape, np.float32)
@@ -393,6 +411,7 @@ class ReconstructionThreadManager:
         self.rec2_centers = None
         self._device_newer = False
         self._rec2_zero = True
+        self._thprojs = None
 
     def _projector_for(self, params: ReconstructionParameters) -> VoxelProjector3DHIP:
         """doCPUrec's params.getProjector(ts, rec2) (:223): a projector over rec2
@@ -449,6 +468,7 @@ class ReconstructionThreadManager:
         saveErrorVol = params.isSaveErrorVolume() if saveErrorVol is None else bool(saveErrorVol)
         saveForAll = params.isSaveErrorVolumeAll() if saveForAll is None else bool(saveForAll)
         P = self._projector_for(params)
+        self._thprojs = None
         avail = params.getAvailableIndexes(self.ts)
         P.setAvailableIndexes(None if len(avail) == len(self.ts.tilts) and avail == list(range(len(avail)))
                               else avail)
@@ -473,6 +493,26 @@ class ReconstructionThreadManager:
             return P.run_ossart(params.nbIterations, params.relaxationCoefficient, params.updateNb,
                                 first_index=first_index)
         raise ValueError(f"reconstruction type {params.reconstructionType} is not on this path")
+
+    def getProjection(self, index: int) -> np.ndarray:
+        """getProjection(int) (:257-270): the forward projection of image
+        `index` from the current reconstruction through the projector
+        (addProjection, project, getProjection(0), clearAllProjections),
+        cached until the reconstruction changes."""
+        if self.projector is None:
+            if self.params is None:
+                raise RuntimeError("getProjection: no reconstruction yet")
+            self._projector_for(self.params)
+        if self._thprojs is None:
+            self._thprojs = {}
+        if index not in self._thprojs:
+            P = self.projector
+            P.clearAllProjections()
+            P.addProjection(int(index))
+            P.project()
+            self._thprojs[index] = P.getProjection(0)
+            P.clearAllProjections()
+        return self._thprojs[index]
 
     def performReconstruction(self, params: ReconstructionParameters, first_index: int = 0):
         """createRec2 then doRec with the parameters' error-volume flags (:83-84, :70-71)."""
