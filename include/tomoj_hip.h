@@ -122,6 +122,11 @@ int tj_get_volume(tj_ctx *ctx, float *zyx, int start_y, int end_y);
 /* Sets every voxel to zero (TomoReconstruction2.resetPixels, :414-428). */
 int tj_reset_volume(tj_ctx *ctx);
 
+/* VoxelProjector3D.setPositivityConstraint (VoxelProjector3D.java:102-104;
+ * ProjectorGPU.setPositivityConstraint, ProjectorGPU.java:149-151): changes the
+ * constraint of an open context (doGPURec sets it after initForIterative). */
+int tj_set_positivity(tj_ctx *ctx, int on);
+
 /* VoxelProjector3D.createErrorVolume (VoxelProjector3D.java:858-864) when `on`
  * (a zero volume of the reconstruction's size on the device), or drops it.
  * While it exists, every backprojection of differences (tj_backproject after

@@ -95,6 +95,7 @@ _SIGS = {
     "tj_set_volume": (C.c_int, [_P, _P, C.c_int, C.c_int]),
     "tj_get_volume": (C.c_int, [_P, _P, C.c_int, C.c_int]),
     "tj_reset_volume": (C.c_int, [_P]),
+    "tj_set_positivity": (C.c_int, [_P, C.c_int]),
     "tj_set_error_volume": (C.c_int, [_P, C.c_int]),
     "tj_get_error_volume": (C.c_int, [_P, _P]),
     "tj_clear_error_volume": (C.c_int, [_P]),

@@ -1045,6 +1045,14 @@ int tj_reset_volume(tj_ctx* c)
     return TJ_OK;
 }
 
+int tj_set_positivity(tj_ctx* c, int on)
+{
+    if (!c) return fail(TJ_EINVAL, "tj_set_positivity: null context");
+    c->geo.positivity = on ? 1 : 0;
+    c->g.positivity = on ? 1 : 0;
+    return TJ_OK;
+}
+
 int tj_set_error_volume(tj_ctx* c, int on)
 {
     if (!c) return fail(TJ_EINVAL, "tj_set_error_volume: null context");

@@ -167,6 +167,11 @@ class VoxelProjector3DHIP:
         call("tj_reset_volume", self._ctx)
 
     # ------------------------------------------------------------------ Projector SPI
+    def setPositivityConstraint(self, on: bool) -> None:
+        """setPositivityConstraint (VoxelProjector3D.java:102-104)."""
+        call("tj_set_positivity", self._ctx, int(bool(on)))
+        self.geometry.positivity = int(bool(on))
+
     def createErrorVolume(self) -> None:
         """VoxelProjector3D.createErrorVolume (:858-864): from now on every
         backprojection of differences also backprojects their squares into a
