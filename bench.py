@@ -409,6 +409,24 @@ def main():
         out["wbp_ramp_hamming_ms"] = (time.perf_counter() - t0) * 1e3
         log(f"[bench] WBP: radermacher {out['wbp_radermacher_ms']:.1f} ms, "
             f"ramp+hamming {out['wbp_ramp_hamming_ms']:.1f} ms")
+        # the drop-in boundary's own cost: TomoReconstruction2.OSSART's loop made
+        # call by call through the Projector interface (per subset 13
+        # addProjection, projectAndDifference, backproject and
+        # clearAllProjections over the C ABI, each synchronous) against the
+        # library's own loop above
+        from tomoj_amd import reconstruction as REC
+        P.reset_volume()
+        REC.ossart(P, E, 1, w.relaxation, w.update)
+        P.reset_volume()
+        k = 3
+        t0 = time.perf_counter()
+        REC.ossart(P, E, k, w.relaxation, w.update)
+        dts = time.perf_counter() - t0
+        out["spi_ossart"] = {"iters_per_s": k / dts, "ms_per_iteration": dts / k * 1e3, "vs_library_loop": (k / dts) / iters,
+                             "calls": "per subset: addProjection x update, projectAndDifference, backproject, "
+                                      "clearAllProjections (C ABI, synchronous)"}
+        log(f"[bench] OS-SART through the Projector interface: {k / dts:.1f} it/s "
+            f"({out['spi_ossart']['vs_library_loop']:.2f} x the library loop)")
         # literal 512^3 SIRT, with its own kernel times and roofline
         c = WL.SIRT_512_CUBE
         if c.name != w.name:
