@@ -51,20 +51,22 @@ def log(*a):
 
 def kernel_bytes(w):
     """Algorithmic HBM bytes per iteration of each timed kernel group (FP32
-    storage).  fp: the forward projection fused with the residual, in the
-    per-projection slab-streaming model (the volume streamed once per
-    projection; the kernel itself reads it about once per subset, so this term
-    exceeds the bytes that cross HBM), plus the measured images read and the
-    differences written.  res: the fold of the per-block error partials and
-    row flags.  bpprep: the backprojection's row table and FP64 pair images
-    (difference read, 16-byte pair written per pixel).  bp: the backprojection
-    kernel, the volume read and written once per subset plus the differences
-    (the model's 4 bytes per pixel; the kernel stages the pair images)."""
+    storage), each the least any implementation of the step must move.  fp:
+    the forward projection fused with the residual, the volume read once per
+    subset launch (every projection of the subset samples all of it) plus the
+    measured images read and the differences written (SURVEY's per-projection
+    streaming model, one volume read per projection, is reported apart as
+    `iteration_model`: it counts reads no implementation needs to make).  res:
+    the fold of the per-block error partials and row flags.  bpprep: the
+    backprojection's row table and FP64 pair images (difference read, 16-byte
+    pair written per pixel).  bp: the backprojection kernel, the volume read
+    and written once per subset plus the differences (4 bytes per pixel; the
+    kernel stages the pair images)."""
     nvox, npx, n, s = w.nvox, w.npx, w.n, w.subsets()
     ntile = (w.W + 255) // 256
     nrb = (w.H + 3) // 4
     return {
-        "fp": n * nvox * 4 + n * npx * 8,
+        "fp": s * nvox * 4 + n * npx * 8,
         "res": n * (nrb * ntile * 8 + w.H * ntile + w.H + 8),
         "bpprep": n * npx * (4 + 16) + s * w.H * 32,
         "bp": s * 2 * nvox * 4 + n * npx * 4,

@@ -480,6 +480,7 @@ typedef struct {
     int bpprep_launches;
     int fused_subsets;  /* subsets whose projectAndDifference ran as the fused
                            forward kernel (counted with timing on or off) */
+    int fast_bp_launches; /* backprojections the fast form took (timing on) */
 } tj_timing;
 int tj_get_timing(tj_ctx *ctx, tj_timing *t);
 
@@ -500,6 +501,11 @@ typedef struct {
     int bp_rows;    /* volume rows per thread: 2, 4 (1, 8 for kernel 2; 8 for 3)   */
     int bp_zchunk;  /* slices per thread: 4, 8 (16 with 2 rows in kernel 3)        */
     int bp_group;   /* images per LDS pass in kernel 3: 2, 3, 4                    */
+    int bp_fast;    /* kernel 3's fast form for a finite volume without positivity
+                       when every image row it meets is finite at an integer
+                       position: 0 off, 1..7 its shape (rows x slices x images:
+                       1 4x8x2, 2 4x8x1, 3 8x8x1, 4 4x16x1, 5 2x16x2, 6 8x4x2,
+                       7 4x4x2)                                                  */
 } tj_tuning;
 int tj_set_tuning(tj_ctx *ctx, const tj_tuning *t);
 int tj_get_tuning(tj_ctx *ctx, tj_tuning *t);

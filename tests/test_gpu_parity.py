@@ -1372,3 +1372,53 @@ def test_slab_reconstruction_equals_whole_volume(tj, oracle, kind):
         m.close()
     vol, _ = reconstruct_in_slabs(ts, p, slab_rows=3)
     assert same(vol, whole)
+
+
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5, 6, 7])
+def test_fast_backprojection_shapes_bit_identical(tj, oracle, shape):
+    """The fast backprojection (finite volume, no positivity, integer image
+    rows) in each of its shapes against the general kernel and the oracle, on
+    a ragged case: two column tiles (the second partial), a partial slice
+    chunk and a partial row group.  The fast kernel must have taken every
+    subset."""
+    W, H, D = 300, 13, 21
+    tilts = np.arange(-60, 61, 8.0)
+    E = oracle.tilt_eulers(tilts)
+    exps = oracle.phantom_project(oracle.ellipsoids(17, 6, W, H, D), E, W, H).astype(np.float32)
+    g = oracle.make_geom(W, H, D, nthreads=8)
+    ref = np.zeros((D, H, W), np.float32)
+    oracle.ossart(g, ref, exps, E, 2, 1.0, 5)
+    vols = {}
+    for f in (0, shape):
+        with make(tj, (W, H, D)) as P:
+            P.load_tilt_series(exps, E)
+            P.set_tuning(bp_fast=f)
+            P.enable_timing(True)
+            P.run_ossart(2, 1.0, 5)
+            vols[f] = P.get_volume()
+            t = P.timing()
+            subsets = 2 * -(-len(tilts) // 5)
+            assert t["fast_bp_launches"] == (subsets if f else 0), (f, t["fast_bp_launches"])
+    assert same(vols[shape], vols[0])
+    assert same(vols[shape], ref)
+
+
+def test_fast_backprojection_steps_aside(tj, oracle):
+    """Rows the fast kernel cannot take (image rows at half-integer positions,
+    so every voxel row interpolates two image rows) leave every launch to the
+    general kernel, with the same result as the oracle."""
+    W, H, D = 64, 9, 12
+    tilts = np.arange(-60, 61, 15.0)
+    E = oracle.tilt_eulers(tilts)
+    exps = oracle.phantom_project(oracle.ellipsoids(23, 4, W, H, D), E, W, H).astype(np.float32)
+    pc = (W - 1) / 2.0, (H - 1) / 2.0 + 0.5, (W - 1) / 2.0, (H - 1) / 2.0 + 0.5
+    with make(tj, (W, H, D), proj_center=pc) as P:
+        P.load_tilt_series(exps, E)
+        P.enable_timing(True)
+        P.run_ossart(1, 1.0, 3)
+        got = P.get_volume()
+        assert P.timing()["fast_bp_launches"] == 0
+    g = oracle.make_geom(W, H, D, pc=list(pc), nthreads=8)
+    ref = np.zeros((D, H, W), np.float32)
+    oracle.ossart(g, ref, exps, E, 1, 1.0, 3)
+    assert same(got, ref)

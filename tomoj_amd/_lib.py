@@ -58,12 +58,14 @@ class tj_timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("fp_ms", C.c_double), ("res_ms", C.c_double),
                 ("bp_ms", C.c_double), ("other_ms", C.c_double), ("fp_launches", C.c_int),
                 ("res_launches", C.c_int), ("bp_launches", C.c_int),
-                ("bpprep_ms", C.c_double), ("bpprep_launches", C.c_int), ("fused_subsets", C.c_int)]
+                ("bpprep_ms", C.c_double), ("bpprep_launches", C.c_int), ("fused_subsets", C.c_int),
+                ("fast_bp_launches", C.c_int)]
 
 
 class tj_tuning(C.Structure):
     _fields_ = [("fp_kernel", C.c_int), ("fp_rows", C.c_int), ("fp_rays", C.c_int), ("fp_zchunk", C.c_int),
-                ("bp_kernel", C.c_int), ("bp_rows", C.c_int), ("bp_zchunk", C.c_int), ("bp_group", C.c_int)]
+                ("bp_kernel", C.c_int), ("bp_rows", C.c_int), ("bp_zchunk", C.c_int), ("bp_group", C.c_int),
+                ("bp_fast", C.c_int)]
 
 
 class tj_job_desc(C.Structure):

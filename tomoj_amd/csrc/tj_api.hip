@@ -30,7 +30,8 @@ hipError_t fp_residual(hipStream_t, const TjGeomDev&, const float*, const TjProj
                        int, int, const float*, double, float*, double*, uint8_t*, uint8_t*, double*,
                        hipEvent_t mid = nullptr);
 hipError_t bp(hipStream_t, const TjGeomDev&, float*, const TjProj*, const int*, int, const float*, const int*,
-              const uint8_t*, TjBpRow*, int, int, int, int, int, int, int, double2*, hipEvent_t mid = nullptr);
+              const uint8_t*, TjBpRow*, int, int, int, int, int, int, int, double2*, hipEvent_t mid = nullptr,
+              int* slow = nullptr, int fast = 0);
 hipError_t zyx_to_yzx(hipStream_t, const float*, float*, int, int, int, int);
 hipError_t yzx_to_zyx(hipStream_t, const float*, float*, int, int, int, int);
 hipError_t plane_flags(hipStream_t, const float*, size_t, int, int, uint8_t*);
@@ -187,7 +188,9 @@ struct tj_ctx {
     int bpairs_cap = 0;
 
     // kernel selection and shapes (tj_set_tuning)
-    tj_tuning tune{0, 2, 4, 4, 0, 4, 8, 3};
+    tj_tuning tune{0, 2, 4, 4, 0, 4, 8, 3, 1};
+    int* bpslow = nullptr;      // [2]: k_bp_rows' flag (a row the fast backprojection cannot take),
+                                // and the count of launches the fast backprojection took
 
     // progress / cancellation
     std::atomic<double> completion{0.0};
@@ -201,6 +204,7 @@ struct tj_ctx {
 
 static void ctx_free(tj_ctx* c)
 {
+    dfree(c->bpslow);
     dfree(c->errvol);
     dfree(c->bpairs);
     dfree(c->prev);
@@ -375,13 +379,14 @@ static int ensure_slots(tj_ctx* c, int n)
     dfree(c->rbad);
     dfree(c->bprows);
     dfree(c->slot_proj);
+    dfree(c->bpslow);
     c->cap = 0;
     const size_t npx = c->npx;
     const size_t ntile = (size_t)(c->g.pw + 255) / 256;  // column tiles of the fused residual
     if (dalloc(&c->sproj, npx * n) || dalloc(&c->snorm, npx * n) || dalloc(&c->sdiff, npx * n) ||
         dalloc(&c->row_err, (size_t)c->g.ph * n * ntile) || dalloc(&c->rowflag, (size_t)c->g.ph * n) ||
         dalloc(&c->rbad, (size_t)c->g.ph * n * ntile) ||
-        dalloc(&c->bprows, (size_t)c->g.vh * n) || dalloc(&c->slot_proj, n)) {
+        dalloc(&c->bprows, (size_t)c->g.vh * n) || dalloc(&c->slot_proj, n) || dalloc(&c->bpslow, 2)) {
         (void)hipGetLastError();
         return fail(TJ_ENOMEM, "cannot allocate %d projection slots of %dx%d", n, c->g.pw, c->g.ph);
     }
@@ -489,7 +494,7 @@ static int do_backproject(tj_ctx* c, const int* slots, const std::vector<int>& h
         if (int rc = ensure_bpairs(c, (int)hslots.size())) return rc;
     TJ_HIP(tjk::bp(c->stream, c->g, bp_target(c), c->pp, slots, (int)hslots.size(), images, image_index,
                    c->rowflag, c->bprows, start_y, end_y, kernel, c->tune.bp_rows, c->tune.bp_zchunk,
-                   c->tune.bp_group, bp_finite(c), c->bpairs));
+                   c->tune.bp_group, bp_finite(c), c->bpairs, nullptr, c->bpslow, c->tune.bp_fast));
     return TJ_OK;
 }
 
@@ -516,7 +521,8 @@ static int do_error_backproject(tj_ctx* c, const int* slots, const std::vector<i
     // the error volume holds sums of squares with the NaN/Inf scrub after
     // every image, so it is always finite
     TJ_HIP(tjk::bp(c->stream, c->g, c->errvol, c->pp, slots, n, c->sdiff, c->iota, c->rowflag, c->bprows, start_y,
-                   end_y, kernel, c->tune.bp_rows, c->tune.bp_zchunk, c->tune.bp_group, 1, c->bpairs));
+                   end_y, kernel, c->tune.bp_rows, c->tune.bp_zchunk, c->tune.bp_group, 1, c->bpairs, nullptr,
+                   c->bpslow, c->tune.bp_fast));
     return TJ_OK;
 }
 
@@ -1321,6 +1327,7 @@ int tj_set_tuning(tj_ctx* c, const tj_tuning* t)
     if (t->fp_rows < 1 || t->bp_rows < 1 || t->fp_rays < 1 || t->bp_zchunk < 0 || t->fp_zchunk < 0 ||
         t->bp_group < 1)
         return fail(TJ_EINVAL, "tj_set_tuning: non-positive shape");
+    if (t->bp_fast < 0 || t->bp_fast > 7) return fail(TJ_EINVAL, "tj_set_tuning: bp_fast out of range");
     c->tune = *t;
     return TJ_OK;
 }
@@ -1860,6 +1867,7 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
     if (c->timing_on) {
         t0 = ev(c, nev++);
         t1 = ev(c, nev++);
+        TJ_HIP(hipMemsetAsync(c->bpslow + 1, 0, sizeof(int), c->stream));
         TJ_HIP(hipEventRecord(t0, c->stream));
     }
     std::vector<std::pair<hipEvent_t, hipEvent_t>> fpev, resev, prepev, bpev;
@@ -1912,7 +1920,7 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
                     if ((rc = ensure_bpairs(c, diffIndex))) return rc;
                 TJ_HIP(tjk::bp(c->stream, c->g, bp_target(c), c->pp, slots, diffIndex, c->sdiff, c->iota,
                                c->rowflag, c->bprows, start_y, end_y, bk, c->tune.bp_rows, c->tune.bp_zchunk,
-                               c->tune.bp_group, bp_finite(c), c->bpairs, q));
+                               c->tune.bp_group, bp_finite(c), c->bpairs, q, c->bpslow, c->tune.bp_fast));
                 if ((rc = do_error_backproject(c, slots, hs, start_y, end_y))) return rc;
                 if (c->timing_on) {
                     TJ_HIP(hipEventRecord(d, c->stream));
@@ -1985,6 +1993,9 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
             tm.bp_launches++;
         }
         tm.other_ms = tm.total_ms - tm.fp_ms - tm.res_ms - tm.bpprep_ms - tm.bp_ms;
+        int nfast = 0;
+        TJ_HIP(hipMemcpy(&nfast, c->bpslow + 1, sizeof(int), hipMemcpyDeviceToHost));
+        tm.fast_bp_launches = nfast;
     }
     c->last_timing = tm;
     c->completion.store(0.0);

@@ -33,6 +33,7 @@ struct AddD { __device__ double operator()(double a, double b) const { return a 
 struct MaxF { __device__ float operator()(float a, float b) const { return a > b ? a : b; } };
 struct MaxD { __device__ double operator()(double a, double b) const { return a > b ? a : b; } };
 struct OrU { __device__ unsigned operator()(unsigned a, unsigned b) const { return a | b; } };
+struct MaxU { __device__ unsigned operator()(unsigned a, unsigned b) const { return a > b ? a : b; } };
 struct MinD { __device__ double operator()(double a, double b) const { return a < b ? a : b; } };
 
 // jcentered of detector row j: VoxelProjector3D.project splits the rows from
@@ -321,7 +322,8 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_row_flags(int pw, int ph, const fl
 // Row table for the single-axis kernel: yy = ((x*e10 + y*e11) + z*e12)*scale + ciy
 // reduces to (y*e11)*scale + ciy when e10 == e12 == 0.
 __global__ void k_bp_rows(TjGeomDev g, const TjProj* __restrict__ pp, const int* __restrict__ slot_proj,
-                          int nslot, const uint8_t* __restrict__ rowflag, TjBpRow* __restrict__ rows)
+                          int nslot, const uint8_t* __restrict__ rowflag, TjBpRow* __restrict__ rows,
+                          int* __restrict__ slow)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int s = blockIdx.y;
@@ -343,6 +345,9 @@ __global__ void k_bp_rows(TjGeomDev g, const TjProj* __restrict__ pp, const int*
         r.yf = yy - (double)r.yb;
         r.shortcut = (r.y1 == r.yb) || (r.yf == 0.0 && rowflag[(size_t)s * g.ph + r.y1] == 0);
         r.finite = rowflag[(size_t)s * g.ph + r.yb] == 0;
+        // a row the fast backprojection cannot take (fractional yy, or a
+        // non-finite value in its image rows): the general kernel runs
+        if (slow && !(r.shortcut && r.finite)) atomicOr(slow, 1);
     }
     rows[(size_t)s * g.vh + j] = r;
 }
@@ -498,8 +503,9 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_bp_lds(
     TjGeomDev g, float* __restrict__ vol, const TjProj* __restrict__ pp,
     const int* __restrict__ slot_proj, int nslot, const float* __restrict__ images,
     const int* __restrict__ image_index, const TjBpRow* __restrict__ rows, int start_y, int end_y,
-    int finite_volume, const double2* __restrict__ pairs)
+    int finite_volume, const double2* __restrict__ pairs, const int* __restrict__ run_if)
 {
+    if (run_if && *run_if == 0) return;  // the fast kernel takes this launch (uniform)
     constexpr int XW = TJ_BLOCK + KC + 8;  // reachable columns + a zero cell at XW-1
     constexpr int NB = (XW + 63) / 64;     // 64-cell copies per staged row
     __shared__ double2 lds[G][R][XW];
@@ -703,6 +709,165 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_bp_lds(
                         bp_accumulate<POS>(acc[kk][r], value);
                     }
                 }
+            }
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (kk < nk && r < nr) vol[(size_t)(j0 + r) * plane + (size_t)(k0 + kk) * g.vw + i] = acc[kk][r];
+}
+
+// Fast backprojection for the common case of a launch: a finite volume, no
+// positivity, and every image row a voxel row meets finite with an integer
+// position (k_bp_rows leaves *slow clear).  Same results as k_bp_lds's finite
+// mode with fewer instructions per update and the staging hidden:
+// - the staged cells start at floor(min xx) over the block, which may be
+//   negative, and cover every column the block's voxels reach; cells of
+//   columns outside the image (and whole rows outside the detector) are
+//   loaded as zero pairs by the buffer range check, so a voxel reads the cell
+//   of floor(xx) with no range test: off the image it adds (0 + f*0) = 0,
+//   exactly TomoJ's "skip" for a finite sum (a voxel holding -0 becomes +0);
+// - for xx >= 0, floor(xx) is (int)xx and xx - floor(xx) is exact, so the
+//   weight is ImageJ's x - (int)x;
+// - one image at a time, double-buffered: the rows of image s+1 stream into
+//   LDS (buffer-to-LDS loads) while image s is added;
+// - slices past the volume in the last chunk read the last slice's cells and
+//   are never stored.
+// The per-image NaN/Inf scrub of TomoJ stays (a finite sum can overflow).
+template <int R, int KC, int WAVES>
+__global__ __launch_bounds__(TJ_BLOCK, WAVES) void k_bp_fast(
+    TjGeomDev g, float* __restrict__ vol, const TjProj* __restrict__ pp, const int* __restrict__ slot_proj,
+    int nslot, const TjBpRow* __restrict__ rows, int start_y, int end_y, const double2* __restrict__ pairs,
+    const int* __restrict__ slow)
+{
+    if (*slow) return;  // the general kernel takes this launch (uniform)
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((int*)slow + 1, 1);  // launches taken (slow[1])
+    constexpr int XW = TJ_BLOCK + KC + 8;
+    constexpr int NB = (XW + 63) / 64;
+    constexpr int NW = TJ_BLOCK / 64;
+    static_assert(R <= NW, "one staging wave per row");
+    __shared__ double2 lds[2][R][XW];
+    __shared__ double s_e0[2], s_e2[2];
+    __shared__ int s_x0[2];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nbx = (g.vw + TJ_BLOCK - 1) / TJ_BLOCK, nby = (end_y - start_y + R - 1) / R;
+    const int nbz = (g.vd + KC - 1) / KC;
+    int bx, by, bz;
+    {
+        // XCD-aware order as in k_bp_lds: a row group's blocks on one XCD
+        const int b = blockIdx.x;
+        if ((nby & 7) == 0) {
+            const int per = nbx * nbz, k = b >> 3;
+            by = (b & 7) + 8 * (k / per);
+            const int rr = k - (k / per) * per;
+            bz = rr / nbx;
+            bx = rr - bz * nbx;
+        } else {
+            bx = b % nbx;
+            by = (b / nbx) % nby;
+            bz = b / (nbx * nby);
+        }
+    }
+    const int i0 = bx * TJ_BLOCK;
+    const int i = i0 + tid;
+    const int j0 = start_y + by * R;
+    const int nr = min(R, end_y - j0);
+    const int k0 = bz * KC;
+    const int nk = min(KC, g.vd - k0);
+    const int ilast = min(i0 + TJ_BLOCK - 1, g.vw - 1);
+    const bool active = i < g.vw;
+    const double x = (active ? i : ilast) - g.vcx;
+    const size_t plane = (size_t)g.vd * g.vw;
+
+    // wave r (< nr) stages image row r of slot s into buffer b: the cells
+    // from floor(min xx) on, one range-checked 16-byte buffer load per lane
+    // and 64-cell copy; lane 0 of wave 0 leaves the slot's constants for the
+    // compute phase
+    auto stage = [&](int sl, int b) {
+        const TjProj& Q = pp[slot_proj[sl]];
+        const double xa = (i0 - g.vcx) * Q.e[0], xb = (ilast - g.vcx) * Q.e[0];
+        const double za = (k0 - g.vcz) * Q.e[2], zb = (k0 + nk - 1 - g.vcz) * Q.e[2];
+        const double m = fmin(fmin((xa + za) + g.pc2, (xa + zb) + g.pc2), fmin((xb + za) + g.pc2, (xb + zb) + g.pc2));
+        const int x0s = __builtin_amdgcn_readfirstlane((int)floor(m));
+        if (wv == 0 && lane == 0) {
+            s_x0[b] = x0s;
+            s_e0[b] = Q.e[0];
+            s_e2[b] = Q.e[2];
+        }
+        if (wv < R) {
+            const int r = wv;
+            bool rok = false;
+            int ryb = 0;
+            if (r < nr) {
+                const TjBpRow row = rows[(size_t)sl * g.vh + j0 + r];
+                rok = row.valid != 0;
+                ryb = row.yb;
+            }
+            rok = __builtin_amdgcn_readfirstlane(rok ? 1 : 0) != 0;
+            ryb = __builtin_amdgcn_readfirstlane(ryb);
+            const double2* prow = pairs + ((size_t)sl * g.ph + (rok ? ryb : 0)) * g.pw;
+            const __amdgpu_buffer_rsrc_t rs =
+                __builtin_amdgcn_make_buffer_rsrc((void*)prow, 0, rok ? g.pw * 16 : 0, 0x00020000);
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                const int c0 = min(64 * c, XW - 64);
+                const int col = x0s + c0 + lane;
+                const uint32_t voff = (col >= 0) ? (uint32_t)col * 16u : 0xFFFFFFF0u;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)&lds[b][r][c0],
+                                                         16, voff, 0, 0, 0);
+            }
+        }
+    };
+
+    if (nslot > 0) stage(0, 0);
+    float acc[KC][R];
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            acc[kk][r] = (active && kk < nk && r < nr)
+                             ? vol[(size_t)(j0 + r) * plane + (size_t)(k0 + kk) * g.vw + i]
+                             : 0.0f;
+    double zk[KC];  // slice positions, the last real slice for the chunk's padding
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk) zk[kk] = (k0 + min(kk, nk - 1)) - g.vcz;
+
+    for (int sl = 0; sl < nslot; ++sl) {
+        const int b = sl & 1;
+        __builtin_amdgcn_s_waitcnt(0);  // this thread's copies of image sl have landed
+        __syncthreads();                // every thread's have, and image sl-1 is done
+        if (sl + 1 < nslot) stage(sl + 1, b ^ 1);
+        const double e2 = s_e2[b];
+        const double xe = x * s_e0[b];
+        const int x0s = s_x0[b];
+        int c[KC];
+        double xf[KC];
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk) {
+            const double xx = (xe + zk[kk] * e2) + g.pc2;
+            const double fl = floor(xx);
+            xf[kk] = xx - fl;
+            c[kk] = (int)fl - x0s;
+        }
+        double2 q[2][R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) q[0][r] = lds[b][r][c[0]];
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk) {
+            if (kk + 1 < KC) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) q[(kk + 1) & 1][r] = lds[b][r][c[kk + 1 < KC ? kk + 1 : kk]];
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const double2 v = q[kk & 1][r];
+                const float a = acc[kk][r] + (float)(v.x + xf[kk] * v.y);
+                acc[kk][r] = (isnan(a) || isinf(a)) ? 0.0f : a;
             }
         }
     }
@@ -1933,10 +2098,11 @@ template <int R, int KC, int G>
 static hipError_t launch_bp_lds(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp,
                                 const int* slot_proj, int nslot, const float* images, const int* image_index,
                                 const TjBpRow* rows, int start_y, int end_y, int finite_volume, double2* pairs,
-                                hipEvent_t mid)
+                                hipEvent_t mid, const int* slow, int fast)
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     const size_t npx = (size_t)g.pw * g.ph;
+    const bool fast_on = slow && fast > 0 && finite_volume && !g.positivity;
     k_bp_pairs<<<dim3((unsigned)std::min<size_t>((npx + TJ_BLOCK - 1) / TJ_BLOCK, 1024), nslot), TJ_BLOCK, 0, st>>>(
         images, image_index, g.pw, g.ph, pairs);
     if (mid) {
@@ -1944,11 +2110,31 @@ static hipError_t launch_bp_lds(hipStream_t st, const TjGeomDev& g, float* vol, 
         if (e != hipSuccess) return e;
     }
     const int ny = end_y - start_y;
+    // the fast kernel (finite volume, no positivity; it leaves the launch to
+    // the general kernel when k_bp_rows flagged a row it cannot take)
+    const int* run_if = nullptr;
+    if (fast_on) {
+        run_if = slow;
+        auto grid_of = [&](int r, int kc) {
+            return (unsigned)(((g.vw + TJ_BLOCK - 1) / TJ_BLOCK) * ((ny + r - 1) / r) * ((g.vd + kc - 1) / kc));
+        };
+#define TJ_BPF(RR, KK, WW) k_bp_fast<RR, KK, WW><<<grid_of(RR, KK), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, rows, start_y, end_y, pairs, slow)
+        switch (fast) {
+        case 1: TJ_BPF(4, 8, 4); break;
+        case 2: TJ_BPF(4, 8, 3); break;
+        case 3: TJ_BPF(4, 4, 4); break;
+        case 4: TJ_BPF(2, 8, 4); break;
+        case 5: TJ_BPF(4, 16, 2); break;
+        case 6: TJ_BPF(2, 16, 3); break;
+        default: TJ_BPF(4, 4, 5); break;
+        }
+#undef TJ_BPF
+    }
     const unsigned grid = (unsigned)(((g.vw + TJ_BLOCK - 1) / TJ_BLOCK) * ((ny + R - 1) / R) * ((g.vd + KC - 1) / KC));
     if (g.positivity)
-        k_bp_lds<R, KC, G, true><<<grid, TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume, pairs);
+        k_bp_lds<R, KC, G, true><<<grid, TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume, pairs, run_if);
     else
-        k_bp_lds<R, KC, G, false><<<grid, TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume, pairs);
+        k_bp_lds<R, KC, G, false><<<grid, TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume, pairs, run_if);
     return hipGetLastError();
 }
 
@@ -1956,7 +2142,7 @@ static hipError_t launch_bp_lds(hipStream_t st, const TjGeomDev& g, float* vol, 
 hipError_t bp(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp, const int* slot_proj, int nslot,
               const float* images, const int* image_index, const uint8_t* rowflag, TjBpRow* rows, int start_y,
               int end_y, int kernel, int rows_per_thread, int kchunk, int group, int finite_volume,
-              double2* pairs, hipEvent_t mid)
+              double2* pairs, hipEvent_t mid, int* slow, int fast)
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nslot <= 0 || end_y <= start_y) return hipSuccess;
@@ -1975,9 +2161,17 @@ hipError_t bp(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp, 
             k_bp_generic<false><<<dim3(bx, ny, g.vd), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, start_y);
         return hipGetLastError();
     }
-    k_bp_rows<<<dim3((g.vh + 255) / 256, nslot), 256, 0, st>>>(g, pp, slot_proj, nslot, rowflag, rows);
+    // `slow` (two device ints, or null): slow[0] set by k_bp_rows when a row
+    // needs the general kernel, slow[1] counts the launches the fast kernel
+    // took; only the LDS path with a fast shape uses them
+    int* slow_flag = (kernel == 3 && fast > 0 && finite_volume && !g.positivity) ? slow : nullptr;
+    if (slow_flag) {
+        const hipError_t e = hipMemsetAsync(slow_flag, 0, sizeof(int), st);
+        if (e != hipSuccess) return e;
+    }
+    k_bp_rows<<<dim3((g.vh + 255) / 256, nslot), 256, 0, st>>>(g, pp, slot_proj, nslot, rowflag, rows, slow_flag);
     if (kernel == 3) {
-#define TJ_BPL(RR, KK, GG) return launch_bp_lds<RR, KK, GG>(st, g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume, pairs, mid)
+#define TJ_BPL(RR, KK, GG) return launch_bp_lds<RR, KK, GG>(st, g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume, pairs, mid, slow_flag, fast)
         if (rows_per_thread == 2) {
             if (kchunk >= 16) { if (group >= 4) TJ_BPL(2, 16, 4); TJ_BPL(2, 16, 2); }
             if (group >= 8) TJ_BPL(2, 8, 8);
