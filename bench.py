@@ -74,8 +74,21 @@ def kernel_bytes(w):
 
 
 # Kernels of one plain OS-SART/SIRT iteration on the LDS path, as rocprofv3
-# names them, grouped as tj_timing groups them.
-ITERATION_KERNELS = {"fp": "k_fp_lds", "res": "k_err_fold", "bpprep": ("k_bp_rows", "k_bp_pairs"), "bp": "k_bp_lds"}
+# names them, grouped as tj_timing groups them.  The backprojection group
+# launches both forms; the row table picks one on the device and the other
+# returns at once.
+ITERATION_KERNELS = {"fp": "k_fp_lds", "res": "k_err_fold", "bpprep": ("k_bp_rows", "k_bp_pairs"),
+                     "bp": ("k_bp_fast", "k_bp_lds")}
+
+
+def working_kernel(group, tim):
+    """The kernel of a timed group that did the group's work: the fast
+    backprojection when its launches ran (tj_timing counts them), else the
+    general one."""
+    if group == "bp":
+        return "k_bp_fast" if tim.get("fast_bp_launches", 0) > 0 else "k_bp_lds"
+    names = ITERATION_KERNELS[group]
+    return names if isinstance(names, str) else names[-1]
 
 
 def load_traffic():
@@ -157,7 +170,7 @@ def roofline_of(w, kernels, tim, steps, traffic):
     launch time against the 8 TB/s HBM peak, its counter-measured traffic, and
     the same launch against the vector issue rate."""
     dom = max(("fp", "bp"), key=lambda k: tim[f"{k}_ms"])
-    kname = ITERATION_KERNELS[dom]
+    kname = working_kernel(dom, tim)
     achieved = kernels[dom]["achieved_GBs"]
     r = {
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -550,11 +550,14 @@ typedef struct {
 /* Tile and slab shapes of the job kernels (picked by tj_batch_autotune). */
 typedef struct {
     int fp_rays;        /* projections per forward tile: 2 or 4                     */
-    int fp_lds_kb;      /* LDS per forward block in KiB (16..128): slices per slab   */
+    int fp_lds_kb;      /* LDS cap per forward block in KiB (16..152): slices per
+                           slab; a launch takes less where more would cost
+                           resident blocks (the registers' limit)               */
     int bp_rows;        /* volume rows per back tile: 2, 4 or 8                      */
     int bp_zchunk;      /* slices per back tile: 4, 8 or 16                          */
     int bp_group;       /* images per LDS pass (derived from the shape)              */
-    int blocks_per_cu;  /* persistent blocks per compute unit: 1..16                 */
+    int blocks_per_cu;  /* persistent blocks per compute unit: 1..16, capped at
+                           what one CU holds of each kernel                      */
     int fp_packed;      /* BF16/FP16 storage: forward projection on packed 16-bit
                            pairs with v_dot2 (8 rows per tile) instead of FP32    */
     int fp_cols;        /* packed forward: 256-column tiles per block, 1 or 2 (two

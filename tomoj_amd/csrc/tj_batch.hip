@@ -264,9 +264,12 @@ __device__ __forceinline__ void fp_slab(const typename V4<A>::t* cell, int zn, i
         fp_slice_f64<A, G, NEARY>(cell, z0 + zlast, zlast * W1, zero, (A)0, wd, ym, bd, td, zlo, zhi, rs);
 }
 
+// At most 128 VGPRs (4 waves per SIMD): the kernel is latency bound and a CU
+// holding four of its blocks hides one block's slab staging under the others'
+// projection.
 template <typename T, typename TE, typename A, int G>
-__global__ __launch_bounds__(NT) void k_job_fp(const TjJob* __restrict__ jobs, const int4* __restrict__ tiles,
-                                               int ntiles, int lds_cells)
+__global__ __launch_bounds__(NT, 4) void k_job_fp(const TjJob* __restrict__ jobs, const int4* __restrict__ tiles,
+                                                  int ntiles, int lds_cells)
 {
     using C4 = typename V4<A>::t;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1178,6 +1181,15 @@ struct BJob {
     size_t bytes = 0;
 };
 
+// A launch resolved, from the storage and the tuning, to one kernel instance
+// with its block size and dynamic LDS.
+struct Kern {
+    const void* fn = nullptr;
+    int block = tjb::NT;
+    int lds = 0;     // dynamic LDS bytes
+    int cells = -1;  // forward kernels: LDS cells (the kernel's last argument); -1 for the backprojection
+};
+
 struct RunPlan {
     std::vector<int> ids;
     int iterations = 0;
@@ -1192,6 +1204,7 @@ struct RunPlan {
         int4* tiles;
         int job0, njobs;  // error launch: jobs [job0, job0+njobs) of djobs
         int pstorage;     // forward launch: storage of the measured projections
+        Kern k;           // forward and back launches: the kernel
     };
     std::vector<L> launches;
 };
@@ -1210,6 +1223,7 @@ struct tj_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int ncu = 256;
+    int lds_cu = 160 * 1024;  // LDS bytes per CU
 };
 
 static void free_plan(RunPlan& p)
@@ -1308,19 +1322,52 @@ static int job_params(BJob* j)
 }
 
 // ---------------------------------------------------------------------------
-// launch helpers (template dispatch on storage and tuning)
+// launch helpers: each launch of a plan resolves, from the storage and the
+// tuning, to one kernel instance with its block size and dynamic LDS (Kern);
+// the persistent grid is then sized from that kernel's own occupancy.
 // ---------------------------------------------------------------------------
-// Dynamic LDS of the forward kernel: the tuned size, raised to what one slice of
-// the widest job needs (W + 1 cells, the replicated column, the zero cells).
+// Blocks of `k` one CU holds at once (0 if the runtime cannot say).
+static int resident_blocks(const Kern& k)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k.fn, k.block, (size_t)k.lds) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return nb;
+}
+
+// Dynamic LDS of a forward kernel: at most `cap` bytes, and no more than lets
+// the CU hold as many blocks as the kernel's registers allow (the forward
+// kernels are latency bound: resident blocks count for more than slices per
+// slab); never below `need` (one slice of the widest job).
+static int fit_lds(Kern k, int cap, int need, int lds_cu)
+{
+    k.lds = 0;
+    const int nb = resident_blocks(k);
+    int l = cap;
+    if (nb > 0) {
+        l = std::min(cap, (lds_cu / nb) & ~1023);
+        // allocation granules may round a block's share up: step down to a size
+        // at which nb blocks are resident
+        for (int m = 0; m < 16 && l - 1024 >= need; ++m) {
+            k.lds = l;
+            if (resident_blocks(k) >= nb) break;
+            l -= 1024;
+        }
+    }
+    return std::max(l, need);
+}
+
 template <typename T, typename TE, typename A, int G>
-static hipError_t launch_fp_t(hipStream_t st, const TjJob* jobs, const int4* tiles, int ntiles, int grid,
-                              int lds_bytes, int max_w)
+static Kern fp_kern_t(int cap, int max_w, int lds_cu)
 {
     using C4 = typename tjb::V4<A>::t;
-    lds_bytes = std::max(lds_bytes, (max_w + 3) * (int)sizeof(C4));
-    const int cells = lds_bytes / (int)sizeof(C4);
-    tjb::k_job_fp<T, TE, A, G><<<grid, tjb::NT, lds_bytes, st>>>(jobs, tiles, ntiles, cells);
-    return hipGetLastError();
+    Kern k;
+    k.fn = (const void*)tjb::k_job_fp<T, TE, A, G>;
+    k.lds = fit_lds(k, cap, (max_w + 3) * (int)sizeof(C4), lds_cu);
+    k.cells = k.lds / (int)sizeof(C4);
+    return k;
 }
 
 template <typename T, typename TE, typename A, int G>
@@ -1332,7 +1379,7 @@ static hipError_t allow_lds(int bytes)
 
 // Largest dynamic LDS any forward launch may ask for (set once per device,
 // before any graph capture).
-static const int kMaxFpLds = 144 * 1024;
+static const int kMaxFpLds = 152 * 1024;
 
 template <typename T, typename TE>
 static hipError_t allow_lds2()
@@ -1368,13 +1415,13 @@ static hipError_t allow_fp_lds()
     return e;
 }
 
+static int fp_lds_cap(const tj_batch_tuning& tu) { return std::min(std::max(tu.fp_lds_kb, 16), 152) * 1024; }
+
 template <typename T, typename TE>
-static hipError_t launch_fp_s(hipStream_t st, const tj_batch_tuning& tu, const TjJob* jobs, const int4* tiles,
-                              int ntiles, int grid, int max_w)
+static Kern fp_kern_s(const tj_batch_tuning& tu, int max_w, int lds_cu)
 {
-    const int lds = std::min(std::max(tu.fp_lds_kb, 16), 128) * 1024;
-    if (tu.fp_rays <= 2) return launch_fp_t<T, TE, float, 2>(st, jobs, tiles, ntiles, grid, lds, max_w);
-    return launch_fp_t<T, TE, float, 4>(st, jobs, tiles, ntiles, grid, lds, max_w);
+    if (tu.fp_rays <= 2) return fp_kern_t<T, TE, float, 2>(fp_lds_cap(tu), max_w, lds_cu);
+    return fp_kern_t<T, TE, float, 4>(fp_lds_cap(tu), max_w, lds_cu);
 }
 
 static bool use_packed(const tj_batch_tuning& tu, int storage)
@@ -1398,46 +1445,36 @@ static int fp_rays_of(const tj_batch_tuning& tu, int storage)
 // packed-pair forward kernel for 16-bit storage (8 rows per tile); one slice
 // of the widest job needs 2 (W + 2) cells plus the padding of the odd half
 template <typename T, typename TE>
-static hipError_t launch_fpp_s(hipStream_t st, const tj_batch_tuning& tu, const TjJob* jobs, const int4* tiles,
-                               int ntiles, int grid, int max_w)
+static Kern fpp_kern_s(const tj_batch_tuning& tu, int max_w, int lds_cu)
 {
-    int lds = std::min(std::max(tu.fp_lds_kb, 16), 128) * 1024;
-    lds = std::max(lds, 2 * (max_w + 16) * (int)sizeof(uint4));
-    const int cells = lds / (int)sizeof(uint4);
     const bool two = fp_cols_of(tu, TJ_STORAGE_BF16) == 2, g2 = fp_rays_of(tu, TJ_STORAGE_BF16) == 2;
-    if (two) {
-        if (g2) tjb::k_job_fpp<T, TE, 2, 2><<<grid, 2 * tjb::NT, lds, st>>>(jobs, tiles, ntiles, cells);
-        else tjb::k_job_fpp<T, TE, 4, 2><<<grid, 2 * tjb::NT, lds, st>>>(jobs, tiles, ntiles, cells);
-    } else {
-        if (g2) tjb::k_job_fpp<T, TE, 2, 1><<<grid, tjb::NT, lds, st>>>(jobs, tiles, ntiles, cells);
-        else tjb::k_job_fpp<T, TE, 4, 1><<<grid, tjb::NT, lds, st>>>(jobs, tiles, ntiles, cells);
-    }
-    return hipGetLastError();
+    Kern k;
+    k.block = two ? 2 * tjb::NT : tjb::NT;
+    if (two) k.fn = g2 ? (const void*)tjb::k_job_fpp<T, TE, 2, 2> : (const void*)tjb::k_job_fpp<T, TE, 4, 2>;
+    else k.fn = g2 ? (const void*)tjb::k_job_fpp<T, TE, 2, 1> : (const void*)tjb::k_job_fpp<T, TE, 4, 1>;
+    k.lds = fit_lds(k, fp_lds_cap(tu), 2 * (max_w + 16) * (int)sizeof(uint4), lds_cu);
+    k.cells = k.lds / (int)sizeof(uint4);
+    return k;
 }
 
 static int fp_rows_of(const tj_batch_tuning& tu, int storage) { return use_packed(tu, storage) ? 8 : 4; }
 
-static hipError_t launch_fp(hipStream_t st, int storage, int pstorage, const tj_batch_tuning& tu,
-                            const TjJob* jobs, const int4* tiles, int ntiles, int grid, int max_w)
+static Kern fp_kern(int storage, int pstorage, const tj_batch_tuning& tu, int max_w, int lds_cu)
 {
-    (void)hipGetLastError();
     const bool pf = pstorage == TJ_STORAGE_F32;
     if (use_packed(tu, storage)) {
         if (storage == TJ_STORAGE_BF16)
-            return pf ? launch_fpp_s<Bf16, float>(st, tu, jobs, tiles, ntiles, grid, max_w)
-                      : launch_fpp_s<Bf16, Bf16>(st, tu, jobs, tiles, ntiles, grid, max_w);
-        return pf ? launch_fpp_s<_Float16, float>(st, tu, jobs, tiles, ntiles, grid, max_w)
-                  : launch_fpp_s<_Float16, _Float16>(st, tu, jobs, tiles, ntiles, grid, max_w);
+            return pf ? fpp_kern_s<Bf16, float>(tu, max_w, lds_cu) : fpp_kern_s<Bf16, Bf16>(tu, max_w, lds_cu);
+        return pf ? fpp_kern_s<_Float16, float>(tu, max_w, lds_cu)
+                  : fpp_kern_s<_Float16, _Float16>(tu, max_w, lds_cu);
     }
     switch (storage) {
-    case TJ_STORAGE_F32: return launch_fp_s<float, float>(st, tu, jobs, tiles, ntiles, grid, max_w);
+    case TJ_STORAGE_F32: return fp_kern_s<float, float>(tu, max_w, lds_cu);
     case TJ_STORAGE_BF16:
-        return pf ? launch_fp_s<Bf16, float>(st, tu, jobs, tiles, ntiles, grid, max_w)
-                  : launch_fp_s<Bf16, Bf16>(st, tu, jobs, tiles, ntiles, grid, max_w);
+        return pf ? fp_kern_s<Bf16, float>(tu, max_w, lds_cu) : fp_kern_s<Bf16, Bf16>(tu, max_w, lds_cu);
     case TJ_STORAGE_F16:
-        return pf ? launch_fp_s<_Float16, float>(st, tu, jobs, tiles, ntiles, grid, max_w)
-                  : launch_fp_s<_Float16, _Float16>(st, tu, jobs, tiles, ntiles, grid, max_w);
-    default: return launch_fp_t<double, double, double, 2>(st, jobs, tiles, ntiles, grid, 128 * 1024, max_w);
+        return pf ? fp_kern_s<_Float16, float>(tu, max_w, lds_cu) : fp_kern_s<_Float16, _Float16>(tu, max_w, lds_cu);
+    default: return fp_kern_t<double, double, double, 2>(128 * 1024, max_w, lds_cu);
     }
 }
 
@@ -1456,49 +1493,55 @@ static BpShape bp_shape_of(const tj_batch_tuning& tu, int storage)
 }
 
 template <typename T, int MODE>
-static hipError_t launch_bp_s(hipStream_t st, const BpShape& s, const TjJob* jobs, const int4* tiles, int ntiles,
-                              int grid)
+static const void* bp_fn_s(const BpShape& s)
 {
-    if (s.r == 2 && s.kc == 16) tjb::k_job_bp<T, float, 2, 16, 2, MODE><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
-    else if (s.r == 2) tjb::k_job_bp<T, float, 2, 8, 4, MODE><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
-    else if (s.r == 8 && s.kc == 16) tjb::k_job_bp<T, float, 8, 16, 2, MODE><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
-    else if (s.r == 8) tjb::k_job_bp<T, float, 8, 8, 2, MODE><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
-    else if (s.kc == 4) tjb::k_job_bp<T, float, 4, 4, 4, MODE><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
-    else if (s.kc == 16) tjb::k_job_bp<T, float, 4, 16, 4, MODE><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
-    else tjb::k_job_bp<T, float, 4, 8, 4, MODE><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
-    return hipGetLastError();
+    if (s.r == 2 && s.kc == 16) return (const void*)tjb::k_job_bp<T, float, 2, 16, 2, MODE>;
+    if (s.r == 2) return (const void*)tjb::k_job_bp<T, float, 2, 8, 4, MODE>;
+    if (s.r == 8 && s.kc == 16) return (const void*)tjb::k_job_bp<T, float, 8, 16, 2, MODE>;
+    if (s.r == 8) return (const void*)tjb::k_job_bp<T, float, 8, 8, 2, MODE>;
+    if (s.kc == 4) return (const void*)tjb::k_job_bp<T, float, 4, 4, 4, MODE>;
+    if (s.kc == 16) return (const void*)tjb::k_job_bp<T, float, 4, 16, 4, MODE>;
+    return (const void*)tjb::k_job_bp<T, float, 4, 8, 4, MODE>;
 }
 
 // 16-bit storage always takes the FP32-cell-position form (mode bit 2)
 template <typename T>
-static hipError_t launch_bp_m(hipStream_t st, int mode, const BpShape& s, const TjJob* jobs, const int4* tiles,
-                              int ntiles, int grid)
+static const void* bp_fn_m(int mode, const BpShape& s)
 {
     switch (mode) {
-    case 0: return launch_bp_s<T, 0>(st, s, jobs, tiles, ntiles, grid);
-    case 1: return launch_bp_s<T, 1>(st, s, jobs, tiles, ntiles, grid);
-    case 4: return launch_bp_s<T, 4>(st, s, jobs, tiles, ntiles, grid);
-    case 5: return launch_bp_s<T, 5>(st, s, jobs, tiles, ntiles, grid);
-    case 6: return launch_bp_s<T, 6>(st, s, jobs, tiles, ntiles, grid);
-    default: return launch_bp_s<T, 7>(st, s, jobs, tiles, ntiles, grid);
+    case 0: return bp_fn_s<T, 0>(s);
+    case 1: return bp_fn_s<T, 1>(s);
+    case 4: return bp_fn_s<T, 4>(s);
+    case 5: return bp_fn_s<T, 5>(s);
+    case 6: return bp_fn_s<T, 6>(s);
+    default: return bp_fn_s<T, 7>(s);
     }
 }
 
 // mode: bit 0 positivity, bit 1 FP32 master volume (16-bit storage only)
-static hipError_t launch_bp(hipStream_t st, int storage, int mode, const tj_batch_tuning& tu, const TjJob* jobs,
-                            const int4* tiles, int ntiles, int grid)
+static Kern bp_kern(int storage, int mode, const tj_batch_tuning& tu)
 {
-    (void)hipGetLastError();
     const BpShape s = bp_shape_of(tu, storage);
+    Kern k;
     switch (storage) {
-    case TJ_STORAGE_F32: return launch_bp_m<float>(st, mode & 1, s, jobs, tiles, ntiles, grid);
-    case TJ_STORAGE_BF16: return launch_bp_m<Bf16>(st, mode | 4, s, jobs, tiles, ntiles, grid);
-    case TJ_STORAGE_F16: return launch_bp_m<_Float16>(st, mode | 4, s, jobs, tiles, ntiles, grid);
+    case TJ_STORAGE_F32: k.fn = bp_fn_m<float>(mode & 1, s); break;
+    case TJ_STORAGE_BF16: k.fn = bp_fn_m<Bf16>(mode | 4, s); break;
+    case TJ_STORAGE_F16: k.fn = bp_fn_m<_Float16>(mode | 4, s); break;
     default:
-        if (mode & 1) tjb::k_job_bp<double, double, 2, 8, 2, 1><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
-        else tjb::k_job_bp<double, double, 2, 8, 2, 0><<<grid, tjb::NT, 0, st>>>(jobs, tiles, ntiles);
-        return hipGetLastError();
+        k.fn = (mode & 1) ? (const void*)tjb::k_job_bp<double, double, 2, 8, 2, 1>
+                          : (const void*)tjb::k_job_bp<double, double, 2, 8, 2, 0>;
     }
+    return k;
+}
+
+// One persistent launch of a resolved kernel over a tile table.
+static hipError_t launch_kern(hipStream_t st, const Kern& k, int grid, const TjJob* jobs, const int4* tiles,
+                              int ntiles)
+{
+    int cells = k.cells;
+    void* args[] = {(void*)&jobs, (void*)&tiles, (void*)&ntiles, (void*)&cells};
+    (void)hipGetLastError();
+    return hipLaunchKernel(k.fn, dim3(grid), dim3(k.block), args, (size_t)k.lds, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1571,10 +1614,16 @@ static std::vector<int4> bp_tiles(const std::vector<BJob*>& js, const std::vecto
     return interleave(groups);
 }
 
-static int grid_for(const tj_batch* b, int ntiles)
+// Persistent grid of a launch: blocks_per_cu blocks per CU, at most as many as
+// the CU holds at once (blocks beyond those would run as a second, partial
+// round and leave the first round's CUs idle at the end), at most one per tile,
+// and a multiple of 8 (tile t runs on XCD t % 8).
+static int grid_for(const tj_batch* b, int ntiles, const Kern& k)
 {
-    const int want = std::max(1, std::min(b->tune.blocks_per_cu, 16)) * b->ncu;
-    int g = std::min(ntiles, want);
+    int per_cu = std::max(1, std::min(b->tune.blocks_per_cu, 16));
+    const int nb = resident_blocks(k);
+    if (nb > 0) per_cu = std::min(per_cu, nb);
+    int g = std::min(ntiles, per_cu * b->ncu);
     g = ((g + 7) / 8) * 8;
     return std::max(g, 8);
 }
@@ -1681,7 +1730,8 @@ static int build_plan(tj_batch* b, const std::vector<int>& ids)
             if (int rc = upload(ft, &dft)) return rc;
             int max_w = 0;
             for (BJob* j : fj) max_w = std::max(max_w, j->d.vol_w);
-            fwd.push_back({0, s, max_w, (int)ft.size(), grid_for(b, (int)ft.size()), dft, 0, 0, ps});
+            const Kern k = fp_kern(s, ps, b->tune, max_w, b->lds_cu);
+            fwd.push_back({0, s, max_w, (int)ft.size(), grid_for(b, (int)ft.size(), k), dft, 0, 0, ps, k});
         }
         for (int mode = 0; mode <= 3; ++mode) {
             std::vector<BJob*> pj;
@@ -1695,7 +1745,8 @@ static int build_plan(tj_batch* b, const std::vector<int>& ids)
             std::vector<int4> bt = bp_tiles(pj, pidx, bp_shape_of(b->tune, s));
             int4* dbt = nullptr;
             if (int rc = upload(bt, &dbt)) return rc;
-            back.push_back({2, s, mode, (int)bt.size(), grid_for(b, (int)bt.size()), dbt, 0, 0, s});
+            const Kern k = bp_kern(s, mode, b->tune);
+            back.push_back({2, s, mode, (int)bt.size(), grid_for(b, (int)bt.size(), k), dbt, 0, 0, s, k});
         }
     }
     p.launches = fwd;
@@ -1719,9 +1770,7 @@ static int build_plan(tj_batch* b, const std::vector<int>& ids)
 static hipError_t enqueue_launch(tj_batch* b, const RunPlan::L& l)
 {
     const RunPlan& p = *b->cur;
-    if (l.kind == 0)
-        return launch_fp(b->stream, l.storage, l.pstorage, p.tune, p.djobs, l.tiles, l.ntiles, l.grid, l.pos);
-    if (l.kind == 2) return launch_bp(b->stream, l.storage, l.pos, p.tune, p.djobs, l.tiles, l.ntiles, l.grid);
+    if (l.kind == 0 || l.kind == 2) return launch_kern(b->stream, l.k, l.grid, p.djobs, l.tiles, l.ntiles);
     (void)hipGetLastError();
     if (l.kind == 3) {
         tjb::k_job_pairs<<<dim3(2048, l.njobs), tjb::NT, 0, b->stream>>>(p.djobs + l.job0);
@@ -1819,6 +1868,10 @@ int tj_batch_open(int device, size_t hbm_budget, tj_batch** out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
         b->ncu = prop.multiProcessorCount;
+    int lds_cu = 0;
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) == hipSuccess &&
+        lds_cu > 0)
+        b->lds_cu = lds_cu;
     if (allow_fp_lds() != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
         delete b;
@@ -2227,7 +2280,7 @@ int tj_batch_errors(tj_batch* b, int id, double* err, int cap)
 int tj_batch_set_tuning(tj_batch* b, const tj_batch_tuning* t)
 {
     if (!b || !t) return bfail(TJ_EINVAL, "null argument");
-    if (t->fp_rays < 1 || t->fp_lds_kb < 16 || t->fp_lds_kb > 128 || t->bp_rows < 1 || t->bp_zchunk < 1 ||
+    if (t->fp_rays < 1 || t->fp_lds_kb < 16 || t->fp_lds_kb > 152 || t->bp_rows < 1 || t->bp_zchunk < 1 ||
         t->bp_group < 1 || t->blocks_per_cu < 1 || t->blocks_per_cu > 16 || t->fp_packed < 0 || t->fp_packed > 1 ||
         t->fp_cols < 0 || t->fp_cols > 2)
         return bfail(TJ_EINVAL, "tj_batch_set_tuning: value out of range");
