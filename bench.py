@@ -77,7 +77,7 @@ def kernel_bytes(w):
 # names them, grouped as tj_timing groups them.  The backprojection group
 # launches both forms; the row table picks one on the device and the other
 # returns at once.
-ITERATION_KERNELS = {"fp": "k_fp_lds", "res": "k_err_fold", "bpprep": ("k_bp_rows", "k_bp_pairs"),
+ITERATION_KERNELS = {"fp": "k_fp_lds", "res": "k_err_fold", "bpprep": "k_bp_prep",
                      "bp": ("k_bp_fast", "k_bp_lds")}
 
 

@@ -506,6 +506,9 @@ typedef struct {
                        position: 0 off, 1..7 its shape (rows x slices x images:
                        1 4x8x2, 2 4x8x1, 3 8x8x1, 4 4x16x1, 5 2x16x2, 6 8x4x2,
                        7 4x4x2)                                                  */
+    int fp_groups;  /* kernel 3: groups of fp_rays projections per block, 1, 2 or
+                       4 (one 256-thread part each, sharing one staged slab);
+                       0 automatic: 4 for launches of up to 4 groups, else 2  */
 } tj_tuning;
 int tj_set_tuning(tj_ctx *ctx, const tj_tuning *t);
 int tj_get_tuning(tj_ctx *ctx, tj_tuning *t);

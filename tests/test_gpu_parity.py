@@ -399,6 +399,12 @@ TUNINGS = [
     dict(fp_kernel=3, fp_rows=1, fp_rays=2, fp_zchunk=4, bp_kernel=3, bp_rows=2, bp_zchunk=16, bp_group=4),
     dict(fp_kernel=3, fp_rows=1, fp_rays=4, fp_zchunk=0, bp_kernel=3, bp_rows=4, bp_zchunk=4, bp_group=2),
     dict(fp_kernel=1, bp_kernel=1),
+    # projection groups sharing one staged slab (parts without projections in
+    # the last set of a launch)
+    dict(fp_kernel=3, fp_rays=2, fp_zchunk=4, fp_groups=4, bp_kernel=3),
+    dict(fp_kernel=3, fp_rays=4, fp_zchunk=0, fp_groups=2, bp_kernel=3),
+    dict(fp_kernel=3, fp_rays=4, fp_zchunk=3, fp_groups=1, bp_kernel=3),
+    dict(fp_kernel=3, fp_rays=4, fp_zchunk=5, fp_groups=4, bp_kernel=3, bp_fast=0),
 ]
 
 

@@ -19,7 +19,7 @@
 
 namespace tjk {
 hipError_t fp(hipStream_t, const TjGeomDev&, const float*, const TjProj*, const int*, int, int, int,
-              const uint8_t*, const float*, float*, float*, int, int, int);
+              const uint8_t*, const float*, float*, float*, int, int, int, int gb = 1);
 hipError_t norm_table(hipStream_t, const TjGeomDev&, const TjProj*, int, float*);
 void set_max_lds(int);
 hipError_t norm_max_fill(hipStream_t, int, int, float*);
@@ -28,10 +28,10 @@ hipError_t residual(hipStream_t, int, int, int, const float*, const int*, const 
 hipError_t row_flags(hipStream_t, int, int, int, const float*, const int*, uint8_t*);
 hipError_t fp_residual(hipStream_t, const TjGeomDev&, const float*, const TjProj*, const int*, int, int, const float*,
                        int, int, const float*, double, float*, double*, uint8_t*, uint8_t*, double*,
-                       hipEvent_t mid = nullptr);
+                       hipEvent_t mid = nullptr, int gb = 1, int* clear = nullptr);
 hipError_t bp(hipStream_t, const TjGeomDev&, float*, const TjProj*, const int*, int, const float*, const int*,
               const uint8_t*, TjBpRow*, int, int, int, int, int, int, int, double2*, hipEvent_t mid = nullptr,
-              int* slow = nullptr, int fast = 0);
+              int* slow = nullptr, int fast = 0, int slow_cleared = 0);
 hipError_t zyx_to_yzx(hipStream_t, const float*, float*, int, int, int, int);
 hipError_t yzx_to_zyx(hipStream_t, const float*, float*, int, int, int, int);
 hipError_t plane_flags(hipStream_t, const float*, size_t, int, int, uint8_t*);
@@ -188,7 +188,7 @@ struct tj_ctx {
     int bpairs_cap = 0;
 
     // kernel selection and shapes (tj_set_tuning)
-    tj_tuning tune{0, 2, 4, 4, 0, 4, 8, 3, 1};
+    tj_tuning tune{0, 2, 4, 4, 0, 4, 8, 3, 1, 0};
     int* bpslow = nullptr;      // [2]: k_bp_rows' flag (a row the fast backprojection cannot take),
                                 // and the count of launches the fast backprojection took
 
@@ -468,7 +468,7 @@ static int do_project(tj_ctx* c, const int* slots, const std::vector<int>& hslot
     const int start = fp_start_row(c, start_y);
     const int kernel = pick_fp_kernel(c, hslots, start);
     TJ_HIP(tjk::fp(c->stream, c->g, c->vol, c->pp, slots, nslot, start, kernel, c->pflag, c->norm_tab, c->sproj,
-                   c->snorm, c->tune.fp_rows, c->tune.fp_rays, c->tune.fp_zchunk));
+                   c->snorm, c->tune.fp_rows, c->tune.fp_rays, c->tune.fp_zchunk, c->tune.fp_groups));
     if (c->geo.long_object_comp) TJ_HIP(tjk::norm_max_fill(c->stream, (int)c->npx, nslot, c->snorm));
     return TJ_OK;
 }
@@ -1328,6 +1328,8 @@ int tj_set_tuning(tj_ctx* c, const tj_tuning* t)
         t->bp_group < 1)
         return fail(TJ_EINVAL, "tj_set_tuning: non-positive shape");
     if (t->bp_fast < 0 || t->bp_fast > 7) return fail(TJ_EINVAL, "tj_set_tuning: bp_fast out of range");
+    if (t->fp_groups != 0 && t->fp_groups != 1 && t->fp_groups != 2 && t->fp_groups != 4)
+        return fail(TJ_EINVAL, "tj_set_tuning: fp_groups must be 0, 1, 2 or 4");
     c->tune = *t;
     return TJ_OK;
 }
@@ -1898,18 +1900,21 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
                 const int start = fp_start_row(c, start_y);
                 const int fk = pick_fp_kernel(c, hs, start);
                 const int bk = pick_bp_kernel(c, hs);
+                bool slow_cleared = false;  // the fold zeroed the backprojection's row flag
                 if (fk == 3 && !c->geo.long_object_comp) {
                     // projectAndDifference in one pass: the LDS forward kernel
                     // writes the differences and the row errors itself
                     ++tm.fused_subsets;
                     TJ_HIP(tjk::fp_residual(c->stream, c->g, c->vol, c->pp, slots, diffIndex, start, c->norm_tab,
                                             c->tune.fp_rays, c->tune.fp_zchunk, c->exps, (double)factorimg,
-                                            c->sdiff, c->row_err, c->rbad, c->rowflag, c->err + off, b));
+                                            c->sdiff, c->row_err, c->rbad, c->rowflag, c->err + off, b,
+                                            c->tune.fp_groups, c->bpslow));
+                    slow_cleared = true;
                     if (c->timing_on) TJ_HIP(hipEventRecord(r, c->stream));
                 } else {
                     TJ_HIP(tjk::fp(c->stream, c->g, c->vol, c->pp, slots, diffIndex, start, fk, c->pflag,
                                    c->norm_tab, c->sproj, c->snorm, c->tune.fp_rows, c->tune.fp_rays,
-                                   c->tune.fp_zchunk));
+                                   c->tune.fp_zchunk, c->tune.fp_groups));
                     if (c->geo.long_object_comp)
                         TJ_HIP(tjk::norm_max_fill(c->stream, (int)c->npx, diffIndex, c->snorm));
                     if (c->timing_on) TJ_HIP(hipEventRecord(b, c->stream));
@@ -1920,7 +1925,8 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
                     if ((rc = ensure_bpairs(c, diffIndex))) return rc;
                 TJ_HIP(tjk::bp(c->stream, c->g, bp_target(c), c->pp, slots, diffIndex, c->sdiff, c->iota,
                                c->rowflag, c->bprows, start_y, end_y, bk, c->tune.bp_rows, c->tune.bp_zchunk,
-                               c->tune.bp_group, bp_finite(c), c->bpairs, q, c->bpslow, c->tune.bp_fast));
+                               c->tune.bp_group, bp_finite(c), c->bpairs, q, c->bpslow, c->tune.bp_fast,
+                               slow_cleared ? 1 : 0));
                 if ((rc = do_error_backproject(c, slots, hs, start_y, end_y))) return rc;
                 if (c->timing_on) {
                     TJ_HIP(hipEventRecord(d, c->stream));

@@ -321,12 +321,11 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_row_flags(int pw, int ph, const fl
 
 // Row table for the single-axis kernel: yy = ((x*e10 + y*e11) + z*e12)*scale + ciy
 // reduces to (y*e11)*scale + ciy when e10 == e12 == 0.
-__global__ void k_bp_rows(TjGeomDev g, const TjProj* __restrict__ pp, const int* __restrict__ slot_proj,
-                          int nslot, const uint8_t* __restrict__ rowflag, TjBpRow* __restrict__ rows,
-                          int* __restrict__ slow)
+__device__ __forceinline__ void bp_row(const TjGeomDev& g, const TjProj* __restrict__ pp,
+                                       const int* __restrict__ slot_proj, int nslot,
+                                       const uint8_t* __restrict__ rowflag, TjBpRow* __restrict__ rows,
+                                       int* __restrict__ slow, int j, int s)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = blockIdx.y;
     if (j >= g.vh || s >= nslot) return;
     const TjProj& Q = pp[slot_proj[s]];
     const double y = j - g.vcy;
@@ -350,6 +349,13 @@ __global__ void k_bp_rows(TjGeomDev g, const TjProj* __restrict__ pp, const int*
         if (slow && !(r.shortcut && r.finite)) atomicOr(slow, 1);
     }
     rows[(size_t)s * g.vh + j] = r;
+}
+
+__global__ void k_bp_rows(TjGeomDev g, const TjProj* __restrict__ pp, const int* __restrict__ slot_proj,
+                          int nslot, const uint8_t* __restrict__ rowflag, TjBpRow* __restrict__ rows,
+                          int* __restrict__ slow)
+{
+    bp_row(g, pp, slot_proj, nslot, rowflag, rows, slow, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
 }
 
 template <bool POS>
@@ -473,19 +479,35 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_bp_generic(
 // neighbour (the last column repeated) -- the two operands of ImageJ's
 // interpolation along x, in the cells' order, so the backprojection stages
 // them with plain buffer-to-LDS copies.
-__global__ __launch_bounds__(TJ_BLOCK) void k_bp_pairs(const float* __restrict__ images,
-                                                       const int* __restrict__ image_index, int pw, int ph,
-                                                       double2* __restrict__ pairs)
+// Pair images of slot s: blocks b0 = 0 .. nb - 1 of a grid-stride loop.
+__device__ __forceinline__ void bp_pairs(const float* __restrict__ images, const int* __restrict__ image_index,
+                                         int pw, int ph, double2* __restrict__ pairs, int s, int b0, int nb)
 {
-    const int s = blockIdx.y;
     const float* img = images + (size_t)image_index[s] * pw * ph;
     double2* out = pairs + (size_t)s * pw * ph;
     const size_t npx = (size_t)pw * ph;
-    for (size_t k = (size_t)blockIdx.x * TJ_BLOCK + threadIdx.x; k < npx; k += (size_t)gridDim.x * TJ_BLOCK) {
+    for (size_t k = (size_t)b0 * TJ_BLOCK + threadIdx.x; k < npx; k += (size_t)nb * TJ_BLOCK) {
         const int x = (int)(k % pw);
         const double ll = img[k], lr = img[x + 1 < pw ? k + 1 : k];
         out[k] = make_double2(ll, lr - ll);
     }
+}
+
+// The backprojection's preparation in one launch: blocks [0, nrb) of each slot
+// (grid y) build its row table (k_bp_rows), the others its pair images
+// (k_bp_pairs).  The two parts are independent.
+__global__ __launch_bounds__(TJ_BLOCK) void k_bp_prep(TjGeomDev g, const TjProj* __restrict__ pp,
+                                                      const int* __restrict__ slot_proj, int nslot,
+                                                      const uint8_t* __restrict__ rowflag,
+                                                      TjBpRow* __restrict__ rows, int* __restrict__ slow, int nrb,
+                                                      const float* __restrict__ images,
+                                                      const int* __restrict__ image_index,
+                                                      double2* __restrict__ pairs)
+{
+    if ((int)blockIdx.x < nrb)
+        bp_row(g, pp, slot_proj, nslot, rowflag, rows, slow, blockIdx.x * TJ_BLOCK + threadIdx.x, blockIdx.y);
+    else
+        bp_pairs(images, image_index, g.pw, g.ph, pairs, blockIdx.y, blockIdx.x - nrb, gridDim.x - nrb);
 }
 
 // Backprojection: a 256-thread block owns 256 voxel columns x, R volume rows and
@@ -923,7 +945,7 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_norm_table(TjGeomDev g, const TjPr
 // blocks sharing one row group run on one XCD back to back and share its L2.
 // NG: the rays of the thread that belong to projections of the launch (the
 // last group of a launch can hold fewer than G); only they are marched.
-template <int G, int NG, bool NEARY>
+template <int G, int NG, bool NEARY, int GB>
 __device__ __forceinline__ void fp_lds_chunks(const TjGeomDev& g, const float* __restrict__ vol, float4* lds4,
                                               int zc, int rowbase, int ns, const float (&ym)[4], double (&xc)[G],
                                               const double (&dirx)[G], float4 (&rs)[G])
@@ -945,12 +967,13 @@ __device__ __forceinline__ void fp_lds_chunks(const TjGeomDev& g, const float* _
     for (int z0 = 0; z0 < g.vd; z0 += zc) {
         const int zn = min(zc, g.vd - z0);
         __syncthreads();
-        // wave w stages slices w, w+4, ...; lanes take consecutive columns
-        // (32-bit offsets within a plane: the plane D*W is far below 2^31)
+        // wave w stages slices w, w + (waves of the block), ...; lanes take
+        // consecutive columns (32-bit offsets within a plane: the plane D*W is
+        // far below 2^31)
         // SU columns per lane per batch: 4*SU loads in flight before the first
         // LDS write.  Rows outside the volume read row 0 (a valid address) and
         // store 0; columns past W read column W-1.
-        for (int zz = wave; zz < zn; zz += TJ_BLOCK / 64) {
+        for (int zz = wave; zz < zn; zz += TJ_BLOCK * GB / 64) {
             const int zo = (z0 + zz) * g.vw;
             float4* dst = lds4 + zz * W1;
             for (int x0 = 0; x0 < W1; x0 += 64 * TJ_SU) {
@@ -1021,14 +1044,19 @@ struct TjFpFuse {
     uint8_t* rbad;      // [nslot][ph][ntile]
 };
 
-template <int G, bool FUSE>
-__global__ __launch_bounds__(TJ_BLOCK) void k_fp_lds(
+// GB: groups of G projections per block, one 256-thread part each; the parts
+// share the staged slab (the slab is read from L2 once per GB G projections).
+// The last set of a launch may hold parts without projections: they stage and
+// wait at the barriers like the others and march no ray.
+template <int G, bool FUSE, int GB>
+__global__ __launch_bounds__(TJ_BLOCK * GB) void k_fp_lds(
     TjGeomDev g, const float* __restrict__ vol, const TjProj* __restrict__ pp,
     const int* __restrict__ slot_proj, int nslot, int start_row, const float* __restrict__ norm_tab, int zc,
     int nrb, int nsub, int ntile, float* __restrict__ proj, float* __restrict__ norm, TjFpFuse fu)
 {
     extern __shared__ float4 lds4[];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x & (TJ_BLOCK - 1);   // column of the tile
+    const int part = GB > 1 ? (int)threadIdx.x / TJ_BLOCK : 0;
     const int b = blockIdx.x;
     const int xcd = b & 7, local = b >> 3;
     const int sub = local % nsub;
@@ -1036,13 +1064,16 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_fp_lds(
     if (rb >= nrb) return;  // whole block
     const int tile = sub % ntile, grp = sub / ntile;
     const int i = tile * TJ_BLOCK + tid;
-    const int s0 = grp * G;
-    const int ns = min(G, nslot - s0);
+    const int s0 = (grp * GB + part) * G;
+    const int ns = min(G, nslot - s0);  // <= 0: a part without projections
+    const int s0c = min(s0, nslot - 1);
     const int j0 = rb * 4;
     const bool col = i < g.pw;
     const double ic = ((col ? i : g.pw - 1) - g.pc0) / g.scale;
 
-    const TjProj& P0 = pp[slot_proj[s0]];
+    // the rows a block stages do not depend on the projection (the host admits
+    // only launches whose projections share the row mapping)
+    const TjProj& P0 = pp[slot_proj[s0c]];
     bool ok[4];
     float ym[4];  // 0 for the last volume row (nearest value), 1 otherwise
     int rowbase = 0;
@@ -1076,7 +1107,7 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_fp_lds(
 #pragma unroll
     for (int k = 0; k < G; ++k) {
         rs[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-        const TjProj& Q = pp[slot_proj[s0 + min(k, ns - 1)]];
+        const TjProj& Q = pp[slot_proj[min(s0 + min(k, max(ns, 1) - 1), nslot - 1)]];
         double ox = ic - Q.yfx * oldy_any - Q.zfx * (-1 - g.vcz);
         ox /= Q.divx;
         // rays beyond the launch's projections stay far outside the volume
@@ -1084,13 +1115,13 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_fp_lds(
         dirx[k] = (k < ns) ? Q.dirx : 0.0;
     }
     if (have_row) {
-        if (tid < 2) lds4[zc * (g.vw + 1) + tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (threadIdx.x < 2) lds4[zc * (g.vw + 1) + threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
         auto run = [&](auto ng) {
             constexpr int NG = decltype(ng)::value;
             if (neary)
-                fp_lds_chunks<G, NG, true>(g, vol, lds4, zc, rowbase, ns, ym, xc, dirx, rs);
+                fp_lds_chunks<G, NG, true, GB>(g, vol, lds4, zc, rowbase, ns, ym, xc, dirx, rs);
             else
-                fp_lds_chunks<G, NG, false>(g, vol, lds4, zc, rowbase, ns, ym, xc, dirx, rs);
+                fp_lds_chunks<G, NG, false, GB>(g, vol, lds4, zc, rowbase, ns, ym, xc, dirx, rs);
         };
         // the smallest supported count of marched rays that covers ns
         if (ns >= G) run(std::integral_constant<int, G>{});
@@ -1101,9 +1132,10 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_fp_lds(
         else run(std::integral_constant<int, G>{});
     }
     if constexpr (FUSE) {
-        __shared__ double red[TJ_BLOCK / 64][G];
-        __shared__ unsigned redb[TJ_BLOCK / 64][G * 4];
-        const int lane = tid & 63, wv = tid >> 6;
+        __shared__ double red[TJ_BLOCK * GB / 64][G];
+        __shared__ unsigned redb[TJ_BLOCK * GB / 64][G * 4];
+        const int lane = tid & 63, wv = (int)threadIdx.x >> 6;  // wave of the block
+        const int w0 = part * (TJ_BLOCK / 64);                   // first wave of the part
         const size_t npx = (size_t)g.pw * g.ph;
 #pragma unroll
         for (int k = 0; k < G; ++k) {
@@ -1153,13 +1185,13 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_fp_lds(
             if (j < g.ph) {
                 unsigned b = 0;
 #pragma unroll
-                for (int w = 0; w < TJ_BLOCK / 64; ++w) b |= redb[w][tid];
+                for (int w = 0; w < TJ_BLOCK / 64; ++w) b |= redb[w0 + w][tid];
                 fu.rbad[((size_t)(s0 + k) * g.ph + j) * ntile + tile] = (uint8_t)b;
             }
             if (r == 0) {
                 double sum = 0.0;
 #pragma unroll
-                for (int w = 0; w < TJ_BLOCK / 64; ++w) sum += red[w][k];
+                for (int w = 0; w < TJ_BLOCK / 64; ++w) sum += red[w0 + w][k];
                 fu.rerr[((size_t)(s0 + k) * nrb + rb) * ntile + tile] = sum;
             }
         }
@@ -1950,7 +1982,7 @@ static int g_max_lds = 65536;
 
 void set_max_lds(int bytes) { g_max_lds = bytes; }
 
-template <int G, bool FUSE = false>
+template <int G, bool FUSE = false, int GB = 1>
 static hipError_t launch_fp_lds(hipStream_t st, const TjGeomDev& g, const float* vol, const TjProj* pp,
                                 const int* slot_proj, int nslot, int start_row, const float* norm_tab, int zc,
                                 float* proj, float* norm, TjFpFuse fu = TjFpFuse{})
@@ -1958,7 +1990,7 @@ static hipError_t launch_fp_lds(hipStream_t st, const TjGeomDev& g, const float*
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     const int per = (g.vw + 1) * (int)sizeof(float4);
     // the fused form's reduction arrays are static LDS beside the staged slices
-    const int fixed = FUSE ? (TJ_BLOCK / 64) * G * (int)(sizeof(double) + 4 * sizeof(unsigned)) : 0;
+    const int fixed = FUSE ? (TJ_BLOCK * GB / 64) * G * (int)(sizeof(double) + 4 * sizeof(unsigned)) : 0;
     int zmax = (g_max_lds - 2 * (int)sizeof(float4) - fixed) / per;
     if (zmax < 1) return hipErrorInvalidValue;
     if (zc <= 0 || zc > zmax) zc = zmax;
@@ -1966,17 +1998,36 @@ static hipError_t launch_fp_lds(hipStream_t st, const TjGeomDev& g, const float*
     const size_t lds = (size_t)(zc * (g.vw + 1) + 2) * sizeof(float4);
     const int nrb = (g.ph + 3) / 4;
     const int ntile = (g.pw + TJ_BLOCK - 1) / TJ_BLOCK;
-    const int ngrp = (nslot + G - 1) / G;
+    const int ngrp = (nslot + G * GB - 1) / (G * GB);
     const int nsub = ntile * ngrp;
     const int nblk = ((nrb + 7) / 8) * 8 * nsub;
     if (lds + fixed > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_fp_lds<G, FUSE>,
+        hipError_t e = hipFuncSetAttribute((const void*)k_fp_lds<G, FUSE, GB>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    k_fp_lds<G, FUSE><<<nblk, TJ_BLOCK, lds, st>>>(g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nrb,
-                                                    nsub, ntile, proj, norm, fu);
+    k_fp_lds<G, FUSE, GB><<<nblk, TJ_BLOCK * GB, lds, st>>>(g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc,
+                                                             nrb, nsub, ntile, proj, norm, fu);
     return hipGetLastError();
+}
+
+// The LDS forward kernel with gb groups of G projections per block (1, 2 or 4;
+// G = 8 takes 1).  gb = 0 picks: a launch of at most 4 groups (an OS-SART
+// subset) in one block per row group and column tile, so the slab is read
+// once; more groups (SIRT) in pairs, which keeps two blocks per CU.
+template <int G, bool FUSE = false>
+static hipError_t launch_fp_lds_gb(int gb, hipStream_t st, const TjGeomDev& g, const float* vol, const TjProj* pp,
+                                   const int* slot_proj, int nslot, int start_row, const float* norm_tab, int zc,
+                                   float* proj, float* norm, TjFpFuse fu = TjFpFuse{})
+{
+    if (gb <= 0) gb = (nslot + G - 1) / G <= 4 ? 4 : 2;
+    if constexpr (G <= 4) {
+        if (gb >= 4)
+            return launch_fp_lds<G, FUSE, 4>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm, fu);
+        if (gb == 2)
+            return launch_fp_lds<G, FUSE, 2>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm, fu);
+    }
+    return launch_fp_lds<G, FUSE, 1>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm, fu);
 }
 
 hipError_t norm_table(hipStream_t st, const TjGeomDev& g, const TjProj* pp, int nproj, float* tab)
@@ -1990,7 +2041,7 @@ hipError_t norm_table(hipStream_t st, const TjGeomDev& g, const TjProj* pp, int 
 // kernel: 1 generic, 2 axis (global loads), 3 axis with LDS staging
 hipError_t fp(hipStream_t st, const TjGeomDev& g, const float* vol, const TjProj* pp, const int* slot_proj,
               int nslot, int start_row, int kernel, const uint8_t* pflag, const float* norm_tab, float* proj,
-              float* norm, int rows, int rpt, int zc)
+              float* norm, int rows, int rpt, int zc, int gb)
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nslot <= 0) return hipSuccess;
@@ -2000,9 +2051,11 @@ hipError_t fp(hipStream_t st, const TjGeomDev& g, const float* vol, const TjProj
         return hipGetLastError();
     }
     if (kernel == 3) {
-        if (rpt <= 2) return launch_fp_lds<2>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm);
-        if (rpt <= 4) return launch_fp_lds<4>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm);
-        return launch_fp_lds<8>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm);
+        if (rpt <= 2)
+            return launch_fp_lds_gb<2>(gb, st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm);
+        if (rpt <= 4)
+            return launch_fp_lds_gb<4>(gb, st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm);
+        return launch_fp_lds_gb<8>(gb, st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm);
     }
     switch (rows) {
     case 1:
@@ -2042,12 +2095,16 @@ hipError_t residual(hipStream_t st, int pw, int ph, int nslot, const float* exps
 
 // Per slot: the sum of the error partials of its row blocks and column tiles
 // (in that order), and each row's non-finite flag over its column tiles.
+// clear (or null): an int zeroed on the way (the backprojection's row flag,
+// so that the next launch needs no memset of its own)
 __global__ __launch_bounds__(TJ_BLOCK) void k_err_fold(int ph, int ntile, const double* __restrict__ rerr,
                                                        const uint8_t* __restrict__ rbad,
-                                                       uint8_t* __restrict__ rowflag, double* __restrict__ err)
+                                                       uint8_t* __restrict__ rowflag, double* __restrict__ err,
+                                                       int* __restrict__ clear)
 {
     __shared__ double sh[TJ_BLOCK];
     const int s = blockIdx.x;
+    if (clear && s == 0 && threadIdx.x == 0) *clear = 0;
     const int nq = ((ph + 3) / 4) * ntile;
     double acc = 0.0;
     for (int q = threadIdx.x; q < nq; q += blockDim.x) acc += rerr[(size_t)s * nq + q];
@@ -2070,18 +2127,24 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_err_fold(int ph, int ntile, const 
 hipError_t fp_residual(hipStream_t st, const TjGeomDev& g, const float* vol, const TjProj* pp, const int* slot_proj,
                        int nslot, int start_row, const float* norm_tab, int rpt, int zc, const float* exps,
                        double factor, float* diff, double* rerr, uint8_t* rbad, uint8_t* rowflag, double* err,
-                       hipEvent_t mid)
+                       hipEvent_t mid, int gb, int* clear)
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nslot <= 0) return hipSuccess;
     const TjFpFuse fu{exps, factor, diff, rerr, rbad};
     hipError_t e;
-    if (rpt <= 2) e = launch_fp_lds<2, true>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nullptr, nullptr, fu);
-    else if (rpt <= 4) e = launch_fp_lds<4, true>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nullptr, nullptr, fu);
-    else e = launch_fp_lds<8, true>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nullptr, nullptr, fu);
+    if (rpt <= 2)
+        e = launch_fp_lds_gb<2, true>(gb, st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nullptr, nullptr,
+                                      fu);
+    else if (rpt <= 4)
+        e = launch_fp_lds_gb<4, true>(gb, st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nullptr, nullptr,
+                                      fu);
+    else
+        e = launch_fp_lds_gb<8, true>(gb, st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nullptr, nullptr,
+                                      fu);
     if (e != hipSuccess) return e;
     if (mid && (e = hipEventRecord(mid, st)) != hipSuccess) return e;
-    k_err_fold<<<nslot, TJ_BLOCK, 0, st>>>(g.ph, (g.pw + TJ_BLOCK - 1) / TJ_BLOCK, rerr, rbad, rowflag, err);
+    k_err_fold<<<nslot, TJ_BLOCK, 0, st>>>(g.ph, (g.pw + TJ_BLOCK - 1) / TJ_BLOCK, rerr, rbad, rowflag, err, clear);
     return hipGetLastError();
 }
 
@@ -2097,14 +2160,16 @@ hipError_t row_flags(hipStream_t st, int pw, int ph, int nslot, const float* ima
 template <int R, int KC, int G>
 static hipError_t launch_bp_lds(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp,
                                 const int* slot_proj, int nslot, const float* images, const int* image_index,
-                                const TjBpRow* rows, int start_y, int end_y, int finite_volume, double2* pairs,
-                                hipEvent_t mid, const int* slow, int fast)
+                                const uint8_t* rowflag, TjBpRow* rows, int start_y, int end_y, int finite_volume,
+                                double2* pairs, hipEvent_t mid, int* slow, int fast)
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     const size_t npx = (size_t)g.pw * g.ph;
     const bool fast_on = slow && fast > 0 && finite_volume && !g.positivity;
-    k_bp_pairs<<<dim3((unsigned)std::min<size_t>((npx + TJ_BLOCK - 1) / TJ_BLOCK, 1024), nslot), TJ_BLOCK, 0, st>>>(
-        images, image_index, g.pw, g.ph, pairs);
+    const int nrb = (g.vh + TJ_BLOCK - 1) / TJ_BLOCK;
+    const unsigned npb = (unsigned)std::min<size_t>((npx + TJ_BLOCK - 1) / TJ_BLOCK, 1024);
+    k_bp_prep<<<dim3(nrb + npb, nslot), TJ_BLOCK, 0, st>>>(g, pp, slot_proj, nslot, rowflag, rows, slow, nrb, images,
+                                                           image_index, pairs);
     if (mid) {
         const hipError_t e = hipEventRecord(mid, st);
         if (e != hipSuccess) return e;
@@ -2142,7 +2207,7 @@ static hipError_t launch_bp_lds(hipStream_t st, const TjGeomDev& g, float* vol, 
 hipError_t bp(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp, const int* slot_proj, int nslot,
               const float* images, const int* image_index, const uint8_t* rowflag, TjBpRow* rows, int start_y,
               int end_y, int kernel, int rows_per_thread, int kchunk, int group, int finite_volume,
-              double2* pairs, hipEvent_t mid, int* slow, int fast)
+              double2* pairs, hipEvent_t mid, int* slow, int fast, int slow_cleared)
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nslot <= 0 || end_y <= start_y) return hipSuccess;
@@ -2161,17 +2226,18 @@ hipError_t bp(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp, 
             k_bp_generic<false><<<dim3(bx, ny, g.vd), TJ_BLOCK, 0, st>>>(g, vol, pp, slot_proj, nslot, images, image_index, start_y);
         return hipGetLastError();
     }
-    // `slow` (two device ints, or null): slow[0] set by k_bp_rows when a row
-    // needs the general kernel, slow[1] counts the launches the fast kernel
-    // took; only the LDS path with a fast shape uses them
+    // `slow` (two device ints, or null): slow[0] set by the row table when a
+    // row needs the general kernel, slow[1] counts the launches the fast kernel
+    // took; only the LDS path with a fast shape uses them.  slow[0] is cleared
+    // here unless the caller's previous launch on the stream (k_err_fold)
+    // cleared it.
     int* slow_flag = (kernel == 3 && fast > 0 && finite_volume && !g.positivity) ? slow : nullptr;
-    if (slow_flag) {
+    if (slow_flag && !slow_cleared) {
         const hipError_t e = hipMemsetAsync(slow_flag, 0, sizeof(int), st);
         if (e != hipSuccess) return e;
     }
-    k_bp_rows<<<dim3((g.vh + 255) / 256, nslot), 256, 0, st>>>(g, pp, slot_proj, nslot, rowflag, rows, slow_flag);
     if (kernel == 3) {
-#define TJ_BPL(RR, KK, GG) return launch_bp_lds<RR, KK, GG>(st, g, vol, pp, slot_proj, nslot, images, image_index, rows, start_y, end_y, finite_volume, pairs, mid, slow_flag, fast)
+#define TJ_BPL(RR, KK, GG) return launch_bp_lds<RR, KK, GG>(st, g, vol, pp, slot_proj, nslot, images, image_index, rowflag, rows, start_y, end_y, finite_volume, pairs, mid, slow_flag, fast)
         if (rows_per_thread == 2) {
             if (kchunk >= 16) { if (group >= 4) TJ_BPL(2, 16, 4); TJ_BPL(2, 16, 2); }
             if (group >= 8) TJ_BPL(2, 8, 8);
@@ -2185,6 +2251,7 @@ hipError_t bp(hipStream_t st, const TjGeomDev& g, float* vol, const TjProj* pp, 
         TJ_BPL(4, 8, 2);
 #undef TJ_BPL
     }
+    k_bp_rows<<<dim3((g.vh + 255) / 256, nslot), 256, 0, st>>>(g, pp, slot_proj, nslot, rowflag, rows, nullptr);
     if (mid) {
         const hipError_t e = hipEventRecord(mid, st);
         if (e != hipSuccess) return e;

@@ -23,7 +23,7 @@ namespace tjk {
 hipError_t norm_table(hipStream_t, const TjGeomDev&, const TjProj*, int, float*);
 hipError_t fp_residual(hipStream_t, const TjGeomDev&, const float*, const TjProj*, const int*, int, int, const float*,
                        int, int, const float*, double, float*, double*, uint8_t*, uint8_t*, double*,
-                       hipEvent_t mid = nullptr);
+                       hipEvent_t mid = nullptr, int gb = 1, int* clear = nullptr);
 hipError_t plane_flags(hipStream_t, const float*, size_t, int, int, uint8_t*);
 void set_max_lds(int);
 }  // namespace tjk
@@ -196,7 +196,7 @@ extern "C" int tj_project_and_difference_noalloc(int device, const tj_geometry* 
     }
     TJ_NA_HIP(tjk::norm_table(st, g, d_pp, n, d_norm));
     TJ_NA_HIP(tjk::fp_residual(st, g, vol_yzx, d_pp, d_slot, n, start, d_norm, 4, 4, measured, factor, diff, d_rerr,
-                               d_rbad, d_rowflag, d_err));
+                               d_rbad, d_rowflag, d_err, nullptr, 0));
     TJ_NA_HIP(hipMemcpyAsync(errors, d_err, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     TJ_NA_HIP(hipStreamSynchronize(st));
     return TJ_OK;
