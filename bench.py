@@ -326,6 +326,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="also run a plain device copy of BYTES (HBM counter calibration)")
     ap.add_argument("--no-forms", action="store_true", help="skip the precision/batched/serving forms")
+    ap.add_argument("--no-serving", action="store_true", help="forms: skip the serving runs")
     ap.add_argument("--precision-study", type=int, nargs=4, metavar=("W", "H", "D", "N"),
                     help="run only the precision study (30 + 2 iterations, 80 ellipsoids, 2%% noise)")
     ap.add_argument("--modes", nargs="+", default=None,
@@ -479,6 +480,7 @@ def main():
             out["batched"][key] = forms.batched_run(64, 15, st, master)
             log(f"[bench] batched 64 jobs {key}: {out['batched'][key]['ms']:.1f} ms for 15 iterations, "
                 f"{out['batched'][key]['gups']:.0f} GUPS")
+    if not args.no_forms and not args.no_serving:
         # the serving configuration: 200 requests of a seeded Poisson stream at
         # 20/s (more work than one GPU does, so latency grows with queueing),
         # then 120 requests at 0.8 x the capacity that run measured (jobs per
