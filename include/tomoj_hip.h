@@ -508,7 +508,11 @@ typedef struct {
                        7 4x4x2)                                                  */
     int fp_groups;  /* kernel 3: groups of fp_rays projections per block, 1, 2 or
                        4 (one 256-thread part each, sharing one staged slab);
-                       0 automatic: 4 for launches of up to 4 groups, else 2  */
+                       0 automatic: 4 with the double-buffered slab, else 4 for
+                       launches of up to 4 groups and 2 for longer ones       */
+    int fp_prefetch; /* kernel 3's slab staging: 0 double-buffered where it fits
+                       (the next chunk's loads in flight while the current one
+                       is projected), 1 a single buffer                        */
 } tj_tuning;
 int tj_set_tuning(tj_ctx *ctx, const tj_tuning *t);
 int tj_get_tuning(tj_ctx *ctx, tj_tuning *t);

@@ -405,6 +405,11 @@ TUNINGS = [
     dict(fp_kernel=3, fp_rays=4, fp_zchunk=0, fp_groups=2, bp_kernel=3),
     dict(fp_kernel=3, fp_rays=4, fp_zchunk=3, fp_groups=1, bp_kernel=3),
     dict(fp_kernel=3, fp_rays=4, fp_zchunk=5, fp_groups=4, bp_kernel=3, bp_fast=0),
+    # the single-buffer slab staging (the rows above use the double-buffered
+    # form wherever it fits: partial last chunks, one to four parts)
+    dict(fp_kernel=3, fp_rays=4, fp_zchunk=5, fp_groups=4, bp_kernel=3, fp_prefetch=1),
+    dict(fp_kernel=3, fp_rays=2, fp_zchunk=0, fp_groups=1, bp_kernel=3, fp_prefetch=1),
+    dict(fp_kernel=3, fp_rays=8, fp_zchunk=2, fp_groups=1, bp_kernel=3, fp_prefetch=1),
 ]
 
 

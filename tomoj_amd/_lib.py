@@ -65,7 +65,7 @@ class tj_timing(C.Structure):
 class tj_tuning(C.Structure):
     _fields_ = [("fp_kernel", C.c_int), ("fp_rows", C.c_int), ("fp_rays", C.c_int), ("fp_zchunk", C.c_int),
                 ("bp_kernel", C.c_int), ("bp_rows", C.c_int), ("bp_zchunk", C.c_int), ("bp_group", C.c_int),
-                ("bp_fast", C.c_int), ("fp_groups", C.c_int)]
+                ("bp_fast", C.c_int), ("fp_groups", C.c_int), ("fp_prefetch", C.c_int)]
 
 
 class tj_job_desc(C.Structure):

@@ -19,7 +19,7 @@
 
 namespace tjk {
 hipError_t fp(hipStream_t, const TjGeomDev&, const float*, const TjProj*, const int*, int, int, int,
-              const uint8_t*, const float*, float*, float*, int, int, int, int gb = 1);
+              const uint8_t*, const float*, float*, float*, int, int, int, int gb = 1, bool pf = true);
 hipError_t norm_table(hipStream_t, const TjGeomDev&, const TjProj*, int, float*);
 void set_max_lds(int);
 hipError_t norm_max_fill(hipStream_t, int, int, float*);
@@ -28,7 +28,7 @@ hipError_t residual(hipStream_t, int, int, int, const float*, const int*, const 
 hipError_t row_flags(hipStream_t, int, int, int, const float*, const int*, uint8_t*);
 hipError_t fp_residual(hipStream_t, const TjGeomDev&, const float*, const TjProj*, const int*, int, int, const float*,
                        int, int, const float*, double, float*, double*, uint8_t*, uint8_t*, double*,
-                       hipEvent_t mid = nullptr, int gb = 1, int* clear = nullptr);
+                       hipEvent_t mid = nullptr, int gb = 1, int* clear = nullptr, bool pf = true);
 hipError_t bp(hipStream_t, const TjGeomDev&, float*, const TjProj*, const int*, int, const float*, const int*,
               const uint8_t*, TjBpRow*, int, int, int, int, int, int, int, double2*, hipEvent_t mid = nullptr,
               int* slow = nullptr, int fast = 0, int slow_cleared = 0);
@@ -188,7 +188,7 @@ struct tj_ctx {
     int bpairs_cap = 0;
 
     // kernel selection and shapes (tj_set_tuning)
-    tj_tuning tune{0, 2, 4, 4, 0, 4, 8, 3, 1, 0};
+    tj_tuning tune{0, 2, 4, 5, 0, 4, 8, 3, 1, 0, 0};
     int* bpslow = nullptr;      // [2]: k_bp_rows' flag (a row the fast backprojection cannot take),
                                 // and the count of launches the fast backprojection took
 
@@ -468,7 +468,8 @@ static int do_project(tj_ctx* c, const int* slots, const std::vector<int>& hslot
     const int start = fp_start_row(c, start_y);
     const int kernel = pick_fp_kernel(c, hslots, start);
     TJ_HIP(tjk::fp(c->stream, c->g, c->vol, c->pp, slots, nslot, start, kernel, c->pflag, c->norm_tab, c->sproj,
-                   c->snorm, c->tune.fp_rows, c->tune.fp_rays, c->tune.fp_zchunk, c->tune.fp_groups));
+                   c->snorm, c->tune.fp_rows, c->tune.fp_rays, c->tune.fp_zchunk, c->tune.fp_groups,
+                   c->tune.fp_prefetch == 0));
     if (c->geo.long_object_comp) TJ_HIP(tjk::norm_max_fill(c->stream, (int)c->npx, nslot, c->snorm));
     return TJ_OK;
 }
@@ -1330,6 +1331,8 @@ int tj_set_tuning(tj_ctx* c, const tj_tuning* t)
     if (t->bp_fast < 0 || t->bp_fast > 7) return fail(TJ_EINVAL, "tj_set_tuning: bp_fast out of range");
     if (t->fp_groups != 0 && t->fp_groups != 1 && t->fp_groups != 2 && t->fp_groups != 4)
         return fail(TJ_EINVAL, "tj_set_tuning: fp_groups must be 0, 1, 2 or 4");
+    if (t->fp_prefetch != 0 && t->fp_prefetch != 1)
+        return fail(TJ_EINVAL, "tj_set_tuning: fp_prefetch must be 0 or 1");
     c->tune = *t;
     return TJ_OK;
 }
@@ -1908,13 +1911,13 @@ int tj_run_ossart(tj_ctx* c, int iterations, double lambda, int update_nb, int f
                     TJ_HIP(tjk::fp_residual(c->stream, c->g, c->vol, c->pp, slots, diffIndex, start, c->norm_tab,
                                             c->tune.fp_rays, c->tune.fp_zchunk, c->exps, (double)factorimg,
                                             c->sdiff, c->row_err, c->rbad, c->rowflag, c->err + off, b,
-                                            c->tune.fp_groups, c->bpslow));
+                                            c->tune.fp_groups, c->bpslow, c->tune.fp_prefetch == 0));
                     slow_cleared = true;
                     if (c->timing_on) TJ_HIP(hipEventRecord(r, c->stream));
                 } else {
                     TJ_HIP(tjk::fp(c->stream, c->g, c->vol, c->pp, slots, diffIndex, start, fk, c->pflag,
                                    c->norm_tab, c->sproj, c->snorm, c->tune.fp_rows, c->tune.fp_rays,
-                                   c->tune.fp_zchunk, c->tune.fp_groups));
+                                   c->tune.fp_zchunk, c->tune.fp_groups, c->tune.fp_prefetch == 0));
                     if (c->geo.long_object_comp)
                         TJ_HIP(tjk::norm_max_fill(c->stream, (int)c->npx, diffIndex, c->snorm));
                     if (c->timing_on) TJ_HIP(hipEventRecord(b, c->stream));

@@ -945,6 +945,123 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_norm_table(TjGeomDev g, const TjPr
 // blocks sharing one row group run on one XCD back to back and share its L2.
 // NG: the rays of the thread that belong to projections of the launch (the
 // last group of a launch can hold fewer than G); only they are marched.
+//
+// One slice of rays: the same arithmetic for both staging forms below.
+template <int G, int NG, bool NEARY>
+__device__ __forceinline__ void fp_lds_slice(const TjGeomDev& g, const float4* lds4, int zo, int zero_idx, float zl,
+                                             const float (&ym)[4], double (&xc)[G], const double (&dirx)[G],
+                                             float4 (&rs)[G])
+{
+    const double nacc = -TJ_ACCURACY;  // x + ACCURACY >= 0  <=>  x >= -ACCURACY
+    const double bx = g.bxmax;
+    const double wm1 = g.vw - 1;
+#pragma unroll
+    for (int k = 0; k < NG; ++k) {
+        xc[k] = xc[k] + dirx[k];
+        const double xv = xc[k] + g.vcx;
+        const bool in = (xv >= nacc) && (xv < bx);
+        // inside, xv is in [-ACCURACY, W): clamping to [0, W-1] keeps xb and,
+        // for xb < W-1, the fraction; at xb = W-1 both taps are equal (the
+        // replicated column) so the fraction does not matter
+        const double xs = fmin(fmax(xv, 0.0), wm1);
+        const int xb = (int)xs;
+        const float xf = ((float)xs - (float)xb) * zl;
+        const int a = in ? zo + xb : zero_idx;
+        const float4 q0 = lds4[a];
+        const float4 q1 = lds4[a + 1];
+        if (NEARY) {
+            rs[k].x += q0.x + (xf * ym[0]) * (q1.x - q0.x);
+            rs[k].y += q0.y + (xf * ym[1]) * (q1.y - q0.y);
+            rs[k].z += q0.z + (xf * ym[2]) * (q1.z - q0.z);
+            rs[k].w += q0.w + (xf * ym[3]) * (q1.w - q0.w);
+        } else {
+            rs[k].x += q0.x + xf * (q1.x - q0.x);
+            rs[k].y += q0.y + xf * (q1.y - q0.y);
+            rs[k].z += q0.z + xf * (q1.z - q0.z);
+            rs[k].w += q0.w + xf * (q1.w - q0.w);
+        }
+    }
+}
+
+// Double-buffered staging (PF): the slab alternates between two LDS buffers of
+// zc * (W + 1) + 2 cells (the slab and its two zero cells).  The loads of
+// chunk c + 1 are issued into registers before chunk c is projected and
+// stored into the other buffer after it, so their latency hides behind the
+// rays and a chunk needs one barrier instead of two.  Thread t owns the cells
+// t + q * (threads of the block), q < TJ_PFQ, of every chunk: the host picks zc
+// with zc * (W + 1) <= TJ_PFQ * threads.  The cells and the arithmetic are the
+// single-buffer form's, so the sums are identical.
+#define TJ_PFQ 4
+template <int G, int NG, bool NEARY, int GB>
+__device__ __forceinline__ void fp_lds_chunks_pf(const TjGeomDev& g, const float* __restrict__ vol, float4* lds4,
+                                                 int zc, int rowbase, const float (&ym)[4], double (&xc)[G],
+                                                 const double (&dirx)[G], float4 (&rs)[G])
+{
+    constexpr int NT = TJ_BLOCK * GB;
+    const int tid = threadIdx.x;
+    const size_t plane = (size_t)g.vd * g.vw;
+    const int W1 = g.vw + 1;
+    const int bufsz = zc * W1 + 2;
+    const int zero_idx = zc * W1;
+    const bool rin0 = rowbase + 0 >= 0 && rowbase + 0 < g.vh, rin1 = rowbase + 1 >= 0 && rowbase + 1 < g.vh;
+    const bool rin2 = rowbase + 2 >= 0 && rowbase + 2 < g.vh, rin3 = rowbase + 3 >= 0 && rowbase + 3 < g.vh;
+    const float* r0 = vol + (size_t)(rin0 ? rowbase + 0 : 0) * plane;
+    const float* r1 = vol + (size_t)(rin1 ? rowbase + 1 : 0) * plane;
+    const float* r2 = vol + (size_t)(rin2 ? rowbase + 2 : 0) * plane;
+    const float* r3 = vol + (size_t)(rin3 ? rowbase + 3 : 0) * plane;
+    // the thread's cells: slice zq within the chunk, offset oq within the
+    // chunk's planes (columns past W read column W-1); zq = zc for none
+    int zq[TJ_PFQ], oq[TJ_PFQ];
+#pragma unroll
+    for (int q = 0; q < TJ_PFQ; ++q) {
+        const int e = tid + q * NT;
+        const int z = e / W1;
+        zq[q] = min(z, zc);
+        oq[q] = z * g.vw + min(e - z * W1, g.vw - 1);
+    }
+    float v[TJ_PFQ][4];
+    auto load = [&](int z0) {
+        const int zn = min(zc, g.vd - z0);
+        const int zo = z0 * g.vw;
+#pragma unroll
+        for (int q = 0; q < TJ_PFQ; ++q)
+            if (zq[q] < zn) {
+                const int o = zo + oq[q];
+                v[q][0] = r0[o];
+                v[q][1] = r1[o];
+                v[q][2] = r2[o];
+                v[q][3] = r3[o];
+            }
+    };
+    auto store = [&](float4* dst, int z0) {
+        const int zn = min(zc, g.vd - z0);
+#pragma unroll
+        for (int q = 0; q < TJ_PFQ; ++q)
+            if (zq[q] < zn)
+                dst[tid + q * NT] = make_float4(rin0 ? v[q][0] : 0.0f, rin1 ? v[q][1] : 0.0f,
+                                                rin2 ? v[q][2] : 0.0f, rin3 ? v[q][3] : 0.0f);
+    };
+    load(0);
+    store(lds4, 0);
+    __syncthreads();
+    int buf = 0;
+    for (int z0 = 0; z0 < g.vd; z0 += zc) {
+        const int zn = min(zc, g.vd - z0);
+        const bool more = z0 + zc < g.vd;  // uniform
+        if (more) load(z0 + zc);
+        const float4* cur = lds4 + buf * bufsz;
+        // the last slice takes the nearest value (getPixel's z == D-1 rule)
+        const int zlast = more ? zn : zn - 1;
+        for (int zz = 0; zz < zn; ++zz)
+            fp_lds_slice<G, NG, NEARY>(g, cur, zz * W1, zero_idx, (zz == zlast) ? 0.0f : 1.0f, ym, xc, dirx, rs);
+        if (more) {
+            store(lds4 + (buf ^ 1) * bufsz, z0 + zc);
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+}
+
 template <int G, int NG, bool NEARY, int GB>
 __device__ __forceinline__ void fp_lds_chunks(const TjGeomDev& g, const float* __restrict__ vol, float4* lds4,
                                               int zc, int rowbase, int ns, const float (&ym)[4], double (&xc)[G],
@@ -954,9 +1071,6 @@ __device__ __forceinline__ void fp_lds_chunks(const TjGeomDev& g, const float* _
     const size_t plane = (size_t)g.vd * g.vw;
     const int W1 = g.vw + 1;
     const int zero_idx = zc * W1;
-    const double nacc = -TJ_ACCURACY;  // x + ACCURACY >= 0  <=>  x >= -ACCURACY
-    const double bx = g.bxmax;
-    const double wm1 = g.vw - 1;
     const bool rin0 = rowbase + 0 >= 0 && rowbase + 0 < g.vh, rin1 = rowbase + 1 >= 0 && rowbase + 1 < g.vh;
     const bool rin2 = rowbase + 2 >= 0 && rowbase + 2 < g.vh, rin3 = rowbase + 3 >= 0 && rowbase + 3 < g.vh;
     const float* r0 = vol + (size_t)(rin0 ? rowbase + 0 : 0) * plane;
@@ -999,36 +1113,8 @@ __device__ __forceinline__ void fp_lds_chunks(const TjGeomDev& g, const float* _
         // the last slice takes the nearest value (getPixel's z == D-1 rule): its
         // fraction is multiplied by 0
         const int zlast = (z0 + zn == g.vd) ? zn - 1 : zn;
-        for (int zz = 0; zz < zn; ++zz) {
-            const float zl = (zz == zlast) ? 0.0f : 1.0f;
-            const int zo = zz * W1;
-#pragma unroll
-            for (int k = 0; k < NG; ++k) {
-                xc[k] = xc[k] + dirx[k];
-                const double xv = xc[k] + g.vcx;
-                const bool in = (xv >= nacc) && (xv < bx);
-                // inside, xv is in [-ACCURACY, W): clamping to [0, W-1] keeps xb and,
-                // for xb < W-1, the fraction; at xb = W-1 both taps are equal (the
-                // replicated column) so the fraction does not matter
-                const double xs = fmin(fmax(xv, 0.0), wm1);
-                const int xb = (int)xs;
-                const float xf = ((float)xs - (float)xb) * zl;
-                const int a = in ? zo + xb : zero_idx;
-                const float4 q0 = lds4[a];
-                const float4 q1 = lds4[a + 1];
-                if (NEARY) {
-                    rs[k].x += q0.x + (xf * ym[0]) * (q1.x - q0.x);
-                    rs[k].y += q0.y + (xf * ym[1]) * (q1.y - q0.y);
-                    rs[k].z += q0.z + (xf * ym[2]) * (q1.z - q0.z);
-                    rs[k].w += q0.w + (xf * ym[3]) * (q1.w - q0.w);
-                } else {
-                    rs[k].x += q0.x + xf * (q1.x - q0.x);
-                    rs[k].y += q0.y + xf * (q1.y - q0.y);
-                    rs[k].z += q0.z + xf * (q1.z - q0.z);
-                    rs[k].w += q0.w + xf * (q1.w - q0.w);
-                }
-            }
-        }
+        for (int zz = 0; zz < zn; ++zz)
+            fp_lds_slice<G, NG, NEARY>(g, lds4, zz * W1, zero_idx, (zz == zlast) ? 0.0f : 1.0f, ym, xc, dirx, rs);
     }
 }
 
@@ -1048,7 +1134,7 @@ struct TjFpFuse {
 // share the staged slab (the slab is read from L2 once per GB G projections).
 // The last set of a launch may hold parts without projections: they stage and
 // wait at the barriers like the others and march no ray.
-template <int G, bool FUSE, int GB>
+template <int G, bool FUSE, int GB, bool PF = false>
 __global__ __launch_bounds__(TJ_BLOCK * GB) void k_fp_lds(
     TjGeomDev g, const float* __restrict__ vol, const TjProj* __restrict__ pp,
     const int* __restrict__ slot_proj, int nslot, int start_row, const float* __restrict__ norm_tab, int zc,
@@ -1115,13 +1201,27 @@ __global__ __launch_bounds__(TJ_BLOCK * GB) void k_fp_lds(
         dirx[k] = (k < ns) ? Q.dirx : 0.0;
     }
     if (have_row) {
-        if (threadIdx.x < 2) lds4[zc * (g.vw + 1) + threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+        // the zero cells after the slab (of both buffers with PF)
+        if (PF) {
+            if (threadIdx.x < 4)
+                lds4[(threadIdx.x >> 1) * (zc * (g.vw + 1) + 2) + zc * (g.vw + 1) + (threadIdx.x & 1)] =
+                    make_float4(0.f, 0.f, 0.f, 0.f);
+        } else if (threadIdx.x < 2) {
+            lds4[zc * (g.vw + 1) + threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
         auto run = [&](auto ng) {
             constexpr int NG = decltype(ng)::value;
-            if (neary)
-                fp_lds_chunks<G, NG, true, GB>(g, vol, lds4, zc, rowbase, ns, ym, xc, dirx, rs);
-            else
-                fp_lds_chunks<G, NG, false, GB>(g, vol, lds4, zc, rowbase, ns, ym, xc, dirx, rs);
+            if constexpr (PF) {
+                if (neary)
+                    fp_lds_chunks_pf<G, NG, true, GB>(g, vol, lds4, zc, rowbase, ym, xc, dirx, rs);
+                else
+                    fp_lds_chunks_pf<G, NG, false, GB>(g, vol, lds4, zc, rowbase, ym, xc, dirx, rs);
+            } else {
+                if (neary)
+                    fp_lds_chunks<G, NG, true, GB>(g, vol, lds4, zc, rowbase, ns, ym, xc, dirx, rs);
+                else
+                    fp_lds_chunks<G, NG, false, GB>(g, vol, lds4, zc, rowbase, ns, ym, xc, dirx, rs);
+            }
         };
         // the smallest supported count of marched rays that covers ns
         if (ns >= G) run(std::integral_constant<int, G>{});
@@ -1985,49 +2085,66 @@ void set_max_lds(int bytes) { g_max_lds = bytes; }
 template <int G, bool FUSE = false, int GB = 1>
 static hipError_t launch_fp_lds(hipStream_t st, const TjGeomDev& g, const float* vol, const TjProj* pp,
                                 const int* slot_proj, int nslot, int start_row, const float* norm_tab, int zc,
-                                float* proj, float* norm, TjFpFuse fu = TjFpFuse{})
+                                float* proj, float* norm, TjFpFuse fu, bool pf)
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
-    const int per = (g.vw + 1) * (int)sizeof(float4);
+    const int W1 = g.vw + 1;
+    const int per = W1 * (int)sizeof(float4);
     // the fused form's reduction arrays are static LDS beside the staged slices
     const int fixed = FUSE ? (TJ_BLOCK * GB / 64) * G * (int)(sizeof(double) + 4 * sizeof(unsigned)) : 0;
-    int zmax = (g_max_lds - 2 * (int)sizeof(float4) - fixed) / per;
-    if (zmax < 1) return hipErrorInvalidValue;
-    if (zc <= 0 || zc > zmax) zc = zmax;
+    // double-buffered form: two slabs, each thread stages at most TJ_PFQ cells
+    // of a chunk; used where two or more slices fit
+    if (pf) {
+        const int zpf = min(((g_max_lds - fixed) / 2 - 2 * (int)sizeof(float4)) / per, TJ_PFQ * TJ_BLOCK * GB / W1);
+        pf = zpf >= 2 && g.vd >= 2;
+        if (pf && (zc <= 0 || zc > zpf)) zc = zpf;
+    }
+    if (!pf) {
+        const int zmax = (g_max_lds - 2 * (int)sizeof(float4) - fixed) / per;
+        if (zmax < 1) return hipErrorInvalidValue;
+        if (zc <= 0 || zc > zmax) zc = zmax;
+    }
     if (zc > g.vd) zc = g.vd;
-    const size_t lds = (size_t)(zc * (g.vw + 1) + 2) * sizeof(float4);
+    const size_t lds = (size_t)(pf ? 2 : 1) * (zc * W1 + 2) * sizeof(float4);
     const int nrb = (g.ph + 3) / 4;
     const int ntile = (g.pw + TJ_BLOCK - 1) / TJ_BLOCK;
     const int ngrp = (nslot + G * GB - 1) / (G * GB);
     const int nsub = ntile * ngrp;
     const int nblk = ((nrb + 7) / 8) * 8 * nsub;
-    if (lds + fixed > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void*)k_fp_lds<G, FUSE, GB>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
-    k_fp_lds<G, FUSE, GB><<<nblk, TJ_BLOCK * GB, lds, st>>>(g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc,
-                                                             nrb, nsub, ntile, proj, norm, fu);
-    return hipGetLastError();
+    auto go = [&](auto kern) -> hipError_t {
+        if (lds + fixed > 65536) {
+            hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        kern<<<nblk, TJ_BLOCK * GB, lds, st>>>(g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nrb, nsub,
+                                               ntile, proj, norm, fu);
+        return hipGetLastError();
+    };
+    return pf ? go(k_fp_lds<G, FUSE, GB, true>) : go(k_fp_lds<G, FUSE, GB, false>);
 }
 
 // The LDS forward kernel with gb groups of G projections per block (1, 2 or 4;
-// G = 8 takes 1).  gb = 0 picks: a launch of at most 4 groups (an OS-SART
-// subset) in one block per row group and column tile, so the slab is read
-// once; more groups (SIRT) in pairs, which keeps two blocks per CU.
+// G = 8 takes 1).  gb = 0 picks 4 with the double-buffered slab (pf; one block
+// per CU hides its own load latency, and 1024 threads stage up to 7 slices of
+// a 512-column slab); with a single buffer, a launch of at most 4 groups (an
+// OS-SART subset) in one block per row group and column tile, so the slab is
+// read once, and more groups (SIRT) in pairs, which keeps two blocks per CU to
+// overlap one block's staging with the other's rays.
 template <int G, bool FUSE = false>
 static hipError_t launch_fp_lds_gb(int gb, hipStream_t st, const TjGeomDev& g, const float* vol, const TjProj* pp,
                                    const int* slot_proj, int nslot, int start_row, const float* norm_tab, int zc,
-                                   float* proj, float* norm, TjFpFuse fu = TjFpFuse{})
+                                   float* proj, float* norm, TjFpFuse fu, bool pf)
 {
-    if (gb <= 0) gb = (nslot + G - 1) / G <= 4 ? 4 : 2;
+    if (gb <= 0) gb = (pf || (nslot + G - 1) / G <= 4) ? 4 : 2;
     if constexpr (G <= 4) {
         if (gb >= 4)
-            return launch_fp_lds<G, FUSE, 4>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm, fu);
+            return launch_fp_lds<G, FUSE, 4>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm, fu,
+                                             pf);
         if (gb == 2)
-            return launch_fp_lds<G, FUSE, 2>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm, fu);
+            return launch_fp_lds<G, FUSE, 2>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm, fu,
+                                             pf);
     }
-    return launch_fp_lds<G, FUSE, 1>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm, fu);
+    return launch_fp_lds<G, FUSE, 1>(st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm, fu, pf);
 }
 
 hipError_t norm_table(hipStream_t st, const TjGeomDev& g, const TjProj* pp, int nproj, float* tab)
@@ -2041,7 +2158,7 @@ hipError_t norm_table(hipStream_t st, const TjGeomDev& g, const TjProj* pp, int 
 // kernel: 1 generic, 2 axis (global loads), 3 axis with LDS staging
 hipError_t fp(hipStream_t st, const TjGeomDev& g, const float* vol, const TjProj* pp, const int* slot_proj,
               int nslot, int start_row, int kernel, const uint8_t* pflag, const float* norm_tab, float* proj,
-              float* norm, int rows, int rpt, int zc, int gb)
+              float* norm, int rows, int rpt, int zc, int gb, bool pf)
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nslot <= 0) return hipSuccess;
@@ -2052,10 +2169,13 @@ hipError_t fp(hipStream_t st, const TjGeomDev& g, const float* vol, const TjProj
     }
     if (kernel == 3) {
         if (rpt <= 2)
-            return launch_fp_lds_gb<2>(gb, st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm);
+            return launch_fp_lds_gb<2>(gb, st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm,
+                                          TjFpFuse{}, pf);
         if (rpt <= 4)
-            return launch_fp_lds_gb<4>(gb, st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm);
-        return launch_fp_lds_gb<8>(gb, st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm);
+            return launch_fp_lds_gb<4>(gb, st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm,
+                                          TjFpFuse{}, pf);
+        return launch_fp_lds_gb<8>(gb, st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, proj, norm,
+                                          TjFpFuse{}, pf);
     }
     switch (rows) {
     case 1:
@@ -2127,7 +2247,7 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_err_fold(int ph, int ntile, const 
 hipError_t fp_residual(hipStream_t st, const TjGeomDev& g, const float* vol, const TjProj* pp, const int* slot_proj,
                        int nslot, int start_row, const float* norm_tab, int rpt, int zc, const float* exps,
                        double factor, float* diff, double* rerr, uint8_t* rbad, uint8_t* rowflag, double* err,
-                       hipEvent_t mid, int gb, int* clear)
+                       hipEvent_t mid, int gb, int* clear, bool pf)
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nslot <= 0) return hipSuccess;
@@ -2135,13 +2255,13 @@ hipError_t fp_residual(hipStream_t st, const TjGeomDev& g, const float* vol, con
     hipError_t e;
     if (rpt <= 2)
         e = launch_fp_lds_gb<2, true>(gb, st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nullptr, nullptr,
-                                      fu);
+                                      fu, pf);
     else if (rpt <= 4)
         e = launch_fp_lds_gb<4, true>(gb, st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nullptr, nullptr,
-                                      fu);
+                                      fu, pf);
     else
         e = launch_fp_lds_gb<8, true>(gb, st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nullptr, nullptr,
-                                      fu);
+                                      fu, pf);
     if (e != hipSuccess) return e;
     if (mid && (e = hipEventRecord(mid, st)) != hipSuccess) return e;
     k_err_fold<<<nslot, TJ_BLOCK, 0, st>>>(g.ph, (g.pw + TJ_BLOCK - 1) / TJ_BLOCK, rerr, rbad, rowflag, err, clear);
