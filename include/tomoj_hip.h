@@ -566,7 +566,10 @@ typedef struct {
     int blocks_per_cu;  /* persistent blocks per compute unit: 1..16, capped at
                            what one CU holds of each kernel                      */
     int fp_packed;      /* BF16/FP16 storage: forward projection on packed 16-bit
-                           pairs with v_dot2 (8 rows per tile) instead of FP32    */
+                           pairs with v_dot2 (8 rows per tile) instead of FP32;
+                           2: the same on a double-buffered slab shared by four
+                           projection groups per 1024-thread block (even widths
+                           up to 2048; other launches take form 1)            */
     int fp_cols;        /* packed forward: 256-column tiles per block, 1 or 2 (two
                            share one staged slab)                                 */
 } tj_batch_tuning;

@@ -172,15 +172,20 @@ def test_half_storage_awkward_shapes(batch, oracle, case):
     exps = oracle.phantom_project(ell, E, pw, ph).astype(np.float32)
     if pos:
         exps -= np.float32(0.5 * exps.mean())
-    with batch.JobEngine(GIB) as eng:
-        jid = eng.add(batch.JobSpec(W, H, D, tilts, storage=storage, positivity=pos, fp32_master=True,
-                                    fp32_projections=True, pw=pw, ph=ph))
-        eng.set_projections(jid, exps)
-        eng.run([jid], 10)
-        eng.convert(jid, "f32")
-        eng.set_projections(jid, exps)
-        eng.run([jid], 2)
-        got = eng.get_volume(jid)
+    vols = []
+    for packed in (1, 2):   # both packed forward kernels (form 2 only takes even widths)
+        with batch.JobEngine(GIB) as eng:
+            eng.set_tuning(fp_packed=packed)
+            jid = eng.add(batch.JobSpec(W, H, D, tilts, storage=storage, positivity=pos, fp32_master=True,
+                                        fp32_projections=True, pw=pw, ph=ph))
+            eng.set_projections(jid, exps)
+            eng.run([jid], 10)
+            eng.convert(jid, "f32")
+            eng.set_projections(jid, exps)
+            eng.run([jid], 2)
+            vols.append(eng.get_volume(jid))
+    assert np.array_equal(vols[0], vols[1])
+    got = vols[1]
     g = oracle.make_geom(W, H, D, pw, ph, positivity=pos, nthreads=8)
     ref = np.zeros((D, H, W), np.float64)
     oracle.ossart(g, ref, exps, E, 12, 1.0, len(E), first_index=0, f64=True)
@@ -202,11 +207,14 @@ def test_half_tile_shapes_do_not_change_results(batch, oracle):
               dict(fp_rays=2, fp_lds_kb=32, bp_rows=8, bp_zchunk=16, blocks_per_cu=4),
               dict(fp_rays=4, fp_lds_kb=48, bp_rows=4, bp_zchunk=4, blocks_per_cu=3),
               dict(fp_rays=4, fp_lds_kb=64, fp_cols=2, bp_rows=4, bp_zchunk=16, blocks_per_cu=4),
-              dict(fp_rays=2, fp_lds_kb=32, fp_cols=2, bp_rows=8, bp_zchunk=8, blocks_per_cu=2)]
+              dict(fp_rays=2, fp_lds_kb=32, fp_cols=2, bp_rows=8, bp_zchunk=8, blocks_per_cu=2),
+              # the double-buffered packed kernel (four projection groups per block)
+              dict(fp_packed=2, fp_rays=2, fp_lds_kb=152, bp_rows=4, bp_zchunk=8, blocks_per_cu=1),
+              dict(fp_packed=2, fp_rays=2, fp_lds_kb=64, bp_rows=4, bp_zchunk=8, blocks_per_cu=2)]
     results = []
     for sh in shapes:
         with batch.JobEngine(GIB) as eng:
-            eng.set_tuning(fp_packed=1, **sh)
+            eng.set_tuning(**{"fp_packed": 1, **sh})
             jid = eng.add(batch.JobSpec(W, H, D, tilts, storage="bf16", fp32_master=True, fp32_projections=True))
             eng.set_projections(jid, exps)
             eng.run([jid], 3)

@@ -537,6 +537,76 @@ __device__ __forceinline__ void fpp_slice_f64(const uint4* cell, int hp, int hb,
     }
 }
 
+// Projects one staged slab of the packed forward kernels: slices z0 .. z0 +
+// zn - 1 of the tile's G rays.  cell: the slab's first cell (rows 0-3; rows
+// 4-7 at + hp), ss cells per slice, odd cells from hb on.  Interpolated slices
+// take FP32 positions rounded to the weight grid; the last volume slice (z0 +
+// zn == D) takes FP64 positions with getPixel's nearest value.
+template <typename T, int G>
+__device__ __forceinline__ void fpp_project(const uint4* cell, int hp, int ss, int hb, int z0, int zn, int D, int W,
+                                            unsigned ymask, const float (&base1)[G], const float (&tn)[G],
+                                            const double* prm, int p0, int n, double ic, const int (&zlo)[G],
+                                            const int (&zhi)[G], float (&rs)[G][8])
+{
+    const int zlast = (z0 + zn == D) ? zn - 1 : zn;
+    const int zend = min(zlast, zn);
+    const float wc1 = (float)(W + 1);
+    // interpolated slices: FP32 positions; a last-row row (ymask) takes
+    // getPixel's nearest value, the weights (1, 0)
+    // LDS byte address of cell index 0 of the slab (uniform)
+    const uint32_t sb = __builtin_amdgcn_readfirstlane((uint32_t)(size_t)(const TJ_LDS uint4*)cell);
+    const float mg = Pair<T>::magic(W);
+    const int mgh = (int)mg >> 1;
+    auto slices = [&](auto nearest_rows) {
+        for (int zz = 0; zz < zend; ++zz) {
+            const float zf = (float)(z0 + zz);
+            // byte address of index zz*ss - mg/2: cell c + mg then sits at
+            // ((c + mg) >> 1) + ((c + mg) & 1) hb
+            const uint32_t sbz = sb + 16u * (uint32_t)(zz * ss - mgh);
+            uint32_t a[G], w[G];
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                const float xs = __builtin_amdgcn_fmed3f(fmaf(zf, tn[k], base1[k]), 0.0f, wc1);
+                const float xm = xs + mg;  // rounded to the weight grid
+                const uint32_t cm = (uint32_t)xm;
+                uint32_t odd = cm & 1u;
+                asm("" : "+v"(odd));  // keeps (odd * hb) a 24-bit multiply-add, not a select
+                uint32_t idx = __umul24(odd, (uint32_t)hb) + (cm >> 1);
+                asm("" : "+v"(idx));
+                a[k] = sbz + (idx << 4);
+                w[k] = Pair<T>::of(__builtin_amdgcn_fractf(xm));
+            }
+            uint4 c0[G], c1[G];
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+                const u4v t0 = *(const TJ_LDS u4v*)(a[k]);
+                const u4v t1 = *(const TJ_LDS u4v*)(a[k] + 16u * (uint32_t)hp);
+                c0[k] = make_uint4(t0.x, t0.y, t0.z, t0.w);
+                c1[k] = make_uint4(t1.x, t1.y, t1.z, t1.w);
+            }
+            if constexpr (decltype(nearest_rows)::value) {
+                const uint32_t wn = Pair<T>::nearest();
+#pragma unroll
+                for (int k = 0; k < G; ++k) {
+                    const uint32_t v[8] = {c0[k].x, c0[k].y, c0[k].z, c0[k].w,
+                                           c1[k].x, c1[k].y, c1[k].z, c1[k].w};
+#pragma unroll
+                    for (int r = 0; r < 8; ++r)
+                        rs[k][r] = Pair<T>::dot(v[r], ((ymask >> r) & 1u) ? wn : w[k], rs[k][r]);
+                }
+            } else {
+                fpp_accumulate<T, G>(c0, c1, w, rs);
+            }
+        }
+    };
+    if (ymask) slices(std::true_type{});
+    else slices(std::false_type{});
+    if (zlast < zn)
+        fpp_slice_f64<T, G>(cell, hp, hb, z0 + zlast, zlast * ss, true, (double)(W - 1), ymask, prm, p0, n, ic, zlo,
+                            zhi, rs);
+}
+
 // Forward projection for 16-bit storage on packed pairs (BF16/FP16 forms).
 // Tile (job, G projections, 8 detector rows, 256-column tile).  Per slice the
 // LDS holds W + 2 cells of 32 bytes: cell c is column x = c - 1 of the 8 volume
@@ -612,8 +682,6 @@ __global__ __launch_bounds__(NT * CT) void k_job_fpp(const TjJob* __restrict__ j
 #pragma unroll
             for (int r = 0; r < RB; ++r) rs[k][r] = 0.0f;
         }
-        const float wc1 = (float)(W + 1);
-        const double wd = (double)(W - 1);
         // item q of this thread: m = tid + q BT = (slice zz, column pair x, x + 1)
         int izz[PF], ix[PF];
 #pragma unroll
@@ -692,62 +760,7 @@ __global__ __launch_bounds__(NT * CT) void k_job_fpp(const TjJob* __restrict__ j
                 }
             }
             __syncthreads();
-            const int zlast = (z0 + zn == D) ? zn - 1 : zn;
-            const int zend = min(zlast, zn);
-            // interpolated slices: FP32 positions; a last-row row (ymask) takes
-            // getPixel's nearest value, the weights (1, 0)
-            // LDS byte address of cell index 0 of the slab (uniform)
-            const uint32_t sb = __builtin_amdgcn_readfirstlane((uint32_t)(size_t)(const TJ_LDS uint4*)cell);
-            const float mg = Pair<T>::magic(W);
-            const int mgh = (int)mg >> 1;
-            auto slices = [&](auto nearest_rows) {
-                for (int zz = 0; zz < zend; ++zz) {
-                    const float zf = (float)(z0 + zz);
-                    // byte address of index zz*ss - mg/2: cell c + mg then sits at
-                    // ((c + mg) >> 1) + ((c + mg) & 1) hb
-                    const uint32_t sbz = sb + 16u * (uint32_t)(zz * ss - mgh);
-                    uint32_t a[G], w[G];
-#pragma unroll
-                    for (int k = 0; k < G; ++k) {
-                        const float xs = __builtin_amdgcn_fmed3f(fmaf(zf, tn[k], base1[k]), 0.0f, wc1);
-                        const float xm = xs + mg;  // rounded to the weight grid
-                        const uint32_t cm = (uint32_t)xm;
-                        uint32_t odd = cm & 1u;
-                        asm("" : "+v"(odd));  // keeps (odd * hb) a 24-bit multiply-add, not a select
-                        uint32_t idx = __umul24(odd, (uint32_t)hb) + (cm >> 1);
-                        asm("" : "+v"(idx));
-                        a[k] = sbz + (idx << 4);
-                        w[k] = Pair<T>::of(__builtin_amdgcn_fractf(xm));
-                    }
-                    uint4 c0[G], c1[G];
-#pragma unroll
-                    for (int k = 0; k < G; ++k) {
-                        typedef uint32_t u4v __attribute__((ext_vector_type(4)));
-                        const u4v t0 = *(const TJ_LDS u4v*)(a[k]);
-                        const u4v t1 = *(const TJ_LDS u4v*)(a[k] + 16u * (uint32_t)hp);
-                        c0[k] = make_uint4(t0.x, t0.y, t0.z, t0.w);
-                        c1[k] = make_uint4(t1.x, t1.y, t1.z, t1.w);
-                    }
-                    if constexpr (decltype(nearest_rows)::value) {
-                        const uint32_t wn = Pair<T>::nearest();
-#pragma unroll
-                        for (int k = 0; k < G; ++k) {
-                            const uint32_t v[8] = {c0[k].x, c0[k].y, c0[k].z, c0[k].w,
-                                                   c1[k].x, c1[k].y, c1[k].z, c1[k].w};
-#pragma unroll
-                            for (int r = 0; r < 8; ++r)
-                                rs[k][r] = Pair<T>::dot(v[r], ((ymask >> r) & 1u) ? wn : w[k], rs[k][r]);
-                        }
-                    } else {
-                        fpp_accumulate<T, G>(c0, c1, w, rs);
-                    }
-                }
-            };
-            if (ymask) slices(std::true_type{});
-            else slices(std::false_type{});
-            if (zlast < zn)
-                fpp_slice_f64<T, G>(cell, hp, hb, z0 + zlast, zlast * ss, true, wd, ymask, J.prm, p0, n, ic, zlo,
-                                    zhi, rs);
+            fpp_project<T, G>(cell, hp, ss, hb, z0, zn, D, W, ymask, base1, tn, J.prm, p0, n, ic, zlo, zhi, rs);
         }
         // projection, norm and difference per (projection, row); error partials
         double e2[G][RB];
@@ -793,6 +806,175 @@ __global__ __launch_bounds__(NT * CT) void k_job_fpp(const TjJob* __restrict__ j
 #pragma unroll
                 for (int w = 1; w < NT / 64; ++w) s += red[4 * h + w][e];
                 gp(J.err_part)[((size_t)p * ph + j) * J.nct + cth] = s;
+            }
+        }
+    }
+}
+
+// The packed forward projection with a double-buffered slab.  A block of
+// PG * 256 threads holds PG parts, each the G rays of its own projections
+// (p0 + part G ...) over the tile's 256 columns, and all parts share the
+// staged slab, so one staging serves PG * G * 256 rays.  The slab alternates
+// between two LDS buffers: the next slab's words are loaded into registers
+// before the current slab is projected and written into the other buffer
+// after it (one barrier per slab).  Each thread stages one column pair of one
+// slice per slab (zc = threads / (W / 2) slices, at most what the LDS holds);
+// rows outside the volume or the detector read zeros through the buffer range
+// check.  Requires an even W; the cells and the arithmetic are k_job_fpp's.
+template <typename T, typename TE, int G, int PG>
+__global__ __launch_bounds__(NT * PG) void k_job_fpq(const TjJob* __restrict__ jobs, const int4* __restrict__ tiles,
+                                                int ntiles, int lds_cells)
+{
+    constexpr int RB = 8;
+    constexpr int BT = NT * PG;
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint4* cells = reinterpret_cast<uint4*>(smem);
+    __shared__ double red[BT / 64][G * RB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int part = tid / NT, ctid = tid - part * NT;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int4 tl = tiles[t];
+        if (tl.x < 0) continue;
+        const TjJob& J = jobs[tl.x];
+        const uint16_t* vol = (const uint16_t*)J.vol;
+        const int W = J.W, D = J.D, pw = J.pw, ph = J.ph, n = J.n;
+        const int WC = W + 2;
+        const int hb = (((WC + 1) / 2 + 3) & ~7) + 4;
+        const int ss = hb + WC / 2;
+        const int half = W >> 1;
+        // slices per slab: one column pair per thread, two buffers of two planes
+        const int zc = min(min(D, BT / half), lds_cells / (4 * ss));
+        const int hp = zc * ss;       // rows 4-7 plane within a buffer
+        const int bufc = 2 * hp;      // cells per buffer
+        const int p0 = tl.y + part * G, j0 = tl.z * RB;
+        const int i = tl.w * NT + ctid;
+        const bool col = i < pw;
+        const double ic = col ? i : pw - 1;
+        const size_t plane = (size_t)D * W;
+        unsigned ymask = 0;
+        unsigned rmask = 0;  // bit r: row r is inside the volume and the detector
+        __amdgpu_buffer_rsrc_t rsc[RB];
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            const int y = j0 + r + J.dy;
+            const bool in = (j0 + r < ph) && y >= 0 && y < J.H;
+            rmask |= in ? (1u << r) : 0u;
+            ymask |= (y == J.H - 1) ? (1u << r) : 0u;
+            rsc[r] = __builtin_amdgcn_make_buffer_rsrc((void*)(vol + (size_t)(in ? y : 0) * plane), 0,
+                                                       in ? (int)(plane * 2) : 0, 0x00020000);
+        }
+        float base1[G], tn[G], rs[G][RB];
+        int zlo[G], zhi[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const double* q = J.prm + PRM * min(p0 + k, n - 1);
+            const double bd = ic * q[0] + q[1];
+            base1[k] = (float)(bd + 1.0);
+            tn[k] = (float)q[2];
+            line_interval(bd, q[2], q[7], -TJ_ACCURACY, (double)W, D, zlo[k], zhi[k]);
+#pragma unroll
+            for (int r = 0; r < RB; ++r) rs[k][r] = 0.0f;
+        }
+        // this thread's item: slice izz of the slab, column pair (ix, ix + 1)
+        const int izz = tid / half;
+        const int ix = 2 * (tid - izz * half);
+        const uint32_t sel = (ix + 2 >= W) ? 0x03020302u : 0x05040302u;
+        uint32_t wa[RB], wb[RB];
+        // cell x + 1 is the word at x; cell x + 2 is (v[x+1], v[x+2]) from the
+        // words at x and x + 2, or (v[W-1], v[W-1]) on the last pair
+        auto fetch = [&](int zs) {
+            if (izz < min(zc, D - zs)) {
+                const uint32_t o0 = (uint32_t)((zs + izz) * W + ix) * 2u;
+                const uint32_t o1 = (ix + 2 >= W) ? o0 : o0 + 4u;
+#pragma unroll
+                for (int r = 0; r < RB; ++r) {
+                    wa[r] = __builtin_amdgcn_raw_buffer_load_b32(rsc[r], o0, 0, 0);
+                    wb[r] = __builtin_amdgcn_raw_buffer_load_b32(rsc[r], o1, 0, 0);
+                }
+            }
+        };
+        auto store = [&](uint4* c, int zs) {
+            if (izz < min(zc, D - zs)) {
+                uint4* d1 = c + ((size_t)izz * ss + hb + (ix >> 1));
+                uint4* d2 = c + ((size_t)izz * ss + (ix >> 1) + 1);
+                d1[0] = make_uint4(wa[0], wa[1], wa[2], wa[3]);
+                d1[hp] = make_uint4(wa[4], wa[5], wa[6], wa[7]);
+                d2[0] = make_uint4(__builtin_amdgcn_perm(wb[0], wa[0], sel), __builtin_amdgcn_perm(wb[1], wa[1], sel),
+                                   __builtin_amdgcn_perm(wb[2], wa[2], sel), __builtin_amdgcn_perm(wb[3], wa[3], sel));
+                d2[hp] = make_uint4(__builtin_amdgcn_perm(wb[4], wa[4], sel), __builtin_amdgcn_perm(wb[5], wa[5], sel),
+                                    __builtin_amdgcn_perm(wb[6], wa[6], sel), __builtin_amdgcn_perm(wb[7], wa[7], sel));
+            }
+        };
+        fetch(0);
+        // the zero cells 0 and W + 1 of every slice of both buffers (the
+        // previous tile's readers passed its final barrier)
+        for (int q = tid; q < 2 * zc; q += BT) {
+            uint4* d = cells + (size_t)(q / zc) * bufc + (size_t)(q % zc) * ss;
+            const uint4 zero4 = make_uint4(0u, 0u, 0u, 0u);
+            d[fpp_cidx(0, hb)] = zero4;
+            d[hp + fpp_cidx(0, hb)] = zero4;
+            d[fpp_cidx(W + 1, hb)] = zero4;
+            d[hp + fpp_cidx(W + 1, hb)] = zero4;
+        }
+        store(cells, 0);
+        __syncthreads();
+        int buf = 0;
+        for (int z0 = 0; z0 < D; z0 += zc) {
+            const int zn = min(zc, D - z0);
+            const bool more = z0 + zc < D;  // uniform
+            if (more) fetch(z0 + zc);
+            fpp_project<T, G>(cells + (size_t)buf * bufc, hp, ss, hb, z0, zn, D, W, ymask, base1, tn, J.prm, p0, n,
+                              ic, zlo, zhi, rs);
+            if (more) {
+                store(cells + (size_t)(buf ^ 1) * bufc, z0 + zc);
+                __syncthreads();
+                buf ^= 1;
+            }
+        }
+        // projection, norm and difference per (projection, row); error partials
+        double e2[G][RB];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int p = p0 + k;
+            const double ninc = J.prm[PRM * min(p, n - 1) + 3];
+            const int cnt = zhi[k] - zlo[k] + 1;
+            const double nrm = (cnt > 0) ? (double)(cnt + (zhi[k] == D - 1 ? 0 : 1)) * ninc : 0.0;
+#pragma unroll
+            for (int r = 0; r < RB; ++r) {
+                const int j = j0 + r;
+                double v2 = 0.0;
+                if (p < n && col && j < ph) {
+                    const size_t o = ((size_t)p * ph + j) * pw + i;
+                    const float norm = ((rmask >> r) & 1u) ? (float)nrm : 0.0f;
+                    const float th = (norm > 0.0f) ? (float)((double)rs[k][r] * ninc) : NAN;
+                    const float ex = ld<float>((const TE*)J.exps, o);
+                    float d = 0.0f;
+                    if (!isnan(ex) && !isnan(th)) {
+                        const double v = (double)((ex - th) / norm) * J.factor;
+                        d = (float)v;
+                        v2 = v * v;
+                    }
+                    st((float*)J.diff, o, d);
+                }
+                e2[k][r] = wave_sum(v2);
+            }
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < G; ++k)
+#pragma unroll
+                for (int r = 0; r < RB; ++r) red[wave][k * RB + r] = e2[k][r];
+        }
+        __syncthreads();
+        // per part: the sum of its 4 waves in order
+        if (tid < PG * G * RB) {
+            const int h = tid / (G * RB), e = tid - h * (G * RB);
+            const int p = tl.y + h * G + e / RB, j = j0 + (e % RB);
+            if (p < n && j < ph && tl.w < J.nct) {
+                double sum = red[4 * h][e];
+#pragma unroll
+                for (int w = 1; w < NT / 64; ++w) sum += red[4 * h + w][e];
+                gp(J.err_part)[((size_t)p * ph + j) * J.nct + tl.w] = sum;
             }
         }
     }
@@ -1215,7 +1397,7 @@ struct tj_batch {
     size_t budget = 0, used = 0;
     int next_id = 1;
     std::map<int, BJob*> jobs;
-    tj_batch_tuning tune{4, 128, 4, 16, 2, 8, 1, 2};  // the autotuner's pick at 512^3 BF16
+    tj_batch_tuning tune{4, 128, 4, 16, 2, 8, 2, 2};  // the autotuner's pick at 512^3 BF16
     // run plans of the recent job sets, most recently used first (a serving
     // loop alternates between a few sets, each with its tile tables and graph)
     std::list<RunPlan> plans;
@@ -1380,6 +1562,11 @@ static hipError_t allow_lds(int bytes)
 // Largest dynamic LDS any forward launch may ask for (set once per device,
 // before any graph capture).
 static const int kMaxFpLds = 152 * 1024;
+// Parts (projection groups sharing one slab) of the double-buffered packed
+// forward kernel, a 1024-thread block, and rays per thread (two keep it within
+// the 128 registers of four waves per SIMD; four spill).
+static const int kFpqParts = 4;
+static const int kFpqRays = 2;
 
 template <typename T, typename TE>
 static hipError_t allow_lds2()
@@ -1393,7 +1580,8 @@ template <typename T, typename TE>
 static hipError_t allow_lds_p()
 {
     const void* fns[] = {(const void*)tjb::k_job_fpp<T, TE, 2, 1>, (const void*)tjb::k_job_fpp<T, TE, 4, 1>,
-                         (const void*)tjb::k_job_fpp<T, TE, 2, 2>, (const void*)tjb::k_job_fpp<T, TE, 4, 2>};
+                         (const void*)tjb::k_job_fpp<T, TE, 2, 2>, (const void*)tjb::k_job_fpp<T, TE, 4, 2>,
+                         (const void*)tjb::k_job_fpq<T, TE, kFpqRays, kFpqParts>};
     hipError_t e = hipSuccess;
     for (const void* f : fns)
         if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxFpLds);
@@ -1444,9 +1632,40 @@ static int fp_rays_of(const tj_batch_tuning& tu, int storage)
 
 // packed-pair forward kernel for 16-bit storage (8 rows per tile); one slice
 // of the widest job needs 2 (W + 2) cells plus the padding of the odd half
-template <typename T, typename TE>
-static Kern fpp_kern_s(const tj_batch_tuning& tu, int max_w, int lds_cu)
+// Cells of one slice of the packed slab (even cells, the odd half from hb on).
+static int fpp_slice_cells(int w)
 {
+    const int wc = w + 2, hb = (((wc + 1) / 2 + 3) & ~7) + 4;
+    return hb + wc / 2;
+}
+
+// The double-buffered packed forward kernel takes a launch whose jobs all have
+// an even width of at most 2 * (its threads) columns (one column pair per
+// thread and slab) and whose widest slice fits the LDS twice.
+static bool fpq_ok(const std::vector<BJob*>& js, const tj_batch_tuning& tu, int storage)
+{
+    if (!use_packed(tu, storage) || tu.fp_packed != 2) return false;
+    for (const BJob* j : js) {
+        const int w = j->d.vol_w;
+        if ((w & 1) || w / 2 > kFpqParts * tjb::NT || 4 * fpp_slice_cells(w) * (int)sizeof(uint4) > kMaxFpLds)
+            return false;
+    }
+    return true;
+}
+
+template <typename T, typename TE>
+static Kern fpp_kern_s(const tj_batch_tuning& tu, int max_w, int lds_cu, bool quad)
+{
+    if (quad) {
+        Kern k;
+        k.block = kFpqParts * tjb::NT;
+        k.fn = (const void*)tjb::k_job_fpq<T, TE, kFpqRays, kFpqParts>;
+        // one 1024-thread block fills a CU: it takes all the LDS a block may
+        // have (zc = 4 slices at W = 512, 2 at W = 1024)
+        k.lds = fit_lds(k, kMaxFpLds, 4 * fpp_slice_cells(max_w) * (int)sizeof(uint4), lds_cu);
+        k.cells = k.lds / (int)sizeof(uint4);
+        return k;
+    }
     const bool two = fp_cols_of(tu, TJ_STORAGE_BF16) == 2, g2 = fp_rays_of(tu, TJ_STORAGE_BF16) == 2;
     Kern k;
     k.block = two ? 2 * tjb::NT : tjb::NT;
@@ -1459,14 +1678,15 @@ static Kern fpp_kern_s(const tj_batch_tuning& tu, int max_w, int lds_cu)
 
 static int fp_rows_of(const tj_batch_tuning& tu, int storage) { return use_packed(tu, storage) ? 8 : 4; }
 
-static Kern fp_kern(int storage, int pstorage, const tj_batch_tuning& tu, int max_w, int lds_cu)
+static Kern fp_kern(int storage, int pstorage, const tj_batch_tuning& tu, int max_w, int lds_cu, bool quad)
 {
     const bool pf = pstorage == TJ_STORAGE_F32;
     if (use_packed(tu, storage)) {
         if (storage == TJ_STORAGE_BF16)
-            return pf ? fpp_kern_s<Bf16, float>(tu, max_w, lds_cu) : fpp_kern_s<Bf16, Bf16>(tu, max_w, lds_cu);
-        return pf ? fpp_kern_s<_Float16, float>(tu, max_w, lds_cu)
-                  : fpp_kern_s<_Float16, _Float16>(tu, max_w, lds_cu);
+            return pf ? fpp_kern_s<Bf16, float>(tu, max_w, lds_cu, quad)
+                      : fpp_kern_s<Bf16, Bf16>(tu, max_w, lds_cu, quad);
+        return pf ? fpp_kern_s<_Float16, float>(tu, max_w, lds_cu, quad)
+                  : fpp_kern_s<_Float16, _Float16>(tu, max_w, lds_cu, quad);
     }
     switch (storage) {
     case TJ_STORAGE_F32: return fp_kern_s<float, float>(tu, max_w, lds_cu);
@@ -1724,13 +1944,16 @@ static int build_plan(tj_batch* b, const std::vector<int>& ids)
                     fidx.push_back(sidx[m]);
                 }
             if (fj.empty()) continue;
-            std::vector<int4> ft =
-                fp_tiles(fj, fidx, fp_rays_of(b->tune, s), fp_rows_of(b->tune, s), fp_cols_of(b->tune, s));
+            // the double-buffered packed kernel's tiles step over all its
+            // parts' projections and cover one 256-column tile each
+            const bool quad = fpq_ok(fj, b->tune, s);
+            std::vector<int4> ft = fp_tiles(fj, fidx, quad ? kFpqRays * kFpqParts : fp_rays_of(b->tune, s),
+                                            fp_rows_of(b->tune, s), quad ? 1 : fp_cols_of(b->tune, s));
             int4* dft = nullptr;
             if (int rc = upload(ft, &dft)) return rc;
             int max_w = 0;
             for (BJob* j : fj) max_w = std::max(max_w, j->d.vol_w);
-            const Kern k = fp_kern(s, ps, b->tune, max_w, b->lds_cu);
+            const Kern k = fp_kern(s, ps, b->tune, max_w, b->lds_cu, quad);
             fwd.push_back({0, s, max_w, (int)ft.size(), grid_for(b, (int)ft.size(), k), dft, 0, 0, ps, k});
         }
         for (int mode = 0; mode <= 3; ++mode) {
@@ -2281,7 +2504,7 @@ int tj_batch_set_tuning(tj_batch* b, const tj_batch_tuning* t)
 {
     if (!b || !t) return bfail(TJ_EINVAL, "null argument");
     if (t->fp_rays < 1 || t->fp_lds_kb < 16 || t->fp_lds_kb > 152 || t->bp_rows < 1 || t->bp_zchunk < 1 ||
-        t->bp_group < 1 || t->blocks_per_cu < 1 || t->blocks_per_cu > 16 || t->fp_packed < 0 || t->fp_packed > 1 ||
+        t->bp_group < 1 || t->blocks_per_cu < 1 || t->blocks_per_cu > 16 || t->fp_packed < 0 || t->fp_packed > 2 ||
         t->fp_cols < 0 || t->fp_cols > 2)
         return bfail(TJ_EINVAL, "tj_batch_set_tuning: value out of range");
     b->tune = *t;
@@ -2366,11 +2589,16 @@ int tj_batch_autotune(tj_batch* b, const int* ids, int nids, tj_batch_tuning* be
     }
     tj_batch_tuning cur = start, bestf = start;
     double tf = 1e300;
-    for (int pk = any_half ? 0 : 1; pk <= 1; ++pk)
-    for (int cols = 1; cols <= ((any_half && pk) ? 2 : 1); ++cols)
+    // the double-buffered packed kernel has one shape (its rays, LDS and
+    // one block per CU are fixed)
+    for (int pk = any_half ? 0 : 1; pk <= (any_half ? 2 : 1); ++pk)
+    for (int cols = 1; cols <= ((any_half && pk == 1) ? 2 : 1); ++cols)
     for (int r : fr)
+        if (pk != 2 || r == kFpqRays)
         for (int l : fl)
-            for (int c : bpc) {
+            if (pk != 2 || l == fl[0])
+            for (int c : bpc)
+                if (pk != 2 || c == bpc[0]) {
                 cur.fp_packed = any_half ? pk : start.fp_packed;
                 cur.fp_cols = (any_half && pk) ? cols : start.fp_cols;
                 cur.fp_rays = r;
