@@ -264,6 +264,30 @@ __device__ __forceinline__ void fp_slab(const typename V4<A>::t* cell, int zn, i
         fp_slice_f64<A, G, NEARY>(cell, z0 + zlast, zlast * W1, zero, (A)0, wd, ym, bd, td, zlo, zhi, rs);
 }
 
+// One element of a volume row read through a buffer resource (element index
+// within the row's plane; a row outside the volume has an empty range and
+// reads 0), widened to the arithmetic type.
+template <typename A>
+__device__ __forceinline__ A bld(const float*, __amdgpu_buffer_rsrc_t rs, uint32_t e)
+{
+    return (A)__uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, e * 4u, 0, 0));
+}
+template <typename A>
+__device__ __forceinline__ A bld(const double*, __amdgpu_buffer_rsrc_t rs, uint32_t e)
+{
+    return (A)__builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, e * 8u, 0, 0));
+}
+template <typename A>
+__device__ __forceinline__ A bld(const _Float16*, __amdgpu_buffer_rsrc_t rs, uint32_t e)
+{
+    return (A)(float)__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(rs, e * 2u, 0, 0));
+}
+template <typename A>
+__device__ __forceinline__ A bld(const Bf16*, __amdgpu_buffer_rsrc_t rs, uint32_t e)
+{
+    return (A)__uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(rs, e * 2u, 0, 0) << 16);
+}
+
 // At most 128 VGPRs (4 waves per SIMD): the kernel is latency bound and a CU
 // holding four of its blocks hides one block's slab staging under the others'
 // projection.
@@ -291,14 +315,15 @@ __global__ __launch_bounds__(NT, 4) void k_job_fp(const TjJob* __restrict__ jobs
         const double ic = col ? i : pw - 1;
         const size_t plane = (size_t)D * W;
         bool rin[4];
-        const T* rp[4];
+        __amdgpu_buffer_rsrc_t rsc[4];  // the 4 volume rows; empty outside the volume
         A ym[4];  // 0 on the last volume row: getPixel takes the nearest value there
         bool neary = false;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int y = j0 + r + J.dy;
             rin[r] = (j0 + r < ph) && y >= 0 && y < J.H;
-            rp[r] = vol + (size_t)(rin[r] ? y : 0) * plane;
+            rsc[r] = __builtin_amdgcn_make_buffer_rsrc((void*)(vol + (size_t)(rin[r] ? y : 0) * plane), 0,
+                                                       rin[r] ? (int)(plane * sizeof(T)) : 0, 0x00020000);
             ym[r] = (y == J.H - 1) ? (A)0 : (A)1;
             neary = neary || (y == J.H - 1);
         }
@@ -329,27 +354,27 @@ __global__ __launch_bounds__(NT, 4) void k_job_fp(const TjJob* __restrict__ jobs
             // wave w stages slices w, w + NT/64, ...; lanes take consecutive
             // columns, SU columns per lane per batch so that 4*SU loads are in
             // flight before the first LDS write.  Rows outside the volume read
-            // row 0 (a valid address) and store 0; columns past W read W-1.
+            // 0 (empty buffer range); columns past W read W-1.
             for (int zz = wave; zz < zn; zz += NT / 64) {
-                const size_t zo = (size_t)(z0 + zz) * W;
+                const uint32_t zo = (uint32_t)((z0 + zz) * W);
                 C4* dst = cell + zz * W1;
                 for (int x0 = 0; x0 < W1; x0 += 64 * SU) {
                     A v[SU][4];
 #pragma unroll
                     for (int u = 0; u < SU; ++u) {
-                        const size_t o = zo + min(x0 + 64 * u + lane, W - 1);
+                        const uint32_t o = zo + (uint32_t)min(x0 + 64 * u + lane, W - 1);
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) v[u][r] = ld<A>(rp[r], o);
+                        for (int r = 0; r < 4; ++r) v[u][r] = bld<A>(vol, rsc[r], o);
                     }
 #pragma unroll
                     for (int u = 0; u < SU; ++u) {
                         const int xx = x0 + 64 * u + lane;
                         if (xx < W1) {
                             C4 c4;
-                            c4.x = rin[0] ? v[u][0] : (A)0;
-                            c4.y = rin[1] ? v[u][1] : (A)0;
-                            c4.z = rin[2] ? v[u][2] : (A)0;
-                            c4.w = rin[3] ? v[u][3] : (A)0;
+                            c4.x = v[u][0];
+                            c4.y = v[u][1];
+                            c4.z = v[u][2];
+                            c4.w = v[u][3];
                             dst[xx] = c4;
                         }
                     }
