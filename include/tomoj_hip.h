@@ -257,6 +257,13 @@ int tj_set_tvm(tj_ctx *ctx, int iterations, double theta, double g, double lambd
  * (room for nproj) and returns its length, or a negative TJ_E* code. */
 int tj_available_indexes(int nproj, int projection_type, int *out);
 
+/* TomoReconstruction2.orderShuffleART (TomoReconstruction2.java:735-781): the
+ * order in which an iteration walks `n` images, each next one the remaining
+ * image whose third Euler row has the smallest |dot product| with the
+ * previous one's (euler: n x 9, row-major).  TomoJ picks the first image with
+ * Math.random() (:755); here the caller passes it.  Writes n indexes. */
+void tj_order_shuffle(const double *euler, int n, int first, int *order);
+
 /* The images tj_run_ossart and tj_run_wbp walk: the list getAvailableIndexes
  * returns (or setAvailableIndexes sets, :258-261), as TomoReconstruction2.WBP
  * (:500) and orderShuffleART (:736) use it; n = 0 restores all.  With a list,

@@ -123,6 +123,7 @@ _SIGS = {
     "tj_set_bayesian": (C.c_int, [_P, C.c_int, C.c_double, C.c_double]),
     "tj_set_bgart": (C.c_int, [_P, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double]),
     "tj_available_indexes": (C.c_int, [C.c_int, C.c_int, _P]),
+    "tj_order_shuffle": (None, [_P, C.c_int, C.c_int, _P]),
     "tj_threshold_projections": (C.c_int, [_P, C.c_double, C.c_double, C.c_int]),
     "tj_max_volume_bytes": (C.c_int, [C.c_int, C.POINTER(C.c_size_t)]),
     "tj_fsc": (C.c_int, [_P, _P, _P]),

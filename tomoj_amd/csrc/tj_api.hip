@@ -28,7 +28,8 @@ hipError_t residual(hipStream_t, int, int, int, const float*, const int*, const 
 hipError_t row_flags(hipStream_t, int, int, int, const float*, const int*, uint8_t*);
 hipError_t fp_residual(hipStream_t, const TjGeomDev&, const float*, const TjProj*, const int*, int, int, const float*,
                        int, int, const float*, double, float*, double*, uint8_t*, uint8_t*, double*,
-                       hipEvent_t mid = nullptr, int gb = 1, int* clear = nullptr, bool pf = true);
+                       hipEvent_t mid = nullptr, int gb = 1, int* clear = nullptr, bool pf = true,
+                       float* proj = nullptr, float* norm = nullptr);
 hipError_t bp(hipStream_t, const TjGeomDev&, float*, const TjProj*, const int*, int, const float*, const int*,
               const uint8_t*, TjBpRow*, int, int, int, int, int, int, int, double2*, hipEvent_t mid = nullptr,
               int* slow = nullptr, int fast = 0, int slow_cleared = 0);
@@ -134,6 +135,7 @@ struct tj_ctx {
     TjBpRow* bprows = nullptr;  // [cap][vh]
     double* err = nullptr;      // [max(cap,nproj)]
     int* slot_proj = nullptr;   // [cap]
+    std::vector<int> uploaded;  // the indices slot_proj holds (upload_added)
     int* iota = nullptr;        // [max(cap,nproj)] 0,1,2,...
     int iota_n = 0;
     double* h_err = nullptr;    // pinned [max(cap,nproj)]
@@ -531,7 +533,7 @@ static int do_error_backproject(tj_ctx* c, const int* slots, const std::vector<i
 // replaced by `first`: each next image is the remaining one whose row 2 of the
 // Euler matrix (euler, n x 9) has the smallest |dot product| with the previous
 // one's.
-void tj_order_shuffle(const double* euler, int n, int first, int* order)
+extern "C" void tj_order_shuffle(const double* euler, int n, int first, int* order)
 {
     std::vector<char> todo(n, 1);
     int cc = 0, im = 0;
@@ -1125,12 +1127,18 @@ int tj_clear_projections(tj_ctx* c)
 
 int tj_num_added(tj_ctx* c) { return c ? (int)c->added.size() : 0; }
 
+// The added indices on the device (slot_proj); a list already there (the
+// backprojection after projectAndDifference of the same subset) is not sent
+// again.
 static int upload_added(tj_ctx* c)
 {
+    const int cap = c->cap;
     int rc = ensure_slots(c, (int)c->added.size());
     if (rc) return rc;
+    if (c->cap == cap && c->uploaded == c->added) return TJ_OK;
     TJ_HIP(hipMemcpyAsync(c->slot_proj, c->added.data(), sizeof(int) * c->added.size(), hipMemcpyHostToDevice,
                           c->stream));
+    c->uploaded = c->added;
     return TJ_OK;
 }
 
@@ -1165,8 +1173,18 @@ int tj_project_and_difference(tj_ctx* c, double factor, int start_y, int end_y, 
         return TJ_OK;
     }
     if ((rc = upload_added(c))) return rc;
-    if ((rc = do_project(c, c->slot_proj, c->added, start_y))) return rc;
-    if ((rc = do_residual(c, c->slot_proj, n, factor, c->err))) return rc;
+    const int start = fp_start_row(c, start_y);
+    if (pick_fp_kernel(c, c->added, start) == 3 && !c->geo.long_object_comp) {
+        // the LDS forward kernel fused with the residual, writing the
+        // projections and norms as well (getProjection after the call)
+        TJ_HIP(tjk::fp_residual(c->stream, c->g, c->vol, c->pp, c->slot_proj, n, start, c->norm_tab,
+                                c->tune.fp_rays, c->tune.fp_zchunk, c->exps, factor, c->sdiff, c->row_err, c->rbad,
+                                c->rowflag, c->err, nullptr, c->tune.fp_groups, nullptr, c->tune.fp_prefetch == 0,
+                                c->sproj, c->snorm));
+    } else {
+        if ((rc = do_project(c, c->slot_proj, c->added, start_y))) return rc;
+        if ((rc = do_residual(c, c->slot_proj, n, factor, c->err))) return rc;
+    }
     TJ_HIP(hipMemcpyAsync(c->h_err, c->err, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     TJ_HIP(hipStreamSynchronize(c->stream));
     if (errors) std::memcpy(errors, c->h_err, sizeof(double) * n);
@@ -2044,6 +2062,7 @@ int tj_run_wbp(tj_ctx* c, int start_y, int end_y)
     const int* slots = c->iota;
     if (!c->avail.empty()) {
         TJ_HIP(hipMemcpy(c->slot_proj, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice));
+        c->uploaded.clear();
         slots = c->slot_proj;
     }
     const float* images;

@@ -54,8 +54,9 @@ struct TjBpRow {
 // Stores the message tj_last_error() returns on the calling thread.
 void tj_record_error(const char* msg);
 
-// TomoReconstruction2.orderShuffleART with the first image fixed (host code).
-void tj_order_shuffle(const double* euler, int n, int first, int* order);
+// TomoReconstruction2.orderShuffleART with the first image fixed (host code;
+// part of the C ABI, declared in tomoj_hip.h as well).
+extern "C" void tj_order_shuffle(const double* euler, int n, int first, int* order);
 
 // Host helpers shared by the context API (tj_api.hip) and the allocation-free
 // entry point (tj_noalloc.hip); tj_geom_to_dev and tj_fp_start_row, which take

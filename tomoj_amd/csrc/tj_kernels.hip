@@ -1128,6 +1128,8 @@ struct TjFpFuse {
     float* diff;        // [nslot][ph][pw]
     double* rerr;       // [nslot][ceil(ph / 4)][ntile]
     uint8_t* rbad;      // [nslot][ph][ntile]
+    float* proj;        // [nslot][ph][pw] or null: the projections and ray norms too,
+    float* norm;        //   as the unfused kernel writes them (getProjection)
 };
 
 // GB: groups of G projections per block, one 256-thread part each; the parts
@@ -1269,7 +1271,12 @@ __global__ __launch_bounds__(TJ_BLOCK * GB) void k_fp_lds(
                         d = (float)value;
                         v2 += value * value;
                     }
-                    fu.diff[((size_t)(s0 + k) * g.ph + j) * g.pw + i] = d;
+                    const size_t o = ((size_t)(s0 + k) * g.ph + j) * g.pw + i;
+                    fu.diff[o] = d;
+                    if (fu.proj) {  // uniform
+                        fu.proj[o] = th;
+                        fu.norm[o] = n;
+                    }
                     bad = !isfinite(d);
                 }
                 const unsigned wbad = __ballot(bad) != 0 ? 1u : 0u;
@@ -2240,18 +2247,18 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_err_fold(int ph, int ntile, const 
 
 // The forward projection of the LDS kernel fused with the residual (the
 // projectAndDifference of one subset): diff, rowflag and the per-slot errors as
-// fp + residual give them (the errors summed in another order), without
-// writing proj and norm.  rerr holds nslot x ceil(ph / 4) x ceil(pw / 256)
+// fp + residual give them (the errors summed in another order), and proj and
+// norm too when they are given.  rerr holds nslot x ceil(ph / 4) x ceil(pw / 256)
 // entries, rbad nslot x ph x ceil(pw / 256).  `mid` (or null) is recorded
 // between the forward kernel and the fold.
 hipError_t fp_residual(hipStream_t st, const TjGeomDev& g, const float* vol, const TjProj* pp, const int* slot_proj,
                        int nslot, int start_row, const float* norm_tab, int rpt, int zc, const float* exps,
                        double factor, float* diff, double* rerr, uint8_t* rbad, uint8_t* rowflag, double* err,
-                       hipEvent_t mid, int gb, int* clear, bool pf)
+                       hipEvent_t mid, int gb, int* clear, bool pf, float* proj, float* norm)
 {
     (void)hipGetLastError();  // drop a stale error left by an earlier runtime call
     if (nslot <= 0) return hipSuccess;
-    const TjFpFuse fu{exps, factor, diff, rerr, rbad};
+    const TjFpFuse fu{exps, factor, diff, rerr, rbad, proj, norm};
     hipError_t e;
     if (rpt <= 2)
         e = launch_fp_lds_gb<2, true>(gb, st, g, vol, pp, slot_proj, nslot, start_row, norm_tab, zc, nullptr, nullptr,

@@ -23,7 +23,8 @@ namespace tjk {
 hipError_t norm_table(hipStream_t, const TjGeomDev&, const TjProj*, int, float*);
 hipError_t fp_residual(hipStream_t, const TjGeomDev&, const float*, const TjProj*, const int*, int, int, const float*,
                        int, int, const float*, double, float*, double*, uint8_t*, uint8_t*, double*,
-                       hipEvent_t mid = nullptr, int gb = 1, int* clear = nullptr, bool pf = true);
+                       hipEvent_t mid = nullptr, int gb = 1, int* clear = nullptr, bool pf = true,
+                       float* proj = nullptr, float* norm = nullptr);
 hipError_t plane_flags(hipStream_t, const float*, size_t, int, int, uint8_t*);
 void set_max_lds(int);
 }  // namespace tjk

@@ -10,10 +10,13 @@ must give the same volume.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 from typing import List, Optional, Sequence
 
 import numpy as np
+
+from . import _lib
 
 JAVA_FLOAT_MAX = 3.4028234663852886e38
 
@@ -40,32 +43,20 @@ def tilt_eulers(tilts_deg: Sequence[float], tilt_axis_deg: float = 0.0, axis_ver
 
 def order_shuffle(eulers: np.ndarray, first_index: int, available: Optional[Sequence[int]] = None) -> List[int]:
     """orderShuffleART (:735-781) with the first pick fixed: each next image is the
-    remaining one with the smallest |row2(E_prev) . row2(E_o)|.  With a list of
-    available images TomoJ runs the chain over positions 0..len-1 using the
-    Euler matrices of images 0..len-1 (:741) and emits the listed images."""
-    rows = np.asarray(eulers, np.float64).reshape(-1, 3, 3)[:, 2, :]
-    n = rows.shape[0] if available is None else len(available)
-    todo = [True] * n
-    order: List[int] = []
-    im = first_index
-    while len(order) < n:
-        if order:
-            mini, omini = float("inf"), -1
-            a = rows[im]
-            for o in range(n):
-                if todo[o]:
-                    s = 0.0
-                    s += a[0] * rows[o][0]
-                    s += a[1] * rows[o][1]
-                    s += a[2] * rows[o][2]
-                    sp = abs(s)
-                    if sp < mini:
-                        mini, omini = sp, o
-            im = omini
-        if todo[im]:
-            order.append(im)
-            todo[im] = False
-    return order if available is None else [int(available[k]) for k in order]
+    remaining one with the smallest |row2(E_prev) . row2(E_o)|, computed by the
+    library's tj_order_shuffle.  With a list of available images TomoJ runs the
+    chain over positions 0..len-1 using the Euler matrices of images 0..len-1
+    (:741) and emits the listed images."""
+    e = np.ascontiguousarray(np.asarray(eulers, np.float64).reshape(-1, 9))
+    n = e.shape[0] if available is None else len(available)
+    if n == 0:
+        return []
+    e = np.ascontiguousarray(e[:n])
+    order = np.zeros(n, np.int32)
+    _lib.load().tj_order_shuffle(e.ctypes.data_as(C.c_void_p), n, int(first_index),
+                                 order.ctypes.data_as(C.c_void_p))
+    out = [int(v) for v in order]
+    return out if available is None else [int(available[k]) for k in out]
 
 
 def osart_iteration(projector, eulers: np.ndarray, vol_w: int, vol_h: int, relaxation: float,
