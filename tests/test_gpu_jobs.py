@@ -245,14 +245,19 @@ def test_result_same_alone_and_in_mix(batch, oracle):
     assert np.array_equal(alone, mixed)
 
 
-def test_tile_shapes_do_not_change_results(batch, oracle):
-    W, H, D = 70, 10, 27
+@pytest.mark.parametrize("size", [(70, 10, 27), (300, 9, 37)], ids=["one-slab", "partial-slabs"])
+def test_tile_shapes_do_not_change_results(batch, oracle, size):
+    """FP32 storage: the one-group kernel (k_job_fp) in several shapes and the
+    shared double-buffered one (k_job_fq; at 300 columns its slabs hold 13
+    slices, so 37 slices end in a partial slab) give bit-identical volumes."""
+    W, H, D = size
     tilts, E, exps = phantom_case(oracle, W, H, D, tilts=np.arange(-60, 61, 10.0))
     results = []
-    shapes = [dict(fp_rays=2, fp_lds_kb=16, bp_rows=2, bp_zchunk=8, blocks_per_cu=1),
-              dict(fp_rays=4, fp_lds_kb=64, bp_rows=4, bp_zchunk=8, blocks_per_cu=4),
-              dict(fp_rays=4, fp_lds_kb=32, bp_rows=4, bp_zchunk=4, blocks_per_cu=8),
-              dict(fp_rays=2, fp_lds_kb=128, bp_rows=2, bp_zchunk=16, blocks_per_cu=2)]
+    shapes = [dict(fp_shared=0, fp_rays=2, fp_lds_kb=16, bp_rows=2, bp_zchunk=8, blocks_per_cu=1),
+              dict(fp_shared=0, fp_rays=4, fp_lds_kb=64, bp_rows=4, bp_zchunk=8, blocks_per_cu=4),
+              dict(fp_shared=0, fp_rays=4, fp_lds_kb=32, bp_rows=4, bp_zchunk=4, blocks_per_cu=8),
+              dict(fp_shared=0, fp_rays=2, fp_lds_kb=128, bp_rows=2, bp_zchunk=16, blocks_per_cu=2),
+              dict(fp_shared=1, bp_rows=4, bp_zchunk=8, blocks_per_cu=4)]
     for sh in shapes:
         with batch.JobEngine(GIB) as eng:
             eng.set_tuning(**sh)

@@ -79,7 +79,7 @@ class tj_job_desc(C.Structure):
 class tj_batch_tuning(C.Structure):
     _fields_ = [("fp_rays", C.c_int), ("fp_lds_kb", C.c_int), ("bp_rows", C.c_int),
                 ("bp_zchunk", C.c_int), ("bp_group", C.c_int), ("blocks_per_cu", C.c_int),
-                ("fp_packed", C.c_int), ("fp_cols", C.c_int)]
+                ("fp_packed", C.c_int), ("fp_cols", C.c_int), ("fp_shared", C.c_int)]
 
 
 # name -> (restype, argtypes); every function returning int is status-checked by call()

@@ -437,6 +437,170 @@ __global__ __launch_bounds__(NT, 4) void k_job_fp(const TjJob* __restrict__ jobs
     }
 }
 
+// The job forward projection with PG parts per block sharing a
+// double-buffered slab (arithmetic in FP32): part h of a 1024-thread block
+// follows the G rays of projections p0 + h G ... over the tile's 256 columns,
+// and all parts project from one staged slab, so one staging serves PG * G
+// projections.  Each thread owns the cells e = tid + q * (threads), q < QF, of
+// every slab: their loads for slab k + 1 are issued before slab k is projected
+// and stored into the other buffer after it (one barrier per slab).  The
+// cells, the positions and the arithmetic are k_job_fp's.
+template <typename T, typename TE, int G, int PG>
+__global__ __launch_bounds__(NT * PG) void k_job_fq(const TjJob* __restrict__ jobs, const int4* __restrict__ tiles,
+                                                int ntiles, int lds_cells)
+{
+    using A = float;
+    using C4 = float4;
+    constexpr int BT = NT * PG;
+    constexpr int QF = 4;  // cells per thread and slab
+    extern __shared__ __align__(16) unsigned char smem[];
+    C4* cells = reinterpret_cast<C4*>(smem);
+    __shared__ double red[BT / 64][G * 4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int part = tid / NT, ctid = tid - part * NT;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int4 tl = tiles[t];
+        if (tl.x < 0) continue;
+        const TjJob& J = jobs[tl.x];
+        const T* vol = (const T*)J.vol;
+        const int W = J.W, D = J.D, pw = J.pw, ph = J.ph, n = J.n;
+        const int W1 = W + 1;
+        // slices per slab: QF cells per thread, two buffers of zc W1 + 2 cells
+        const int zc = min(min(D, (QF * BT) / W1), (lds_cells / 2 - 2) / W1);
+        const int bufc = zc * W1 + 2;
+        const int zero = zc * W1;
+        const int p0 = tl.y + part * G, j0 = tl.z * 4, ct = tl.w;
+        const int i = ct * NT + ctid;
+        const bool col = i < pw;
+        const double ic = col ? i : pw - 1;
+        const size_t plane = (size_t)D * W;
+        bool rin[4];
+        __amdgpu_buffer_rsrc_t rsc[4];
+        A ym[4];
+        bool neary = false;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int y = j0 + r + J.dy;
+            rin[r] = (j0 + r < ph) && y >= 0 && y < J.H;
+            rsc[r] = __builtin_amdgcn_make_buffer_rsrc((void*)(vol + (size_t)(rin[r] ? y : 0) * plane), 0,
+                                                       rin[r] ? (int)(plane * sizeof(T)) : 0, 0x00020000);
+            ym[r] = (y == J.H - 1) ? (A)0 : (A)1;
+            neary = neary || (y == J.H - 1);
+        }
+        A base[G], tn[G], rs[G][4];
+        double bd[G], td[G];
+        int zlo[G], zhi[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const double* q = J.prm + PRM * min(p0 + k, n - 1);
+            bd[k] = ic * q[0] + q[1];
+            td[k] = q[2];
+            base[k] = (A)bd[k];
+            tn[k] = (A)td[k];
+            line_interval(bd[k], td[k], q[7], -TJ_ACCURACY, (double)W, D, zlo[k], zhi[k]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) rs[k][r] = (A)0;
+        }
+        // this thread's cells: slice zq of the slab (zc for none), element
+        // offset oq within the slab's planes (columns past W read W - 1)
+        int zq[QF];
+        uint32_t oq[QF];
+#pragma unroll
+        for (int q = 0; q < QF; ++q) {
+            const int e = tid + q * BT;
+            const int z = e / W1;
+            zq[q] = min(z, zc);
+            oq[q] = (uint32_t)(z * W + min(e - z * W1, W - 1));
+        }
+        A v[QF][4];
+        auto fetch = [&](int zs) {
+            const int zn = min(zc, D - zs);
+            const uint32_t zo = (uint32_t)(zs * W);
+#pragma unroll
+            for (int q = 0; q < QF; ++q)
+                if (zq[q] < zn) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) v[q][r] = bld<A>(vol, rsc[r], zo + oq[q]);
+                }
+        };
+        auto store = [&](C4* c, int zs) {
+            const int zn = min(zc, D - zs);
+#pragma unroll
+            for (int q = 0; q < QF; ++q)
+                if (zq[q] < zn) c[tid + q * BT] = make_float4(v[q][0], v[q][1], v[q][2], v[q][3]);
+        };
+        fetch(0);
+        if (tid < 4) cells[(tid >> 1) * bufc + zero + (tid & 1)] = make_float4(0.f, 0.f, 0.f, 0.f);
+        store(cells, 0);
+        __syncthreads();
+        const A wm1 = (A)(W - 1);
+        int buf = 0;
+        for (int z0 = 0; z0 < D; z0 += zc) {
+            const int zn = min(zc, D - z0);
+            const bool more = z0 + zc < D;  // uniform
+            if (more) fetch(z0 + zc);
+            const int zlast = more ? zn : zn - 1;
+            const C4* cur = cells + (size_t)buf * bufc;
+            if (neary)
+                fp_slab<A, G, true>(cur, zn, zlast, z0, W1, zero, wm1, ym, base, tn, bd, td, zlo, zhi, rs);
+            else
+                fp_slab<A, G, false>(cur, zn, zlast, z0, W1, zero, wm1, ym, base, tn, bd, td, zlo, zhi, rs);
+            if (more) {
+                store(cells + (size_t)(buf ^ 1) * bufc, z0 + zc);
+                __syncthreads();
+                buf ^= 1;
+            }
+        }
+        // projection, norm and difference per (projection, row); error partials
+        double e2[G][4];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            const int p = p0 + k;
+            const double* q = J.prm + PRM * min(p, n - 1);
+            const double ninc = q[3];
+            const int cnt = zhi[k] - zlo[k] + 1;
+            const double nrm = (cnt > 0) ? (double)(cnt + (zhi[k] == D - 1 ? 0 : 1)) * ninc : 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int j = j0 + r;
+                double v2 = 0.0;
+                if (p < n && col && j < ph) {
+                    const size_t o = ((size_t)p * ph + j) * pw + i;
+                    const A norm = rin[r] ? (A)nrm : (A)0;
+                    const A th = (norm > (A)0) ? (A)((double)rs[k][r] * ninc) : (A)NAN;
+                    const A ex = ld<A>((const TE*)J.exps, o);
+                    A d = (A)0;
+                    if (!isnan(ex) && !isnan(th)) {
+                        const double vv = (double)((ex - th) / norm) * J.factor;
+                        d = (A)vv;
+                        v2 = vv * vv;
+                    }
+                    st((A*)J.diff, o, d);
+                }
+                e2[k][r] = wave_sum(v2);
+            }
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < G; ++k)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) red[wave][k * 4 + r] = e2[k][r];
+        }
+        __syncthreads();
+        // per part: the sum of its 4 waves in order
+        if (tid < PG * G * 4) {
+            const int h = tid / (G * 4), e = tid - h * (G * 4);
+            const int p = tl.y + h * G + e / 4, j = j0 + (e & 3);
+            if (p < n && j < ph) {
+                double sum = red[4 * h][e];
+#pragma unroll
+                for (int w = 1; w < NT / 64; ++w) sum += red[4 * h + w][e];
+                gp(J.err_part)[((size_t)p * ph + j) * J.nct + ct] = sum;
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Forward projection for 16-bit storage on packed pairs.  Each LDS cell holds,
 // for 8 volume rows, the two neighbours (v[x], v[x+1]) of a column as the two
@@ -1422,7 +1586,11 @@ struct tj_batch {
     size_t budget = 0, used = 0;
     int next_id = 1;
     std::map<int, BJob*> jobs;
-    tj_batch_tuning tune{4, 128, 4, 16, 2, 8, 2, 2};  // the autotuner's pick at 512^3 BF16
+    // the autotuner's pick at 512^3 BF16; the FP32 forward one group per
+    // 256-thread block (k_job_fq measured 4.28 ms against 3.81 ms at 512^3:
+    // its 2 rays per thread and one block per CU lose more than the shared
+    // slab saves; the autotuner tries it per job mix)
+    tj_batch_tuning tune{4, 128, 4, 16, 2, 8, 2, 2, 0};
     // run plans of the recent job sets, most recently used first (a serving
     // loop alternates between a few sets, each with its tile tables and graph)
     std::list<RunPlan> plans;
@@ -1592,12 +1760,19 @@ static const int kMaxFpLds = 152 * 1024;
 // the 128 registers of four waves per SIMD; four spill).
 static const int kFpqParts = 4;
 static const int kFpqRays = 2;
+// The same for the FP32-arithmetic forward kernel k_job_fq (4 rows per tile,
+// 4 cells per thread and slab).
+static const int kFqParts = 4;
+static const int kFqRays = 2;
 
 template <typename T, typename TE>
 static hipError_t allow_lds2()
 {
     hipError_t e = allow_lds<T, TE, float, 2>(kMaxFpLds);
     if (e == hipSuccess) e = allow_lds<T, TE, float, 4>(kMaxFpLds);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)tjb::k_job_fq<T, TE, kFqRays, kFqParts>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, kMaxFpLds);
     return e;
 }
 
@@ -1630,9 +1805,22 @@ static hipError_t allow_fp_lds()
 
 static int fp_lds_cap(const tj_batch_tuning& tu) { return std::min(std::max(tu.fp_lds_kb, 16), 152) * 1024; }
 
+// k_job_fq takes launches whose widest slab row (W + 1 cells) fits its
+// threads' 4 cells each
+static bool fq_ok(int max_w, const tj_batch_tuning& tu) { return tu.fp_shared && max_w + 1 <= 4 * kFqParts * tjb::NT; }
+
 template <typename T, typename TE>
 static Kern fp_kern_s(const tj_batch_tuning& tu, int max_w, int lds_cu)
 {
+    if (fq_ok(max_w, tu)) {
+        Kern k;
+        k.block = kFqParts * tjb::NT;
+        k.fn = (const void*)tjb::k_job_fq<T, TE, kFqRays, kFqParts>;
+        // one 1024-thread block fills a CU: all the LDS a block may have
+        k.lds = fit_lds(k, kMaxFpLds, 2 * (max_w + 3) * (int)sizeof(float4), lds_cu);
+        k.cells = k.lds / (int)sizeof(float4);
+        return k;
+    }
     if (tu.fp_rays <= 2) return fp_kern_t<T, TE, float, 2>(fp_lds_cap(tu), max_w, lds_cu);
     return fp_kern_t<T, TE, float, 4>(fp_lds_cap(tu), max_w, lds_cu);
 }
@@ -1969,15 +2157,17 @@ static int build_plan(tj_batch* b, const std::vector<int>& ids)
                     fidx.push_back(sidx[m]);
                 }
             if (fj.empty()) continue;
-            // the double-buffered packed kernel's tiles step over all its
-            // parts' projections and cover one 256-column tile each
-            const bool quad = fpq_ok(fj, b->tune, s);
-            std::vector<int4> ft = fp_tiles(fj, fidx, quad ? kFpqRays * kFpqParts : fp_rays_of(b->tune, s),
-                                            fp_rows_of(b->tune, s), quad ? 1 : fp_cols_of(b->tune, s));
-            int4* dft = nullptr;
-            if (int rc = upload(ft, &dft)) return rc;
+            // the double-buffered kernels' tiles step over all their parts'
+            // projections and cover one 256-column tile each
             int max_w = 0;
             for (BJob* j : fj) max_w = std::max(max_w, j->d.vol_w);
+            const bool quad = fpq_ok(fj, b->tune, s);
+            const bool shared = !use_packed(b->tune, s) && s != TJ_STORAGE_F64 && fq_ok(max_w, b->tune);
+            const int fg = quad ? kFpqRays * kFpqParts : shared ? kFqRays * kFqParts : fp_rays_of(b->tune, s);
+            std::vector<int4> ft =
+                fp_tiles(fj, fidx, fg, fp_rows_of(b->tune, s), (quad || shared) ? 1 : fp_cols_of(b->tune, s));
+            int4* dft = nullptr;
+            if (int rc = upload(ft, &dft)) return rc;
             const Kern k = fp_kern(s, ps, b->tune, max_w, b->lds_cu, quad);
             fwd.push_back({0, s, max_w, (int)ft.size(), grid_for(b, (int)ft.size(), k), dft, 0, 0, ps, k});
         }
@@ -2530,7 +2720,7 @@ int tj_batch_set_tuning(tj_batch* b, const tj_batch_tuning* t)
     if (!b || !t) return bfail(TJ_EINVAL, "null argument");
     if (t->fp_rays < 1 || t->fp_lds_kb < 16 || t->fp_lds_kb > 152 || t->bp_rows < 1 || t->bp_zchunk < 1 ||
         t->bp_group < 1 || t->blocks_per_cu < 1 || t->blocks_per_cu > 16 || t->fp_packed < 0 || t->fp_packed > 2 ||
-        t->fp_cols < 0 || t->fp_cols > 2)
+        t->fp_cols < 0 || t->fp_cols > 2 || t->fp_shared < 0 || t->fp_shared > 1)
         return bfail(TJ_EINVAL, "tj_batch_set_tuning: value out of range");
     b->tune = *t;
     return TJ_OK;
@@ -2616,16 +2806,19 @@ int tj_batch_autotune(tj_batch* b, const int* ids, int nids, tj_batch_tuning* be
     double tf = 1e300;
     // the double-buffered packed kernel has one shape (its rays, LDS and
     // one block per CU are fixed)
+    // k_job_fq (shared, double-buffered) likewise has one shape
     for (int pk = any_half ? 0 : 1; pk <= (any_half ? 2 : 1); ++pk)
+    for (int sh = 0; sh <= 1; ++sh)
     for (int cols = 1; cols <= ((any_half && pk == 1) ? 2 : 1); ++cols)
     for (int r : fr)
-        if (pk != 2 || r == kFpqRays)
+        if ((pk != 2 || r == kFpqRays) && (!sh || r == kFqRays))
         for (int l : fl)
-            if (pk != 2 || l == fl[0])
+            if ((pk != 2 && !sh) || l == fl[0])
             for (int c : bpc)
-                if (pk != 2 || c == bpc[0]) {
+                if ((pk != 2 && !sh) || c == bpc[0]) {
                 cur.fp_packed = any_half ? pk : start.fp_packed;
                 cur.fp_cols = (any_half && pk) ? cols : start.fp_cols;
+                cur.fp_shared = sh;
                 cur.fp_rays = r;
                 cur.fp_lds_kb = l;
                 cur.blocks_per_cu = c;
@@ -2665,6 +2858,7 @@ int tj_batch_autotune(tj_batch* b, const int* ids, int nids, tj_batch_tuning* be
     fin.fp_lds_kb = bestf.fp_lds_kb;
     fin.fp_packed = bestf.fp_packed;
     fin.fp_cols = bestf.fp_cols;
+    fin.fp_shared = bestf.fp_shared;
     double tot_best = 1e300;
     tj_batch_tuning pick = fin;
     for (int c : {bestf.blocks_per_cu, bestb.blocks_per_cu}) {
