@@ -94,6 +94,12 @@ def test_project_and_difference(tj, oracle, name):
         for s in range(len(order)):
             assert same(P.getDifferenceImage(s), diffs[s]), (name, s)
         assert np.allclose(errs, ref_errs, rtol=1e-12, atol=0)
+        # getProjection after projectAndDifference: on the single-axis cases the
+        # fused forward + residual kernel wrote the projections and norms too
+        ref_p, ref_n = oracle.project(g, vol, E[order])
+        for s in range(len(order)):
+            assert same(P.getProjection(s), ref_p[s]), (name, s)
+            assert same(P.getNormImage(s), ref_n[s]), (name, s)
 
 
 @pytest.mark.parametrize("name", list(CASES))
