@@ -580,9 +580,11 @@ typedef struct {
     int fp_cols;        /* packed forward: 256-column tiles per block, 1 or 2 (two
                            share one staged slab)                                 */
     int fp_shared;      /* FP32-arithmetic forward (FP32 storage, or 16-bit without
-                           the packed form): 1 four projection groups of 2 rays
-                           per 1024-thread block sharing a double-buffered slab
-                           (widths up to 4095), 0 one group per 256-thread block */
+                           the packed form): 1 (default) four projection groups
+                           of 4 rays per 1024-thread block sharing a
+                           double-buffered slab (widths up to 3071; wider
+                           launches take form 0), 0 one group per 256-thread
+                           block                                                  */
 } tj_batch_tuning;
 
 #define TJ_RUN_GRAPH 1  /* tj_batch_run: capture the iterations in one HIP graph */

@@ -248,7 +248,7 @@ def test_result_same_alone_and_in_mix(batch, oracle):
 @pytest.mark.parametrize("size", [(70, 10, 27), (300, 9, 37)], ids=["one-slab", "partial-slabs"])
 def test_tile_shapes_do_not_change_results(batch, oracle, size):
     """FP32 storage: the one-group kernel (k_job_fp) in several shapes and the
-    shared double-buffered one (k_job_fq; at 300 columns its slabs hold 13
+    shared double-buffered one (k_job_fq; at 300 columns its slabs hold 10
     slices, so 37 slices end in a partial slab) give bit-identical volumes."""
     W, H, D = size
     tilts, E, exps = phantom_case(oracle, W, H, D, tilts=np.arange(-60, 61, 10.0))

@@ -213,25 +213,38 @@ __device__ __forceinline__ void fp_slice_f64(const typename V4<A>::t* cell, int 
     fp_accumulate<A, G, NEARY>(q0, q1, xf, ym, rs);
 }
 
+// The FP64 ray constants (x = z * td + bd) of the thread's G rays, from the
+// job's parameter table (column ic, projections p0 .. p0 + G - 1).
+template <int G>
+__device__ __forceinline__ void ray_f64(const double* prm, int p0, int n, double ic, double (&bd)[G], double (&td)[G])
+{
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+        const double* q = prm + PRM * min(p0 + k, n - 1);
+        bd[k] = ic * q[0] + q[1];
+        td[k] = q[2];
+    }
+}
+
 // One LDS slab of the forward projection: zn slices for the G rays of the
 // thread.  Interpolated samples take their position in the arithmetic type A
 // (the bilinear value is continuous in x, so rounding moves it by a rounding
 // error).  getPixel's nearest-value samples -- the last slice (zlast) and, with
 // NEARY, the last volume row (ym = 0) -- are discontinuous in x, so there the
-// column comes from the FP64 position, as TomoJ computes it.  The main loop
-// has no branch: addresses, then LDS reads, then arithmetic.
-template <typename A, int G, bool NEARY>
+// column comes from the FP64 position, as TomoJ computes it: f64(neary, z, zo,
+// zl) projects such a slice (fp_slice_f64 with the rays' FP64 constants).  The
+// main loop has no branch: addresses, then LDS reads, then arithmetic, for all
+// G rays at once, or with BYRAY ray by ray (fewer registers live: 2 cells of
+// one ray instead of 2 G).
+template <typename A, int G, bool NEARY, typename F64, bool BYRAY = false>
 __device__ __forceinline__ void fp_slab(const typename V4<A>::t* cell, int zn, int zlast, int z0, int W1, int zero,
                                         A wm1, const A (&ym)[4], const A (&base)[G], const A (&tn)[G],
-                                        const double (&bd)[G], const double (&td)[G], const int (&zlo)[G],
-                                        const int (&zhi)[G], A (&rs)[G][4])
+                                        const int (&zlo)[G], const int (&zhi)[G], A (&rs)[G][4], F64 f64)
 {
     using C4 = typename V4<A>::t;
-    const double wd = (double)wm1;
     const int zend = min(zlast, zn);
     if (NEARY) {
-        for (int zz = 0; zz < zend; ++zz)
-            fp_slice_f64<A, G, true>(cell, z0 + zz, zz * W1, zero, (A)1, wd, ym, bd, td, zlo, zhi, rs);
+        for (int zz = 0; zz < zend; ++zz) f64(std::true_type{}, z0 + zz, zz * W1, (A)1);
     } else {
         for (int zz = 0; zz < zend; ++zz) {
             const int z = z0 + zz;
@@ -251,17 +264,27 @@ __device__ __forceinline__ void fp_slab(const typename V4<A>::t* cell, int zn, i
                 xf[k] = frac_of(xs, xb);
                 a[k] = ((z >= zlo[k]) && (z <= zhi[k])) ? zo + xb : zero;
             }
-            C4 q0[G], q1[G];
+            if constexpr (BYRAY) {
 #pragma unroll
-            for (int k = 0; k < G; ++k) {
-                q0[k] = cell[a[k]];
-                q1[k] = cell[a[k] + 1];
+                for (int k = 0; k < G; ++k) {
+                    const C4 q0 = cell[a[k]], q1 = cell[a[k] + 1];
+                    rs[k][0] += fma(xf[k], q1.x - q0.x, q0.x);
+                    rs[k][1] += fma(xf[k], q1.y - q0.y, q0.y);
+                    rs[k][2] += fma(xf[k], q1.z - q0.z, q0.z);
+                    rs[k][3] += fma(xf[k], q1.w - q0.w, q0.w);
+                }
+            } else {
+                C4 q0[G], q1[G];
+#pragma unroll
+                for (int k = 0; k < G; ++k) {
+                    q0[k] = cell[a[k]];
+                    q1[k] = cell[a[k] + 1];
+                }
+                fp_accumulate<A, G, false>(q0, q1, xf, ym, rs);
             }
-            fp_accumulate<A, G, false>(q0, q1, xf, ym, rs);
         }
     }
-    if (zlast < zn)
-        fp_slice_f64<A, G, NEARY>(cell, z0 + zlast, zlast * W1, zero, (A)0, wd, ym, bd, td, zlo, zhi, rs);
+    if (zlast < zn) f64(std::integral_constant<bool, NEARY>{}, z0 + zlast, zlast * W1, (A)0);
 }
 
 // One element of a volume row read through a buffer resource (element index
@@ -382,10 +405,13 @@ __global__ __launch_bounds__(NT, 4) void k_job_fp(const TjJob* __restrict__ jobs
             }
             __syncthreads();
             const int zlast = (z0 + zn == D) ? zn - 1 : zn;
+            auto f64 = [&](auto ny, int z, int zo, A zl) {
+                fp_slice_f64<A, G, decltype(ny)::value>(cell, z, zo, zero, zl, (double)wm1, ym, bd, td, zlo, zhi, rs);
+            };
             if (neary)
-                fp_slab<A, G, true>(cell, zn, zlast, z0, W1, zero, wm1, ym, base, tn, bd, td, zlo, zhi, rs);
+                fp_slab<A, G, true>(cell, zn, zlast, z0, W1, zero, wm1, ym, base, tn, zlo, zhi, rs, f64);
             else
-                fp_slab<A, G, false>(cell, zn, zlast, z0, W1, zero, wm1, ym, base, tn, bd, td, zlo, zhi, rs);
+                fp_slab<A, G, false>(cell, zn, zlast, z0, W1, zero, wm1, ym, base, tn, zlo, zhi, rs, f64);
         }
         // projection, norm and difference per (projection, row); error partials
         double e2[G][4];
@@ -445,6 +471,11 @@ __global__ __launch_bounds__(NT, 4) void k_job_fp(const TjJob* __restrict__ jobs
 // every slab: their loads for slab k + 1 are issued before slab k is projected
 // and stored into the other buffer after it (one barrier per slab).  The
 // cells, the positions and the arithmetic are k_job_fp's.
+// Cells each k_job_fq thread stages per slab (held in registers while the
+// previous slab is projected): with 4, at 4 rays per thread, they spill around
+// the slice loop (a SIRT iteration at 512^3 7.31 against 6.78 ms with 3).
+constexpr int FQ_CELLS = 3;
+
 template <typename T, typename TE, int G, int PG>
 __global__ __launch_bounds__(NT * PG) void k_job_fq(const TjJob* __restrict__ jobs, const int4* __restrict__ tiles,
                                                 int ntiles, int lds_cells)
@@ -452,12 +483,14 @@ __global__ __launch_bounds__(NT * PG) void k_job_fq(const TjJob* __restrict__ jo
     using A = float;
     using C4 = float4;
     constexpr int BT = NT * PG;
-    constexpr int QF = 4;  // cells per thread and slab
+    constexpr int QF = FQ_CELLS;
     extern __shared__ __align__(16) unsigned char smem[];
     C4* cells = reinterpret_cast<C4*>(smem);
     __shared__ double red[BT / 64][G * 4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int part = tid / NT, ctid = tid - part * NT;
+    // the part is uniform over a wave (NT is a multiple of 64): in a scalar
+    // register the parameter rows of its projections are scalar loads
+    const int part = __builtin_amdgcn_readfirstlane(tid / NT), ctid = tid - part * NT;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int4 tl = tiles[t];
         if (tl.x < 0) continue;
@@ -488,18 +521,19 @@ __global__ __launch_bounds__(NT * PG) void k_job_fq(const TjJob* __restrict__ jo
             neary = neary || (y == J.H - 1);
         }
         A base[G], tn[G], rs[G][4];
-        double bd[G], td[G];
         int zlo[G], zhi[G];
+        {
+            double bd[G], td[G];
+            ray_f64<G>(J.prm, p0, n, ic, bd, td);
 #pragma unroll
-        for (int k = 0; k < G; ++k) {
-            const double* q = J.prm + PRM * min(p0 + k, n - 1);
-            bd[k] = ic * q[0] + q[1];
-            td[k] = q[2];
-            base[k] = (A)bd[k];
-            tn[k] = (A)td[k];
-            line_interval(bd[k], td[k], q[7], -TJ_ACCURACY, (double)W, D, zlo[k], zhi[k]);
+            for (int k = 0; k < G; ++k) {
+                base[k] = (A)bd[k];
+                tn[k] = (A)td[k];
+                line_interval(bd[k], td[k], J.prm[PRM * min(p0 + k, n - 1) + 7], -TJ_ACCURACY, (double)W, D, zlo[k],
+                              zhi[k]);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) rs[k][r] = (A)0;
+                for (int r = 0; r < 4; ++r) rs[k][r] = (A)0;
+            }
         }
         // this thread's cells: slice zq of the slab (zc for none), element
         // offset oq within the slab's planes (columns past W read W - 1)
@@ -541,18 +575,27 @@ __global__ __launch_bounds__(NT * PG) void k_job_fq(const TjJob* __restrict__ jo
             if (more) fetch(z0 + zc);
             const int zlast = more ? zn : zn - 1;
             const C4* cur = cells + (size_t)buf * bufc;
+            // the FP64 ray constants are read again where a slice needs them
+            // (they would hold 4 G registers through the slab loop)
+            auto f64 = [&](auto ny, int z, int zo, A zl) {
+                double bd[G], td[G];
+                ray_f64<G>(J.prm, p0, n, ic, bd, td);
+                fp_slice_f64<A, G, decltype(ny)::value>(cur, z, zo, zero, zl, (double)wm1, ym, bd, td, zlo, zhi, rs);
+            };
             if (neary)
-                fp_slab<A, G, true>(cur, zn, zlast, z0, W1, zero, wm1, ym, base, tn, bd, td, zlo, zhi, rs);
+                fp_slab<A, G, true, decltype(f64), true>(cur, zn, zlast, z0, W1, zero, wm1, ym, base, tn, zlo, zhi, rs,
+                                                         f64);
             else
-                fp_slab<A, G, false>(cur, zn, zlast, z0, W1, zero, wm1, ym, base, tn, bd, td, zlo, zhi, rs);
+                fp_slab<A, G, false, decltype(f64), true>(cur, zn, zlast, z0, W1, zero, wm1, ym, base, tn, zlo, zhi,
+                                                          rs, f64);
             if (more) {
                 store(cells + (size_t)(buf ^ 1) * bufc, z0 + zc);
                 __syncthreads();
                 buf ^= 1;
             }
         }
-        // projection, norm and difference per (projection, row); error partials
-        double e2[G][4];
+        // projection, norm and difference per (projection, row); error
+        // partials, each wave's sum stored as soon as it is formed
 #pragma unroll
         for (int k = 0; k < G; ++k) {
             const int p = p0 + k;
@@ -577,14 +620,9 @@ __global__ __launch_bounds__(NT * PG) void k_job_fq(const TjJob* __restrict__ jo
                     }
                     st((A*)J.diff, o, d);
                 }
-                e2[k][r] = wave_sum(v2);
+                const double e2 = wave_sum(v2);
+                if (lane == 0) red[wave][k * 4 + r] = e2;
             }
-        }
-        if (lane == 0) {
-#pragma unroll
-            for (int k = 0; k < G; ++k)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) red[wave][k * 4 + r] = e2[k][r];
         }
         __syncthreads();
         // per part: the sum of its 4 waves in order
@@ -1586,11 +1624,10 @@ struct tj_batch {
     size_t budget = 0, used = 0;
     int next_id = 1;
     std::map<int, BJob*> jobs;
-    // the autotuner's pick at 512^3 BF16; the FP32 forward one group per
-    // 256-thread block (k_job_fq measured 4.28 ms against 3.81 ms at 512^3:
-    // its 2 rays per thread and one block per CU lose more than the shared
-    // slab saves; the autotuner tries it per job mix)
-    tj_batch_tuning tune{4, 128, 4, 16, 2, 8, 2, 2, 0};
+    // the autotuner's pick at 512^3 BF16; the FP32 forward on the shared
+    // double-buffered slab (k_job_fq: per SIRT iteration 6.78 against 7.28 ms
+    // with k_job_fp at 512^3, 31.4 against 43.7 ms at 1024x1024x512)
+    tj_batch_tuning tune{4, 128, 4, 16, 2, 8, 2, 2, 1};
     // run plans of the recent job sets, most recently used first (a serving
     // loop alternates between a few sets, each with its tile tables and graph)
     std::list<RunPlan> plans;
@@ -1763,7 +1800,7 @@ static const int kFpqRays = 2;
 // The same for the FP32-arithmetic forward kernel k_job_fq (4 rows per tile,
 // 4 cells per thread and slab).
 static const int kFqParts = 4;
-static const int kFqRays = 2;
+static const int kFqRays = 4;
 
 template <typename T, typename TE>
 static hipError_t allow_lds2()
@@ -1806,8 +1843,11 @@ static hipError_t allow_fp_lds()
 static int fp_lds_cap(const tj_batch_tuning& tu) { return std::min(std::max(tu.fp_lds_kb, 16), 152) * 1024; }
 
 // k_job_fq takes launches whose widest slab row (W + 1 cells) fits its
-// threads' 4 cells each
-static bool fq_ok(int max_w, const tj_batch_tuning& tu) { return tu.fp_shared && max_w + 1 <= 4 * kFqParts * tjb::NT; }
+// threads' FQ_CELLS cells each
+static bool fq_ok(int max_w, const tj_batch_tuning& tu)
+{
+    return tu.fp_shared && max_w + 1 <= tjb::FQ_CELLS * kFqParts * tjb::NT;
+}
 
 template <typename T, typename TE>
 static Kern fp_kern_s(const tj_batch_tuning& tu, int max_w, int lds_cu)
