@@ -789,7 +789,8 @@ __device__ __forceinline__ void fpp_project(const uint4* cell, int hp, int ss, i
             const float zf = (float)(z0 + zz);
             // byte address of index zz*ss - mg/2: cell c + mg then sits at
             // ((c + mg) >> 1) + ((c + mg) & 1) hb
-            const uint32_t sbz = sb + 16u * (uint32_t)(zz * ss - mgh);
+            uint32_t sbz = sb + 16u * (uint32_t)(zz * ss - mgh);
+            asm("" : "+s"(sbz));  // one v_lshl_add per ray below, not an add and a shift-add
             uint32_t a[G], w[G];
 #pragma unroll
             for (int k = 0; k < G; ++k) {
@@ -1038,6 +1039,12 @@ __global__ __launch_bounds__(NT * CT) void k_job_fpp(const TjJob* __restrict__ j
     }
 }
 
+// Cells per plane of a k_job_fpq slab buffer.  A compile-time plane stride
+// lets the read of a cell's rows 4-7 take the stride as the LDS instruction's
+// immediate offset (16 x 2432 bytes); the host gives the kernel the 4 planes of
+// its two buffers (152 KiB).
+constexpr int FPQ_PLANE = 2432;
+
 // The packed forward projection with a double-buffered slab.  A block of
 // PG * 256 threads holds PG parts, each the G rays of its own projections
 // (p0 + part G ...) over the tile's 256 columns, and all parts share the
@@ -1048,9 +1055,10 @@ __global__ __launch_bounds__(NT * CT) void k_job_fpp(const TjJob* __restrict__ j
 // slice per slab (zc = threads / (W / 2) slices, at most what the LDS holds);
 // rows outside the volume or the detector read zeros through the buffer range
 // check.  Requires an even W; the cells and the arithmetic are k_job_fpp's.
+// The slab planes are FPQ_PLANE cells apart (zc ss <= FPQ_PLANE).
 template <typename T, typename TE, int G, int PG>
 __global__ __launch_bounds__(NT * PG) void k_job_fpq(const TjJob* __restrict__ jobs, const int4* __restrict__ tiles,
-                                                int ntiles, int lds_cells)
+                                                int ntiles, int /*lds_cells: 4 FPQ_PLANE*/)
 {
     constexpr int RB = 8;
     constexpr int BT = NT * PG;
@@ -1069,10 +1077,10 @@ __global__ __launch_bounds__(NT * PG) void k_job_fpq(const TjJob* __restrict__ j
         const int hb = (((WC + 1) / 2 + 3) & ~7) + 4;
         const int ss = hb + WC / 2;
         const int half = W >> 1;
-        // slices per slab: one column pair per thread, two buffers of two planes
-        const int zc = min(min(D, BT / half), lds_cells / (4 * ss));
-        const int hp = zc * ss;       // rows 4-7 plane within a buffer
-        const int bufc = 2 * hp;      // cells per buffer
+        // slices per slab: one column pair per thread, as many as a plane holds
+        const int zc = min(min(D, BT / half), FPQ_PLANE / ss);
+        constexpr int hp = FPQ_PLANE;    // rows 4-7 plane within a buffer
+        constexpr int bufc = 2 * hp;     // cells per buffer
         const int p0 = tl.y + part * G, j0 = tl.z * RB;
         const int i = tl.w * NT + ctid;
         const bool col = i < pw;
@@ -1900,8 +1908,7 @@ static bool fpq_ok(const std::vector<BJob*>& js, const tj_batch_tuning& tu, int 
     if (!use_packed(tu, storage) || tu.fp_packed != 2) return false;
     for (const BJob* j : js) {
         const int w = j->d.vol_w;
-        if ((w & 1) || w / 2 > kFpqParts * tjb::NT || 4 * fpp_slice_cells(w) * (int)sizeof(uint4) > kMaxFpLds)
-            return false;
+        if ((w & 1) || w / 2 > kFpqParts * tjb::NT || fpp_slice_cells(w) > tjb::FPQ_PLANE) return false;
     }
     return true;
 }
@@ -1913,9 +1920,10 @@ static Kern fpp_kern_s(const tj_batch_tuning& tu, int max_w, int lds_cu, bool qu
         Kern k;
         k.block = kFpqParts * tjb::NT;
         k.fn = (const void*)tjb::k_job_fpq<T, TE, kFpqRays, kFpqParts>;
-        // one 1024-thread block fills a CU: it takes all the LDS a block may
-        // have (zc = 4 slices at W = 512, 2 at W = 1024)
-        k.lds = fit_lds(k, kMaxFpLds, 4 * fpp_slice_cells(max_w) * (int)sizeof(uint4), lds_cu);
+        // one 1024-thread block fills a CU: two buffers of two fixed planes
+        // (zc = 4 slices at W = 512, 2 at W = 1024)
+        static_assert(4 * tjb::FPQ_PLANE * (int)sizeof(uint4) <= kMaxFpLds, "k_job_fpq slab buffers");
+        k.lds = 4 * tjb::FPQ_PLANE * (int)sizeof(uint4);
         k.cells = k.lds / (int)sizeof(uint4);
         return k;
     }
