@@ -269,6 +269,33 @@ def test_tile_shapes_do_not_change_results(batch, oracle, size):
         assert np.array_equal(results[0], r)
 
 
+def test_f32_forward_forms_on_shorter_narrower_detector(batch, oracle):
+    """FP32 storage with positivity on a detector shorter and narrower than the
+    volume (rows outside the detector stay untouched, rays that miss the
+    volume read zero cells): the shared double-buffered forward (k_job_fq,
+    fp_shared 1) and the one-group one (k_job_fp, fp_shared 0) give
+    bit-identical volumes, within the FP32 bound of the FP64 oracle."""
+    W, H, D, pw, ph = 52, 14, 20, 44, 10
+    tilts = list(np.arange(-50, 51, 10.0))
+    E = oracle.tilt_eulers(tilts)
+    ell = oracle.ellipsoids(13, 5, pw, ph, D)
+    exps = oracle.phantom_project(ell, E, pw, ph).astype(np.float32) - np.float32(1.0)
+    got = []
+    for sh in (1, 0):
+        with batch.JobEngine(GIB) as eng:
+            eng.set_tuning(fp_shared=sh)
+            jid = eng.add(batch.JobSpec(W, H, D, tilts, storage="f32", positivity=True, pw=pw, ph=ph))
+            eng.set_projections(jid, exps)
+            eng.run([jid], 3)
+            got.append(eng.get_volume(jid))
+    assert np.array_equal(got[0], got[1])
+    g = oracle.make_geom(W, H, D, pw, ph, positivity=True, nthreads=8)
+    ref = np.zeros((D, H, W), np.float64)
+    oracle.ossart(g, ref, exps, E, 3, 1.0, len(E), f64=True)
+    assert rel_l2(got[0], ref) <= 1e-5, rel_l2(got[0], ref)
+    assert (got[0] >= 0).all()
+
+
 def test_autotune_keeps_volumes_and_caches(batch, oracle, tmp_path):
     W, H, D = 64, 16, 24
     tilts, E, exps = phantom_case(oracle, W, H, D)
