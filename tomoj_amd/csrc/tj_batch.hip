@@ -1348,6 +1348,10 @@ __global__ __launch_bounds__(NT) void k_job_bp(const TjJob* __restrict__ jobs, c
                     // address is one shift-add)
                     const uint32_t cg = __builtin_amdgcn_readfirstlane(
                         (uint32_t)(size_t)(const TJ_LDS uint32_t*)&ldsf[buf][g][0][0] - 4u * (uint32_t)s_x0[buf][g]);
+                    // 1.0 in a scalar register: with both halves converted from FP32
+                    // registers the weight pair is one packed conversion
+                    float one = 1.0f;
+                    asm("" : "+s"(one));
                     // 4 slices at a time: positions, then all LDS reads, then the sums
 #pragma unroll
                     for (int k4 = 0; k4 < KC; k4 += 4) {
@@ -1358,7 +1362,7 @@ __global__ __launch_bounds__(NT) void k_job_bp(const TjJob* __restrict__ jobs, c
                             const float xs =
                                 __builtin_amdgcn_fmed3f(fmaf((float)(k0 + k4 + u), ms, xe1), 0.0f, cmax);
                             c[u] = (int)xs;
-                            wt[u] = h2_t{(_Float16)1.0f, (_Float16)__builtin_amdgcn_fractf(xs)};
+                            wt[u] = h2_t{(_Float16)one, (_Float16)__builtin_amdgcn_fractf(xs)};
                         }
                         uint32_t q[4][R];
 #pragma unroll
