@@ -213,6 +213,32 @@ __device__ __forceinline__ void fp_slice_f64(const typename V4<A>::t* cell, int 
     fp_accumulate<A, G, NEARY>(q0, q1, xf, ym, rs);
 }
 
+// x clamped to [0, hi] (hi >= 0, x finite): one v_med3 in FP32.
+template <typename A>
+__device__ __forceinline__ A clamp0(A x, A hi)
+{
+    return fmin(fmax(x, (A)0), hi);
+}
+template <>
+__device__ __forceinline__ float clamp0<float>(float x, float hi)
+{
+    return __builtin_amdgcn_fmed3f(x, 0.0f, hi);
+}
+
+// The A4 cell at LDS byte address a.
+template <typename A>
+__device__ __forceinline__ typename V4<A>::t lds_cell(uint32_t a)
+{
+    typedef A v4 __attribute__((ext_vector_type(4)));
+    const v4 t = *(const TJ_LDS v4*)a;
+    typename V4<A>::t c;
+    c.x = t.x;
+    c.y = t.y;
+    c.z = t.z;
+    c.w = t.w;
+    return c;
+}
+
 // The FP64 ray constants (x = z * td + bd) of the thread's G rays, from the
 // job's parameter table (column ic, projections p0 .. p0 + G - 1).
 template <int G>
@@ -250,7 +276,13 @@ __device__ __forceinline__ void fp_slab(const typename V4<A>::t* cell, int zn, i
             const int z = z0 + zz;
             const int zo = zz * W1;
             const A zf = (A)z;
-            int a[G];
+            // LDS byte address of the slice's first cell, kept in a scalar
+            // register, so a cell address is one shift-add of its index; the
+            // zero cell counted from the slice
+            uint32_t sz = (uint32_t)(size_t)(const TJ_LDS C4*)(cell + zo);
+            asm("" : "+s"(sz));
+            const int zrel = zero - zo;
+            uint32_t a[G];
             A xf[G];
 #pragma unroll
             for (int k = 0; k < G; ++k) {
@@ -259,15 +291,16 @@ __device__ __forceinline__ void fp_slab(const typename V4<A>::t* cell, int zn, i
                 // fraction 0 (getPixel), x >= W-1 the last column with fraction
                 // 0 (its replicated neighbour makes the fraction irrelevant).
                 // Outside, the lane reads the zero cell.
-                const A xs = fmin(fmax(fma(zf, tn[k], base[k]), (A)0), wm1);
+                const A xs = clamp0(fma(zf, tn[k], base[k]), wm1);
                 const int xb = (int)xs;
                 xf[k] = frac_of(xs, xb);
-                a[k] = ((z >= zlo[k]) && (z <= zhi[k])) ? zo + xb : zero;
+                const int c = ((z >= zlo[k]) && (z <= zhi[k])) ? xb : zrel;
+                a[k] = sz + (uint32_t)sizeof(C4) * (uint32_t)c;
             }
             if constexpr (BYRAY) {
 #pragma unroll
                 for (int k = 0; k < G; ++k) {
-                    const C4 q0 = cell[a[k]], q1 = cell[a[k] + 1];
+                    const C4 q0 = lds_cell<A>(a[k]), q1 = lds_cell<A>(a[k] + (uint32_t)sizeof(C4));
                     rs[k][0] += fma(xf[k], q1.x - q0.x, q0.x);
                     rs[k][1] += fma(xf[k], q1.y - q0.y, q0.y);
                     rs[k][2] += fma(xf[k], q1.z - q0.z, q0.z);
@@ -277,8 +310,8 @@ __device__ __forceinline__ void fp_slab(const typename V4<A>::t* cell, int zn, i
                 C4 q0[G], q1[G];
 #pragma unroll
                 for (int k = 0; k < G; ++k) {
-                    q0[k] = cell[a[k]];
-                    q1[k] = cell[a[k] + 1];
+                    q0[k] = lds_cell<A>(a[k]);
+                    q1[k] = lds_cell<A>(a[k] + (uint32_t)sizeof(C4));
                 }
                 fp_accumulate<A, G, false>(q0, q1, xf, ym, rs);
             }
