@@ -9,6 +9,12 @@
 #include <math.h>
 #include <type_traits>
 
+// LDS (address space 3) pointers: some kernels keep LDS byte addresses as
+// 32-bit values in registers and turn them into such pointers where they read
+// them; the host pass, which sees 64-bit pointers, would warn about each cast.
+#define TJ_LDS __attribute__((address_space(3)))
+#pragma clang diagnostic ignored "-Wint-to-pointer-cast"
+
 namespace tj {
 
 // ---------------------------------------------------------------------------
@@ -946,15 +952,29 @@ __global__ __launch_bounds__(TJ_BLOCK) void k_norm_table(TjGeomDev g, const TjPr
 // NG: the rays of the thread that belong to projections of the launch (the
 // last group of a launch can hold fewer than G); only they are marched.
 //
+// The float4 at LDS byte address a.
+__device__ __forceinline__ float4 lds_f4(uint32_t a)
+{
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    const f4v t = *(const TJ_LDS f4v*)a;
+    return make_float4(t.x, t.y, t.z, t.w);
+}
+
 // One slice of rays: the same arithmetic for both staging forms below.
-template <int G, int NG, bool NEARY>
-__device__ __forceinline__ void fp_lds_slice(const TjGeomDev& g, const float4* lds4, int zo, int zero_idx, float zl,
+template <int G, int NG, bool NEARY, bool LAST>
+__device__ __forceinline__ void fp_lds_slice(const TjGeomDev& g, const float4* lds4, int zo, int zero_idx,
                                              const float (&ym)[4], double (&xc)[G], const double (&dirx)[G],
                                              float4 (&rs)[G])
 {
     const double nacc = -TJ_ACCURACY;  // x + ACCURACY >= 0  <=>  x >= -ACCURACY
     const double bx = g.bxmax;
     const double wm1 = g.vw - 1;
+    // LDS byte address of the slice's first cell, kept in a scalar register, so
+    // a cell address is one shift-add of its index; the zero cell counted from
+    // the slice
+    uint32_t sz = __builtin_amdgcn_readfirstlane((uint32_t)(size_t)(const TJ_LDS float4*)(lds4 + zo));
+    asm("" : "+s"(sz));
+    const int zrel = zero_idx - zo;
 #pragma unroll
     for (int k = 0; k < NG; ++k) {
         xc[k] = xc[k] + dirx[k];
@@ -965,10 +985,12 @@ __device__ __forceinline__ void fp_lds_slice(const TjGeomDev& g, const float4* l
         // replicated column) so the fraction does not matter
         const double xs = fmin(fmax(xv, 0.0), wm1);
         const int xb = (int)xs;
-        const float xf = ((float)xs - (float)xb) * zl;
-        const int a = in ? zo + xb : zero_idx;
-        const float4 q0 = lds4[a];
-        const float4 q1 = lds4[a + 1];
+        // the last slice takes the nearest value (getPixel's z == D-1 rule):
+        // its fraction times 0
+        float xf = (float)xs - (float)xb;
+        if (LAST) xf = xf * 0.0f;
+        const uint32_t a = sz + 16u * (uint32_t)(in ? xb : zrel);
+        const float4 q0 = lds_f4(a), q1 = lds_f4(a + 16u);
         if (NEARY) {
             rs[k].x += q0.x + (xf * ym[0]) * (q1.x - q0.x);
             rs[k].y += q0.y + (xf * ym[1]) * (q1.y - q0.y);
@@ -1052,8 +1074,9 @@ __device__ __forceinline__ void fp_lds_chunks_pf(const TjGeomDev& g, const float
         const float4* cur = lds4 + buf * bufsz;
         // the last slice takes the nearest value (getPixel's z == D-1 rule)
         const int zlast = more ? zn : zn - 1;
-        for (int zz = 0; zz < zn; ++zz)
-            fp_lds_slice<G, NG, NEARY>(g, cur, zz * W1, zero_idx, (zz == zlast) ? 0.0f : 1.0f, ym, xc, dirx, rs);
+        for (int zz = 0; zz < zlast; ++zz)
+            fp_lds_slice<G, NG, NEARY, false>(g, cur, zz * W1, zero_idx, ym, xc, dirx, rs);
+        if (zlast < zn) fp_lds_slice<G, NG, NEARY, true>(g, cur, zlast * W1, zero_idx, ym, xc, dirx, rs);
         if (more) {
             store(lds4 + (buf ^ 1) * bufsz, z0 + zc);
             __syncthreads();
@@ -1113,8 +1136,9 @@ __device__ __forceinline__ void fp_lds_chunks(const TjGeomDev& g, const float* _
         // the last slice takes the nearest value (getPixel's z == D-1 rule): its
         // fraction is multiplied by 0
         const int zlast = (z0 + zn == g.vd) ? zn - 1 : zn;
-        for (int zz = 0; zz < zn; ++zz)
-            fp_lds_slice<G, NG, NEARY>(g, lds4, zz * W1, zero_idx, (zz == zlast) ? 0.0f : 1.0f, ym, xc, dirx, rs);
+        for (int zz = 0; zz < zlast; ++zz)
+            fp_lds_slice<G, NG, NEARY, false>(g, lds4, zz * W1, zero_idx, ym, xc, dirx, rs);
+        if (zlast < zn) fp_lds_slice<G, NG, NEARY, true>(g, lds4, zlast * W1, zero_idx, ym, xc, dirx, rs);
     }
 }
 
