@@ -19,8 +19,9 @@ Roofline figures: `roofline` is the dominant kernel (algorithmic bytes per
 launch over its HIP-event time, with its counter-measured traffic and its VALU
 issue fraction); `hbm_iteration` is the counter-measured HBM bytes of a whole
 iteration (profiles/traffic.json) over the measured iteration time; the
-`iteration_model` entry is SURVEY's per-projection streaming model, a model
-figure that counts more volume reads than the kernels make.
+`iteration_model` entry is the per-projection streaming model (one volume
+read per projection), a model figure that counts more volume reads than the
+kernels make.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-forms]
        python bench.py --workload sirt_512x512x512_121 ...   (another timed workload)
@@ -54,7 +55,7 @@ def kernel_bytes(w):
     storage), each the least any implementation of the step must move.  fp:
     the forward projection fused with the residual, the volume read once per
     subset launch (every projection of the subset samples all of it) plus the
-    measured images read and the differences written (SURVEY's per-projection
+    measured images read and the differences written (the per-projection
     streaming model, one volume read per projection, is reported apart as
     `iteration_model`: it counts reads no implementation needs to make).  res:
     the fold of the per-block error partials and row flags.  bpprep: the
@@ -393,7 +394,7 @@ def main():
         # honest bandwidth figure of the path (it is bound by VALU issue and
         # LDS, not by HBM)
         "hbm_iteration": hbm_iteration(w, tim, args.steps, ms, traffic),
-        # SURVEY's per-projection slab-streaming model: it counts one volume
+        # the per-projection slab-streaming model: it counts one volume
         # read per projection, which the LDS kernels never make (they read the
         # volume about once per subset), so it is a model figure, above the
         # bytes that cross HBM, not a measurement of HBM use; it carries no
