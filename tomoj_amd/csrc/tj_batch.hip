@@ -496,19 +496,19 @@ __global__ __launch_bounds__(NT, 4) void k_job_fp(const TjJob* __restrict__ jobs
     }
 }
 
-// The job forward projection with PG parts per block sharing a
-// double-buffered slab (arithmetic in FP32): part h of a 1024-thread block
-// follows the G rays of projections p0 + h G ... over the tile's 256 columns,
-// and all parts project from one staged slab, so one staging serves PG * G
-// projections.  Each thread owns the cells e = tid + q * (threads), q < QF, of
-// every slab: their loads for slab k + 1 are issued before slab k is projected
-// and stored into the other buffer after it (one barrier per slab).  The
-// cells, the positions and the arithmetic are k_job_fp's.
 // Cells each k_job_fq thread stages per slab (held in registers while the
 // previous slab is projected): with 4, at 4 rays per thread, they spill around
 // the slice loop (a SIRT iteration at 512^3 7.31 against 6.78 ms with 3).
 constexpr int FQ_CELLS = 3;
 
+// The job forward projection with PG parts per block sharing a
+// double-buffered slab (arithmetic in FP32): part h of a 1024-thread block
+// follows the G rays of projections p0 + h G ... over the tile's 256 columns,
+// and all parts project from one staged slab, so one staging serves PG * G
+// projections.  Each thread owns the cells e = tid + q * (threads),
+// q < FQ_CELLS, of every slab: their loads for slab k + 1 are issued before
+// slab k is projected and stored into the other buffer after it (one barrier
+// per slab).  The cells, the positions and the arithmetic are k_job_fp's.
 template <typename T, typename TE, int G, int PG>
 __global__ __launch_bounds__(NT * PG) void k_job_fq(const TjJob* __restrict__ jobs, const int4* __restrict__ tiles,
                                                 int ntiles, int lds_cells)
@@ -1843,7 +1843,7 @@ static const int kMaxFpLds = 152 * 1024;
 static const int kFpqParts = 4;
 static const int kFpqRays = 2;
 // The same for the FP32-arithmetic forward kernel k_job_fq (4 rows per tile,
-// 4 cells per thread and slab).
+// FQ_CELLS cells per thread and slab).
 static const int kFqParts = 4;
 static const int kFqRays = 4;
 
