@@ -282,10 +282,21 @@ def test_order_shuffle_perpendicular_chain(oracle):
 
 
 def test_python_order_shuffle_matches_oracle(oracle):
+    """The library's orderShuffleART (tj_order_shuffle, host code) against the
+    oracle: uniform tilts, and seeded uneven tilt sets with a repeated angle
+    (equal |cos| distances, where the first minimum in index order wins)."""
     from tomoj_amd.reconstruction import order_shuffle
     E = oracle.tilt_eulers(np.arange(-60, 61, 1.0))
     for first in (0, 17, 60, 120):
         assert list(order_shuffle(E, first)) == list(oracle.order_shuffle(E, first))
+    rng = np.random.default_rng(41)
+    for n in (2, 7, 41, 80):
+        tilts = np.sort(rng.uniform(-70.0, 70.0, n))
+        if n > 2:
+            tilts[n // 2] = tilts[n // 2 - 1]
+        E = oracle.tilt_eulers(tilts)
+        for first in (0, n // 3, n - 1):
+            assert list(order_shuffle(E, first)) == list(oracle.order_shuffle(E, first))
 
 
 def test_sirt_fixed_point(oracle):
