@@ -157,6 +157,8 @@ AWKWARD = [
     (64, 14, 30, 64, 10, "f16", True),    # detector shorter than the volume, positivity after each image
     (50, 16, 17, 40, 16, "bf16", True),   # detector narrower than the volume, positivity
     (96, 24, 40, 96, 24, "f16", False),   # word staging, several row tiles
+    (2048, 6, 5, 2048, 6, "bf16", False),  # widest double-buffered slab: one slice per slab
+    (2050, 5, 4, 2050, 5, "f16", False),   # wider: the single-buffer kernel takes both forms
 ]
 
 
