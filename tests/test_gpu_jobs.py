@@ -76,6 +76,28 @@ def test_f32_storage_within_parity_bound(batch, oracle, shape):
     assert rel_l2(got, ref) <= 1e-5, rel_l2(got, ref)
 
 
+@pytest.mark.parametrize("shape", [(3071, 4, 3), (3073, 3, 2)], ids=["3071-shared", "3073-one-group"])
+def test_f32_storage_at_the_shared_forward_width_limit(batch, oracle, shape):
+    """3071 columns: the widest shared FP32 forward slab (one slice per slab);
+    3073: the one-group kernel takes the launch.  Both forms give
+    bit-identical volumes.  The FP32 positions round to an ulp of 2^-12 px at
+    x ~ 3000, ten times that of 300 columns, so the bound against the FP64
+    oracle is 1e-4 here (3.5e-5 and 4.5e-5 measured)."""
+    W, H, D = shape
+    tilts, E, exps = phantom_case(oracle, W, H, D, tilts=np.arange(-57, 58, 9.5))
+    vols = []
+    for sh in (1, 0):
+        with batch.JobEngine(GIB) as eng:
+            eng.set_tuning(fp_shared=sh)
+            jid = eng.add(batch.JobSpec(W, H, D, tilts, storage="f32"))
+            eng.set_projections(jid, exps)
+            eng.run([jid], 3)
+            vols.append(eng.get_volume(jid))
+    assert np.array_equal(vols[0], vols[1])
+    ref, _ = oracle_sirt(oracle, W, H, D, exps, E, 3)
+    assert rel_l2(vols[0], ref) <= 1e-4, rel_l2(vols[0], ref)
+
+
 def test_positivity_and_taller_volume(batch, oracle):
     """Positivity after each image; a volume taller than the detector (rows
     outside the detector stay untouched, rays through them are NaN)."""
