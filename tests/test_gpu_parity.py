@@ -270,6 +270,25 @@ def test_ossart_loop_bit_exact(tj, oracle, update):
     assert rel_l2(got, ref64) < 1e-5
 
 
+def test_wide_detector_ossart_bit_exact(tj, oracle):
+    """3000 columns: a slab slice of the forward kernel is 48 KB, too wide for
+    two slices per buffer of the double-buffered slab, so the launch takes the
+    single-buffer slab (and 12 column tiles); the loop stays bit-identical."""
+    W, H, D = 3000, 4, 6
+    tilts = np.arange(-60, 61, 12.0)
+    E = oracle.tilt_eulers(tilts)
+    ell = oracle.ellipsoids(23, 5, W, H, D)
+    exps = oracle.phantom_project(ell, E, W, H).astype(np.float32)
+    with make(tj, (W, H, D)) as P:
+        P.load_tilt_series(exps, E)
+        err = P.run_ossart(2, 1.0, 4, first_index=2)
+        got = P.get_volume()
+    ref = np.zeros((D, H, W), np.float32)
+    ref_err = oracle.ossart(ogeom(oracle, P), ref, exps, E, 2, 1.0, 4, first_index=2)
+    assert same(got, ref)
+    assert np.allclose(err[:, [0, 1, 3]], ref_err[:, [0, 1, 3]], rtol=1e-12)
+
+
 def test_stacked_dual_axis_series_bit_exact(tj, oracle):
     """A dual-axis reconstruction as TomoJ's dual application runs it: OS-SART
     over a TiltSeriesStack (TiltSeriesStack.java:84-95), the second series'
