@@ -366,6 +366,27 @@ def test_synthetic_job_matches_oracle_phantom(batch, oracle):
     assert np.allclose(got, clean.astype(np.float32), rtol=1e-6, atol=1e-5)
 
 
+def test_tunings_out_of_range_are_refused(batch, tj):
+    """tj_batch_set_tuning and tj_set_tuning refuse values outside their ranges
+    and keep the tuning they had; a new engine reports the documented defaults
+    (packed double-buffered 16-bit forward, shared FP32 forward)."""
+    from tomoj_amd import TjError
+    with batch.JobEngine(GIB) as eng:
+        before = eng.get_tuning()
+        assert before["fp_packed"] == 2 and before["fp_shared"] == 1
+        for bad in (dict(fp_shared=2), dict(fp_shared=-1), dict(fp_packed=3), dict(fp_lds_kb=200),
+                    dict(blocks_per_cu=17)):
+            with pytest.raises(TjError):
+                eng.set_tuning(**bad)
+            assert eng.get_tuning() == before
+    with tj.VoxelProjector3DHIP((16, 8, 8)) as P:
+        before = P.get_tuning()
+        for bad in (dict(fp_prefetch=2), dict(fp_groups=3)):
+            with pytest.raises(TjError):
+                P.set_tuning(**bad)
+            assert P.get_tuning() == before
+
+
 def test_bad_jobs_are_rejected(batch, oracle):
     with batch.JobEngine(GIB) as eng:
         with pytest.raises(batch._lib.TjError):
